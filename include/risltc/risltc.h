@@ -1,0 +1,349 @@
+/*
+ * risltc.h -- C-ABI of the MI355X (gfx950) RIS-LTC shading pass.
+ *
+ * This header is the drop-in boundary. It replaces the Vulkan shading subpass
+ * of the reference renderer: descriptor set 0 of the shading pass
+ * (src/main.c:845-856) plus the full-screen draw in subpass 1
+ * (src/main.c:2050-2055) become one call, risltc_shade(). The host data that
+ * the reference uploads keeps its byte layout:
+ *   - light records: exactly what write_lights() packs (src/main.c:456-490),
+ *     i.e. polygonal_light_upload_t (src/polygonal_light.h:104-111) followed
+ *     by max_vertex_count vertices of 16 bytes each;
+ *   - LTC tables: the two UNORM16 texture arrays that load_ltc_table()
+ *     builds (src/ltc_table.c:82-116) and ltc_constants_t
+ *     (src/ltc_table.h:23-35);
+ *   - per-frame constants: per_frame_constants_t (src/main.h:537-553), the
+ *     std140 UBO of src/shaders/shared_constants.glsl:21-60;
+ *   - mesh buffers: the quantised *.vks layout of mesh_t (src/scene.h:47-70):
+ *     2 x uint32 positions per vertex, 4 x uint16 normal+uv per vertex, one
+ *     uint8 material index per triangle (src/scene.c:79-83);
+ *   - visibility buffer: R32_UINT primitive ids, 0xFFFFFFFF = background,
+ *     bit 31 = light (src/shaders/visibility_pass.vert.glsl:30-37).
+ *
+ * Conventions follow the reference host code: caller-allocated structs,
+ * int return with 0 = success (src/ltc_table.c:38-42), plain C types only,
+ * extern "C" for C++ callers (src/polygonal_light.h:21-28). Large per-pixel
+ * buffers (visibility, output, debug records) are DEVICE pointers (HIP
+ * device memory) owned by the caller; everything else is host memory that
+ * is copied before the call returns. A context is not thread-safe.
+ */
+#ifndef RISLTC_H
+#define RISLTC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Error codes                                                               */
+/* ------------------------------------------------------------------------ */
+#define RISLTC_OK 0
+#define RISLTC_E_ARG 1         /* invalid argument (null pointer, bad size)  */
+#define RISLTC_E_HIP 2         /* HIP runtime error                          */
+#define RISLTC_E_FORMAT 3      /* malformed input data (file, record)        */
+#define RISLTC_E_UNSUPPORTED 4 /* setting outside what the kernels support   */
+#define RISLTC_E_STATE 5       /* call order: e.g. shade before set_lights   */
+#define RISLTC_E_IO 6          /* file could not be opened or read           */
+
+/* Largest light polygon (vertices before clipping). The reference clipper
+ * encodes the count in 3 bits and so handles at most 7
+ * (src/shaders/polygon_clipping.glsl:39-42); this build extends it to 8. */
+#define RISLTC_MAX_LIGHT_VERTEX_COUNT 8
+/* Vertices after clipping: one more than before (src/main.c:191-204). */
+#define RISLTC_MAX_POLYGON_VERTEX_COUNT 9
+
+/* ------------------------------------------------------------------------ */
+/* Host data model (byte-compatible with the reference)                      */
+/* ------------------------------------------------------------------------ */
+
+/* polygonal_light_upload_t (src/polygonal_light.h:104-111); 48 bytes. In the
+ * buffer each header is followed by max_vertex_count vertices, 16 bytes each
+ * (xyz + padding); write_lights() repeats vertex 0 after the last vertex
+ * (src/main.c:483-488). */
+typedef struct risltc_light_upload_s {
+    float surface_radiance[3];
+    float padding_1;
+    float plane[4];
+    uint32_t vertex_count;
+    float padding_2[3];
+} risltc_light_upload_t;
+
+#define RISLTC_LIGHT_RECORD_SIZE(max_vertex_count) (48u + 16u * (uint32_t)(max_vertex_count))
+
+/* polygonal_light_t (src/polygonal_light.h:73-99): host-side light in plane
+ * space, turned into world space by risltc_update_polygonal_light(). */
+typedef struct risltc_polygonal_light_s {
+    float rotation_angles[3];
+    float scaling_x;
+    float translation[3];
+    float scaling_y;
+    float radiant_flux[3];
+    float inv_scaling_x;
+    float surface_radiance[3];
+    float inv_scaling_y;
+    float plane[4];
+    uint32_t vertex_count;
+    uint32_t texturing_technique;
+    uint32_t texture_index;
+    uint32_t padding_0;
+    float rotation[3][4];
+    float area, rcp_area;
+    float padding_1[2];
+    char* texture_file_path;
+    float* vertices_plane_space; /* vertex i at [4i+0], [4i+1] */
+    float* vertices_world_space; /* vertex i at [4i+0 .. 4i+2] */
+} risltc_polygonal_light_t;
+
+/* ltc_constants_t (src/ltc_table.h:23-35); 32 bytes. */
+typedef struct risltc_ltc_constants_s {
+    float fresnel_index_factor, fresnel_index_summand;
+    float roughness_factor, roughness_summand;
+    float inclination_factor, inclination_summand;
+    float padding[2];
+} risltc_ltc_constants_t;
+
+/* per_frame_constants_t (src/main.h:537-553); 256 bytes, std140 offsets of
+ * the UBO in src/shaders/shared_constants.glsl:21-60. VkExtent2D is two
+ * uint32_t. */
+typedef struct risltc_frame_constants_s {
+    float mesh_dequantization_factor[3], padding_0, mesh_dequantization_summand[3];
+    float error_factor;
+    float world_to_projection_space[4][4];
+    float pixel_to_ray_direction_world_space[3][4];
+    float camera_position_world_space[3];
+    float mis_visibility_estimate;
+    uint32_t viewport_size[2];
+    int32_t cursor_position[2];
+    float exposure_factor;
+    float roughness_factor;
+    uint32_t noise_resolution_mask[2];
+    uint32_t noise_texture_index_mask;
+    uint32_t padding_3[3];
+    uint32_t noise_random_numbers[4];
+    risltc_ltc_constants_t ltc_constants;
+} risltc_frame_constants_t;
+
+/* first_person_camera_t (src/camera.h:29-49). */
+typedef struct risltc_camera_s {
+    float position_world_space[3];
+    float rotation_z;
+    float rotation_x;
+    float vertical_fov;
+    float near_plane, far_plane;
+    float speed;
+    int32_t rotate_camera;
+    float rotation_x_0, rotation_z_0;
+} risltc_camera_t;
+
+/* Constant material. It stands in for the three material textures that
+ * get_shading_data() samples (src/shaders/shading_pass.frag.glsl:629-635):
+ * base_color is the base colour texel, specular is the (r, g, b) texel of
+ * the specular texture (g = linear roughness, b = metalicity), and the
+ * normal-map texel is the flat (0.5, 0.5). Project extension: texture
+ * sampling is outside the shading-pass boundary. */
+typedef struct risltc_material_s {
+    float base_color[3];
+    float specular[3];
+} risltc_material_t;
+
+/* mis_heuristic_t (src/main.h:58-76) */
+#define RISLTC_MIS_BALANCE 0
+#define RISLTC_MIS_POWER 1
+#define RISLTC_MIS_WEIGHTED 2
+#define RISLTC_MIS_OPTIMAL_CLAMPED 3
+#define RISLTC_MIS_OPTIMAL 4
+
+/* light_sampling_strategies_t (src/main.h:78-83) */
+#define RISLTC_LIGHT_UNIFORM 0
+#define RISLTC_LIGHT_RESERVOIR 1
+
+/* sample_polygon_technique_t (src/polygonal_light.h:30-45); only
+ * sample_polygon_ltc_cp (the reference default, src/main.c:227) is built. */
+#define RISLTC_SAMPLE_POLYGON_LTC_CP 4
+
+/* RIS source distribution */
+#define RISLTC_PROPOSAL_UNIFORM 0 /* reference: p = 1/N (shading_pass.frag.glsl:733) */
+#define RISLTC_PROPOSAL_LEARNED 1 /* 0.5 * MLP cluster proposal + 0.5 / N     */
+
+/* Render settings. The fields mirror render_settings_t (src/main.h:123-156)
+ * and the shader defines built from it (src/main.c:962-991); what the
+ * reference fixes at compile time is a runtime value here. */
+typedef struct risltc_settings_s {
+    uint32_t sample_count;               /* SAMPLE_COUNT (must be 1 on the GPU)   */
+    uint32_t sample_count_light;         /* LIGHT_SAMPLES (samples per pixel)     */
+    uint32_t mis_heuristic;              /* RISLTC_MIS_*                          */
+    uint32_t light_sampling;             /* RISLTC_LIGHT_*                        */
+    uint32_t polygon_sampling_technique; /* RISLTC_SAMPLE_POLYGON_LTC_CP          */
+    uint32_t ris_candidate_count;        /* M; the reference hard-codes 32
+                                            (shading_pass.frag.glsl:726)          */
+    uint32_t fast_atan;                  /* USE_FAST_ATAN                          */
+    uint32_t shadow_rays;                /* 1 = trace shadow rays (reference)     */
+    uint32_t proposal;                   /* RISLTC_PROPOSAL_*                     */
+    uint32_t exact_math;                 /* 1 = IEEE div/sqrt everywhere: results
+                                            bit-identical to the FP32 oracle      */
+    uint32_t reserved[6];
+} risltc_settings_t;
+
+/* Fills settings with the reference defaults (src/main.c:220-236): SAMPLE_COUNT
+ * 1, one light sample, optimal clamped MIS, reservoir light sampling, LTC_CP,
+ * M = 32, shadow rays on, exact_math off. */
+void risltc_default_settings(risltc_settings_t* settings);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side helpers that restate the reference's CPU code                   */
+/* ------------------------------------------------------------------------ */
+
+/* update_polygonal_light (src/polygonal_light.c:44-98). vertices_world_space
+ * must be allocated (4 floats per vertex). */
+void risltc_update_polygonal_light(risltc_polygonal_light_t* light);
+
+/* write_lights (src/main.c:456-490): calls risltc_update_polygonal_light on
+ * every light, then packs the records (stride RISLTC_LIGHT_RECORD_SIZE(
+ * max_vertex_count)). Unlike the reference, which writes vertex 0 once after
+ * the last vertex and leaves later slots untouched, every unused slot holds
+ * vertex 0. Returns the number of bytes written or 0 on error. */
+size_t risltc_write_lights(risltc_polygonal_light_t* lights, uint32_t light_count,
+                           uint32_t max_vertex_count, void* out, size_t out_size);
+
+/* write_constants (src/main.c:2902-2946) with get_world_to_projection_space
+ * (src/camera.c:24-83) and set_noise_constants (src/noise_table.c:24-28).
+ * noise_random_seed is the noise_table_t::random_seed that the reference
+ * increments each frame; animate_noise selects wang_random_number() numbers
+ * instead of i * 0x123456. The LTC constants are copied from ltc. */
+void risltc_write_frame_constants(risltc_frame_constants_t* out, const risltc_camera_t* camera,
+                                  uint32_t width, uint32_t height,
+                                  const float mesh_dequantization_factor[3],
+                                  const float mesh_dequantization_summand[3],
+                                  const risltc_ltc_constants_t* ltc, float exposure_factor,
+                                  float roughness_factor, float mis_visibility_estimate,
+                                  uint32_t noise_random_seed, int animate_noise);
+
+/* ltc_constants_t as load_ltc_table computes them (src/ltc_table.c:184-191). */
+void risltc_ltc_constants_for(risltc_ltc_constants_t* out, uint32_t roughness_count,
+                              uint32_t inclination_count, uint32_t fresnel_count);
+
+/* Reads <directory>/fit<i>.dat for i < fresnel_count and quantises them the
+ * way load_ltc_table does (src/ltc_table.c:31-116). rgba16 receives
+ * res*res*4*fresnel_count values, rg16 res*res*2*fresnel_count values; pass
+ * NULL buffers to query *out_resolution first. Returns RISLTC_E_IO when a file
+ * is missing, RISLTC_E_FORMAT on inconsistent resolutions or short files. */
+int risltc_load_ltc_fit(const char* directory, uint32_t fresnel_count, uint32_t* out_resolution,
+                        uint16_t* rgba16, uint16_t* rg16, size_t capacity_values);
+
+/* ------------------------------------------------------------------------ */
+/* Context                                                                   */
+/* ------------------------------------------------------------------------ */
+typedef struct risltc_context_s risltc_context_t;
+
+/* Creates a context on HIP device device_index. */
+int risltc_create(risltc_context_t** out_context, int device_index);
+void risltc_destroy(risltc_context_t* context);
+/* Text of the last error on this context (never NULL). */
+const char* risltc_last_error(const risltc_context_t* context);
+
+/* Light records as packed by write_lights: count records of stride
+ * RISLTC_LIGHT_RECORD_SIZE(max_vertex_count). max_vertex_count <= 8,
+ * every vertex_count in [3, max_vertex_count]. Host pointer. */
+int risltc_set_lights(risltc_context_t* context, const void* records, uint32_t count,
+                      uint32_t max_vertex_count);
+
+/* LTC tables as load_ltc_table builds them: rgba16 holds fresnel_count slices
+ * of roughness_count x inclination_count texels (roughness fastest), 4
+ * channels; rg16 the same with 2 channels. Host pointers. */
+int risltc_set_ltc_table(risltc_context_t* context, const uint16_t* rgba16, const uint16_t* rg16,
+                         uint32_t roughness_count, uint32_t inclination_count,
+                         uint32_t fresnel_count);
+
+/* Mesh in the *.vks layout (src/scene.h:47-70) plus a constant material per
+ * material index. Uploads it and builds the LBVH used for shadow rays
+ * (30-bit Morton codes of triangle centroids, sorted, Karras hierarchy,
+ * escape links). Light triangles (bit 31 of the second position word) are
+ * part of the hierarchy, as they are in the reference BLAS
+ * (src/scene.c:176-187). Host pointers. */
+int risltc_set_scene(risltc_context_t* context, const uint32_t* positions,
+                     const uint16_t* normals_and_tex_coords, const uint8_t* material_indices,
+                     uint64_t triangle_count, const float dequantization_factor[3],
+                     const float dequantization_summand[3], const risltc_material_t* materials,
+                     uint32_t material_count);
+
+/* Primary visibility: writes the R32_UINT visibility buffer (primitive id of
+ * the closest hit, bit 31 set for light triangles, 0xFFFFFFFF for
+ * background) for the camera rays of constants. This is the step before the
+ * shading pass (src/shaders/visibility_pass.vert.glsl:27-38), done by ray
+ * casting into the LBVH instead of rasterisation. out_visibility: device
+ * pointer, width*height. stream: hipStream_t or NULL. */
+int risltc_render_visibility(risltc_context_t* context, const risltc_frame_constants_t* constants,
+                             uint32_t* out_visibility, void* stream);
+
+/* The shading pass: main() of src/shaders/shading_pass.frag.glsl:674-770 for
+ * every pixel of constants->viewport_size. visibility: device pointer,
+ * width*height. out_rgba32f: device pointer, width*height*4 floats. */
+int risltc_shade(risltc_context_t* context, const risltc_frame_constants_t* constants,
+                 const risltc_settings_t* settings, const uint32_t* visibility, float* out_rgba32f,
+                 void* stream);
+
+/* Running mean of frames, the accumulation pass of
+ * src/shaders/accum_pass.frag.glsl:45-53: accum = (accum * n + cur) / (n + 1).
+ * Device pointers, pixel_count * 4 floats. */
+int risltc_accumulate(risltc_context_t* context, float* accum_rgba32f, const float* frame_rgba32f,
+                      uint32_t pixel_count, uint32_t accum_num, void* stream);
+
+/* Any-hit shadow-ray queries against the scene LBVH with the semantics of
+ * get_polygon_visibility (src/shaders/shading_pass.frag.glsl:112-129):
+ * origin, direction, [t_min, t_max], opaque, no culling, terminate on first
+ * hit. rays: device pointer, 8 floats per ray (ox, oy, oz, t_min, dx, dy,
+ * dz, t_max). out_hit: device pointer, one uint32 (0/1) per ray. */
+int risltc_trace_shadow_rays(risltc_context_t* context, const float* rays, uint32_t ray_count,
+                             uint32_t* out_hit, void* stream);
+
+/* Debug variant of risltc_shade for parity tests. For every pixel p and
+ * light sample j it writes one record of RISLTC_SAMPLE_RECORD_FLOATS 32-bit
+ * words at sample_records[(p * spp + j) * RISLTC_SAMPLE_RECORD_FLOATS]
+ * (layout: risltc_sample_record_t) and, when candidate_records is not NULL,
+ * M records of RISLTC_CANDIDATE_RECORD_WORDS words per pixel sample
+ * (risltc_candidate_record_t). Device pointers. */
+#define RISLTC_SAMPLE_RECORD_FLOATS 40
+#define RISLTC_CANDIDATE_RECORD_WORDS 4
+typedef struct risltc_sample_record_s {
+    int32_t light_index;     /* RIS-selected light, -1 if none               */
+    float w_sum;             /* reservoir weight sum                         */
+    float p_hat;             /* target value of the selected light           */
+    float W;                 /* w_sum / (M * p_hat), 0 if p_hat == 0         */
+    uint32_t clip_diffuse;   /* clipped vertex count, shading space          */
+    uint32_t clip_specular;  /* clipped vertex count, cosine space           */
+    float psa_diffuse;       /* projected solid angle, shading space         */
+    float psa_specular;      /* projected solid angle, cosine space          */
+    float ray_dir[2][3];     /* world-space shadow ray directions            */
+    float ray_tmax[2];       /* ray t_max (t_min = 1e-3)                     */
+    uint32_t ray_state[2];   /* 0 no sample, 1 lambert <= 0, 2 traced hit,
+                                3 traced miss, 4 untraced (shadow rays off)  */
+    float estimate_visible[2][3];  /* MIS estimate if visible                */
+    float estimate_occluded[2][3]; /* MIS estimate if occluded               */
+    float color[3];          /* evaluate_polygonal_light_shading_peters()    */
+    uint32_t noise_seed_end; /* LCG state after this sample                  */
+    float pad[6];
+} risltc_sample_record_t;
+typedef struct risltc_candidate_record_s {
+    int32_t light_index;
+    uint32_t clip_counts;    /* diffuse | specular << 8                      */
+    float p_hat;
+    float w;
+} risltc_candidate_record_t;
+
+int risltc_shade_debug(risltc_context_t* context, const risltc_frame_constants_t* constants,
+                       const risltc_settings_t* settings, const uint32_t* visibility,
+                       float* out_rgba32f, float* sample_records, uint32_t* candidate_records,
+                       void* stream);
+
+/* Milliseconds of the last risltc_shade call per kernel, measured with HIP
+ * events on the call's stream: [0] shade, [1] shadow-ray trace, [2] resolve,
+ * [3] learned-proposal inference. Entries not run are 0. */
+int risltc_last_timings(const risltc_context_t* context, float out_ms[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RISLTC_H */

@@ -1,0 +1,247 @@
+"""ctypes binding of the CPU oracle (oracle/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker. The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from risltc_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborc.so")
+REF_LIB_PATH = os.path.join(HERE, "_ref", "libref_host.so")
+
+_lib = None
+_ref = None
+
+f32p = C.POINTER(C.c_float)
+f64p = C.POINTER(C.c_double)
+u32p = C.POINTER(C.c_uint32)
+u16p = C.POINTER(C.c_uint16)
+u8p = C.POINTER(C.c_uint8)
+i32p = C.POINTER(C.c_int32)
+
+
+def build(reference=None):
+    """make -C oracle (also builds oracle/_ref when the reference tree exists)."""
+    cmd = ["make", "-s", "-C", HERE]
+    if reference:
+        cmd.append("REF=%s" % reference)
+    subprocess.run(cmd, check=True)
+
+
+class OrcBvh(C.Structure):
+    _fields_ = [("leaf_count", C.c_uint32), ("node_box", f32p), ("node_left", i32p),
+                ("node_right", i32p), ("node_escape", i32p), ("node_parent", i32p),
+                ("leaf_tri", f32p), ("leaf_prim", u32p), ("sorted_keys", C.POINTER(C.c_uint64))]
+
+
+class OrcScene(C.Structure):
+    _fields_ = [("positions", u32p), ("normals_uv", u16p), ("material_indices", u8p),
+                ("triangle_count", C.c_uint64), ("materials", C.POINTER(abi.Material)),
+                ("material_count", C.c_uint32), ("light_records", u8p), ("light_count", C.c_uint32),
+                ("light_max_vertex_count", C.c_uint32), ("ltc_rgba16", u16p), ("ltc_rg16", u16p),
+                ("ltc_res_roughness", C.c_uint32), ("ltc_res_inclination", C.c_uint32),
+                ("ltc_fresnel_count", C.c_uint32), ("bvh", C.POINTER(OrcBvh)),
+                ("proposal_pdf", f32p), ("cluster_of_light", u32p), ("cluster_offsets", u32p),
+                ("cluster_lights", u32p), ("cluster_count", C.c_uint32)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "orc_murmur_mix": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+            "orc_murmur_finalize": (C.c_uint32, [C.c_uint32]),
+            "orc_noise_seed": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+            "orc_noise_next": (C.c_float, [u32p]),
+            "orc_wang_random_number": (C.c_uint32, [C.c_uint32]),
+            "prj_atanf": (C.c_float, [C.c_float]),
+            "prj_asinf": (C.c_float, [C.c_float]),
+            "prj_acosf": (C.c_float, [C.c_float]),
+            "prj_sinf": (C.c_float, [C.c_float]),
+            "prj_cosf": (C.c_float, [C.c_float]),
+            "orc_update_polygonal_light": (None, [C.POINTER(abi.PolygonalLight)]),
+            "orc_matrix_inverse": (None, [f32p, f32p]),
+            "orc_get_world_to_projection_space": (None, [f32p, C.POINTER(abi.Camera), C.c_float]),
+            "orc_write_frame_constants": (None, [C.POINTER(abi.FrameConstants), C.POINTER(abi.Camera),
+                                                 C.c_uint32, C.c_uint32, f32p, f32p,
+                                                 C.POINTER(abi.LtcConstants), C.c_float, C.c_float,
+                                                 C.c_float, C.c_uint32, C.c_int]),
+            "orc_quantize_ltc_entry": (None, [f32p, u16p, u16p]),
+            "orc_ltc_constants": (None, [C.POINTER(abi.LtcConstants), C.c_uint32, C.c_uint32, C.c_uint32]),
+            "orc_clip_rule": (C.c_int, [C.c_int, C.c_uint32, C.POINTER(C.c_int)]),
+            "orc_clip_rotation": (C.c_int, [C.c_int, C.c_uint32]),
+            "orc_bvh_build": (C.c_int, [C.POINTER(OrcBvh), f32p, C.c_uint32]),
+            "orc_bvh_free": (None, [C.POINTER(OrcBvh)]),
+            "orc_morton_part_1_by_2": (C.c_uint32, [C.c_uint32]),
+            "orc_bvh_any_hit": (C.c_int, [C.POINTER(OrcBvh), f32p, f32p, C.c_float, C.c_float, u32p]),
+            "orc_bvh_closest_hit": (C.c_uint32, [C.POINTER(OrcBvh), f32p, f32p, C.c_float, C.c_float, f32p]),
+            "orc_brute_any_hit": (C.c_int, [f32p, C.c_uint32, f32p, f32p, C.c_float, C.c_float]),
+            "orc_dequantize_mesh": (None, [u32p, C.c_uint64, f32p, f32p, f32p]),
+            "orc_render_visibility": (None, [C.POINTER(OrcBvh), u32p, C.POINTER(abi.FrameConstants), u32p, C.c_int]),
+            "orc_shade_pixels_f32": (C.c_int, [C.POINTER(OrcScene), C.POINTER(abi.FrameConstants),
+                                               C.POINTER(abi.Settings), u32p, u32p, C.c_uint32, f32p, f32p, u32p]),
+            "orc_shade_pixels_f64": (C.c_int, [C.POINTER(OrcScene), C.POINTER(abi.FrameConstants),
+                                               C.POINTER(abi.Settings), u32p, u32p, C.c_uint32, f64p, f32p, u32p]),
+            "orc_shade_frame_f32": (C.c_int, [C.POINTER(OrcScene), C.POINTER(abi.FrameConstants),
+                                              C.POINTER(abi.Settings), u32p, u32p, C.c_uint32, f32p, C.c_int]),
+            "orc_clip_polygon_f32": (C.c_uint32, [C.c_uint32, f32p]),
+            "orc_clip_polygon_f64": (C.c_uint32, [C.c_uint32, f64p]),
+            "orc_calculate_ltc_f32": (C.c_float, [C.c_uint32, f32p]),
+            "orc_calculate_ltc_f64": (C.c_double, [C.c_uint32, f64p]),
+            "orc_prepare_psa_f32": (C.c_float, [C.c_uint32, f32p, f32p]),
+            "orc_prepare_psa_f64": (C.c_double, [C.c_uint32, f64p, f64p]),
+            "orc_sample_psa_f32": (None, [C.c_uint32, f32p, f32p, C.c_uint32, f32p]),
+            "orc_sample_psa_f64": (None, [C.c_uint32, f64p, f64p, C.c_uint32, f64p]),
+            "orc_evaluate_brdf_f32": (None, [f32p, f32p, f32p]),
+            "orc_evaluate_brdf_f64": (None, [f64p, f64p, f64p]),
+            "orc_ltc_coefficients_f32": (None, [C.POINTER(OrcScene), C.POINTER(abi.LtcConstants), C.c_float,
+                                                C.c_float, f32p, f32p, f32p, f32p]),
+            "orc_ltc_fetch_f32": (C.c_float, [C.POINTER(OrcScene), C.c_int, C.c_int, C.c_float, C.c_float,
+                                              C.c_float]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's own host code compiled by oracle/Makefile, or None."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB_PATH):
+        R = C.CDLL(REF_LIB_PATH)
+        R.update_polygonal_light.argtypes = [C.POINTER(abi.PolygonalLight)]
+        R.update_polygonal_light.restype = None
+        R.ref_matrix_inverse.argtypes = [f32p, f32p]
+        R.ref_matrix_inverse.restype = None
+        R.ref_wang_random_number.argtypes = [C.c_uint32]
+        R.ref_wang_random_number.restype = C.c_uint32
+        _ref = R
+    return _ref
+
+
+def ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Oracle:
+    """The oracle bound to one synthetic scene (risltc_amd.synth.Scene)."""
+
+    def __init__(self, scene, ltc_rgba16, ltc_rg16, bvh=True):
+        self.L = lib()
+        self.scene = scene
+        m = scene.mesh
+        self.positions = np.ascontiguousarray(m.positions, dtype=np.uint32)
+        self.normals_uv = np.ascontiguousarray(m.normals_uv, dtype=np.uint16)
+        self.material_index = np.ascontiguousarray(m.material_index, dtype=np.uint8)
+        self.materials = scene.materials_ctypes()
+        self.records = np.frombuffer(scene.light_records, dtype=np.uint8).copy()
+        self.rgba = np.ascontiguousarray(ltc_rgba16, dtype=np.uint16)
+        self.rg = np.ascontiguousarray(ltc_rg16, dtype=np.uint16)
+        res = self.rgba.shape[1]
+        layers = self.rgba.shape[0]
+        self.ltc_constants = abi.LtcConstants()
+        self.L.orc_ltc_constants(C.byref(self.ltc_constants), res, res, layers)
+        s = OrcScene()
+        s.positions = ptr(self.positions, u32p)
+        s.normals_uv = ptr(self.normals_uv, u16p)
+        s.material_indices = ptr(self.material_index, u8p)
+        s.triangle_count = m.triangle_count
+        s.materials = C.cast(self.materials, C.POINTER(abi.Material))
+        s.material_count = len(self.materials)
+        s.light_records = ptr(self.records, u8p)
+        s.light_count = scene.light_count
+        s.light_max_vertex_count = scene.light_vmax
+        s.ltc_rgba16 = ptr(self.rgba, u16p)
+        s.ltc_rg16 = ptr(self.rg, u16p)
+        s.ltc_res_roughness = res
+        s.ltc_res_inclination = res
+        s.ltc_fresnel_count = layers
+        self.bvh = None
+        if bvh:
+            self.triangles = self.dequantized_triangles()
+            self.bvh = OrcBvh()
+            self.L.orc_bvh_build(C.byref(self.bvh), ptr(self.triangles, f32p), m.triangle_count)
+            s.bvh = C.pointer(self.bvh)
+        self.cscene = s
+
+    def __del__(self):
+        try:
+            if self.bvh is not None:
+                self.L.orc_bvh_free(C.byref(self.bvh))
+        except Exception:
+            pass
+
+    def dequantized_triangles(self):
+        m = self.scene.mesh
+        out = np.empty((m.triangle_count, 9), dtype=np.float32)
+        f = np.ascontiguousarray(m.dequantization_factor, dtype=np.float32)
+        s = np.ascontiguousarray(m.dequantization_summand, dtype=np.float32)
+        self.L.orc_dequantize_mesh(ptr(self.positions, u32p), m.triangle_count, ptr(f, f32p), ptr(s, f32p),
+                                   ptr(out, f32p))
+        return out
+
+    def frame_constants(self, exposure=1.5, roughness_factor=1.0, mis_visibility_estimate=0.5,
+                        noise_seed=0, animate=False, camera=None):
+        sc = self.scene
+        c = abi.FrameConstants()
+        f = np.ascontiguousarray(sc.mesh.dequantization_factor, dtype=np.float32)
+        s = np.ascontiguousarray(sc.mesh.dequantization_summand, dtype=np.float32)
+        self.L.orc_write_frame_constants(C.byref(c), C.byref(camera or sc.camera), sc.width, sc.height,
+                                         ptr(f, f32p), ptr(s, f32p), C.byref(self.ltc_constants), exposure,
+                                         roughness_factor, mis_visibility_estimate, noise_seed,
+                                         1 if animate else 0)
+        return c
+
+    def render_visibility(self, constants, threads=8):
+        out = np.empty(self.scene.width * self.scene.height, dtype=np.uint32)
+        self.L.orc_render_visibility(C.byref(self.bvh), ptr(self.positions, u32p), C.byref(constants),
+                                     ptr(out, u32p), threads)
+        return out
+
+    def shade(self, constants, settings, visibility, pixels=None, fp64=False, records=False,
+              candidates=False):
+        vis = np.ascontiguousarray(visibility, dtype=np.uint32)
+        if pixels is None:
+            count = vis.size
+            pix = None
+        else:
+            pix = np.ascontiguousarray(pixels, dtype=np.uint32)
+            count = pix.size
+        out = np.zeros((count, 4), dtype=np.float64 if fp64 else np.float32)
+        spp = settings.sample_count_light
+        M = settings.ris_candidate_count
+        rec = np.zeros((count, spp, abi.SAMPLE_RECORD_FLOATS), dtype=np.float32) if records else None
+        cand = (np.zeros((count, spp, M, abi.CANDIDATE_RECORD_WORDS), dtype=np.uint32)
+                if candidates else None)
+        fn = self.L.orc_shade_pixels_f64 if fp64 else self.L.orc_shade_pixels_f32
+        rc = fn(C.byref(self.cscene), C.byref(constants), C.byref(settings), ptr(vis, u32p),
+                ptr(pix, u32p) if pix is not None else None, count,
+                ptr(out, f64p if fp64 else f32p), ptr(rec, f32p) if rec is not None else None,
+                ptr(cand, u32p) if cand is not None else None)
+        if rc != 0:
+            raise RuntimeError("oracle shade failed: %s" % abi.ERROR_NAMES.get(rc, rc))
+        return out, rec, cand
+
+    def shade_frame(self, constants, settings, visibility, pixels=None, threads=8):
+        vis = np.ascontiguousarray(visibility, dtype=np.uint32)
+        pix = None if pixels is None else np.ascontiguousarray(pixels, dtype=np.uint32)
+        count = vis.size if pix is None else pix.size
+        out = np.zeros((count, 4), dtype=np.float32)
+        rc = self.L.orc_shade_frame_f32(C.byref(self.cscene), C.byref(constants), C.byref(settings),
+                                        ptr(vis, u32p), ptr(pix, u32p) if pix is not None else None, count,
+                                        ptr(out, f32p), threads)
+        if rc != 0:
+            raise RuntimeError("oracle shade failed: %s" % abi.ERROR_NAMES.get(rc, rc))
+        return out

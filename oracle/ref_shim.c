@@ -1,0 +1,8 @@
+/* Exports the static inline helpers of the reference's src/math_utilities.h
+ * (compiled from the reference tree by oracle/Makefile into oracle/_ref) so
+ * that tests can compare the oracle's restatements against them. Test
+ * infrastructure only. */
+#include "math_utilities.h"
+
+void ref_matrix_inverse(float inverse[4][4], const float matrix[4][4]) { matrix_inverse(inverse, matrix); }
+uint32_t ref_wang_random_number(uint32_t seed) { return wang_random_number(seed); }
