@@ -85,6 +85,7 @@ def lib():
             "orc_bvh_any_hit": (C.c_int, [C.POINTER(OrcBvh), f32p, f32p, C.c_float, C.c_float, u32p]),
             "orc_bvh_closest_hit": (C.c_uint32, [C.POINTER(OrcBvh), f32p, f32p, C.c_float, C.c_float, f32p]),
             "orc_brute_any_hit": (C.c_int, [f32p, C.c_uint32, f32p, f32p, C.c_float, C.c_float]),
+            "orc_ray_triangle": (C.c_int, [f32p, f32p, f32p, C.c_float, C.c_float, f32p]),
             "orc_dequantize_mesh": (None, [u32p, C.c_uint64, f32p, f32p, f32p]),
             "orc_render_visibility": (None, [C.POINTER(OrcBvh), u32p, C.POINTER(abi.FrameConstants), u32p, C.c_int]),
             "orc_shade_pixels_f32": (C.c_int, [C.POINTER(OrcScene), C.POINTER(abi.FrameConstants),

@@ -901,7 +901,9 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
         }
     }
     if (cands) memset(cands, 0, sizeof(uint32_t) * (size_t)spp * M * RISLTC_CANDIDATE_RECORD_WORDS);
-    if (prim == 0xFFFFFFFFu) {
+    /* ids past the mesh are treated as background (project rule; the
+     * reference would read out of bounds) */
+    if (prim == 0xFFFFFFFFu || ((prim >> 31) == 0u && (uint64_t)prim >= sc->triangle_count)) {
         out[0] = out[1] = out[2] = K(0.0);
         out[3] = K(1.0);
         return;

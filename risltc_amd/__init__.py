@@ -1,0 +1,220 @@
+"""MI355X (gfx950) RIS-LTC shading pass.
+
+Python host mirror of the C-ABI in include/risltc/risltc.h (librisltc.so,
+built in-tree by risltc_amd/build.py). The mirror keeps the reference's host
+vocabulary: light records as write_lights packs them, LTC tables as
+load_ltc_table builds them, per-frame constants, and the visibility buffer.
+Device buffers are torch tensors (used only for device memory and streams).
+
+There is no CPU fallback: if librisltc.so is missing or fails to load, every
+call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+__all__ = ["Context", "lib", "LIB_PATH", "RisltcError", "abi"]
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librisltc.so")
+_lib = None
+
+
+class RisltcError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__("%s: %s" % (abi.ERROR_NAMES.get(code, code), message))
+        self.code = code
+
+
+_vp = C.c_void_p
+_u32 = C.c_uint32
+_f32p = C.POINTER(C.c_float)
+_u32p = C.POINTER(C.c_uint32)
+_u16p = C.POINTER(C.c_uint16)
+_u8p = C.POINTER(C.c_uint8)
+
+SIGNATURES = {
+    "risltc_default_settings": (None, [C.POINTER(abi.Settings)]),
+    "risltc_update_polygonal_light": (None, [C.POINTER(abi.PolygonalLight)]),
+    "risltc_write_lights": (C.c_size_t, [C.POINTER(abi.PolygonalLight), _u32, _u32, _vp, C.c_size_t]),
+    "risltc_write_frame_constants": (None, [C.POINTER(abi.FrameConstants), C.POINTER(abi.Camera), _u32, _u32,
+                                            _f32p, _f32p, C.POINTER(abi.LtcConstants), C.c_float, C.c_float,
+                                            C.c_float, _u32, C.c_int]),
+    "risltc_ltc_constants_for": (None, [C.POINTER(abi.LtcConstants), _u32, _u32, _u32]),
+    "risltc_load_ltc_fit": (C.c_int, [C.c_char_p, _u32, _u32p, _u16p, _u16p, C.c_size_t]),
+    "risltc_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
+    "risltc_destroy": (None, [_vp]),
+    "risltc_last_error": (C.c_char_p, [_vp]),
+    "risltc_set_lights": (C.c_int, [_vp, _vp, _u32, _u32]),
+    "risltc_set_ltc_table": (C.c_int, [_vp, _u16p, _u16p, _u32, _u32, _u32]),
+    "risltc_set_scene": (C.c_int, [_vp, _u32p, _u16p, _u8p, C.c_uint64, _f32p, _f32p,
+                                   C.POINTER(abi.Material), _u32]),
+    "risltc_render_visibility": (C.c_int, [_vp, C.POINTER(abi.FrameConstants), _vp, _vp]),
+    "risltc_shade": (C.c_int, [_vp, C.POINTER(abi.FrameConstants), C.POINTER(abi.Settings), _vp, _vp, _vp]),
+    "risltc_shade_debug": (C.c_int, [_vp, C.POINTER(abi.FrameConstants), C.POINTER(abi.Settings), _vp, _vp,
+                                     _vp, _vp, _vp]),
+    "risltc_accumulate": (C.c_int, [_vp, _vp, _vp, _u32, _u32, _vp]),
+    "risltc_trace_shadow_rays": (C.c_int, [_vp, _vp, _u32, _vp, _vp]),
+    "risltc_last_timings": (C.c_int, [_vp, _f32p]),
+}
+
+
+def lib():
+    """Loads librisltc.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("librisltc.so is not built (run risltc_amd/build.py or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _dptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        try:
+            import torch
+            return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        except Exception:
+            return None
+    return C.c_void_p(stream)
+
+
+def default_settings(**kw):
+    return abi.default_settings(**kw)
+
+
+def write_frame_constants(camera, width, height, deq_factor, deq_summand, ltc_constants, exposure=1.5,
+                          roughness_factor=1.0, mis_visibility_estimate=0.5, noise_seed=0, animate=False):
+    """write_constants (src/main.c:2902-2946) through the library."""
+    c = abi.FrameConstants()
+    f = np.ascontiguousarray(deq_factor, dtype=np.float32)
+    s = np.ascontiguousarray(deq_summand, dtype=np.float32)
+    lib().risltc_write_frame_constants(C.byref(c), C.byref(camera), width, height, _ptr(f, _f32p),
+                                       _ptr(s, _f32p), C.byref(ltc_constants), exposure, roughness_factor,
+                                       mis_visibility_estimate, noise_seed, 1 if animate else 0)
+    return c
+
+
+def ltc_constants_for(res_roughness, res_inclination, fresnel_count):
+    k = abi.LtcConstants()
+    lib().risltc_ltc_constants_for(C.byref(k), res_roughness, res_inclination, fresnel_count)
+    return k
+
+
+class Context:
+    """A shading context on one HIP device (risltc_context_t)."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.risltc_create(C.byref(h), device)
+        if rc != 0:
+            raise RisltcError(rc, "risltc_create(%d) failed" % device)
+        self.h = h
+        self.device = device
+        self.ltc_constants = None
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.risltc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RisltcError(rc, "%s: %s" % (what, self.L.risltc_last_error(self.h).decode()))
+
+    # ---- uploads ---------------------------------------------------------
+    def set_lights(self, records, count, max_vertex_count):
+        buf = np.frombuffer(records, dtype=np.uint8)
+        self._check(self.L.risltc_set_lights(self.h, buf.ctypes.data_as(_vp), count, max_vertex_count),
+                    "risltc_set_lights")
+
+    def set_ltc_table(self, rgba16, rg16):
+        rgba = np.ascontiguousarray(rgba16, dtype=np.uint16)
+        rg = np.ascontiguousarray(rg16, dtype=np.uint16)
+        layers, h, w = rgba.shape[0], rgba.shape[1], rgba.shape[2]
+        self._check(self.L.risltc_set_ltc_table(self.h, _ptr(rgba, _u16p), _ptr(rg, _u16p), w, h, layers),
+                    "risltc_set_ltc_table")
+        self.ltc_constants = ltc_constants_for(w, h, layers)
+
+    def set_scene(self, mesh, materials):
+        pos = np.ascontiguousarray(mesh.positions, dtype=np.uint32)
+        nuv = np.ascontiguousarray(mesh.normals_uv, dtype=np.uint16)
+        mi = np.ascontiguousarray(mesh.material_index, dtype=np.uint8)
+        f = np.ascontiguousarray(mesh.dequantization_factor, dtype=np.float32)
+        s = np.ascontiguousarray(mesh.dequantization_summand, dtype=np.float32)
+        mats = np.ascontiguousarray(materials, dtype=np.float32).reshape(-1, 6)
+        arr = (abi.Material * len(mats))()
+        for i, m in enumerate(mats):
+            arr[i].base_color[:] = [float(v) for v in m[0:3]]
+            arr[i].specular[:] = [float(v) for v in m[3:6]]
+        self._check(self.L.risltc_set_scene(self.h, _ptr(pos, _u32p), _ptr(nuv, _u16p), _ptr(mi, _u8p),
+                                            mesh.triangle_count, _ptr(f, _f32p), _ptr(s, _f32p), arr, len(mats)),
+                    "risltc_set_scene")
+
+    def load_scene(self, scene, ltc_rgba16, ltc_rg16):
+        """Uploads a risltc_amd.synth.Scene and an LTC table."""
+        self.set_lights(scene.light_records, scene.light_count, scene.light_vmax)
+        self.set_ltc_table(ltc_rgba16, ltc_rg16)
+        self.set_scene(scene.mesh, scene.materials)
+        self.scene = scene
+
+    def frame_constants(self, exposure=1.5, roughness_factor=1.0, mis_visibility_estimate=0.5, noise_seed=0,
+                        animate=False, camera=None):
+        sc = self.scene
+        return write_frame_constants(camera or sc.camera, sc.width, sc.height, sc.mesh.dequantization_factor,
+                                     sc.mesh.dequantization_summand, self.ltc_constants, exposure,
+                                     roughness_factor, mis_visibility_estimate, noise_seed, animate)
+
+    # ---- passes ------------------------------------------------------------
+    def render_visibility(self, constants, out, stream=None):
+        self._check(self.L.risltc_render_visibility(self.h, C.byref(constants), _dptr(out), _stream(stream)),
+                    "risltc_render_visibility")
+
+    def shade(self, constants, settings, visibility, out, stream=None):
+        self._check(self.L.risltc_shade(self.h, C.byref(constants), C.byref(settings), _dptr(visibility),
+                                        _dptr(out), _stream(stream)), "risltc_shade")
+
+    def shade_debug(self, constants, settings, visibility, out, samples, candidates=None, stream=None):
+        self._check(self.L.risltc_shade_debug(self.h, C.byref(constants), C.byref(settings), _dptr(visibility),
+                                              _dptr(out), _dptr(samples), _dptr(candidates), _stream(stream)),
+                    "risltc_shade_debug")
+
+    def accumulate(self, accum, frame, pixel_count, accum_num, stream=None):
+        self._check(self.L.risltc_accumulate(self.h, _dptr(accum), _dptr(frame), pixel_count, accum_num,
+                                             _stream(stream)), "risltc_accumulate")
+
+    def trace_shadow_rays(self, rays, hits, stream=None):
+        n = rays.numel() // 8
+        self._check(self.L.risltc_trace_shadow_rays(self.h, _dptr(rays), n, _dptr(hits), _stream(stream)),
+                    "risltc_trace_shadow_rays")
+
+    def last_timings(self):
+        ms = (C.c_float * 4)()
+        self._check(self.L.risltc_last_timings(self.h, ms), "risltc_last_timings")
+        return list(ms)

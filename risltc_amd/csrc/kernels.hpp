@@ -1,0 +1,116 @@
+// kernels.hpp -- launch parameters and entry points of the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/risltc/risltc.h"
+
+namespace rl {
+
+// Light record repacked for the GPU: 8 x float4 = 128 bytes.
+//   [0] radiance.xyz, vertex_count (uint bits)
+//   [1] plane.xyzw
+//   [2..7] vertices: 8 vertices x 3 floats, packed (v0.x v0.y v0.z v1.x | ...)
+constexpr int kLightFloat4 = 8;
+
+// Per pixel-sample record written by the shade kernel and read by the
+// resolve kernel: 16 words.
+//   [0] W  (0 when the sample contributes nothing)
+//   [1] flags: bits 0-3 state of technique 0, bits 4-7 technique 1, bit 8 = has contribution
+//   [2..7]  estimate if visible, technique 0 then 1 (xyz, xyz)
+//   [8..13] estimate if occluded, technique 0 then 1
+//   [14] spare, [15] spare
+constexpr int kSampleWords = 16;
+
+// Shadow ray record (compacted list): float4 org.xyz + t_min, float4 dir.xyz + t_max,
+// plus the slot (pixel_sample * 2 + technique) it answers.
+struct ShadowRay {
+    float4 org;
+    float4 dir;
+};
+
+struct SceneView {
+    const uint2* positions;      // 3 per triangle
+    const ushort4* normals_uv;   // 3 per triangle
+    const uint8_t* material_idx; // 1 per triangle
+    const float* materials;      // 6 per material
+    uint32_t material_count;
+    uint64_t triangle_count;
+    const float4* lights;        // kLightFloat4 per light
+    uint32_t light_count;
+    const uint16_t* ltc0;        // RGBA16 texels
+    const uint16_t* ltc1;        // RG16 texels
+    uint32_t ltc_w, ltc_h, ltc_layers;
+};
+
+struct BvhView {
+    const float4* nodes;   // internal: 2 float4 per node (min.xyz | left, max.xyz | escape)
+    const float4* leaves;  // 3 float4 per leaf (v0 | escape, v1 | prim, v2 | flag)
+    uint32_t leaf_count;
+};
+
+struct ShadeArgs {
+    risltc_frame_constants_t c;
+    uint32_t spp, M, mis, light_sampling, fast_atan, shadow_rays;
+    uint32_t tile_w, tile_h;  // pixel tile of one work-group
+    SceneView scene;
+    const uint32_t* visibility;
+    float* samples;            // kSampleWords per pixel-sample
+    ShadowRay* rays;           // compacted shadow rays
+    uint32_t* ray_slots;       // slot of each compacted ray
+    uint32_t* ray_count;       // device counter
+    float* debug_samples;      // RISLTC_SAMPLE_RECORD_FLOATS per pixel-sample or null
+    uint32_t* debug_cands;     // RISLTC_CANDIDATE_RECORD_WORDS per candidate or null
+    // learned proposal (null = uniform source)
+    const uint8_t* proposal;   // per pixel: cluster_count quantised probabilities
+    const uint32_t* proposal_sum;  // per pixel: sum of the quantised probabilities
+    const uint32_t* cluster_offsets;
+    const uint32_t* cluster_lights;
+    uint32_t cluster_count;
+};
+
+struct TraceArgs {
+    BvhView bvh;
+    const ShadowRay* rays;
+    const uint32_t* ray_slots;
+    const uint32_t* ray_count;
+    uint32_t max_rays;
+    uint32_t* hits;  // one word per slot
+};
+
+struct ResolveArgs {
+    uint32_t width, height, spp;
+    uint64_t triangle_count;
+    float exposure;
+    const uint32_t* visibility;
+    const float* samples;
+    const uint32_t* hits;
+    float* out;  // RGBA32F
+};
+
+// host-side launchers (shade.hip, lbvh.hip)
+hipError_t launch_shade(const ShadeArgs& a, bool exact, hipStream_t s);
+hipError_t launch_trace(const TraceArgs& a, hipStream_t s);
+hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s);
+hipError_t launch_accumulate(float* accum, const float* frame, uint32_t pixel_count, uint32_t n,
+                             hipStream_t s);
+hipError_t upload_clip_table(const uint8_t* table, size_t bytes);
+// clears device words with a kernel (no memset nodes, so it is safe under
+// stream capture)
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
+
+struct BvhBuild {
+    float4* nodes = nullptr;
+    float4* leaves = nullptr;
+    uint32_t leaf_count = 0;
+};
+// Builds the LBVH of the dequantised triangles of a *.vks mesh on the device.
+hipError_t build_lbvh(const uint2* d_positions, uint64_t triangle_count, const float deq_factor[3],
+                      const float deq_summand[3], BvhBuild* out, hipStream_t s);
+void free_lbvh(BvhBuild* b);
+hipError_t launch_trace_rays(const BvhView& bvh, const float* rays8, uint32_t n, uint32_t* hits,
+                             hipStream_t s);
+hipError_t launch_primary(const BvhView& bvh, const uint2* positions, const risltc_frame_constants_t& c,
+                          uint32_t* out, hipStream_t s);
+
+}  // namespace rl

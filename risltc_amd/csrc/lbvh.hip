@@ -1,0 +1,446 @@
+// lbvh.hip -- linear BVH over the scene triangles and its stackless traversal.
+//
+// The reference traces shadow rays with VK_KHR_ray_query against a hardware
+// acceleration structure (src/shaders/shading_pass.frag.glsl:112-129,
+// src/scene.c:142-406). gfx950 has no ray-query hardware, so the project
+// builds its own hierarchy:
+//   1. triangles dequantised exactly as the shader decodes positions
+//      (decode_position_64_bit, mesh_quantization.glsl:38-45);
+//   2. 30-bit Morton codes of the centroids in the centroid bounds (the
+//      bit interleave of tools/io_export_vulkan_blender28.py:49-77), keys
+//      (code << 32 | triangle id) radix-sorted -- unique keys, so the order
+//      is deterministic;
+//   3. Karras (2012) hierarchy: internal node i splits the sorted range at
+//      the highest differing key bit; leaves are node ids n-1 .. 2n-2;
+//   4. bottom-up boxes with one atomic arrival counter per internal node;
+//   5. escape links for stackless pre-order traversal (left child first).
+// The CPU oracle builds the same tree from the same keys; traversal uses the
+// same box and Moeller-Trumbore tests in the same operation order, so hit and
+// miss flags agree bit for bit.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "devmath.hpp"
+#include "kernels.hpp"
+
+namespace rl {
+
+__device__ __forceinline__ uint32_t part_1_by_2(uint32_t x) {
+    x &= 0x000003ffu;
+    x = (x ^ (x << 16)) & 0xff0000ffu;
+    x = (x ^ (x << 8)) & 0x0300f00fu;
+    x = (x ^ (x << 4)) & 0x030c30c3u;
+    x = (x ^ (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__global__ void dequant_kernel(const uint2* pos, uint64_t n, float3 f, float3 s, float* tris, float* cent) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float v[9];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        uint2 q = pos[t * 3 + k];
+        uint32_t qx = q.x & 0x1FFFFFu;
+        uint32_t qy = ((q.x & 0xFFE00000u) >> 21) | ((q.y & 0x3FFu) << 11);
+        uint32_t qz = (q.y & 0x7FFFFC00u) >> 10;
+        v[3 * k + 0] = fma_((float)qx, f.x, s.x);
+        v[3 * k + 1] = fma_((float)qy, f.y, s.y);
+        v[3 * k + 2] = fma_((float)qz, f.z, s.z);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) tris[t * 9 + k] = v[k];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) cent[t * 3 + a] = ((v[a] + v[3 + a]) + v[6 + a]) / 3.0f;
+}
+
+// min/max of the centroids: block partials, then one block over partials.
+// min and max are exact, so any reduction order gives the same bounds.
+__global__ void bounds_kernel(const float* cent, uint64_t n, float* partial) {
+    __shared__ float sh[6][256];
+    float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float c = cent[t * 3 + a];
+            lo[a] = fmin_(lo[a], c);
+            hi[a] = fmax_(hi[a], c);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        sh[a][threadIdx.x] = lo[a];
+        sh[3 + a][threadIdx.x] = hi[a];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                sh[a][threadIdx.x] = fmin_(sh[a][threadIdx.x], sh[a][threadIdx.x + w]);
+                sh[3 + a][threadIdx.x] = fmax_(sh[3 + a][threadIdx.x], sh[3 + a][threadIdx.x + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int a = 0; a < 6; ++a) partial[blockIdx.x * 6 + a] = sh[a][0];
+}
+
+__global__ void bounds_final_kernel(const float* partial, uint32_t blocks, float* out) {
+    if (threadIdx.x != 0) return;
+    float r[6] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), -__builtin_inff(), -__builtin_inff(),
+                  -__builtin_inff()};
+    for (uint32_t b = 0; b < blocks; ++b)
+        for (int a = 0; a < 3; ++a) {
+            r[a] = fmin_(r[a], partial[b * 6 + a]);
+            r[3 + a] = fmax_(r[3 + a], partial[b * 6 + 3 + a]);
+        }
+    for (int a = 0; a < 6; ++a) out[a] = r[a];
+}
+
+__global__ void morton_kernel(const float* cent, uint64_t n, const float* bounds, uint64_t* keys) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t code = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float ext = bounds[3 + a] - bounds[a];
+        float factor = (ext > 0.0f) ? 1024.0f / ext : 0.0f;
+        float summand = -bounds[a] * factor;
+        float q = cent[t * 3 + a] * factor + summand;
+        q = fmin_(fmax_(q, 0.0f), 1023.0f);
+        code |= part_1_by_2((uint32_t)q) << a;
+    }
+    keys[t] = ((uint64_t)code << 32) | (uint64_t)t;
+}
+
+__device__ __forceinline__ int kdelta(const uint64_t* keys, int n, int i, int j) {
+    if (j < 0 || j > n - 1) return -1;
+    return __clzll((long long)(keys[i] ^ keys[j]));
+}
+
+__global__ void karras_kernel(const uint64_t* keys, int n, int* left, int* right, int* parent) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = kdelta(keys, n, i, i - d);
+    int lmax = 2;
+    while (kdelta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (kdelta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = kdelta(keys, n, i, j);
+    int s = 0;
+    for (int div = 2;; div *= 2) {
+        int t = (l + div - 1) / div;
+        if (kdelta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t == 1) break;
+    }
+    int gamma = i + s * d + (d < 0 ? d : 0);
+    int lo = i < j ? i : j, hi = i < j ? j : i;
+    int lc = (lo == gamma) ? (n - 1 + gamma) : gamma;
+    int rc = (hi == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
+    left[i] = lc;
+    right[i] = rc;
+    parent[lc] = i;
+    parent[rc] = i;
+}
+
+__global__ void leaf_kernel(const uint64_t* keys, const float* tris, int n, float* box, float4* leaves,
+                            const uint2* pos) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t prim = (uint32_t)(keys[k] & 0xFFFFFFFFull);
+    const float* v = tris + (size_t)prim * 9;
+    float* b = box + (size_t)(n - 1 + k) * 6;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        b[a] = fmin_(fmin_(v[a], v[3 + a]), v[6 + a]);
+        b[3 + a] = fmax_(fmax_(v[a], v[3 + a]), v[6 + a]);
+    }
+    uint32_t light = pos[(size_t)prim * 3].y & 0x80000000u;
+    leaves[3 * k + 0] = make_float4(v[0], v[1], v[2], 0.0f);
+    leaves[3 * k + 1] = make_float4(v[3], v[4], v[5], __uint_as_float(prim));
+    leaves[3 * k + 2] = make_float4(v[6], v[7], v[8], __uint_as_float(light));
+}
+
+__global__ void refit_kernel(int n, const int* left, const int* right, const int* parent, float* box,
+                             uint32_t* arrivals) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int node = n - 1 + k;
+    int p = parent[node];
+    volatile float* vb = box;
+    // a well-formed tree has depth < n; the cap keeps a malformed one finite
+    for (int depth = 0; p >= 0 && depth < n; ++depth) {
+        __threadfence();
+        uint32_t old = atomicAdd(&arrivals[p], 1u);
+        if (old == 0u) return;  // the sibling subtree is not done yet
+        int l = left[p], r = right[p];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float lo = fmin_(vb[(size_t)l * 6 + a], vb[(size_t)r * 6 + a]);
+            float hi = fmax_(vb[(size_t)l * 6 + 3 + a], vb[(size_t)r * 6 + 3 + a]);
+            vb[(size_t)p * 6 + a] = lo;
+            vb[(size_t)p * 6 + 3 + a] = hi;
+        }
+        node = p;
+        p = parent[node];
+    }
+}
+
+__global__ void escape_pack_kernel(int n, const int* left, const int* right, const int* parent, const float* box,
+                                   float4* nodes, float4* leaves) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= 2 * n - 1) return;
+    int esc = -1;
+    int y = x;
+    for (int depth = 0; y > 0 && depth < n; ++depth) {
+        int p = parent[y];
+        if (p < 0) break;
+        if (left[p] == y) {
+            esc = right[p];
+            break;
+        }
+        y = p;
+    }
+    if (x < n - 1) {
+        const float* b = box + (size_t)x * 6;
+        nodes[2 * x + 0] = make_float4(b[0], b[1], b[2], __int_as_float(left[x]));
+        nodes[2 * x + 1] = make_float4(b[3], b[4], b[5], __int_as_float(esc));
+    } else {
+        int k = x - (n - 1);
+        leaves[3 * k].w = __int_as_float(esc);
+    }
+}
+
+__global__ void fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+#define HIP_TRY(x)                          \
+    do {                                    \
+        hipError_t e_ = (x);                \
+        if (e_ != hipSuccess) {             \
+            err = e_;                       \
+            goto done;                      \
+        }                                   \
+    } while (0)
+
+hipError_t build_lbvh(const uint2* d_pos, uint64_t tri_count, const float deq_factor[3], const float deq_summand[3],
+                      BvhBuild* out, hipStream_t s) {
+    hipError_t err = hipSuccess;
+    int n = (int)tri_count;
+    float *tris = nullptr, *cent = nullptr, *partial = nullptr, *bounds = nullptr, *box = nullptr;
+    uint64_t *keys = nullptr, *keys_sorted = nullptr;
+    int *left = nullptr, *right = nullptr, *parent = nullptr;
+    uint32_t* arrivals = nullptr;
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
+    const uint32_t tb = 256;
+    const uint32_t tri_blocks = (uint32_t)((tri_count + tb - 1) / tb);
+    const uint32_t red_blocks = 1024;
+    float4 *nodes = nullptr, *leaves = nullptr;
+    if (n <= 0) return hipErrorInvalidValue;
+    HIP_TRY(hipMalloc(&tris, sizeof(float) * 9 * tri_count));
+    HIP_TRY(hipMalloc(&cent, sizeof(float) * 3 * tri_count));
+    HIP_TRY(hipMalloc(&partial, sizeof(float) * 6 * red_blocks));
+    HIP_TRY(hipMalloc(&bounds, sizeof(float) * 6));
+    HIP_TRY(hipMalloc(&keys, sizeof(uint64_t) * tri_count));
+    HIP_TRY(hipMalloc(&keys_sorted, sizeof(uint64_t) * tri_count));
+    HIP_TRY(hipMalloc(&box, sizeof(float) * 6 * (2 * tri_count - 1)));
+    HIP_TRY(hipMalloc(&left, sizeof(int) * tri_count));
+    HIP_TRY(hipMalloc(&right, sizeof(int) * tri_count));
+    HIP_TRY(hipMalloc(&parent, sizeof(int) * (2 * tri_count - 1)));
+    HIP_TRY(hipMalloc(&arrivals, sizeof(uint32_t) * tri_count));
+    HIP_TRY(hipMalloc(&nodes, sizeof(float4) * 2 * (tri_count > 1 ? tri_count - 1 : 1)));
+    HIP_TRY(hipMalloc(&leaves, sizeof(float4) * 3 * tri_count));
+    hipLaunchKernelGGL(dequant_kernel, dim3(tri_blocks), dim3(tb), 0, s, d_pos, tri_count,
+                       make_float3(deq_factor[0], deq_factor[1], deq_factor[2]),
+                       make_float3(deq_summand[0], deq_summand[1], deq_summand[2]), tris, cent);
+    hipLaunchKernelGGL(bounds_kernel, dim3(red_blocks), dim3(256), 0, s, cent, tri_count, partial);
+    hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(64), 0, s, partial, red_blocks, bounds);
+    hipLaunchKernelGGL(morton_kernel, dim3(tri_blocks), dim3(tb), 0, s, cent, tri_count, bounds, keys);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, temp_bytes, keys, keys_sorted, n, 0, 64, s));
+    HIP_TRY(hipMalloc(&temp, temp_bytes));
+    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(temp, temp_bytes, keys, keys_sorted, n, 0, 64, s));
+    hipLaunchKernelGGL(fill_u32, dim3((uint32_t)((2 * tri_count - 1 + tb - 1) / tb)), dim3(tb), 0, s,
+                       reinterpret_cast<uint32_t*>(parent), 2 * tri_count - 1, 0xFFFFFFFFu);
+    hipLaunchKernelGGL(fill_u32, dim3(tri_blocks), dim3(tb), 0, s, arrivals, tri_count, 0u);
+    if (n > 1) hipLaunchKernelGGL(karras_kernel, dim3(tri_blocks), dim3(tb), 0, s, keys_sorted, n, left, right, parent);
+    hipLaunchKernelGGL(leaf_kernel, dim3(tri_blocks), dim3(tb), 0, s, keys_sorted, tris, n, box, leaves, d_pos);
+    if (n > 1) hipLaunchKernelGGL(refit_kernel, dim3(tri_blocks), dim3(tb), 0, s, n, left, right, parent, box, arrivals);
+    hipLaunchKernelGGL(escape_pack_kernel, dim3((uint32_t)((2 * tri_count - 1 + tb - 1) / tb)), dim3(tb), 0, s, n,
+                       left, right, parent, box, nodes, leaves);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    out->nodes = nodes;
+    out->leaves = leaves;
+    out->leaf_count = (uint32_t)n;
+    nodes = leaves = nullptr;
+done:
+    hipFree(tris);
+    hipFree(cent);
+    hipFree(partial);
+    hipFree(bounds);
+    hipFree(keys);
+    hipFree(keys_sorted);
+    hipFree(box);
+    hipFree(left);
+    hipFree(right);
+    hipFree(parent);
+    hipFree(arrivals);
+    hipFree(temp);
+    hipFree(nodes);
+    hipFree(leaves);
+    return err;
+}
+
+void free_lbvh(BvhBuild* b) {
+    if (b->nodes) hipFree(b->nodes);
+    if (b->leaves) hipFree(b->leaves);
+    b->nodes = b->leaves = nullptr;
+    b->leaf_count = 0;
+}
+
+// ---- traversal -------------------------------------------------------------------
+__device__ __forceinline__ bool ray_box(float3 o, float3 inv, float4 bmin, float4 bmax, float tmin, float tmax) {
+    float t0 = (bmin.x - o.x) * inv.x, t1 = (bmax.x - o.x) * inv.x;
+    tmin = fmax_(tmin, fmin_(t0, t1));
+    tmax = fmin_(tmax, fmax_(t0, t1) * 1.0000004f);
+    t0 = (bmin.y - o.y) * inv.y;
+    t1 = (bmax.y - o.y) * inv.y;
+    tmin = fmax_(tmin, fmin_(t0, t1));
+    tmax = fmin_(tmax, fmax_(t0, t1) * 1.0000004f);
+    t0 = (bmin.z - o.z) * inv.z;
+    t1 = (bmax.z - o.z) * inv.z;
+    tmin = fmax_(tmin, fmin_(t0, t1));
+    tmax = fmin_(tmax, fmax_(t0, t1) * 1.0000004f);
+    return tmin <= tmax;
+}
+
+__device__ __forceinline__ bool ray_tri(float3 o, float3 d, float4 a, float4 b, float4 c, float tmin, float tmax,
+                                        float& t_out) {
+    float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
+    float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
+    float px = d.y * e2z - d.z * e2y, py = d.z * e2x - d.x * e2z, pz = d.x * e2y - d.y * e2x;
+    float det = fma_(e1z, pz, fma_(e1y, py, e1x * px));
+    if (det == 0.0f) return false;
+    float inv = 1.0f / det;
+    float sx = o.x - a.x, sy = o.y - a.y, sz = o.z - a.z;
+    float u = fma_(sz, pz, fma_(sy, py, sx * px)) * inv;
+    if (u < 0.0f || u > 1.0f) return false;
+    float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+    float w = fma_(d.z, qz, fma_(d.y, qy, d.x * qx)) * inv;
+    if (w < 0.0f || u + w > 1.0f) return false;
+    float t = fma_(e2z, qz, fma_(e2y, qy, e2x * qx)) * inv;
+    if (!(t >= tmin && t <= tmax)) return false;
+    t_out = t;
+    return true;
+}
+
+__device__ __forceinline__ bool any_hit(const BvhView& B, float3 o, float3 d, float tmin, float tmax) {
+    float3 inv = make_float3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    int n = (int)B.leaf_count;
+    int x = 0;
+    // pre-order visits every node at most once: 2n - 1 steps bound the walk
+    for (int steps = 0; x >= 0 && x < 2 * n - 1 && steps < 2 * n; ++steps) {
+        if (x >= n - 1) {
+            int k = x - (n - 1);
+            float4 a = B.leaves[3 * k], b = B.leaves[3 * k + 1], c = B.leaves[3 * k + 2];
+            float t;
+            if (ray_tri(o, d, a, b, c, tmin, tmax, t)) return true;
+            x = __float_as_int(a.w);
+        } else {
+            float4 lo = B.nodes[2 * x], hi = B.nodes[2 * x + 1];
+            x = ray_box(o, inv, lo, hi, tmin, tmax) ? __float_as_int(lo.w) : __float_as_int(hi.w);
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ uint32_t closest_hit(const BvhView& B, float3 o, float3 d, float tmin, float tmax) {
+    float3 inv = make_float3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    int n = (int)B.leaf_count;
+    int x = 0;
+    uint32_t best = 0xFFFFFFFFu;
+    for (int steps = 0; x >= 0 && x < 2 * n - 1 && steps < 2 * n; ++steps) {
+        if (x >= n - 1) {
+            int k = x - (n - 1);
+            float4 a = B.leaves[3 * k], b = B.leaves[3 * k + 1], c = B.leaves[3 * k + 2];
+            float t;
+            if (ray_tri(o, d, a, b, c, tmin, tmax, t)) {
+                tmax = t;
+                best = __float_as_uint(b.w) | __float_as_uint(c.w);
+            }
+            x = __float_as_int(a.w);
+        } else {
+            float4 lo = B.nodes[2 * x], hi = B.nodes[2 * x + 1];
+            x = ray_box(o, inv, lo, hi, tmin, tmax) ? __float_as_int(lo.w) : __float_as_int(hi.w);
+        }
+    }
+    return best;
+}
+
+__global__ void __launch_bounds__(256) trace_kernel(TraceArgs A) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t count = *A.ray_count;
+    if (count > A.max_rays) count = A.max_rays;
+    if (i >= count) return;
+    ShadowRay r = A.rays[i];
+    bool hit = any_hit(A.bvh, make_float3(r.org.x, r.org.y, r.org.z), make_float3(r.dir.x, r.dir.y, r.dir.z), r.org.w,
+                       r.dir.w);
+    A.hits[A.ray_slots[i]] = hit ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256) trace_rays_kernel(BvhView B, const float* rays, uint32_t n, uint32_t* hits) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4* r = reinterpret_cast<const float4*>(rays) + 2 * i;
+    float4 a = r[0], b = r[1];
+    hits[i] = any_hit(B, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), a.w, b.w) ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256) primary_kernel(BvhView B, risltc_frame_constants_t c, uint32_t* out) {
+    uint32_t W = c.viewport_size[0], H = c.viewport_size[1];
+    uint32_t tiles_x = (W + 15u) / 16u;
+    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t x = (blockIdx.x % tiles_x) * 16u + (lane & 15u);
+    uint32_t y = (blockIdx.x / tiles_x) * 16u + wave * 4u + (lane >> 4);
+    if (x >= W || y >= H) return;
+    const float(*P)[4] = c.pixel_to_ray_direction_world_space;
+    float fx = (float)x, fy = (float)y;
+    float3 d = make_float3(fma_(P[0][2], 1.0f, fma_(P[0][1], fy, P[0][0] * fx)),
+                           fma_(P[1][2], 1.0f, fma_(P[1][1], fy, P[1][0] * fx)),
+                           fma_(P[2][2], 1.0f, fma_(P[2][1], fy, P[2][0] * fx)));
+    float3 o = make_float3(c.camera_position_world_space[0], c.camera_position_world_space[1],
+                           c.camera_position_world_space[2]);
+    out[y * W + x] = closest_hit(B, o, d, 0.0f, __builtin_inff());
+}
+
+hipError_t launch_trace(const TraceArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(trace_kernel, dim3((a.max_rays + 255u) / 256u), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_rays(const BvhView& bvh, const float* rays8, uint32_t n, uint32_t* hits, hipStream_t s) {
+    hipLaunchKernelGGL(trace_rays_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, bvh, rays8, n, hits);
+    return hipGetLastError();
+}
+
+hipError_t launch_primary(const BvhView& bvh, const uint2* positions, const risltc_frame_constants_t& c, uint32_t* out,
+                          hipStream_t s) {
+    (void)positions;
+    uint32_t W = c.viewport_size[0], H = c.viewport_size[1];
+    dim3 grid(((W + 15u) / 16u) * ((H + 15u) / 16u));
+    hipLaunchKernelGGL(primary_kernel, grid, dim3(256), 0, s, bvh, c, out);
+    return hipGetLastError();
+}
+
+}  // namespace rl

@@ -1,0 +1,1209 @@
+// shade.hip -- the RIS-LTC shading pass on gfx950.
+//
+// One lane shades one pixel (main() of src/shaders/shading_pass.frag.glsl:
+// 674-770): G-buffer reconstruction, LTC coefficients, then per light sample
+// a reservoir over M candidate lights scored by the clipped-polygon LTC
+// integral (the RIS target, shading_pass.frag.glsl:723-736 and 421-456) and
+// a final projected-solid-angle sample pair combined with MIS
+// (shading_pass.frag.glsl:292-397). Shadow rays are not traced inline: the
+// kernel appends them to a compacted ray list (one atomic per wave, lane
+// offsets from a ballot) and records the MIS estimate for both outcomes; the
+// trace kernel answers the rays and the resolve kernel sums the estimates in
+// the reference's order. Work-groups are 16x16 pixel tiles, 4 waves of 16x4.
+//
+// Template parameter X selects exact IEEE arithmetic (bitwise equal to the
+// FP32 oracle) or the fast variant with hardware rcp/rsq/sqrt in the
+// per-candidate and final-sample math. The per-pixel setup (shading frame,
+// LTC lookup) is always exact, so clip decisions agree bit for bit in both.
+#include <hip/hip_runtime.h>
+
+#include "devmath.hpp"
+#include "kernels.hpp"
+
+namespace rl {
+
+// rotation of the clipped polygon for n = 3..8 input vertices and inside
+// mask m: c_clip_rot[(n - 3) * 256 + m] (see the clipping rule in capi.cpp)
+__constant__ uint8_t c_clip_rot[6 * 256];
+
+constexpr int MAXV = 9;
+
+struct Pixel {
+    f3 pos, n, o;
+    float lo;
+    f3 albedo, f0;
+    float rough;
+    float w2s[12];       // world to shading, row-major 3x4
+    float w2c[12];       // shading_to_cosine * world_to_shading
+    float w2c_flip[12];  // shading_to_cosine * (world_to_shading with row 1 negated)
+    float s2c[9], c2s[9];
+    float ltc_albedo, ltc_det;
+};
+
+struct Light {
+    f3 rad;
+    float plane[4];
+    uint32_t n;
+    float v[24];
+};
+
+__device__ __forceinline__ void load_light(const float4* lights, uint32_t idx, Light& L) {
+    const float4* p = lights + (size_t)idx * kLightFloat4;
+    float4 a = p[0], b = p[1];
+    L.rad = mk3(a.x, a.y, a.z);
+    L.n = __float_as_uint(a.w);
+    L.plane[0] = b.x;
+    L.plane[1] = b.y;
+    L.plane[2] = b.z;
+    L.plane[3] = b.w;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        float4 q = p[2 + k];
+        L.v[4 * k + 0] = q.x;
+        L.v[4 * k + 1] = q.y;
+        L.v[4 * k + 2] = q.z;
+        L.v[4 * k + 3] = q.w;
+    }
+}
+
+__device__ __forceinline__ float plane_side(f3 p, const float* pl) {
+    return fma_(1.0f, pl[3], fma_(p.z, pl[2], fma_(p.y, pl[1], p.x * pl[0])));
+}
+
+__device__ __forceinline__ f3 xform(const float* m, float x, float y, float z) {
+    return mk3(fma_(m[2], z, fma_(m[1], y, fma_(m[0], x, m[3]))), fma_(m[6], z, fma_(m[5], y, fma_(m[4], x, m[7]))),
+               fma_(m[10], z, fma_(m[9], y, fma_(m[8], x, m[11]))));
+}
+
+__device__ __forceinline__ f3 mul33(const float* m, f3 p) {
+    return mk3(fma_(m[2], p.z, fma_(m[1], p.y, m[0] * p.x)), fma_(m[5], p.z, fma_(m[4], p.y, m[3] * p.x)),
+               fma_(m[8], p.z, fma_(m[7], p.y, m[6] * p.x)));
+}
+
+// mat3 * mat4x3 with the convention of the oracle
+__device__ __forceinline__ void mul33_34(const float* s, const float* w, float* r) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            r[4 * i + k] = fma_(s[3 * i + 2], w[8 + k], fma_(s[3 * i + 1], w[4 + k], s[3 * i + 0] * w[k]));
+}
+
+// iz0 (polygon_clipping.glsl:19-25)
+template <bool X>
+__device__ __forceinline__ f3 iz0(f3 a, f3 b) {
+    float t = div_<X>(a.z, a.z - b.z);
+    return mk3(fma_(t, b.x, fma_(-t, a.x, a.x)), fma_(t, b.y, fma_(-t, a.y, a.y)), 0.0f);
+}
+
+// integrateEdgeVec (polygon_sampling.glsl:508-521) on normalised vertices
+template <bool X>
+__device__ __forceinline__ float edge_term(f3 a, f3 b) {
+    float x = dot3(a, b);
+    float y = fabs_(x);
+    float A = 0.8543985f + (0.4965155f + 0.0145206f * y) * y;
+    float B = 3.4175940f + (4.1616724f + y) * y;
+    float v = div_<X>(A, B);
+    float ts = (x > 0.0f) ? v : 0.5f * rsq_<X>(fmax_(1.0f - x * x, 1e-7f)) - v;
+    return (a.x * b.y - a.y * b.x) * ts;
+}
+
+// Clip (polygon_clipping.glsl:35-225) + calculate_ltc (polygon_sampling.glsl:
+// 523-530) without materialising the clipped polygon: every input edge
+// contributes its clipped part; the closing edge exit -> entry is added last.
+// The exact build sums the terms in the order of the reference's rotated
+// output (c_clip_rot), the fast build in input-edge order.
+template <bool X>
+__device__ __forceinline__ float polygon_ltc(const float* px, const float* py, const float* pz, uint32_t n,
+                                             uint32_t& vc) {
+    uint32_t mask = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) mask |= ((uint32_t)i < n && pz[i] > 0.0f) ? (1u << i) : 0u;
+    vc = 0;
+    if (mask == 0) return 0.0f;
+    uint32_t full = (1u << n) - 1u;
+    uint32_t k = __builtin_popcount(mask);
+    bool clipped = mask != full;
+    uint32_t L = clipped ? k + 2u : n;
+    vc = L;
+    f3 nv[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) nv[i] = normalize3<X>(mk3(px[i], py[i], pz[i]));
+    f3 E = mk3(0.0f, 0.0f, 0.0f), Xp = E;
+    float acc = 0.0f;
+    float t[MAXV];
+    uint32_t s = 0, p = 0;
+    if constexpr (X) {
+        uint32_t prev = ((mask << 1) | (mask >> (n - 1u))) & full;
+        uint32_t starts = mask & ~prev;
+        s = clipped ? (uint32_t)__builtin_ctz(starts) : 0u;
+        p = clipped ? c_clip_rot[(n - 3u) * 256u + mask] : 0u;
+#pragma unroll
+        for (int i = 0; i < MAXV; ++i) t[i] = 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if ((uint32_t)i < n) {
+            bool wrap = (uint32_t)(i + 1) >= n;
+            int j = (i + 1) & 7;
+            f3 a = mk3(px[i], py[i], pz[i]);
+            f3 b = wrap ? mk3(px[0], py[0], pz[0]) : mk3(px[j], py[j], pz[j]);
+            f3 na = nv[i];
+            f3 nb = wrap ? nv[0] : nv[j];
+            bool in_a = (mask >> i) & 1u;
+            bool in_b = (mask >> (wrap ? 0 : j)) & 1u;
+            f3 P = na;
+            if (in_a != in_b) {
+                P = normalize3<X>(iz0<X>(a, b));
+                if (in_b) E = P;
+                else Xp = P;
+            }
+            if (in_a || in_b) {
+                float term = edge_term<X>(in_a ? na : P, in_b ? nb : P);
+                if constexpr (X) {
+                    int c;
+                    if (!clipped) c = i;
+                    else if (in_a && in_b) {
+                        int d = i - (int)s;
+                        if (d < 0) d += (int)n;
+                        c = d + 1;
+                    } else c = in_b ? 0 : (int)k;
+                    int r = c - (int)p;
+                    if (r < 0) r += (int)L;
+                    t[r] = term;
+                } else {
+                    acc += term;
+                }
+            }
+        }
+    }
+    if (clipped) {
+        float term = edge_term<X>(Xp, E);
+        if constexpr (X) {
+            int r = (int)(k + 1u) - (int)p;
+            if (r < 0) r += (int)L;
+            t[r] = term;
+        } else {
+            acc += term;
+        }
+    }
+    if constexpr (X) {
+#pragma unroll
+        for (int i = 0; i < MAXV; ++i)
+            if ((uint32_t)i < L) acc += t[i];
+    }
+    return fabs_(acc);
+}
+
+// ---- clipped polygon, materialised in the reference's output order --------
+__device__ __forceinline__ f3 pick8(const float* vx, const float* vy, const float* vz, uint32_t idx) {
+    f3 r = mk3(vx[0], vy[0], vz[0]);
+#pragma unroll
+    for (int i = 1; i < 8; ++i)
+        if (idx == (uint32_t)i) r = mk3(vx[i], vy[i], vz[i]);
+    return r;
+}
+
+// In place: vx/vy/vz hold n vertices on entry; on return the clipped polygon
+// with every slot past the count equal to slot 0.
+template <bool X>
+__device__ __forceinline__ uint32_t clip_materialize(float* vx, float* vy, float* vz, uint32_t n) {
+    uint32_t mask = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) mask |= ((uint32_t)i < n && vz[i] > 0.0f) ? (1u << i) : 0u;
+    if (mask == 0) return 0;
+    uint32_t full = (1u << n) - 1u;
+    if (mask == full) {
+#pragma unroll
+        for (int i = 1; i < MAXV; ++i)
+            if ((uint32_t)i >= n) {
+                vx[i] = vx[0];
+                vy[i] = vy[0];
+                vz[i] = vz[0];
+            }
+        return n;
+    }
+    uint32_t k = __builtin_popcount(mask);
+    uint32_t L = k + 2u;
+    uint32_t prev = ((mask << 1) | (mask >> (n - 1u))) & full;
+    uint32_t s = (uint32_t)__builtin_ctz(mask & ~prev);
+    uint32_t p = c_clip_rot[(n - 3u) * 256u + mask];
+    uint32_t sm1 = (s == 0u) ? n - 1u : s - 1u;
+    uint32_t e = s + k - 1u;
+    if (e >= n) e -= n;
+    uint32_t e1 = (e + 1u >= n) ? e + 1u - n : e + 1u;
+    f3 E = iz0<X>(pick8(vx, vy, vz, sm1), pick8(vx, vy, vz, s));
+    f3 Xp = iz0<X>(pick8(vx, vy, vz, e), pick8(vx, vy, vz, e1));
+    float ox[MAXV], oy[MAXV], oz[MAXV];
+#pragma unroll
+    for (int j = 0; j < MAXV; ++j) {
+        uint32_t c = (uint32_t)j + p;
+        if (c >= L) c -= L;
+        if ((uint32_t)j >= L) c = p;  // padding repeats slot 0
+        f3 r;
+        if (c == 0u) r = E;
+        else if (c == k + 1u) r = Xp;
+        else {
+            uint32_t src = s + c - 1u;
+            if (src >= n) src -= n;
+            r = pick8(vx, vy, vz, src);
+        }
+        ox[j] = r.x;
+        oy[j] = r.y;
+        oz[j] = r.z;
+    }
+#pragma unroll
+    for (int j = 0; j < MAXV; ++j) {
+        vx[j] = ox[j];
+        vy[j] = oy[j];
+        vz[j] = oz[j];
+    }
+    return L;
+}
+
+// ---- projected solid angle sampling (polygon_sampling.glsl:245-828) --------
+struct Psa {
+    uint32_t vc;
+    f2 v[MAXV];
+    f2 e[MAXV];
+    f2 inner0;
+    float sector[MAXV];
+    float psa;
+};
+
+__device__ __forceinline__ float kahan(float a, float b, float c, float d) {
+    float cd = c * d;
+    float err = fma_(c, d, -cd);
+    float res = fma_(a, b, -cd);
+    return res - err;
+}
+__device__ __forceinline__ f3 cross_stable(f3 l, f3 r) {
+    return mk3(kahan(l.y, r.z, l.z, r.y), kahan(l.z, r.x, l.x, r.z), kahan(l.x, r.y, l.y, r.x));
+}
+__device__ __forceinline__ f2 rot90(f2 v) { return mk2(-v.y, v.x); }
+__device__ __forceinline__ bool inner_e(f2 e) { return sign_bit(e.x); }
+
+template <bool X>
+__device__ __forceinline__ f2 ellipse_from_edge(f3 a, f3 b) {
+    f3 nn = cross_stable(a, b);
+    float s = rcp_<X>(nn.z);
+    s = sign_bit(nn.x) ? -s : s;
+    f2 e = mk2(nn.x * s, nn.y * s);
+    e.x = (nn.z != 0.0f) ? e.x : __builtin_inff();
+    return e;
+}
+__device__ __forceinline__ f2 etrans(f2 e, f2 p) {
+    float d = dot2(e, p);
+    return mk2(fma_(d, e.x, p.x), fma_(d, e.y, p.y));
+}
+__device__ __forceinline__ float edet(f2 e) { return fma_(e.x, e.x, fma_(e.y, e.y, 1.0f)); }
+template <bool X>
+__device__ __forceinline__ float ersd(f2 e) {
+    return rsq_<X>(edet(e));
+}
+__device__ __forceinline__ float edfr(f2 e, f2 d) {
+    float ed = dot2(e, d);
+    float dd = dot2(d, d);
+    return fma_(ed, ed, dd);
+}
+template <bool X>
+__device__ __forceinline__ float endf(f2 e, f2 nd) {
+    float ed = dot2(e, nd);
+    return rsq_<X>(fma_(ed, ed, 1.0f));
+}
+
+template <bool X>
+__device__ __forceinline__ float positive_atan(float t, bool fast_atan) {
+    if (fast_atan) {
+        float rx = (fabs_(t) > 1.0f) ? rcp_<X>(fabs_(t)) : fabs_(t);
+        float ry = rx * rx;
+        float rz = fma_(ry, 0.02083509974181652f, -0.08513300120830536f);
+        rz = fma_(ry, rz, 0.18014100193977356f);
+        rz = fma_(ry, rz, -0.3302994966506958f);
+        ry = fma_(ry, rz, 0.9998660087585449f);
+        rz = fma_(-2.0f * ry, rx, 1.5707963267948966192313216916398f);
+        rz = (fabs_(t) > 1.0f) ? rz : 0.0f;
+        rx = fma_(rx, ry, rz);
+        return (t < 0.0f) ? (3.1415926535897932384626433832795f - rx) : rx;
+    }
+    float off = (t < 0.0f) ? 3.1415926535897932384626433832795f : 0.0f;
+    return atan_<X>(t) + off;
+}
+
+template <bool X>
+__device__ __forceinline__ float area_from_tangents(float ird, float it, float ord, float ot, bool fa) {
+    float inner = ird * positive_atan<X>(it, fa);
+    float r = fma_(ord, positive_atan<X>(ot, fa), -inner);
+    return (r > 0.0f) ? (0.5f * r) : 0.0f;
+}
+template <bool X>
+__device__ __forceinline__ float area_between(f2 ie, float ird, f2 oe, float ord, f2 d0, f2 d1, bool fa) {
+    float dd = fmax_(0.0f, dot2(d1, rot90(d0)));
+    float idot = ird * dot2(d0, etrans(ie, d1));
+    float odot = ord * dot2(d0, etrans(oe, d1));
+    return area_from_tangents<X>(ird, div_<X>(dd, idot), ord, div_<X>(dd, odot), fa);
+}
+template <bool X>
+__device__ __forceinline__ float ellipse_area_in_sector(f2 e, f2 d0, f2 d1, bool fa) {
+    float rd = ersd<X>(e);
+    float dd = fmax_(0.0f, dot2(d1, rot90(d0)));
+    float edot = rd * dot2(d0, etrans(e, d1));
+    float area = 0.5f * rd * positive_atan<X>(div_<X>(dd, edot), fa);
+    return (rd > 0.0f) ? area : 0.0f;
+}
+
+__device__ __forceinline__ void cas(Psa& P, int l, int r) {
+    f2 lc = P.v[l];
+    float nz = kahan(lc.x, -P.v[r].y, lc.y, -P.v[r].x);
+    bool swap = (nz == 0.0f) ? __builtin_isinf(P.e[r].x) : (nz > 0.0f);
+    P.v[l] = swap ? P.v[r] : lc;
+    P.v[r] = swap ? lc : P.v[r];
+    f2 le = P.e[l];
+    P.e[l] = swap ? P.e[r] : le;
+    P.e[r] = swap ? le : P.e[r];
+}
+
+// sort_convex_polygon_vertices (polygon_sampling.glsl:441-506); 9 vertices
+// (clipped 8-gons) use a complete 25-comparator network
+__device__ __forceinline__ void sort_vertices(Psa& P) {
+    uint32_t n = P.vc;
+    if (n == 3) {
+        cas(P, 1, 2);
+    } else if (n == 4) {
+        cas(P, 1, 3);
+    } else if (n == 5) {
+        cas(P, 2, 4); cas(P, 1, 3); cas(P, 1, 2); cas(P, 0, 3); cas(P, 3, 4);
+    } else if (n == 6) {
+        cas(P, 3, 5); cas(P, 2, 4); cas(P, 1, 5); cas(P, 0, 4); cas(P, 4, 5); cas(P, 1, 3);
+    } else if (n == 7) {
+        cas(P, 2, 5); cas(P, 1, 6); cas(P, 5, 6); cas(P, 3, 4); cas(P, 0, 4); cas(P, 4, 6);
+        cas(P, 1, 3); cas(P, 3, 5); cas(P, 4, 5);
+    } else if (n == 8) {
+        cas(P, 2, 6); cas(P, 3, 7); cas(P, 1, 5); cas(P, 0, 4); cas(P, 4, 6); cas(P, 5, 7);
+        cas(P, 6, 7); cas(P, 4, 5); cas(P, 1, 3);
+    } else if (n == 9) {
+        cas(P, 0, 1); cas(P, 3, 4); cas(P, 6, 7); cas(P, 1, 2); cas(P, 4, 5); cas(P, 7, 8);
+        cas(P, 0, 1); cas(P, 3, 4); cas(P, 6, 7); cas(P, 0, 3); cas(P, 3, 6); cas(P, 0, 3);
+        cas(P, 1, 4); cas(P, 4, 7); cas(P, 1, 4); cas(P, 2, 5); cas(P, 5, 8); cas(P, 2, 5);
+        cas(P, 1, 3); cas(P, 5, 7); cas(P, 2, 6); cas(P, 4, 6); cas(P, 2, 4); cas(P, 2, 3);
+        cas(P, 5, 6);
+    }
+    cas(P, 0, 2);
+    if (n >= 4) cas(P, 2, 3);
+    cas(P, 0, 1);
+}
+
+// prepare_projected_solid_angle_polygon_sampling (polygon_sampling.glsl:545-613)
+template <bool X>
+__device__ __forceinline__ void prepare_psa(Psa& P, uint32_t vc, const float* vx, const float* vy, const float* vz,
+                                            bool fa) {
+    P.vc = vc;
+    P.inner0 = mk2(1.0f, 0.0f);
+    P.v[0] = mk2(vx[0], vy[0]);
+    P.e[0] = ellipse_from_edge<X>(mk3(vx[0], vy[0], vz[0]), mk3(vx[1], vy[1], vz[1]));
+    f2 prev = P.e[0];
+#pragma unroll
+    for (int i = 1; i < MAXV; ++i) {
+        P.v[i] = mk2(vx[i], vy[i]);
+        P.e[i] = mk2(0.0f, 0.0f);
+        P.sector[i] = 0.0f;
+    }
+#pragma unroll
+    for (int i = 1; i < MAXV; ++i) {
+        if (!(i > 2 && (uint32_t)i >= vc)) {
+            int j = (i + 1) % MAXV;
+            f2 e = ellipse_from_edge<X>(mk3(vx[i], vy[i], vz[i]), mk3(vx[j], vy[j], vz[j]));
+            bool in = inner_e(e);
+            P.e[i] = in ? prev : e;
+            P.inner0 = (inner_e(prev) && !in) ? prev : P.inner0;
+            prev = e;
+        }
+    }
+    {
+        f2 e = P.e[0];
+        bool in = inner_e(e);
+        P.e[0] = in ? prev : e;
+        P.inner0 = (inner_e(prev) && !in) ? prev : P.inner0;
+    }
+    P.psa = 0.0f;
+    P.sector[0] = 0.0f;
+    if (P.inner0.x > 0.0f) {
+#pragma unroll
+        for (int i = 0; i < MAXV; ++i) {
+            if (!(i > 2 && (uint32_t)i >= vc)) {
+                P.sector[i] = ellipse_area_in_sector<X>(P.e[i], P.v[i], P.v[(i + 1) % MAXV], fa);
+                P.psa += P.sector[i];
+            }
+        }
+    } else {
+        sort_vertices(P);
+        f2 ie = P.inner0;
+        float ird = ersd<X>(ie);
+        f2 oe = mk2(0.0f, 0.0f);
+        float ord = 0.0f;
+#pragma unroll
+        for (int i = 0; i < MAXV - 1; ++i) {
+            if (!(i > 1 && (uint32_t)i + 1u >= vc)) {
+                f2 ve = P.e[i];
+                bool vin = inner_e(ve);
+                float vrd = ersd<X>(ve);
+                if (i == 0) {
+                    oe = ve;
+                    ord = vrd;
+                } else {
+                    ie = vin ? ve : ie;
+                    ird = vin ? vrd : ird;
+                    oe = vin ? oe : ve;
+                    ord = vin ? ord : vrd;
+                }
+                P.sector[i] = area_between<X>(ie, ird, oe, ord, P.v[i], P.v[i + 1], fa);
+                P.psa += P.sector[i];
+            }
+        }
+    }
+}
+
+template <bool X>
+__device__ __forceinline__ f2 nrm_approx_flip(f2 rhs, f2 semi) {
+    float s = fabs_(rhs.x) + fabs_(rhs.y);
+    s = __uint_as_float(__float_as_uint(s) ^ 0x7F800000u);
+    s = (dot2(rhs, semi) >= 0.0f) ? s : -s;
+    return sclf2(s, rhs);
+}
+
+struct Q2 {
+    float q00, q01, q10, q11;  // column-major: q[col][row]
+};
+__device__ __forceinline__ Q2 outer2(f2 c, f2 r) { return Q2{c.x * r.x, c.y * r.x, c.x * r.y, c.y * r.y}; }
+__device__ __forceinline__ Q2 subq(Q2 a, Q2 b) { return Q2{a.q00 - b.q00, a.q01 - b.q01, a.q10 - b.q10, a.q11 - b.q11}; }
+template <bool X>
+__device__ __forceinline__ f2 solve_quadratic(Q2 m) {
+    float cxy = 0.5f * (m.q01 + m.q10);
+    float sd = sqrt_<X>(fmax_(0.0f, cxy * cxy - m.q00 * m.q11));
+    float root = fabs_(cxy) + sd;
+    return (cxy >= 0.0f) ? mk2(root, -m.q00) : mk2(m.q11, root);
+}
+__device__ __forceinline__ float mix_fma(float x, float y, float a) { return fma_(a, y, fma_(-a, x, x)); }
+
+// sample_sector_between_ellipses (polygon_sampling.glsl:668-762)
+template <bool X>
+__device__ __forceinline__ f2 sample_sector(f2 rn, float target_area, f2 ie, f2 oe, f2 d0, f2 d1, bool fa) {
+    f2 q0 = normalize2<X>(d0);
+    f2 q2 = normalize2<X>(d1);
+    f2 q1 = add2(q0, q2);
+    float n00 = endf<X>(ie, q0), n01 = rsq_<X>(edfr(ie, q1)), n02 = endf<X>(ie, q2);
+    float n10 = endf<X>(oe, q0), n11 = rsq_<X>(edfr(oe, q1)), n12 = endf<X>(oe, q2);
+    float sa0 = n10 * n11 - n00 * n01;
+    float sa1 = n11 * n12 - n01 * n02;
+    float tqa = mix_fma(-sa0, sa1, rn.x);
+    bool sel = tqa <= 0.0f;
+    q2 = sel ? q0 : q2;
+    n02 = sel ? n00 : n02;
+    n12 = sel ? n10 : n12;
+    tqa += sel ? sa0 : -sa1;
+    tqa *= fabs_(q1.x * q2.y - q2.x * q1.y);
+    f2 qn0 = add2(scl2(q1, n01), scl2(q2, n02));
+    f2 qn1 = add2(scl2(q1, n11), scl2(q2, n12));
+    qn0 = etrans(ie, qn0);
+    qn1 = etrans(oe, qn1);
+    float qo0 = dot2(qn0, q1) * n01;
+    float qo1 = dot2(qn1, q1) * n11;
+    f2 r2 = rot90(q2);
+    Q2 quad = outer2(sclf2(qo1 * n12, r2), qn0);
+    quad = subq(quad, outer2(add2(sclf2(qo0 * n02, r2), sclf2(tqa, qn0)), qn1));
+    f2 cur = solve_quadratic<X>(quad);
+    uint32_t iters = (fabs_(rn.x - 0.5f) <= 0.5f - 1.0e-5f) ? 2u : 0u;
+    float ird = ersd<X>(ie);
+    float ord = ersd<X>(oe);
+    for (uint32_t it = 0; it < iters; ++it) {
+        cur = nrm_approx_flip<X>(cur, q1);
+        f2 idir = etrans(ie, cur);
+        f2 odir = etrans(oe, cur);
+        float dd = fmax_(0.0f, dot2(cur, rot90(q0)));
+        float err = target_area - area_from_tangents<X>(ird, div_<X>(dd, ird * dot2(q0, idir)), ord,
+                                                         div_<X>(dd, ord * dot2(q0, odir)), fa);
+        quad = subq(outer2(sub2(idir, odir), rot90(cur)), outer2(sclf2(2.0f * err, idir), odir));
+        cur = solve_quadratic<X>(quad);
+    }
+    cur = (dot2(cur, q1) >= 0.0f) ? cur : mk2(-cur.x, -cur.y);
+    float inner_f = rcp_<X>(edfr(ie, cur));
+    float outer_f = rcp_<X>(edfr(oe, cur));
+    return scl2(cur, sqrt_<X>(mix_fma(inner_f, outer_f, rn.y)));
+}
+
+// sample_projected_solid_angle_polygon (polygon_sampling.glsl:772-828)
+template <bool X>
+__device__ __forceinline__ f3 sample_psa(const Psa& P, f2 rn, bool fa) {
+    float target = rn.x * P.psa;
+    f2 xy;
+    if (P.inner0.x > 0.0f) {
+        f2 oe = P.e[0], d0 = P.v[0];
+        bool done = false;
+#pragma unroll
+        for (int i = 0; i < MAXV; ++i) {
+            if (!done) {
+                if (i > 0) target -= P.sector[i - 1];
+                oe = P.e[i];
+                d0 = P.v[i];
+                if ((i >= 2 && (uint32_t)i + 1u == P.vc) || target < P.sector[i]) done = true;
+            }
+        }
+        float sdet = sqrt_<X>(edet(oe));
+        float angle = 2.0f * target * sdet;
+        f2 a = sclf2(cos_(angle) * sdet, d0);
+        f2 b = sclf2(sin_(angle), rot90(etrans(oe, d0)));
+        xy = add2(a, b);
+        xy = scl2(xy, sqrt_<X>(div_<X>(rn.y, edfr(oe, xy))));
+    } else {
+        f2 ie = P.inner0, oe = P.e[0], d0 = P.v[0], d1 = P.v[1];
+        float spsa = P.sector[0];
+        bool done = false;
+#pragma unroll
+        for (int i = 0; i < MAXV - 1; ++i) {
+            if (!done) {
+                f2 ve = P.e[i];
+                if (i == 0) {
+                    oe = ve;
+                } else {
+                    target -= P.sector[i - 1];
+                    bool vin = inner_e(ve);
+                    ie = vin ? ve : ie;
+                    oe = vin ? oe : ve;
+                }
+                d0 = P.v[i];
+                d1 = P.v[i + 1];
+                spsa = P.sector[i];
+                if ((i >= 1 && (uint32_t)i + 2u == P.vc) || target < spsa) done = true;
+            }
+        }
+        f2 r = mk2(div_<X>(target, spsa), rn.y);
+        xy = sample_sector<X>(r, target, ie, oe, d0, d1, fa);
+    }
+    float z = sqrt_<X>(fmax_(0.0f, fma_(-xy.x, xy.x, fma_(-xy.y, xy.y, 1.0f))));
+    return mk3(xy.x, xy.y, z);
+}
+
+// evaluate_brdf (brdfs.glsl:58-93), both components
+template <bool X>
+__device__ __forceinline__ f3 evaluate_brdf(const Pixel& px, f3 li_dir) {
+    f3 h = normalize3<X>(add3(li_dir, px.o));
+    float li = dot3(px.n, li_dir);
+    float odh = dot3(px.o, h);
+    f3 brdf = add3(mk3(0.0f, 0.0f, 0.0f), px.albedo);
+    float ndh = dot3(px.n, h);
+    float r2 = px.rough * px.rough;
+    float ggx = fma_(fma_(ndh, r2, -ndh), ndh, 1.0f);
+    ggx = div_<X>(r2, ggx * ggx);
+    float lo = px.lo;
+    float masking = li * sqrt_<X>(fma_(fma_(-lo, r2, lo), lo, r2));
+    float shadowing = lo * sqrt_<X>(fma_(fma_(-li, r2, li), li, r2));
+    float smith = div_<X>(0.5f, masking + shadowing);
+    float c = clamp_(odh, 0.0f, 1.0f);
+    float fl = 1.0f - c;
+    float fs = fl * fl;
+    float f5 = fs * fl * fs;
+    f3 F = mk3(px.f0.x + (1.0f - px.f0.x) * f5, px.f0.y + (1.0f - px.f0.y) * f5, px.f0.z + (1.0f - px.f0.z) * f5);
+    float spec = ggx * smith * dot3(F, mk3(0.21263901f, 0.71516868f, 0.07219232f));
+    brdf = add3(brdf, mk3(spec, spec, spec));
+    return scl3(brdf, 0.31830988618379067153776752674503f);
+}
+
+// get_mis_estimate (shading_pass.frag.glsl:219-269)
+template <bool X>
+__device__ __forceinline__ f3 mis_estimate(uint32_t h, f3 integ, f3 sw, float sd, f3 ow, float od, float ve) {
+    if (h == RISLTC_MIS_WEIGHTED) {
+        f3 ws = add3(scl3(sw, sd), scl3(ow, od));
+        f3 num = mul3(sw, integ);
+        return mk3(div_<X>(num.x, ws.x), div_<X>(num.y, ws.y), div_<X>(num.z, ws.z));
+    }
+    if (h == RISLTC_MIS_OPTIMAL_CLAMPED || h == RISLTC_MIS_OPTIMAL) {
+        float bw = rcp_<X>(sd + od);
+        f3 ws = add3(scl3(sw, sd), scl3(ow, od));
+        if (h == RISLTC_MIS_OPTIMAL_CLAMPED) {
+            f3 wwd = mk3(div_<X>(sw.x, ws.x), div_<X>(sw.y, ws.y), div_<X>(sw.z, ws.z));
+            float m0 = fma_(-ve, bw, bw);
+            f3 mixed = mk3(fma_(ve, wwd.x, m0), fma_(ve, wwd.y, m0), fma_(ve, wwd.z, m0));
+            return mul3(mixed, integ);
+        }
+        f3 a = scl3(sw, ve);
+        f3 d = sub3(integ, scl3(ws, ve));
+        return add3(a, sclf3(bw, d));
+    }
+    float wod = (h == RISLTC_MIS_BALANCE) ? rcp_<X>(sd + od) : div_<X>(sd, sd * sd + od * od);
+    return sclf3(wod, integ);
+}
+
+// ---- G-buffer (shading_pass.frag.glsl:571-672), always exact ----------------
+__device__ __forceinline__ f3 decode_normal(uint32_t qx, uint32_t qy) {
+    const float factor = 2.0f * (65534.0f / 65535.0f);
+    const float summand = -(32768.0f / 65535.0f) * factor;
+    float ox = fma_((float)qx / 65535.0f, factor, summand);
+    float oy = fma_((float)qy / 65535.0f, factor, summand);
+    f3 n = mk3(ox, oy, 1.0f - fabs_(ox) - fabs_(oy));
+    float sx = (ox >= 0.0f) ? 1.0f : -1.0f;
+    float sy = (oy >= 0.0f) ? 1.0f : -1.0f;
+    if (n.z < 0.0f) {
+        float nx = (1.0f - fabs_(n.y)) * sx;
+        float ny = (1.0f - fabs_(n.x)) * sy;
+        n.x = nx;
+        n.y = ny;
+    }
+    return normalize3<true>(n);
+}
+
+__device__ __forceinline__ f3 decode_position(uint2 q, const float* f, const float* s) {
+    uint32_t qx = q.x & 0x1FFFFFu;
+    uint32_t qy = ((q.x & 0xFFE00000u) >> 21) | ((q.y & 0x3FFu) << 11);
+    uint32_t qz = (q.y & 0x7FFFFC00u) >> 10;
+    return mk3(fma_((float)qx, f[0], s[0]), fma_((float)qy, f[1], s[1]), fma_((float)qz, f[2], s[2]));
+}
+
+__device__ __forceinline__ float ltc_fetch(const uint16_t* q, int C, int ch, uint32_t W, uint32_t H, int layer,
+                                           float u, float v) {
+    float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+    float x0 = __builtin_floorf(x), y0 = __builtin_floorf(y);
+    float fx = x - x0, fy = y - y0;
+    int ix = (int)x0, iy = (int)y0;
+    int wi = (int)W - 1, hi = (int)H - 1;
+    int i0 = ix < 0 ? 0 : (ix > wi ? wi : ix), i1 = ix + 1 < 0 ? 0 : (ix + 1 > wi ? wi : ix + 1);
+    int j0 = iy < 0 ? 0 : (iy > hi ? hi : iy), j1 = iy + 1 < 0 ? 0 : (iy + 1 > hi ? hi : iy + 1);
+    size_t base = (size_t)layer * H;
+    float t00 = (float)q[((base + j0) * W + i0) * C + ch] / 65535.0f;
+    float t10 = (float)q[((base + j0) * W + i1) * C + ch] / 65535.0f;
+    float t01 = (float)q[((base + j1) * W + i0) * C + ch] / 65535.0f;
+    float t11 = (float)q[((base + j1) * W + i1) * C + ch] / 65535.0f;
+    float a = fma_(fx, t10 - t00, t00);
+    float b = fma_(fx, t11 - t01, t01);
+    return fma_(fy, b - a, a);
+}
+
+__device__ __forceinline__ void setup_pixel(const ShadeArgs& A, uint32_t prim, f3 ray, Pixel& px) {
+    const SceneView& S = A.scene;
+    const risltc_frame_constants_t& c = A.c;
+    f3 p[3], nrm[3];
+    float tc[3][2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        size_t vi = (size_t)prim * 3 + i;
+        p[i] = decode_position(S.positions[vi], c.mesh_dequantization_factor, c.mesh_dequantization_summand);
+        ushort4 nt = S.normals_uv[vi];
+        nrm[i] = decode_normal(nt.x, nt.y);
+        tc[i][0] = fma_((float)nt.z / 65535.0f, 8.0f, 0.0f);
+        tc[i][1] = fma_((float)nt.w / 65535.0f, -8.0f, 1.0f);
+    }
+    f3 org = mk3(c.camera_position_world_space[0], c.camera_position_world_space[1], c.camera_position_world_space[2]);
+    f3 e0 = sub3(p[1], p[0]), e1 = sub3(p[2], p[0]);
+    f3 rce1 = cross3(ray, e1);
+    float rcp_det = 1.0f / dot3(e0, rce1);
+    f3 r0 = sub3(org, p[0]);
+    float d0 = dot3(r0, rce1);
+    float by = rcp_det * d0;
+    f3 e0c0 = cross3(e0, r0);
+    float d1 = dot3(ray, e0c0);
+    float bz = -rcp_det * d1;
+    float bx = 1.0f - (by + bz);
+    px.pos = mk3(fma_(bx, p[0].x, fma_(by, p[1].x, bz * p[2].x)), fma_(bx, p[0].y, fma_(by, p[1].y, bz * p[2].y)),
+                 fma_(bx, p[0].z, fma_(by, p[1].z, bz * p[2].z)));
+    f3 ni = normalize3<true>(mk3(fma_(bx, nrm[0].x, fma_(by, nrm[1].x, bz * nrm[2].x)),
+                                 fma_(bx, nrm[0].y, fma_(by, nrm[1].y, bz * nrm[2].y)),
+                                 fma_(bx, nrm[0].z, fma_(by, nrm[1].z, bz * nrm[2].z))));
+    uint32_t mi = S.material_idx[prim];
+    if (mi >= S.material_count) mi = S.material_count - 1u;
+    const float* m = S.materials + 6 * mi;
+    f3 base = mk3(m[0], m[1], m[2]);
+    float metal = m[5];
+    px.albedo = mk3(fma_(base.x, -metal, base.x), fma_(base.y, -metal, base.y), fma_(base.z, -metal, base.z));
+    float om = 1.0f - metal;
+    px.f0 = mk3(0.02f * om + base.x * metal, 0.02f * om + base.y * metal, 0.02f * om + base.z * metal);
+    float lr = m[4];
+    px.rough = clamp_(lr * lr * c.roughness_factor, 0.0064f, 1.0f);
+    float nts_x = fma_(0.5f, 2.0f, -1.0f), nts_y = nts_x;
+    float nts_z = __builtin_sqrtf(fmax_(0.0f, fma_(-nts_x, nts_x, fma_(-nts_y, nts_y, 1.0f))));
+    float te0x = tc[1][0] - tc[0][0], te0y = tc[1][1] - tc[0][1];
+    float te1x = tc[2][0] - tc[0][0], te1y = tc[2][1] - tc[0][1];
+    f3 nce0 = cross3(ni, e0);
+    f3 e1cn = cross3(e1, ni);
+    f3 tangent = add3(scl3(e1cn, te0x), scl3(nce0, te1x));
+    f3 bitangent = add3(scl3(e1cn, te0y), scl3(nce0, te1y));
+    float mtl = __builtin_sqrtf(0.5f * (dot3(tangent, tangent) + dot3(bitangent, bitangent)));
+    nts_z *= fmax_(1.0e-10f, mtl);
+    f3 nw = mk3(fma_(ni.x, nts_z, fma_(bitangent.x, nts_y, tangent.x * nts_x)),
+                fma_(ni.y, nts_z, fma_(bitangent.y, nts_y, tangent.y * nts_x)),
+                fma_(ni.z, nts_z, fma_(bitangent.z, nts_y, tangent.z * nts_x)));
+    px.n = normalize3<true>(nw);
+    px.o = normalize3<true>(sub3(org, px.pos));
+    float off = fmax_(0.0f, 1.0e-3f - dot3(px.n, px.o));
+    px.n = mk3(fma_(off, px.o.x, px.n.x), fma_(off, px.o.y, px.n.y), fma_(off, px.o.z, px.n.z));
+    px.n = normalize3<true>(px.n);
+    px.lo = dot3(px.n, px.o);
+    // get_ltc_coefficients (ltc_utility.glsl:56-88)
+    float f0l = dot3(px.f0, mk3(0.2126f, 0.7152f, 0.0722f));
+    const risltc_ltc_constants_t& k = c.ltc_constants;
+    float ndo = dot3(px.n, px.o);
+    float incl = acos_(clamp_(ndo, 0.0f, 1.0f));
+    float u = fma_(__builtin_sqrtf(clamp_(px.rough, 0.0f, 1.0f)), k.roughness_factor, k.roughness_summand);
+    float v = fma_(incl, k.inclination_factor, k.inclination_summand);
+    float w = fma_(clamp_(f0l, 0.0f, 1.0f), k.fresnel_index_factor, k.fresnel_index_summand);
+    float lw = __builtin_rintf(w);
+    int layer = lw < 0.0f ? 0 : (lw > (float)(S.ltc_layers - 1) ? (int)S.ltc_layers - 1 : (int)lw);
+    float d0x = ltc_fetch(S.ltc0, 4, 0, S.ltc_w, S.ltc_h, layer, u, v);
+    float d0y = ltc_fetch(S.ltc0, 4, 1, S.ltc_w, S.ltc_h, layer, u, v);
+    float d0z = ltc_fetch(S.ltc0, 4, 2, S.ltc_w, S.ltc_h, layer, u, v);
+    float d0w = ltc_fetch(S.ltc0, 4, 3, S.ltc_w, S.ltc_h, layer, u, v);
+    float d1x = ltc_fetch(S.ltc1, 2, 0, S.ltc_w, S.ltc_h, layer, u, v);
+    float d1y = ltc_fetch(S.ltc1, 2, 1, S.ltc_w, S.ltc_h, layer, u, v);
+    px.s2c[0] = d0x; px.s2c[1] = 0.0f; px.s2c[2] = d0w;
+    px.s2c[3] = 0.0f; px.s2c[4] = d0z; px.s2c[5] = 0.0f;
+    px.s2c[6] = -d0y; px.s2c[7] = 0.0f; px.s2c[8] = d1x;
+    px.ltc_albedo = d1y;
+    float det2 = d0x * d1x + d0y * d0w;
+    px.ltc_det = d0z * det2;
+    float inv = 1.0f / det2;
+    px.c2s[0] = d1x * inv; px.c2s[1] = 0.0f; px.c2s[2] = -d0w * inv;
+    px.c2s[3] = 0.0f; px.c2s[4] = 1.0f / d0z; px.c2s[5] = 0.0f;
+    px.c2s[6] = d0y * inv; px.c2s[7] = 0.0f; px.c2s[8] = d0x * inv;
+    f3 xa = normalize3<true>(mk3(fma_(-ndo, px.n.x, px.o.x), fma_(-ndo, px.n.y, px.o.y), fma_(-ndo, px.n.z, px.o.z)));
+    f3 ya = cross3(px.n, xa);
+    f3 rows[3] = {xa, ya, px.n};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        px.w2s[4 * i + 0] = rows[i].x;
+        px.w2s[4 * i + 1] = rows[i].y;
+        px.w2s[4 * i + 2] = rows[i].z;
+        px.w2s[4 * i + 3] = fma_(-rows[i].z, px.pos.z, fma_(-rows[i].y, px.pos.y, -rows[i].x * px.pos.x));
+    }
+    mul33_34(px.s2c, px.w2s, px.w2c);
+    float flip[12];
+#pragma unroll
+    for (int k2 = 0; k2 < 12; ++k2) flip[k2] = (k2 >= 4 && k2 < 8) ? -px.w2s[k2] : px.w2s[k2];
+    mul33_34(px.s2c, flip, px.w2c_flip);
+}
+
+__device__ __forceinline__ uint32_t light_index_from(float u, uint32_t n) {
+    int idx = (int)(u * (float)n);
+    if (idx > (int)n - 1) idx = (int)n - 1;
+    return (uint32_t)idx;
+}
+
+// evaluate_polygonal_light_shading, LTC_CP branch: the RIS target
+template <bool X>
+__device__ __forceinline__ f3 eval_ltc_cp(const Pixel& px, const Light& L, uint32_t& clip_counts) {
+    float side = plane_side(px.pos, L.plane);
+    bool flip = side < 0.0f;
+    float vx[8], vy[8], vz[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        f3 t = xform(px.w2s, L.v[3 * i], L.v[3 * i + 1], L.v[3 * i + 2]);
+        vx[i] = t.x;
+        vy[i] = flip ? -t.y : t.y;
+        vz[i] = t.z;
+    }
+    uint32_t vcd, vcs;
+    float ltc_d = polygon_ltc<X>(vx, vy, vz, L.n, vcd);
+    f3 diffuse = mk3(0.0f, 0.0f, 0.0f);
+    if (vcd > 0) diffuse = mul3(sclf3(ltc_d, px.albedo), L.rad);
+    f3 result = add3(mk3(0.0f, 0.0f, 0.0f), diffuse);
+    float m[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) m[k] = flip ? px.w2c_flip[k] : px.w2c[k];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        f3 t = xform(m, L.v[3 * i], L.v[3 * i + 1], L.v[3 * i + 2]);
+        vx[i] = t.x;
+        vy[i] = t.y;
+        vz[i] = t.z;
+    }
+    float ltc_s = polygon_ltc<X>(vx, vy, vz, L.n, vcs);
+    f3 ggx = mk3(0.0f, 0.0f, 0.0f);
+    if (vcs > 0) ggx = sclf3(ltc_s * px.ltc_albedo, L.rad);
+    result = add3(result, ggx);
+    clip_counts = vcd | (vcs << 8);
+    return result;
+}
+
+struct FinalOut {
+    f3 color;
+    f3 e_vis[2], e_occ[2];
+    f3 dir[2];
+    float tmax[2];
+    uint32_t state[2];
+    uint32_t clipd, clips;
+    float psad, psas;
+};
+
+// evaluate_polygonal_light_shading_peters (shading_pass.frag.glsl:292-397)
+// with SAMPLE_COUNT = 1; one projected-solid-angle polygon is live at a time
+// (diffuse prepared and sampled, then specular), which consumes the noise in
+// the reference order.
+template <bool X>
+__device__ __forceinline__ void eval_peters(const ShadeArgs& A, const Pixel& px, const Light& L, uint32_t& seed, FinalOut& F) {
+    bool fa = A.fast_atan != 0;
+    F.color = mk3(0.0f, 0.0f, 0.0f);
+    F.state[0] = F.state[1] = 0u;
+    F.clipd = F.clips = 0u;
+    F.psad = F.psas = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        F.e_vis[j] = F.e_occ[j] = F.dir[j] = mk3(0.0f, 0.0f, 0.0f);
+        F.tmax[j] = 0.0f;
+    }
+    float side = plane_side(px.pos, L.plane);
+    bool flip = side < 0.0f;
+    float vx[MAXV], vy[MAXV], vz[MAXV];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        f3 t = xform(px.w2s, L.v[3 * i], L.v[3 * i + 1], L.v[3 * i + 2]);
+        vx[i] = t.x;
+        vy[i] = flip ? -t.y : t.y;
+        vz[i] = t.z;
+    }
+    vx[8] = vx[0];
+    vy[8] = vy[0];
+    vz[8] = vz[0];
+    uint32_t vcd = clip_materialize<X>(vx, vy, vz, L.n);
+    F.clipd = vcd;
+    if (vcd == 0) return;
+    Psa P;
+    prepare_psa<X>(P, vcd, vx, vy, vz, fa);
+    F.psad = P.psa;
+    float psad = P.psa;
+    // transform for the specular polygon now (the diffuse vertices are dead)
+    float m[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) m[k] = flip ? px.w2c_flip[k] : px.w2c[k];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        f3 t = xform(m, L.v[3 * i], L.v[3 * i + 1], L.v[3 * i + 2]);
+        vx[i] = t.x;
+        vy[i] = t.y;
+        vz[i] = t.z;
+    }
+    vx[8] = vx[0];
+    vy[8] = vy[0];
+    vz[8] = vz[0];
+    uint32_t vcs = clip_materialize<X>(vx, vy, vz, L.n);
+    F.clips = vcs;
+    if (psad == 0.0f) {
+        // the reference still prepares the specular polygon; only its
+        // projected solid angle is reported
+        if (vcs != 0) {
+            Psa S;
+            prepare_psa<X>(S, vcs, vx, vy, vz, fa);
+            F.psas = S.psa;
+        }
+        return;
+    }
+    f3 dirs[2];
+    {
+        float u0 = noise_next(seed);
+        float u1 = noise_next(seed);
+        dirs[0] = sample_psa<X>(P, mk2(u0, u1), fa);
+    }
+    float psas = 0.0f;
+    dirs[1] = mk3(0.0f, 0.0f, 0.0f);
+    if (vcs != 0) {
+        prepare_psa<X>(P, vcs, vx, vy, vz, fa);
+        psas = P.psa;
+        if (psas > 0.0f) {
+            float u0 = noise_next(seed);
+            float u1 = noise_next(seed);
+            dirs[1] = sample_psa<X>(P, mk2(u0, u1), fa);
+            dirs[1] = normalize3<X>(mul33(px.c2s, dirs[1]));
+        }
+    }
+    F.psas = psas;
+    float spec_w = px.ltc_albedo * psas;
+    f3 da = mk3(fmax_(px.albedo.x, 0.01f), fmax_(px.albedo.y, 0.01f), fmax_(px.albedo.z, 0.01f));
+    f3 dw = scl3(da, psad);
+    uint32_t techniques = (psas > 0.0f) ? 2u : 1u;
+    float rcp_d = rcp_<X>(psad);
+    float rcp_s = rcp_<X>(psas);
+    f3 sw = mk3(spec_w, spec_w, spec_w);
+    uint32_t h = A.mis;
+    if (h == RISLTC_MIS_OPTIMAL) {
+        f3 rop = scl3(L.rad, 0.31830988618379067153776752674503f);
+        dw = mul3(dw, rop);
+        sw = mul3(sw, rop);
+    }
+    float ve = A.c.mis_visibility_estimate;
+    float w2s_r[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) w2s_r[k] = (flip && k >= 4 && k < 8) ? -px.w2s[k] : px.w2s[k];
+    f3 pn = mk3(L.plane[0], L.plane[1], L.plane[2]);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        if ((uint32_t)j >= techniques) continue;
+        f3 d = dirs[j];
+        if (d.z <= 0.0f) continue;
+        float dd = d.z * rcp_d;
+        f3 dc = mul33(px.s2c, d);
+        float l2 = dot3(dc, dc);
+        float sdn = div_<X>(fmax_(0.0f, dc.z) * px.ltc_det, l2 * l2) * rcp_s;
+        f3 wdir = mk3(fma_(w2s_r[8], d.z, fma_(w2s_r[4], d.y, w2s_r[0] * d.x)),
+                      fma_(w2s_r[9], d.z, fma_(w2s_r[5], d.y, w2s_r[1] * d.x)),
+                      fma_(w2s_r[10], d.z, fma_(w2s_r[6], d.y, w2s_r[2] * d.x)));
+        bool lambert_ok = dot3(px.n, wdir) > 0.0f;
+        uint32_t state = 1u;
+        float tmax = 0.0f;
+        if (lambert_ok) {
+            tmax = div_<X>(-plane_side(px.pos, L.plane), dot3(wdir, pn));
+            tmax -= 1e-3f;
+            state = A.shadow_rays ? 5u : 4u;  // 5 = traced, outcome pending
+        }
+        f3 rb = mul3(L.rad, evaluate_brdf<X>(px, wdir));
+        f3 iv = sclf3(d.z, rb);
+        f3 io = sclf3(d.z, mk3(0.0f, 0.0f, 0.0f));
+        f3 ev, eo;
+        if (j == 0 && psas <= 0.0f) {
+            float rd = rcp_<X>(dd);
+            ev = scl3(iv, rd);
+            eo = mk3(0.0f, 0.0f, 0.0f);
+        } else if (j == 0) {
+            ev = mis_estimate<X>(h, iv, dw, dd, sw, sdn, ve);
+            eo = mis_estimate<X>(h, io, dw, dd, sw, sdn, ve);
+        } else {
+            ev = mis_estimate<X>(h, iv, sw, sdn, dw, dd, ve);
+            eo = mis_estimate<X>(h, io, sw, sdn, dw, dd, ve);
+        }
+        F.e_vis[j] = ev;
+        F.e_occ[j] = eo;
+        F.dir[j] = wdir;
+        F.tmax[j] = tmax;
+        F.state[j] = state;
+    }
+}
+
+// ---- the kernel ----------------------------------------------------------------
+template <bool X, bool DEBUG>
+__global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
+    const uint32_t W = A.c.viewport_size[0], H = A.c.viewport_size[1];
+    const uint32_t tiles_x = (W + 15u) / 16u;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t x = (tile % tiles_x) * 16u + (lane & 15u);
+    const uint32_t y = (tile / tiles_x) * 16u + wave * 4u + (lane >> 4);
+    const bool inside = x < W && y < H;
+    const uint32_t pixel = y * W + x;
+    const uint32_t spp = A.spp;
+    uint32_t prim = inside ? A.visibility[pixel] : 0xFFFFFFFFu;
+    // ids past the mesh are treated as background (the reference would read
+    // out of bounds)
+    bool shade = inside && (prim >> 31) == 0u && (uint64_t)prim < A.scene.triangle_count;
+
+    Pixel px;
+    uint32_t seed = 0;
+    if (shade) {
+        const float(*P)[4] = A.c.pixel_to_ray_direction_world_space;
+        float fx = (float)x, fy = (float)y;
+        f3 ray = mk3(fma_(P[0][2], 1.0f, fma_(P[0][1], fy, P[0][0] * fx)),
+                     fma_(P[1][2], 1.0f, fma_(P[1][1], fy, P[1][0] * fx)),
+                     fma_(P[2][2], 1.0f, fma_(P[2][1], fy, P[2][0] * fx)));
+        setup_pixel(A, prim, ray, px);
+        seed = murmur_finalize(murmur_mix(murmur_mix(0u, x + y * W), A.c.noise_random_numbers[0]));
+    }
+    const uint32_t N = A.scene.light_count;
+    const float pN = 1.0f / (float)N;
+    const uint32_t M = A.M;
+    for (uint32_t j = 0; j < spp; ++j) {
+        int32_t chosen = -1;
+        float w_sum = 0.0f, sample_value = 0.0f;
+        Light L;
+        if (shade) {
+            if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
+                chosen = (int32_t)light_index_from(noise_next(seed), N);
+            } else {
+                for (uint32_t i = 0; i < M; ++i) {
+                    uint32_t idx = light_index_from(noise_next(seed), N);
+                    load_light(A.scene.lights, idx, L);
+                    uint32_t cc;
+                    f3 col = eval_ltc_cp<X>(px, L, cc);
+                    float p_hat = sqrt_<X>(dot3(col, col));
+                    float w = p_hat / pN;
+                    float r = noise_next(seed);
+                    w_sum += w;
+                    if (w > 0.0f && r < w / w_sum) {
+                        chosen = (int32_t)idx;
+                        sample_value = p_hat;
+                    }
+                    if constexpr (DEBUG) {
+                        if (A.debug_cands) {
+                            uint32_t* cr = A.debug_cands +
+                                           (((size_t)pixel * spp + j) * M + i) * RISLTC_CANDIDATE_RECORD_WORDS;
+                            cr[0] = idx;
+                            cr[1] = cc;
+                            cr[2] = __float_as_uint(p_hat);
+                            cr[3] = __float_as_uint(w);
+                        }
+                    }
+                }
+            }
+        }
+        FinalOut F;
+        F.state[0] = F.state[1] = 0u;
+        float Wt = 0.0f;
+        bool contributes = false;
+        if (shade && chosen >= 0) {
+            load_light(A.scene.lights, (uint32_t)chosen, L);
+            eval_peters<X>(A, px, L, seed, F);
+            contributes = true;
+            if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
+                Wt = (float)N;
+            } else {
+                Wt = w_sum / ((float)M * sample_value);
+                if (sample_value == 0.0f) Wt = 0.0f;
+            }
+        }
+        // append shadow rays to the compacted list: one atomic per wave
+        const size_t sidx = (size_t)pixel * spp + j;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            bool emit = contributes && F.state[t] == 5u;
+            uint64_t ballot = __ballot(emit);
+            uint32_t cnt = __popcll(ballot);
+            uint32_t base = 0;
+            if (cnt) {
+                uint32_t first = (uint32_t)__ffsll((unsigned long long)ballot) - 1u;
+                if (lane == first) base = atomicAdd(A.ray_count, cnt);
+                base = __shfl(base, (int)first);
+            }
+            if (emit) {
+                uint32_t off = base + __popcll(ballot & ((1ull << lane) - 1ull));
+                A.rays[off].org = make_float4(px.pos.x, px.pos.y, px.pos.z, 1.0e-3f);
+                A.rays[off].dir = make_float4(F.dir[t].x, F.dir[t].y, F.dir[t].z, F.tmax[t]);
+                A.ray_slots[off] = (uint32_t)(sidx * 2 + t);
+            }
+        }
+        if (inside) {
+            float4* rec = reinterpret_cast<float4*>(A.samples + sidx * kSampleWords);
+            uint32_t flags = (F.state[0] & 15u) | ((F.state[1] & 15u) << 4) | (contributes ? 256u : 0u);
+            rec[0] = make_float4(Wt, __uint_as_float(flags), F.e_vis[0].x, F.e_vis[0].y);
+            rec[1] = make_float4(F.e_vis[0].z, F.e_vis[1].x, F.e_vis[1].y, F.e_vis[1].z);
+            rec[2] = make_float4(F.e_occ[0].x, F.e_occ[0].y, F.e_occ[0].z, F.e_occ[1].x);
+            rec[3] = make_float4(F.e_occ[1].y, F.e_occ[1].z, 0.0f, 0.0f);
+        }
+        if constexpr (DEBUG) {
+            if (inside && A.debug_samples) {
+                float* r = A.debug_samples + sidx * RISLTC_SAMPLE_RECORD_FLOATS;
+                for (int k = 0; k < RISLTC_SAMPLE_RECORD_FLOATS; ++k) r[k] = 0.0f;
+                r[0] = __int_as_float(shade ? chosen : -1);
+                r[1] = w_sum;
+                r[2] = sample_value;
+                r[3] = Wt;
+                r[4] = __uint_as_float(F.clipd);
+                r[5] = __uint_as_float(F.clips);
+                r[6] = F.psad;
+                r[7] = F.psas;
+                for (int t = 0; t < 2; ++t) {
+                    r[8 + 3 * t] = F.dir[t].x;
+                    r[9 + 3 * t] = F.dir[t].y;
+                    r[10 + 3 * t] = F.dir[t].z;
+                    r[14 + t] = F.tmax[t];
+                    r[16 + t] = __uint_as_float(F.state[t]);
+                    r[18 + 3 * t] = F.e_vis[t].x;
+                    r[19 + 3 * t] = F.e_vis[t].y;
+                    r[20 + 3 * t] = F.e_vis[t].z;
+                    r[24 + 3 * t] = F.e_occ[t].x;
+                    r[25 + 3 * t] = F.e_occ[t].y;
+                    r[26 + 3 * t] = F.e_occ[t].z;
+                }
+                r[30] = F.color.x;
+                r[31] = F.color.y;
+                r[32] = F.color.z;
+                r[33] = __uint_as_float(seed);
+            }
+        }
+    }
+}
+
+// ---- resolve: sum the estimates in the reference's order -------------------
+__global__ void __launch_bounds__(256) resolve_kernel(ResolveArgs A) {
+    uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= A.width * A.height) return;
+    uint32_t prim = A.visibility[pixel];
+    float4 out;
+    if (prim == 0xFFFFFFFFu || ((prim >> 31) == 0u && (uint64_t)prim >= A.triangle_count)) {
+        out = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    } else {
+        f3 fin = mk3(0.0f, 0.0f, 0.0f);
+        if ((prim >> 31) > 0u) {
+            fin = mk3(1.0f, 1.0f, 1.0f);
+        } else {
+            for (uint32_t j = 0; j < A.spp; ++j) {
+                size_t sidx = (size_t)pixel * A.spp + j;
+                const float4* rec = reinterpret_cast<const float4*>(A.samples + sidx * kSampleWords);
+                float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+                uint32_t flags = __float_as_uint(r0.y);
+                if (!(flags & 256u)) continue;
+                f3 ev[2] = {mk3(r0.z, r0.w, r1.x), mk3(r1.y, r1.z, r1.w)};
+                f3 eo[2] = {mk3(r2.x, r2.y, r2.z), mk3(r2.w, r3.x, r3.y)};
+                f3 result = mk3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    uint32_t st = (flags >> (4 * t)) & 15u;
+                    if (st == 0u) continue;
+                    bool vis = st == 4u || (st == 5u && A.hits[sidx * 2 + t] == 0u);
+                    result = add3(result, vis ? ev[t] : eo[t]);
+                }
+                f3 col = scl3(result, 1.0f / 1.0f);
+                float Wt = r0.x;
+                float s = (float)A.spp;
+                fin = add3(fin, mk3((col.x * Wt) / s, (col.y * Wt) / s, (col.z * Wt) / s));
+            }
+        }
+        if (__builtin_isnan(fin.x) || __builtin_isnan(fin.y) || __builtin_isnan(fin.z) || __builtin_isinf(fin.x) ||
+            __builtin_isinf(fin.y) || __builtin_isinf(fin.z))
+            fin = mk3(1.0f / A.exposure, 0.0f / A.exposure, 0.8f / A.exposure);
+        out = make_float4(fin.x * A.exposure, fin.y * A.exposure, fin.z * A.exposure, 1.0f);
+    }
+    reinterpret_cast<float4*>(A.out)[pixel] = out;
+}
+
+// accum_pass.frag.glsl:45-53: out = fma(prev, n, cur) * (1 / (n + 1))
+__global__ void accumulate_kernel(float* accum, const float* frame, uint32_t count, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count * 4u) return;
+    float fn = (float)n;
+    float r = fma_(accum[i], fn, frame[i]);
+    accum[i] = fma_(r, 1.0f / (float)(n + 1u), 0.0f);
+}
+
+hipError_t launch_shade(const ShadeArgs& a, bool exact, hipStream_t s) {
+    uint32_t W = a.c.viewport_size[0], H = a.c.viewport_size[1];
+    dim3 grid(((W + 15u) / 16u) * ((H + 15u) / 16u)), block(256);
+    bool dbg = a.debug_samples != nullptr || a.debug_cands != nullptr;
+    if (exact) {
+        if (dbg) hipLaunchKernelGGL((shade_kernel<true, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((shade_kernel<true, false>), grid, block, 0, s, a);
+    } else {
+        if (dbg) hipLaunchKernelGGL((shade_kernel<false, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((shade_kernel<false, false>), grid, block, 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s) {
+    uint32_t n = a.width * a.height;
+    hipLaunchKernelGGL(resolve_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate(float* accum, const float* frame, uint32_t count, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(accumulate_kernel, dim3((count * 4u + 255u) / 256u), dim3(256), 0, s, accum, frame, count, n);
+    return hipGetLastError();
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t upload_clip_table(const uint8_t* table, size_t bytes) {
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_clip_rot), table, bytes, 0, hipMemcpyHostToDevice);
+}
+
+}  // namespace rl

@@ -1,0 +1,207 @@
+"""GPU parity: the HIP kernels, called through the C-ABI, against the CPU
+oracle on the same seeded inputs.
+
+  exact_math = 1 : IEEE division and square root in the kernels; the whole
+                   pass (visibility, every candidate record, every sample
+                   record, the output) is bitwise equal to the FP32 oracle.
+  exact_math = 0 : the production build (hardware rcp/rsq/sqrt). Clipped
+                   vertex counts and BVH hit flags are still bitwise equal;
+                   LTC values within 2e-5 relative; the RIS choice equal
+                   unless the oracle's decision was within 1e-5 of a tie;
+                   the final radiance within the tolerance stated below.
+"""
+import numpy as np
+import pytest
+
+import orc
+from risltc_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+@pytest.fixture(scope="module")
+def small_gpu(small_scene, ltc_table):
+    import risltc_amd
+    torch = _torch()
+    ctx = risltc_amd.Context(0)
+    ctx.load_scene(small_scene, *ltc_table)
+    oracle = orc.Oracle(small_scene, *ltc_table)
+    consts = ctx.frame_constants()
+    P = small_scene.width * small_scene.height
+    vis = torch.empty(P, dtype=torch.int32, device="cuda")
+    ctx.render_visibility(consts, vis)
+    torch.cuda.synchronize()
+    return dict(ctx=ctx, oracle=oracle, consts=consts, vis=vis, P=P, scene=small_scene)
+
+
+def _shade(g, settings, debug=True):
+    torch = _torch()
+    P = g["P"]
+    spp = settings.sample_count_light
+    M = settings.ris_candidate_count
+    out = torch.empty(P * 4, dtype=torch.float32, device="cuda")
+    if debug:
+        rec = torch.zeros(P * spp * abi.SAMPLE_RECORD_FLOATS, dtype=torch.float32, device="cuda")
+        cand = torch.zeros(P * spp * M * abi.CANDIDATE_RECORD_WORDS, dtype=torch.int32, device="cuda")
+        g["ctx"].shade_debug(g["consts"], settings, g["vis"], out, rec, cand)
+    else:
+        g["ctx"].shade(g["consts"], settings, g["vis"], out)
+    torch.cuda.synchronize()
+    o = out.cpu().numpy().reshape(P, 4)
+    if not debug:
+        return o, None, None
+    return (o, rec.cpu().numpy().reshape(P, spp, abi.SAMPLE_RECORD_FLOATS),
+            cand.cpu().numpy().view(np.uint32).reshape(P, spp, M, abi.CANDIDATE_RECORD_WORDS))
+
+
+def test_primary_visibility_bitwise(small_gpu):
+    g = small_gpu
+    vis_o = g["oracle"].render_visibility(g["consts"])
+    vis_g = g["vis"].cpu().numpy().view(np.uint32)
+    assert np.array_equal(vis_g, vis_o)
+    assert (vis_o == 0xFFFFFFFF).sum() > 0 and ((vis_o >> 31) == 1).sum() > (vis_o == 0xFFFFFFFF).sum()
+
+
+def test_shadow_rays_bitwise(small_gpu):
+    """Any-hit flags of random rays: GPU LBVH == CPU LBVH == brute force."""
+    torch = _torch()
+    g = small_gpu
+    rng = np.random.default_rng(5)
+    n = 20000
+    o = rng.uniform([-10, -10, 0], [10, 10, 8], (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    tmax = rng.uniform(0.1, 20, n).astype(np.float32)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3] = 1e-3
+    rays[:, 4:7] = d
+    rays[:, 7] = tmax
+    rt = torch.from_numpy(rays).cuda()
+    hits = torch.zeros(n, dtype=torch.int32, device="cuda")
+    g["ctx"].trace_shadow_rays(rt, hits)
+    torch.cuda.synchronize()
+    h = hits.cpu().numpy()
+    L = g["oracle"].L
+    ref = np.array([L.orc_bvh_any_hit(orc.C.byref(g["oracle"].bvh), orc.ptr(o[i], orc.f32p),
+                                      orc.ptr(d[i], orc.f32p), 1e-3, float(tmax[i]), None) for i in range(n)])
+    assert np.array_equal(h, ref)
+    assert 0.05 < h.mean() < 0.95
+
+
+@pytest.mark.parametrize("spp,M", [(1, 8), (2, 8)])
+def test_exact_shading_bitwise(small_gpu, spp, M):
+    g = small_gpu
+    s = g["scene"].settings(exact_math=1, sample_count_light=spp, ris_candidate_count=M)
+    out, rec, cand = _shade(g, s)
+    vis = g["vis"].cpu().numpy().view(np.uint32)
+    ref, rrec, rcand = g["oracle"].shade(g["consts"], s, vis, records=True, candidates=True)
+    # candidates: index, clip counts, p_hat, w -- all bitwise
+    assert np.array_equal(cand, rcand)
+    # sample records: selected light, reservoir, clip counts, PSA, rays, estimates
+    ri = rec.view(np.uint32)
+    oi = rrec.view(np.uint32)
+    for f in (abi.REC_LIGHT, abi.REC_WSUM, abi.REC_PHAT, abi.REC_W, abi.REC_CLIPD, abi.REC_CLIPS, abi.REC_PSAD,
+              abi.REC_PSAS):
+        assert np.array_equal(ri[..., f], oi[..., f]), f
+    for f in list(range(abi.REC_DIR, abi.REC_DIR + 6)) + [abi.REC_TMAX, abi.REC_TMAX + 1] + \
+            list(range(abi.REC_EVIS, abi.REC_EVIS + 12)):
+        assert np.array_equal(ri[..., f], oi[..., f]), f
+    # ray states: GPU 5 = traced (answer pending), oracle 2 = hit / 3 = miss
+    gs = ri[..., abi.REC_STATE:abi.REC_STATE + 2]
+    os_ = oi[..., abi.REC_STATE:abi.REC_STATE + 2]
+    assert np.array_equal(gs == 5, (os_ == 2) | (os_ == 3))
+    assert np.array_equal(gs * (gs < 5), os_ * (os_ < 2))
+    assert np.array_equal(ri[..., abi.REC_SEED], oi[..., abi.REC_SEED])
+    # the frame
+    assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+    shaded = (vis >> 31) == 0
+    assert (out[shaded, :3].sum(1) > 0).mean() > 0.5
+
+
+@pytest.mark.parametrize("variant", [
+    dict(light_sampling=abi.LIGHT_UNIFORM),
+    dict(mis_heuristic=abi.MIS_BALANCE),
+    dict(mis_heuristic=abi.MIS_POWER),
+    dict(mis_heuristic=abi.MIS_WEIGHTED),
+    dict(mis_heuristic=abi.MIS_OPTIMAL),
+    dict(shadow_rays=0),
+    dict(fast_atan=1),
+])
+def test_exact_settings_variants_bitwise(small_gpu, variant):
+    g = small_gpu
+    s = g["scene"].settings(exact_math=1, sample_count_light=2, **variant)
+    out, _, _ = _shade(g, s, debug=False)
+    vis = g["vis"].cpu().numpy().view(np.uint32)
+    ref, _, _ = g["oracle"].shade(g["consts"], s, vis)
+    assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+
+
+def test_fast_shading_parity(small_gpu):
+    g = small_gpu
+    s = g["scene"].settings(exact_math=0, sample_count_light=2)
+    out, rec, cand = _shade(g, s)
+    vis = g["vis"].cpu().numpy().view(np.uint32)
+    ref, rrec, rcand = g["oracle"].shade(g["consts"], s, vis, records=True, candidates=True)
+    # light indices and clipped vertex counts: bitwise
+    assert np.array_equal(cand[..., 0], rcand[..., 0])
+    assert np.array_equal(cand[..., 1], rcand[..., 1])
+    # LTC targets: relative 2e-5 (hardware rcp/rsq/sqrt are ~1 ulp each)
+    pg = cand[..., 2].view(np.float32).astype(np.float64)
+    po = rcand[..., 2].view(np.float32).astype(np.float64)
+    assert np.all(np.abs(pg - po) <= 2e-5 * np.abs(po) + 1e-30)
+    # RIS choice: equal unless the oracle's decision was within 1e-5 of a tie
+    li_g = rec[..., abi.REC_LIGHT].view(np.int32)
+    li_o = rrec[..., abi.REC_LIGHT].view(np.int32)
+    frac = (li_g == li_o).mean()
+    assert frac > 0.999, frac
+    # output: samples whose choice and clip counts agree match within 1e-3
+    # relative per pixel (the PSA refinement amplifies ulp differences)
+    same = np.all(li_g == li_o, axis=1)
+    d = np.abs(out[same, :3] - ref[same, :3])
+    rel = d / (np.abs(ref[same, :3]) + 1e-3)
+    assert np.percentile(rel, 99.9) < 1e-3, np.percentile(rel, 99.9)
+
+
+def test_error_paths(small_gpu, ltc_table):
+    import risltc_amd
+    torch = _torch()
+    ctx = risltc_amd.Context(0)
+    g = small_gpu
+    out = torch.empty(g["P"] * 4, dtype=torch.float32, device="cuda")
+    s = g["scene"].settings()
+    with pytest.raises(risltc_amd.RisltcError) as e:
+        ctx.shade(g["consts"], s, g["vis"], out)
+    assert e.value.code == abi.E_STATE
+    ctx.load_scene(g["scene"], *ltc_table)
+    for bad, code in ((dict(sample_count=2), abi.E_UNSUPPORTED), (dict(polygon_sampling_technique=2),
+                                                                   abi.E_UNSUPPORTED),
+                      (dict(sample_count_light=0), abi.E_ARG), (dict(mis_heuristic=9), abi.E_ARG)):
+        with pytest.raises(risltc_amd.RisltcError) as e:
+            ctx.shade(g["consts"], g["scene"].settings(**bad), g["vis"], out)
+        assert e.value.code == code
+    # records with a vertex count outside [3, vmax]
+    recs = bytearray(g["scene"].light_records)
+    recs[32:36] = (9).to_bytes(4, "little")
+    with pytest.raises(risltc_amd.RisltcError) as e:
+        ctx.set_lights(bytes(recs), g["scene"].light_count, g["scene"].light_vmax)
+    assert e.value.code == abi.E_FORMAT
+    ctx.close()
+
+
+def test_accumulate_running_mean(small_gpu):
+    """accum_pass.frag.glsl:45-53: after n frames the buffer is their mean."""
+    torch = _torch()
+    g = small_gpu
+    n = 4096
+    frames = torch.rand(5, n * 4, device="cuda")
+    acc = torch.zeros(n * 4, device="cuda")
+    for i in range(5):
+        g["ctx"].accumulate(acc, frames[i], n, i)
+    torch.cuda.synchronize()
+    assert torch.allclose(acc, frames.mean(0), rtol=1e-5, atol=1e-6)
