@@ -85,7 +85,8 @@ struct ResolveArgs {
     const uint32_t* visibility;
     const float* samples;
     const uint32_t* hits;
-    float* out;  // RGBA32F
+    float* out;            // RGBA32F
+    float* debug_samples;  // when set: ray outcomes and colours are written back
 };
 
 // host-side launchers (shade.hip, lbvh.hip)

@@ -1039,7 +1039,15 @@ __global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
             }
         }
         FinalOut F;
+        F.color = mk3(0.0f, 0.0f, 0.0f);
         F.state[0] = F.state[1] = 0u;
+        F.clipd = F.clips = 0u;
+        F.psad = F.psas = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            F.e_vis[t] = F.e_occ[t] = F.dir[t] = mk3(0.0f, 0.0f, 0.0f);
+            F.tmax[t] = 0.0f;
+        }
         float Wt = 0.0f;
         bool contributes = false;
         if (shade && chosen >= 0) {
@@ -1137,14 +1145,22 @@ __global__ void __launch_bounds__(256) resolve_kernel(ResolveArgs A) {
                 f3 ev[2] = {mk3(r0.z, r0.w, r1.x), mk3(r1.y, r1.z, r1.w)};
                 f3 eo[2] = {mk3(r2.x, r2.y, r2.z), mk3(r2.w, r3.x, r3.y)};
                 f3 result = mk3(0.0f, 0.0f, 0.0f);
+                float* dbg = A.debug_samples ? A.debug_samples + sidx * RISLTC_SAMPLE_RECORD_FLOATS : nullptr;
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
                     uint32_t st = (flags >> (4 * t)) & 15u;
                     if (st == 0u) continue;
-                    bool vis = st == 4u || (st == 5u && A.hits[sidx * 2 + t] == 0u);
+                    bool hit = st == 5u && A.hits[sidx * 2 + t] != 0u;
+                    bool vis = st == 4u || (st == 5u && !hit);
                     result = add3(result, vis ? ev[t] : eo[t]);
+                    if (dbg && st == 5u) dbg[16 + t] = __uint_as_float(hit ? 2u : 3u);
                 }
                 f3 col = scl3(result, 1.0f / 1.0f);
+                if (dbg) {
+                    dbg[30] = col.x;
+                    dbg[31] = col.y;
+                    dbg[32] = col.z;
+                }
                 float Wt = r0.x;
                 float s = (float)A.spp;
                 fin = add3(fin, mk3((col.x * Wt) / s, (col.y * Wt) / s, (col.z * Wt) / s));
