@@ -43,7 +43,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="city")
     ap.add_argument("--exact", action="store_true", help="exact-math kernels (parity build)")
-    ap.add_argument("--cpu-pixels", type=int, default=16384)
+    ap.add_argument("--cpu-pixels", type=int, default=524288)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (known HBM bytes) for counter scaling")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
@@ -51,6 +53,16 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        n = 1 << 28  # 2^28 floats = 1 GiB read + 1 GiB written per copy
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}))
+        return
+
     import risltc_amd
     from risltc_amd import ltcfit, synth
 

@@ -324,7 +324,8 @@ typedef struct risltc_sample_record_s {
     float estimate_occluded[2][3]; /* MIS estimate if occluded               */
     float color[3];          /* evaluate_polygonal_light_shading_peters()    */
     uint32_t noise_seed_end; /* LCG state after this sample                  */
-    float pad[6];
+    float position[3];       /* shading position (shadow ray origin)         */
+    float pad[3];
 } risltc_sample_record_t;
 typedef struct risltc_candidate_record_s {
     int32_t light_index;

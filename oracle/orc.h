@@ -205,6 +205,7 @@ float orc_ltc_fetch_f32(const orc_scene_t* scene, int table, int channel, float 
 #define ORC_REC_EOCC 24
 #define ORC_REC_COLOR 30
 #define ORC_REC_SEED 33
+#define ORC_REC_POS 34
 
 #ifdef __cplusplus
 }

@@ -779,6 +779,11 @@ static V3 eval_peters(pix_t* px, const light_t* l, float* rec) {
     const A34* w2s = (side < K(0.0)) ? &px->w2s_flip : &px->ltc.world_to_shading;
     const A34* w2c = (side < K(0.0)) ? &px->w2c_flip : &px->w2c;
     psa_t pd, ps;
+    if (rec) {
+        rec[ORC_REC_POS + 0] = (float)px->sd.position.x;
+        rec[ORC_REC_POS + 1] = (float)px->sd.position.y;
+        rec[ORC_REC_POS + 2] = (float)px->sd.position.z;
+    }
     V3 vs[MAXV];
     for (uint32_t i = 0; i != l->vertex_count; ++i) vs[i] = xform(w2s, l->v[i]);
     uint32_t vcd = clip_polygon(l->vertex_count, vs);

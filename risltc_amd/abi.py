@@ -21,7 +21,7 @@ PROPOSAL_UNIFORM, PROPOSAL_LEARNED = 0, 1
 
 # risltc_sample_record_t word offsets
 REC_LIGHT, REC_WSUM, REC_PHAT, REC_W, REC_CLIPD, REC_CLIPS, REC_PSAD, REC_PSAS = range(8)
-REC_DIR, REC_TMAX, REC_STATE, REC_EVIS, REC_EOCC, REC_COLOR, REC_SEED = 8, 14, 16, 18, 24, 30, 33
+REC_DIR, REC_TMAX, REC_STATE, REC_EVIS, REC_EOCC, REC_COLOR, REC_SEED, REC_POS = 8, 14, 16, 18, 24, 30, 33, 34
 
 
 def light_record_size(max_vertex_count):

@@ -126,9 +126,13 @@ __device__ __forceinline__ float polygon_ltc(const float* px, const float* py, c
     bool clipped = mask != full;
     uint32_t L = clipped ? k + 2u : n;
     vc = L;
-    f3 nv[8];
+    // slots n..8 of px/py/pz hold vertex 0 (the light records repeat it), so
+    // edge i always ends at slot i + 1: static indices, no address selects
+    f3 nv[9];
 #pragma unroll
     for (int i = 0; i < 8; ++i) nv[i] = normalize3<X>(mk3(px[i], py[i], pz[i]));
+    nv[8] = nv[0];
+    const uint32_t mask9 = mask | ((mask & 1u) << n);
     f3 E = mk3(0.0f, 0.0f, 0.0f), Xp = E;
     float acc = 0.0f;
     float t[MAXV];
@@ -144,14 +148,12 @@ __device__ __forceinline__ float polygon_ltc(const float* px, const float* py, c
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         if ((uint32_t)i < n) {
-            bool wrap = (uint32_t)(i + 1) >= n;
-            int j = (i + 1) & 7;
             f3 a = mk3(px[i], py[i], pz[i]);
-            f3 b = wrap ? mk3(px[0], py[0], pz[0]) : mk3(px[j], py[j], pz[j]);
+            f3 b = mk3(px[i + 1], py[i + 1], pz[i + 1]);
             f3 na = nv[i];
-            f3 nb = wrap ? nv[0] : nv[j];
-            bool in_a = (mask >> i) & 1u;
-            bool in_b = (mask >> (wrap ? 0 : j)) & 1u;
+            f3 nb = nv[i + 1];
+            bool in_a = (mask9 >> i) & 1u;
+            bool in_b = (mask9 >> (i + 1)) & 1u;
             f3 P = na;
             if (in_a != in_b) {
                 P = normalize3<X>(iz0<X>(a, b));
@@ -790,7 +792,7 @@ template <bool X>
 __device__ __forceinline__ f3 eval_ltc_cp(const Pixel& px, const Light& L, uint32_t& clip_counts) {
     float side = plane_side(px.pos, L.plane);
     bool flip = side < 0.0f;
-    float vx[8], vy[8], vz[8];
+    float vx[9], vy[9], vz[9];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         f3 t = xform(px.w2s, L.v[3 * i], L.v[3 * i + 1], L.v[3 * i + 2]);
@@ -798,6 +800,9 @@ __device__ __forceinline__ f3 eval_ltc_cp(const Pixel& px, const Light& L, uint3
         vy[i] = flip ? -t.y : t.y;
         vz[i] = t.z;
     }
+    vx[8] = vx[0];
+    vy[8] = vy[0];
+    vz[8] = vz[0];
     uint32_t vcd, vcs;
     float ltc_d = polygon_ltc<X>(vx, vy, vz, L.n, vcd);
     f3 diffuse = mk3(0.0f, 0.0f, 0.0f);
@@ -813,6 +818,9 @@ __device__ __forceinline__ f3 eval_ltc_cp(const Pixel& px, const Light& L, uint3
         vy[i] = t.y;
         vz[i] = t.z;
     }
+    vx[8] = vx[0];
+    vy[8] = vy[0];
+    vz[8] = vz[0];
     float ltc_s = polygon_ltc<X>(vx, vy, vz, L.n, vcs);
     f3 ggx = mk3(0.0f, 0.0f, 0.0f);
     if (vcs > 0) ggx = sclf3(ltc_s * px.ltc_albedo, L.rad);
@@ -1118,6 +1126,11 @@ __global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
                 r[31] = F.color.y;
                 r[32] = F.color.z;
                 r[33] = __uint_as_float(seed);
+                if (contributes) {
+                    r[34] = px.pos.x;
+                    r[35] = px.pos.y;
+                    r[36] = px.pos.z;
+                }
             }
         }
     }

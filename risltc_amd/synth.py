@@ -309,7 +309,7 @@ def small_scene(seed=0, width=256, height=256, light_count=64, spp=1, candidates
                    candidates, seed)
 
 
-def city_scene(seed=1, width=1920, height=1080, light_count=1024, grid=290, cell=4.0, spp=4,
+def city_scene(seed=1, width=1920, height=1080, light_count=1024, grid=270, cell=4.0, spp=4,
                candidates=32):
     """Procedural city: grid x grid boxes (12 triangles each) on a ground plane,
     heights log-uniform in [1, 50] m; lights are convex 3-8-gons."""
@@ -327,13 +327,22 @@ def city_scene(seed=1, width=1920, height=1080, light_count=1024, grid=290, cell
     hi = np.stack([cx + fx / 2, cy + fy / 2, h], axis=1)
     btris, bnrm, buv = _boxes_vectorized(lo, hi)
     bmat = np.repeat(rng.integers(0, 64, nb), 12)
-    ground = np.array([[(-half, -half, 0), (half, -half, 0), (half, half, 0)],
-                       [(-half, -half, 0), (half, half, 0), (-half, half, 0)]], dtype=np.float64)
+    # the ground is tessellated into one quad per cell: a single huge ground
+    # triangle would inflate every LBVH box on its path to the whole city
+    gx0 = (ii.ravel() * cell - half)[:, None]
+    gy0 = (jj.ravel() * cell - half)[:, None]
+    zero = np.zeros_like(gx0)
+    c00 = np.concatenate([gx0, gy0, zero], 1)
+    c10 = np.concatenate([gx0 + cell, gy0, zero], 1)
+    c11 = np.concatenate([gx0 + cell, gy0 + cell, zero], 1)
+    c01 = np.concatenate([gx0, gy0 + cell, zero], 1)
+    ground = np.concatenate([np.stack([c00, c10, c11], 1), np.stack([c00, c11, c01], 1)])
+    ng = ground.shape[0]
+    guv = np.tile(np.array([[(0, 0), (1, 0), (1, 1)]], dtype=np.float64), (ng, 1, 1))
     tris = np.concatenate([ground, btris])
-    nrm = np.concatenate([np.tile(np.array([0, 0, 1.0]), (2, 3, 1)), bnrm])
-    uvs = np.concatenate([np.array([[(0, 0), (1, 0), (1, 1)], [(0, 0), (1, 1), (0, 1)]], dtype=np.float64),
-                          buv])
-    mats = np.concatenate([[0, 0], bmat])
+    nrm = np.concatenate([np.tile(np.array([0, 0, 1.0]), (ng, 3, 1)), bnrm])
+    uvs = np.concatenate([guv, buv])
+    mats = np.concatenate([np.zeros(ng, dtype=np.int64), bmat])
     lights = []
     for i in range(light_count):
         n = int(rng.integers(3, 9))

@@ -1,0 +1,114 @@
+"""Parity at the measurement workload: the 1920x1080 city (~1M triangles,
+1024 convex 3-8-gon lights, M = 32, 4 spp), GPU frame against the oracle on
+a seeded 4096-pixel subset, and per-sample records at reduced resolution."""
+import numpy as np
+import pytest
+
+import orc
+from risltc_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def city(ltc_table):
+    import torch
+    import risltc_amd
+    scene = synth.city_scene()
+    ctx = risltc_amd.Context(0)
+    ctx.load_scene(scene, *ltc_table)
+    consts = ctx.frame_constants()
+    P = scene.width * scene.height
+    vis = torch.empty(P, dtype=torch.int32, device="cuda")
+    ctx.render_visibility(consts, vis)
+    torch.cuda.synchronize()
+    oracle = orc.Oracle(scene, *ltc_table)
+    rng = np.random.default_rng(4096)
+    vis_h = vis.cpu().numpy().view(np.uint32)
+    # the subset: 4096 random pixels that see geometry (not background / lights)
+    shaded = np.flatnonzero((vis_h >> 31) == 0)
+    pix = np.sort(rng.choice(shaded, 4096, replace=False)).astype(np.uint32)
+    return dict(scene=scene, ctx=ctx, consts=consts, vis=vis, vis_h=vis_h, oracle=oracle, pix=pix, P=P)
+
+
+def _frame(c, settings):
+    import torch
+    out = torch.empty(c["P"] * 4, dtype=torch.float32, device="cuda")
+    c["ctx"].shade(c["consts"], settings, c["vis"], out)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().reshape(-1, 4)
+
+
+def test_city_visibility_bitwise(city):
+    vis_o = city["oracle"].render_visibility(city["consts"], threads=16)
+    assert np.array_equal(city["vis_h"], vis_o)
+
+
+def test_city_exact_subset_bitwise(city):
+    s = city["scene"].settings(exact_math=1)
+    out = _frame(city, s)
+    ref, _, _ = city["oracle"].shade(city["consts"], s, city["vis_h"], pixels=city["pix"])
+    got = out[city["pix"]]
+    bad = np.argwhere(np.any(got.view(np.uint32) != ref.view(np.uint32), axis=1)).ravel()
+    assert bad.size == 0, (bad.size, city["pix"][bad[:5]])
+    # most city pixels are shadowed by buildings, a fifth or so are lit
+    assert (got[:, :3].sum(1) > 0).mean() > 0.15
+
+
+def test_city_fast_vs_fp64(city):
+    """Per-pixel radiance of the fast build and of the exact build against the
+    FP64 oracle on the subset: the fast build is as accurate as IEEE FP32
+    (quantiles of the relative error within 2x + 1e-6), and the RIS choices of
+    the FP64 and FP32 oracles agree on > 99% of samples."""
+    s_fast = city["scene"].settings(exact_math=0)
+    s_exact = city["scene"].settings(exact_math=1)
+    fast = _frame(city, s_fast)[city["pix"], :3].astype(np.float64)
+    exact = _frame(city, s_exact)[city["pix"], :3].astype(np.float64)
+    r64, rec64, _ = city["oracle"].shade(city["consts"], s_exact, city["vis_h"], pixels=city["pix"], fp64=True,
+                                         records=True)
+    r32, rec32, _ = city["oracle"].shade(city["consts"], s_exact, city["vis_h"], pixels=city["pix"], records=True)
+    li64 = rec64[..., abi.REC_LIGHT].view(np.int32)
+    li32 = rec32[..., abi.REC_LIGHT].view(np.int32)
+    assert (li64 == li32).mean() > 0.99
+    same = np.all(li64 == li32, axis=1)
+    ref = r64[same, :3]
+    scale = np.abs(ref) + 1e-3 * np.abs(ref).mean()
+    ef = np.abs(fast[same] - ref) / scale
+    ex = np.abs(exact[same] - ref) / scale
+    for q in (50, 90, 99):
+        assert np.percentile(ef, q) <= 2.0 * np.percentile(ex, q) + 1e-6, (q, np.percentile(ef, q),
+                                                                          np.percentile(ex, q))
+
+
+def test_city_records_reduced_resolution(ltc_table):
+    """Every candidate and sample record bitwise (exact build) on a 320x180
+    rendering of the city camera."""
+    import torch
+    import risltc_amd
+    scene = synth.city_scene(width=320, height=180)
+    ctx = risltc_amd.Context(0)
+    ctx.load_scene(scene, *ltc_table)
+    consts = ctx.frame_constants()
+    P = scene.width * scene.height
+    vis = torch.empty(P, dtype=torch.int32, device="cuda")
+    ctx.render_visibility(consts, vis)
+    s = scene.settings(exact_math=1)
+    out = torch.empty(P * 4, dtype=torch.float32, device="cuda")
+    rec = torch.zeros(P * s.sample_count_light * abi.SAMPLE_RECORD_FLOATS, device="cuda")
+    cand = torch.zeros(P * s.sample_count_light * s.ris_candidate_count * abi.CANDIDATE_RECORD_WORDS,
+                       dtype=torch.int32, device="cuda")
+    ctx.shade_debug(consts, s, vis, out, rec, cand)
+    torch.cuda.synchronize()
+    oracle = orc.Oracle(scene, *ltc_table)
+    vis_h = vis.cpu().numpy().view(np.uint32)
+    ref, rrec, rcand = oracle.shade(consts, s, vis_h, records=True, candidates=True)
+    spp, M = s.sample_count_light, s.ris_candidate_count
+    assert np.array_equal(cand.cpu().numpy().view(np.uint32).reshape(P, spp, M, 4), rcand)
+    g = rec.cpu().numpy().reshape(P, spp, abi.SAMPLE_RECORD_FLOATS).view(np.uint32)
+    o = rrec.view(np.uint32)
+    fields = list(range(0, 30)) + [33]
+    assert np.array_equal(g[..., fields], o[..., fields])
+    assert np.array_equal(out.cpu().numpy().reshape(P, 4).view(np.uint32), ref.view(np.uint32))
+    # the workload exercises clipping, 8-gons and both PSA cases
+    clipd = o[..., abi.REC_CLIPD]
+    assert (clipd == 9).any() and ((clipd > 0) & (clipd < 9)).any()
