@@ -260,9 +260,10 @@ int risltc_set_ltc_table(risltc_context_t* context, const uint16_t* rgba16, cons
 /* Mesh in the *.vks layout (src/scene.h:47-70) plus a constant material per
  * material index. Uploads it and builds the LBVH used for shadow rays
  * (30-bit Morton codes of triangle centroids, sorted, Karras hierarchy,
- * escape links). Light triangles (bit 31 of the second position word) are
- * part of the hierarchy, as they are in the reference BLAS
- * (src/scene.c:176-187). Host pointers. */
+ * nodes stored in depth-first pre-order with escape links). Light triangles
+ * (bit 31 of the second position word) are part of the hierarchy, as they
+ * are in the reference BLAS (src/scene.c:176-187). Host pointers. At most
+ * 400,000,000 triangles (RISLTC_E_UNSUPPORTED beyond). */
 int risltc_set_scene(risltc_context_t* context, const uint32_t* positions,
                      const uint16_t* normals_and_tex_coords, const uint8_t* material_indices,
                      uint64_t triangle_count, const float dequantization_factor[3],

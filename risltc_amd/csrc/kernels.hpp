@@ -43,11 +43,19 @@ struct SceneView {
     uint32_t ltc_w, ltc_h, ltc_layers;
 };
 
+// The LBVH as one array of 16-byte units in depth-first pre-order (left
+// child first). Internal node: 2 units (min.xyz | escape unit, max.xyz | 0);
+// a box hit continues at the next record, a miss at the escape unit (the
+// first unit past the node's subtree). Leaf: 3 units (v0.xyz | -1,
+// v1.xyz | triangle id, v2.xyz | light flag bit 31); it continues at the next
+// record. Traversal ends at unit `units`; two zero units of padding follow.
 struct BvhView {
-    const float4* nodes;   // internal: 2 float4 per node (min.xyz | left, max.xyz | escape)
-    const float4* leaves;  // 3 float4 per leaf (v0 | escape, v1 | prim, v2 | flag)
+    const float4* rec;
+    uint32_t units;       // 5 * leaf_count - 2
     uint32_t leaf_count;
 };
+// largest triangle count whose pre-order units fit a signed 32-bit index
+constexpr uint64_t kMaxBvhTriangles = 400000000ull;
 
 struct ShadeArgs {
     risltc_frame_constants_t c;
@@ -101,8 +109,8 @@ hipError_t upload_clip_table(const uint8_t* table, size_t bytes);
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
 
 struct BvhBuild {
-    float4* nodes = nullptr;
-    float4* leaves = nullptr;
+    float4* rec = nullptr;
+    uint32_t units = 0;
     uint32_t leaf_count = 0;
 };
 // Builds the LBVH of the dequantised triangles of a *.vks mesh on the device.

@@ -13,7 +13,9 @@
 //   3. Karras (2012) hierarchy: internal node i splits the sorted range at
 //      the highest differing key bit; leaves are node ids n-1 .. 2n-2;
 //   4. bottom-up boxes with one atomic arrival counter per internal node;
-//   5. escape links for stackless pre-order traversal (left child first).
+//   5. the nodes written in depth-first pre-order (left child first) with
+//      escape links, for stackless traversal: a box hit continues at the
+//      next record in memory, so a descent reads consecutive cache lines.
 // The CPU oracle builds the same tree from the same keys; traversal uses the
 // same box and Moeller-Trumbore tests in the same operation order, so hit and
 // miss flags agree bit for bit.
@@ -122,7 +124,8 @@ __device__ __forceinline__ int kdelta(const uint64_t* keys, int n, int i, int j)
     return __clzll((long long)(keys[i] ^ keys[j]));
 }
 
-__global__ void karras_kernel(const uint64_t* keys, int n, int* left, int* right, int* parent) {
+__global__ void karras_kernel(const uint64_t* keys, int n, int* left, int* right, int* parent, int* first,
+                              int* count) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -148,10 +151,11 @@ __global__ void karras_kernel(const uint64_t* keys, int n, int* left, int* right
     right[i] = rc;
     parent[lc] = i;
     parent[rc] = i;
+    first[i] = lo;
+    count[i] = hi - lo + 1;
 }
 
-__global__ void leaf_kernel(const uint64_t* keys, const float* tris, int n, float* box, float4* leaves,
-                            const uint2* pos) {
+__global__ void leaf_kernel(const uint64_t* keys, const float* tris, int n, float* box) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     uint32_t prim = (uint32_t)(keys[k] & 0xFFFFFFFFull);
@@ -162,10 +166,6 @@ __global__ void leaf_kernel(const uint64_t* keys, const float* tris, int n, floa
         b[a] = fmin_(fmin_(v[a], v[3 + a]), v[6 + a]);
         b[3 + a] = fmax_(fmax_(v[a], v[3 + a]), v[6 + a]);
     }
-    uint32_t light = pos[(size_t)prim * 3].y & 0x80000000u;
-    leaves[3 * k + 0] = make_float4(v[0], v[1], v[2], 0.0f);
-    leaves[3 * k + 1] = make_float4(v[3], v[4], v[5], __uint_as_float(prim));
-    leaves[3 * k + 2] = make_float4(v[6], v[7], v[8], __uint_as_float(light));
 }
 
 __global__ void refit_kernel(int n, const int* left, const int* right, const int* parent, float* box,
@@ -193,28 +193,42 @@ __global__ void refit_kernel(int n, const int* left, const int* right, const int
     }
 }
 
-__global__ void escape_pack_kernel(int n, const int* left, const int* right, const int* parent, const float* box,
-                                   float4* nodes, float4* leaves) {
+// Writes node x (Karras id) at its pre-order unit. Before x in pre-order
+// come its `depth` ancestors, and the subtrees hanging left of the root-to-x
+// path: they hold the a leaves left of x's range and a - r internal nodes,
+// r being the number of right turns on the path. Internal nodes take 2
+// units, leaves 3, so x starts at unit 2 (depth + a - r) + 3 a and its
+// subtree of m leaves spans 5 m - 2 units.
+__global__ void layout_kernel(int n, const int* right, const int* parent, const int* first, const int* count,
+                              const float* box, const uint64_t* keys, const float* tris, const uint2* pos,
+                              float4* rec) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= 2 * n - 1) return;
-    int esc = -1;
+    int depth = 0, turns = 0;
     int y = x;
-    for (int depth = 0; y > 0 && depth < n; ++depth) {
+    for (int steps = 0; steps < n; ++steps) {
         int p = parent[y];
         if (p < 0) break;
-        if (left[p] == y) {
-            esc = right[p];
-            break;
-        }
+        ++depth;
+        if (right[p] == y) ++turns;
         y = p;
     }
     if (x < n - 1) {
+        int a = first[x];
+        int u = 2 * (depth + a - turns) + 3 * a;
+        int esc = u + 5 * count[x] - 2;
         const float* b = box + (size_t)x * 6;
-        nodes[2 * x + 0] = make_float4(b[0], b[1], b[2], __int_as_float(left[x]));
-        nodes[2 * x + 1] = make_float4(b[3], b[4], b[5], __int_as_float(esc));
+        rec[u] = make_float4(b[0], b[1], b[2], __int_as_float(esc));
+        rec[u + 1] = make_float4(b[3], b[4], b[5], 0.0f);
     } else {
         int k = x - (n - 1);
-        leaves[3 * k].w = __int_as_float(esc);
+        int u = 2 * (depth + k - turns) + 3 * k;
+        uint32_t prim = (uint32_t)(keys[k] & 0xFFFFFFFFull);
+        const float* v = tris + (size_t)prim * 9;
+        uint32_t light = pos[(size_t)prim * 3].y & 0x80000000u;
+        rec[u] = make_float4(v[0], v[1], v[2], __int_as_float(-1));
+        rec[u + 1] = make_float4(v[3], v[4], v[5], __uint_as_float(prim));
+        rec[u + 2] = make_float4(v[6], v[7], v[8], __uint_as_float(light));
     }
 }
 
@@ -239,14 +253,16 @@ hipError_t build_lbvh(const uint2* d_pos, uint64_t tri_count, const float deq_fa
     float *tris = nullptr, *cent = nullptr, *partial = nullptr, *bounds = nullptr, *box = nullptr;
     uint64_t *keys = nullptr, *keys_sorted = nullptr;
     int *left = nullptr, *right = nullptr, *parent = nullptr;
+    int *first = nullptr, *count = nullptr;
     uint32_t* arrivals = nullptr;
     void* temp = nullptr;
     size_t temp_bytes = 0;
     const uint32_t tb = 256;
     const uint32_t tri_blocks = (uint32_t)((tri_count + tb - 1) / tb);
     const uint32_t red_blocks = 1024;
-    float4 *nodes = nullptr, *leaves = nullptr;
-    if (n <= 0) return hipErrorInvalidValue;
+    float4* rec = nullptr;
+    const uint64_t units = 5 * tri_count - 2;
+    if (n <= 0 || tri_count > kMaxBvhTriangles) return hipErrorInvalidValue;
     HIP_TRY(hipMalloc(&tris, sizeof(float) * 9 * tri_count));
     HIP_TRY(hipMalloc(&cent, sizeof(float) * 3 * tri_count));
     HIP_TRY(hipMalloc(&partial, sizeof(float) * 6 * red_blocks));
@@ -258,8 +274,11 @@ hipError_t build_lbvh(const uint2* d_pos, uint64_t tri_count, const float deq_fa
     HIP_TRY(hipMalloc(&right, sizeof(int) * tri_count));
     HIP_TRY(hipMalloc(&parent, sizeof(int) * (2 * tri_count - 1)));
     HIP_TRY(hipMalloc(&arrivals, sizeof(uint32_t) * tri_count));
-    HIP_TRY(hipMalloc(&nodes, sizeof(float4) * 2 * (tri_count > 1 ? tri_count - 1 : 1)));
-    HIP_TRY(hipMalloc(&leaves, sizeof(float4) * 3 * tri_count));
+    HIP_TRY(hipMalloc(&first, sizeof(int) * tri_count));
+    HIP_TRY(hipMalloc(&count, sizeof(int) * tri_count));
+    HIP_TRY(hipMalloc(&rec, sizeof(float4) * (units + 2)));
+    // the two padding units after the last record read as zeros
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(rec + units), 8, 0u);
     hipLaunchKernelGGL(dequant_kernel, dim3(tri_blocks), dim3(tb), 0, s, d_pos, tri_count,
                        make_float3(deq_factor[0], deq_factor[1], deq_factor[2]),
                        make_float3(deq_summand[0], deq_summand[1], deq_summand[2]), tris, cent);
@@ -273,17 +292,19 @@ hipError_t build_lbvh(const uint2* d_pos, uint64_t tri_count, const float deq_fa
     hipLaunchKernelGGL(fill_u32, dim3((uint32_t)((2 * tri_count - 1 + tb - 1) / tb)), dim3(tb), 0, s,
                        reinterpret_cast<uint32_t*>(parent), 2 * tri_count - 1, 0xFFFFFFFFu);
     hipLaunchKernelGGL(fill_u32, dim3(tri_blocks), dim3(tb), 0, s, arrivals, tri_count, 0u);
-    if (n > 1) hipLaunchKernelGGL(karras_kernel, dim3(tri_blocks), dim3(tb), 0, s, keys_sorted, n, left, right, parent);
-    hipLaunchKernelGGL(leaf_kernel, dim3(tri_blocks), dim3(tb), 0, s, keys_sorted, tris, n, box, leaves, d_pos);
+    if (n > 1)
+        hipLaunchKernelGGL(karras_kernel, dim3(tri_blocks), dim3(tb), 0, s, keys_sorted, n, left, right, parent, first,
+                           count);
+    hipLaunchKernelGGL(leaf_kernel, dim3(tri_blocks), dim3(tb), 0, s, keys_sorted, tris, n, box);
     if (n > 1) hipLaunchKernelGGL(refit_kernel, dim3(tri_blocks), dim3(tb), 0, s, n, left, right, parent, box, arrivals);
-    hipLaunchKernelGGL(escape_pack_kernel, dim3((uint32_t)((2 * tri_count - 1 + tb - 1) / tb)), dim3(tb), 0, s, n,
-                       left, right, parent, box, nodes, leaves);
+    hipLaunchKernelGGL(layout_kernel, dim3((uint32_t)((2 * tri_count - 1 + tb - 1) / tb)), dim3(tb), 0, s, n, right,
+                       parent, first, count, box, keys_sorted, tris, d_pos, rec);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
-    out->nodes = nodes;
-    out->leaves = leaves;
+    out->rec = rec;
+    out->units = (uint32_t)units;
     out->leaf_count = (uint32_t)n;
-    nodes = leaves = nullptr;
+    rec = nullptr;
 done:
     hipFree(tris);
     hipFree(cent);
@@ -296,16 +317,17 @@ done:
     hipFree(right);
     hipFree(parent);
     hipFree(arrivals);
+    hipFree(first);
+    hipFree(count);
     hipFree(temp);
-    hipFree(nodes);
-    hipFree(leaves);
+    hipFree(rec);
     return err;
 }
 
 void free_lbvh(BvhBuild* b) {
-    if (b->nodes) hipFree(b->nodes);
-    if (b->leaves) hipFree(b->leaves);
-    b->nodes = b->leaves = nullptr;
+    if (b->rec) hipFree(b->rec);
+    b->rec = nullptr;
+    b->units = 0;
     b->leaf_count = 0;
 }
 
@@ -345,21 +367,22 @@ __device__ __forceinline__ bool ray_tri(float3 o, float3 d, float4 a, float4 b, 
     return true;
 }
 
+// Every step moves forward in pre-order (by 2, by 3 or to an escape unit
+// past the node's subtree), so `units` steps bound the walk; reads of a
+// record at unit x < units touch x .. x + 2, inside the padded array.
 __device__ __forceinline__ bool any_hit(const BvhView& B, float3 o, float3 d, float tmin, float tmax) {
     float3 inv = make_float3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    int n = (int)B.leaf_count;
+    const int end = (int)B.units;
     int x = 0;
-    // pre-order visits every node at most once: 2n - 1 steps bound the walk
-    for (int steps = 0; x >= 0 && x < 2 * n - 1 && steps < 2 * n; ++steps) {
-        if (x >= n - 1) {
-            int k = x - (n - 1);
-            float4 a = B.leaves[3 * k], b = B.leaves[3 * k + 1], c = B.leaves[3 * k + 2];
+    for (int steps = 0; x < end && steps < end; ++steps) {
+        float4 a = B.rec[x], b = B.rec[x + 1], c = B.rec[x + 2];
+        int w = __float_as_int(a.w);
+        if (w < 0) {
             float t;
             if (ray_tri(o, d, a, b, c, tmin, tmax, t)) return true;
-            x = __float_as_int(a.w);
+            x += 3;
         } else {
-            float4 lo = B.nodes[2 * x], hi = B.nodes[2 * x + 1];
-            x = ray_box(o, inv, lo, hi, tmin, tmax) ? __float_as_int(lo.w) : __float_as_int(hi.w);
+            x = ray_box(o, inv, a, b, tmin, tmax) ? x + 2 : w;
         }
     }
     return false;
@@ -367,22 +390,21 @@ __device__ __forceinline__ bool any_hit(const BvhView& B, float3 o, float3 d, fl
 
 __device__ __forceinline__ uint32_t closest_hit(const BvhView& B, float3 o, float3 d, float tmin, float tmax) {
     float3 inv = make_float3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    int n = (int)B.leaf_count;
+    const int end = (int)B.units;
     int x = 0;
     uint32_t best = 0xFFFFFFFFu;
-    for (int steps = 0; x >= 0 && x < 2 * n - 1 && steps < 2 * n; ++steps) {
-        if (x >= n - 1) {
-            int k = x - (n - 1);
-            float4 a = B.leaves[3 * k], b = B.leaves[3 * k + 1], c = B.leaves[3 * k + 2];
+    for (int steps = 0; x < end && steps < end; ++steps) {
+        float4 a = B.rec[x], b = B.rec[x + 1], c = B.rec[x + 2];
+        int w = __float_as_int(a.w);
+        if (w < 0) {
             float t;
             if (ray_tri(o, d, a, b, c, tmin, tmax, t)) {
                 tmax = t;
                 best = __float_as_uint(b.w) | __float_as_uint(c.w);
             }
-            x = __float_as_int(a.w);
+            x += 3;
         } else {
-            float4 lo = B.nodes[2 * x], hi = B.nodes[2 * x + 1];
-            x = ray_box(o, inv, lo, hi, tmin, tmax) ? __float_as_int(lo.w) : __float_as_int(hi.w);
+            x = ray_box(o, inv, a, b, tmin, tmax) ? x + 2 : w;
         }
     }
     return best;
