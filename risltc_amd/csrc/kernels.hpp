@@ -77,13 +77,22 @@ struct ShadeArgs {
     uint32_t cluster_count;
 };
 
+// The shadow-ray list is cut into kTraceQueues equal slices, one per XCD:
+// work-group b runs on XCD b mod 8, starts on slice b mod 8 and takes rays
+// from that slice's counter (queues[kQueueStride * q]) until it is empty,
+// then moves on to the next slice. The counters start at zero.
+constexpr uint32_t kTraceQueues = 8;
+constexpr uint32_t kQueueStride = 32;  // words: one 128-byte line per counter
+
 struct TraceArgs {
     BvhView bvh;
     const ShadowRay* rays;
     const uint32_t* ray_slots;
     const uint32_t* ray_count;
     uint32_t max_rays;
-    uint32_t* hits;  // one word per slot
+    uint32_t* hits;     // one word per slot
+    uint32_t* queues;   // kTraceQueues * kQueueStride words, zeroed
+    uint32_t refill;    // a wave fetches rays once this many lanes are idle
 };
 
 struct ResolveArgs {

@@ -410,15 +410,84 @@ __device__ __forceinline__ uint32_t closest_hit(const BvhView& B, float3 o, floa
     return best;
 }
 
+// Persistent any-hit traversal of the shadow-ray list. Each lane holds one
+// ray; a lane whose ray is done takes a new one as soon as `refill` lanes of
+// its wave are idle (one atomic per wave for all of them), so long rays do
+// not hold a wave's other 63 lanes idle. The walk per ray is any_hit's.
+// Every wave leaves the loop: each refill either hands out rays, each of
+// which ends within `units` steps, or moves to the next slice, and after
+// kTraceQueues slices there is nothing left to fetch.
 __global__ void __launch_bounds__(256) trace_kernel(TraceArgs A) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t count = *A.ray_count;
     if (count > A.max_rays) count = A.max_rays;
-    if (i >= count) return;
-    ShadowRay r = A.rays[i];
-    bool hit = any_hit(A.bvh, make_float3(r.org.x, r.org.y, r.org.z), make_float3(r.dir.x, r.dir.y, r.dir.z), r.org.w,
-                       r.dir.w);
-    A.hits[A.ray_slots[i]] = hit ? 1u : 0u;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const float4* rec = A.bvh.rec;
+    const int end = (int)A.bvh.units;
+    uint32_t q = blockIdx.x % kTraceQueues;
+    uint32_t tried = 0;
+    bool active = false;
+    float3 o = make_float3(0.0f, 0.0f, 0.0f), d = o, inv = o;
+    float tmin = 0.0f, tmax = 0.0f;
+    int x = 0, steps = 0;
+    uint32_t slot = 0;
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (tried < kTraceQueues && (n_idle >= A.refill || n_idle == 64u)) {
+            uint32_t q0 = (uint32_t)(((uint64_t)count * q) / kTraceQueues);
+            uint32_t q1 = (uint32_t)(((uint64_t)count * (q + 1u)) / kTraceQueues);
+            uint32_t len = q1 - q0;
+            uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&A.queues[q * kQueueStride], n_idle);
+            base = (uint32_t)__shfl((int)base, (int)leader);
+            if (!active) {
+                uint32_t j = base + (uint32_t)__popcll(idle & below);
+                if (j < len) {
+                    uint32_t i = q0 + j;
+                    ShadowRay r = A.rays[i];
+                    slot = A.ray_slots[i];
+                    o = make_float3(r.org.x, r.org.y, r.org.z);
+                    d = make_float3(r.dir.x, r.dir.y, r.dir.z);
+                    inv = make_float3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+                    tmin = r.org.w;
+                    tmax = r.dir.w;
+                    x = 0;
+                    steps = 0;
+                    active = true;
+                }
+            }
+            if (base + n_idle >= len) {
+                q = (q + 1u) % kTraceQueues;
+                ++tried;
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (tried >= kTraceQueues) break;
+            continue;
+        }
+#pragma unroll 1
+        for (int it = 0; it < 8; ++it) {
+            if (active) {
+                float4 a = rec[x], b = rec[x + 1], c = rec[x + 2];
+                int w = __float_as_int(a.w);
+                bool hit = false;
+                if (w < 0) {
+                    float t;
+                    hit = ray_tri(o, d, a, b, c, tmin, tmax, t);
+                    x += 3;
+                } else {
+                    x = ray_box(o, inv, a, b, tmin, tmax) ? x + 2 : w;
+                }
+                ++steps;
+                if (hit || x >= end || steps >= end) {
+                    A.hits[slot] = hit ? 1u : 0u;
+                    active = false;
+                }
+            }
+        }
+    }
 }
 
 __global__ void __launch_bounds__(256) trace_rays_kernel(BvhView B, const float* rays, uint32_t n, uint32_t* hits) {
@@ -446,8 +515,17 @@ __global__ void __launch_bounds__(256) primary_kernel(BvhView B, risltc_frame_co
     out[y * W + x] = closest_hit(B, o, d, 0.0f, __builtin_inff());
 }
 
+// One resident grid: as many 256-lane work-groups as fit on the device at
+// once, a multiple of the XCD count.
 hipError_t launch_trace(const TraceArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(trace_kernel, dim3((a.max_rays + 255u) / 256u), dim3(256), 0, s, a);
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_kernel, 256, 0);
+    if (e != hipSuccess) return e;
+    uint32_t blocks = (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)(per_cu > 0 ? per_cu : 1);
+    blocks = (blocks + kTraceQueues - 1u) / kTraceQueues * kTraceQueues;
+    hipLaunchKernelGGL(trace_kernel, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
