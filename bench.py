@@ -54,11 +54,12 @@ def main():
 
     import torch
     if args.copy_calibration:
-        n = 1 << 28  # 2^28 floats = 1 GiB read + 1 GiB written per copy
+        # an elementwise kernel (not a blit) that reads 1 GiB and writes 1 GiB
+        n = 1 << 28
         src = torch.ones(n, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}))
         return

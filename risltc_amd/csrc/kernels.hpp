@@ -37,6 +37,8 @@ struct SceneView {
     uint32_t material_count;
     uint64_t triangle_count;
     const float4* lights;        // kLightFloat4 per light
+    const float4* spheres;       // per light: bounding sphere centre.xyz | radius
+    float sphere_bound;          // max over lights of |centre| + radius
     uint32_t light_count;
     const uint16_t* ltc0;        // RGBA16 texels
     const uint16_t* ltc1;        // RG16 texels
@@ -57,8 +59,12 @@ struct BvhView {
 // largest triangle count whose pre-order units fit a signed 32-bit index
 constexpr uint64_t kMaxBvhTriangles = 400000000ull;
 
+// light bounding spheres are staged in LDS up to this many lights
+constexpr uint32_t kLdsSphereMax = 2048;
+
 struct ShadeArgs {
     risltc_frame_constants_t c;
+    uint32_t lds_spheres;      // 1: the kernel stages the spheres in LDS
     uint32_t spp, M, mis, light_sampling, fast_atan, shadow_rays;
     uint32_t tile_w, tile_h;  // pixel tile of one work-group
     SceneView scene;

@@ -33,10 +33,8 @@ struct Pixel {
     float lo;
     f3 albedo, f0;
     float rough;
-    float w2s[12];       // world to shading, row-major 3x4
-    float w2c[12];       // shading_to_cosine * world_to_shading
-    float w2c_flip[12];  // shading_to_cosine * (world_to_shading with row 1 negated)
-    float s2c[9], c2s[9];
+    float w2s[12];  // world to shading, row-major 3x4
+    float s2c[9];   // shading to cosine (the LTC matrix; five non-zero entries)
     float ltc_albedo, ltc_det;
 };
 
@@ -89,6 +87,27 @@ __device__ __forceinline__ void mul33_34(const float* s, const float* w, float* 
             r[4 * i + k] = fma_(s[3 * i + 2], w[8 + k], fma_(s[3 * i + 1], w[4 + k], s[3 * i + 0] * w[k]));
 }
 
+// shading_to_cosine * world_to_shading, with row 1 of world_to_shading
+// negated for a shading point behind the light (the mirrored polygon).
+// Derived on demand rather than kept per pixel: the candidate loop of the
+// fast build never needs it, and 24 fewer live registers there.
+__device__ __forceinline__ void world_to_cosine(const Pixel& px, bool flip, float* m) {
+    float w[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) w[k] = (flip && k >= 4 && k < 8) ? -px.w2s[k] : px.w2s[k];
+    mul33_34(px.s2c, w, m);
+}
+
+// inverse of shading_to_cosine (get_ltc_coefficients, ltc_utility.glsl:80-87)
+__device__ __forceinline__ void cosine_to_shading(const Pixel& px, float* c2s) {
+    const float d0x = px.s2c[0], d0w = px.s2c[2], d0z = px.s2c[4], d0y = -px.s2c[6], d1x = px.s2c[8];
+    float det2 = d0x * d1x + d0y * d0w;
+    float inv = 1.0f / det2;
+    c2s[0] = d1x * inv; c2s[1] = 0.0f; c2s[2] = -d0w * inv;
+    c2s[3] = 0.0f; c2s[4] = 1.0f / d0z; c2s[5] = 0.0f;
+    c2s[6] = d0y * inv; c2s[7] = 0.0f; c2s[8] = d0x * inv;
+}
+
 // iz0 (polygon_clipping.glsl:19-25)
 template <bool X>
 __device__ __forceinline__ f3 iz0(f3 a, f3 b) {
@@ -104,7 +123,10 @@ __device__ __forceinline__ float edge_term(f3 a, f3 b) {
     float A = 0.8543985f + (0.4965155f + 0.0145206f * y) * y;
     float B = 3.4175940f + (4.1616724f + y) * y;
     float v = div_<X>(A, B);
-    float ts = (x > 0.0f) ? v : 0.5f * rsq_<X>(fmax_(1.0f - x * x, 1e-7f)) - v;
+    float ts = v;
+    // edges longer than 90 degrees are rare: keep the reciprocal square
+    // root off the common path
+    if (!(x > 0.0f)) ts = 0.5f * rsq_<X>(fmax_(1.0f - x * x, 1e-7f)) - v;
     return (a.x * b.y - a.y * b.x) * ts;
 }
 
@@ -760,10 +782,6 @@ __device__ __forceinline__ void setup_pixel(const ShadeArgs& A, uint32_t prim, f
     px.ltc_albedo = d1y;
     float det2 = d0x * d1x + d0y * d0w;
     px.ltc_det = d0z * det2;
-    float inv = 1.0f / det2;
-    px.c2s[0] = d1x * inv; px.c2s[1] = 0.0f; px.c2s[2] = -d0w * inv;
-    px.c2s[3] = 0.0f; px.c2s[4] = 1.0f / d0z; px.c2s[5] = 0.0f;
-    px.c2s[6] = d0y * inv; px.c2s[7] = 0.0f; px.c2s[8] = d0x * inv;
     f3 xa = normalize3<true>(mk3(fma_(-ndo, px.n.x, px.o.x), fma_(-ndo, px.n.y, px.o.y), fma_(-ndo, px.n.z, px.o.z)));
     f3 ya = cross3(px.n, xa);
     f3 rows[3] = {xa, ya, px.n};
@@ -774,11 +792,6 @@ __device__ __forceinline__ void setup_pixel(const ShadeArgs& A, uint32_t prim, f
         px.w2s[4 * i + 2] = rows[i].z;
         px.w2s[4 * i + 3] = fma_(-rows[i].z, px.pos.z, fma_(-rows[i].y, px.pos.y, -rows[i].x * px.pos.x));
     }
-    mul33_34(px.s2c, px.w2s, px.w2c);
-    float flip[12];
-#pragma unroll
-    for (int k2 = 0; k2 < 12; ++k2) flip[k2] = (k2 >= 4 && k2 < 8) ? -px.w2s[k2] : px.w2s[k2];
-    mul33_34(px.s2c, flip, px.w2c_flip);
 }
 
 __device__ __forceinline__ uint32_t light_index_from(float u, uint32_t n) {
@@ -808,19 +821,32 @@ __device__ __forceinline__ f3 eval_ltc_cp(const Pixel& px, const Light& L, uint3
     f3 diffuse = mk3(0.0f, 0.0f, 0.0f);
     if (vcd > 0) diffuse = mul3(sclf3(ltc_d, px.albedo), L.rad);
     f3 result = add3(mk3(0.0f, 0.0f, 0.0f), diffuse);
-    float m[12];
+    if constexpr (X) {
+        // the reference's composed world-to-cosine matrix, as the oracle
+        float m[12];
+        world_to_cosine(px, flip, m);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) m[k] = flip ? px.w2c_flip[k] : px.w2c[k];
+        for (int i = 0; i < 8; ++i) {
+            f3 t = xform(m, L.v[3 * i], L.v[3 * i + 1], L.v[3 * i + 2]);
+            vx[i] = t.x;
+            vy[i] = t.y;
+            vz[i] = t.z;
+        }
+        vx[8] = vx[0];
+        vy[8] = vy[0];
+        vz[8] = vz[0];
+    } else {
+        // cosine space = shading_to_cosine * (the mirrored shading-space
+        // polygon above); the LTC matrix has only five non-zero entries
+        const float a = px.s2c[0], b = px.s2c[2], c = px.s2c[4], d = px.s2c[6], e = px.s2c[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        f3 t = xform(m, L.v[3 * i], L.v[3 * i + 1], L.v[3 * i + 2]);
-        vx[i] = t.x;
-        vy[i] = t.y;
-        vz[i] = t.z;
+        for (int i = 0; i < 9; ++i) {
+            float x = vx[i], z = vz[i];
+            vx[i] = fma_(a, x, b * z);
+            vy[i] = c * vy[i];
+            vz[i] = fma_(d, x, e * z);
+        }
     }
-    vx[8] = vx[0];
-    vy[8] = vy[0];
-    vz[8] = vz[0];
     float ltc_s = polygon_ltc<X>(vx, vy, vz, L.n, vcs);
     f3 ggx = mk3(0.0f, 0.0f, 0.0f);
     if (vcs > 0) ggx = sclf3(ltc_s * px.ltc_albedo, L.rad);
@@ -877,8 +903,7 @@ __device__ __forceinline__ void eval_peters(const ShadeArgs& A, const Pixel& px,
     float psad = P.psa;
     // transform for the specular polygon now (the diffuse vertices are dead)
     float m[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) m[k] = flip ? px.w2c_flip[k] : px.w2c[k];
+    world_to_cosine(px, flip, m);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         f3 t = xform(m, L.v[3 * i], L.v[3 * i + 1], L.v[3 * i + 2]);
@@ -916,7 +941,9 @@ __device__ __forceinline__ void eval_peters(const ShadeArgs& A, const Pixel& px,
             float u0 = noise_next(seed);
             float u1 = noise_next(seed);
             dirs[1] = sample_psa<X>(P, mk2(u0, u1), fa);
-            dirs[1] = normalize3<X>(mul33(px.c2s, dirs[1]));
+            float c2s[9];
+            cosine_to_shading(px, c2s);
+            dirs[1] = normalize3<X>(mul33(c2s, dirs[1]));
         }
     }
     F.psas = psas;
@@ -981,9 +1008,79 @@ __device__ __forceinline__ void eval_peters(const ShadeArgs& A, const Pixel& px,
     }
 }
 
+// ---- candidate culling ----------------------------------------------------------
+// A candidate whose polygon lies entirely below the shading point's horizon
+// (every vertex z <= 0 in shading space) and entirely below the clipping
+// plane of the LTC cosine space has clip counts 0 | 0 and p_hat = 0, so it
+// changes nothing in the reservoir (w = 0 is never taken and adds exactly 0
+// to w_sum). Such candidates are recognised from the light's bounding
+// sphere: centre c, radius r, plane normal m and offset o give
+// m.v + o <= m.c + o + |m| r for every vertex v. The margin is 1e-4 of the
+// magnitudes involved, far above the few-ulp error of the vertex transform,
+// so a culled candidate is one whose clip masks are zero in both builds.
+struct Cull {
+    float dx, dy, dz, dw, dn, dm;  // shading-space z plane, |normal|, margin
+    float cx, cy, cz, cw, cn, cm;  // cosine-space z plane
+};
+
+__device__ __forceinline__ Cull make_cull(const Pixel& px, float bound) {
+    Cull k;
+    float plen = __builtin_sqrtf(dot3(px.pos, px.pos));
+    k.dx = px.w2s[8];
+    k.dy = px.w2s[9];
+    k.dz = px.w2s[10];
+    k.dw = px.w2s[11];
+    k.dn = __builtin_sqrtf(fma_(k.dz, k.dz, fma_(k.dy, k.dy, k.dx * k.dx)));
+    k.dm = 1.0e-4f * k.dn * (bound + plen);
+    // row 2 of shading_to_cosine is (s2c[6], 0, s2c[8])
+    const float d = px.s2c[6], e = px.s2c[8];
+    k.cx = fma_(d, px.w2s[0], e * px.w2s[8]);
+    k.cy = fma_(d, px.w2s[1], e * px.w2s[9]);
+    k.cz = fma_(d, px.w2s[2], e * px.w2s[10]);
+    k.cw = fma_(d, px.w2s[3], e * px.w2s[11]);
+    k.cn = __builtin_sqrtf(fma_(k.cz, k.cz, fma_(k.cy, k.cy, k.cx * k.cx)));
+    k.cm = 1.0e-4f * k.cn * (bound + plen);
+    return k;
+}
+
+// NaN anywhere makes both comparisons false: never culled
+__device__ __forceinline__ bool culled(const Cull& k, float4 s) {
+    float hd = fma_(k.dn, s.w, fma_(k.dz, s.z, fma_(k.dy, s.y, fma_(k.dx, s.x, k.dw))));
+    float hc = fma_(k.cn, s.w, fma_(k.cz, s.z, fma_(k.cy, s.y, fma_(k.cx, s.x, k.cw))));
+    return hd < -k.dm && hc < -k.cm;
+}
+
+// The noise LCG advanced by d steps (d < 128) in 7 affine jumps: stepping
+// twice with (a, c) is one step with (a * a, a * c + c).
+struct LcgStep {
+    uint32_t a, c;
+};
+constexpr LcgStep lcg_pow2(int b) {
+    LcgStep r{1664525u, 1013904223u};
+    for (int i = 0; i < b; ++i) r = LcgStep{r.a * r.a, r.a * r.c + r.c};
+    return r;
+}
+__device__ __forceinline__ uint32_t lcg_jump(uint32_t s, uint32_t d) {
+    constexpr LcgStep j0 = lcg_pow2(0), j1 = lcg_pow2(1), j2 = lcg_pow2(2), j3 = lcg_pow2(3),
+                      j4 = lcg_pow2(4), j5 = lcg_pow2(5), j6 = lcg_pow2(6);
+    if (d & 1u) s = j0.a * s + j0.c;
+    if (d & 2u) s = j1.a * s + j1.c;
+    if (d & 4u) s = j2.a * s + j2.c;
+    if (d & 8u) s = j3.a * s + j3.c;
+    if (d & 16u) s = j4.a * s + j4.c;
+    if (d & 32u) s = j5.a * s + j5.c;
+    if (d & 64u) s = j6.a * s + j6.c;
+    return s;
+}
+
 // ---- the kernel ----------------------------------------------------------------
 template <bool X, bool DEBUG>
 __global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
+    extern __shared__ float4 sh_spheres[];
+    if (A.lds_spheres) {
+        for (uint32_t k = threadIdx.x; k < A.scene.light_count; k += blockDim.x) sh_spheres[k] = A.scene.spheres[k];
+        __syncthreads();
+    }
     const uint32_t W = A.c.viewport_size[0], H = A.c.viewport_size[1];
     const uint32_t tiles_x = (W + 15u) / 16u;
     const uint32_t tile = blockIdx.x;
@@ -999,6 +1096,7 @@ __global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
     bool shade = inside && (prim >> 31) == 0u && (uint64_t)prim < A.scene.triangle_count;
 
     Pixel px;
+    Cull cull;
     uint32_t seed = 0;
     if (shade) {
         const float(*P)[4] = A.c.pixel_to_ray_direction_world_space;
@@ -1007,6 +1105,7 @@ __global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
                      fma_(P[1][2], 1.0f, fma_(P[1][1], fy, P[1][0] * fx)),
                      fma_(P[2][2], 1.0f, fma_(P[2][1], fy, P[2][0] * fx)));
         setup_pixel(A, prim, ray, px);
+        cull = make_cull(px, A.scene.sphere_bound);
         seed = murmur_finalize(murmur_mix(murmur_mix(0u, x + y * W), A.c.noise_random_numbers[0]));
     }
     const uint32_t N = A.scene.light_count;
@@ -1020,27 +1119,62 @@ __global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
             if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
                 chosen = (int32_t)light_index_from(noise_next(seed), N);
             } else {
-                for (uint32_t i = 0; i < M; ++i) {
-                    uint32_t idx = light_index_from(noise_next(seed), N);
-                    load_light(A.scene.lights, idx, L);
-                    uint32_t cc;
-                    f3 col = eval_ltc_cp<X>(px, L, cc);
-                    float p_hat = sqrt_<X>(dot3(col, col));
-                    float w = p_hat / pN;
-                    float r = noise_next(seed);
-                    w_sum += w;
-                    if (w > 0.0f && r < w / w_sum) {
-                        chosen = (int32_t)idx;
-                        sample_value = p_hat;
+                // Candidates in chunks of 32: first every candidate's light
+                // index and cull test (cheap, wave-uniform), then the full
+                // target of the survivors in their original order, the noise
+                // state jumped to each survivor. Culled candidates have
+                // w = 0 and leave the reservoir exactly as it was.
+                for (uint32_t base = 0; base < M; base += 32u) {
+                    const uint32_t cnt = (M - base < 32u) ? M - base : 32u;
+                    const uint32_t seed_chunk = seed;
+                    uint32_t live = 0;
+                    for (uint32_t t = 0; t < cnt; ++t) {
+                        uint32_t idx = light_index_from(noise_next(seed), N);
+                        (void)noise_next(seed);
+                        float4 sp = A.lds_spheres ? sh_spheres[idx] : A.scene.spheres[idx];
+                        bool cut = culled(cull, sp);
+                        live |= cut ? 0u : (1u << t);
+                        if constexpr (DEBUG) {
+                            if (A.debug_cands && cut) {
+                                uint32_t* cr = A.debug_cands + (((size_t)pixel * spp + j) * M + base + t) *
+                                                                   RISLTC_CANDIDATE_RECORD_WORDS;
+                                cr[0] = idx;
+                                cr[1] = 0u;
+                                cr[2] = 0u;
+                                cr[3] = 0u;
+                            }
+                        }
                     }
-                    if constexpr (DEBUG) {
-                        if (A.debug_cands) {
-                            uint32_t* cr = A.debug_cands +
-                                           (((size_t)pixel * spp + j) * M + i) * RISLTC_CANDIDATE_RECORD_WORDS;
-                            cr[0] = idx;
-                            cr[1] = cc;
-                            cr[2] = __float_as_uint(p_hat);
-                            cr[3] = __float_as_uint(w);
+                    uint32_t s = seed_chunk, at = 0;
+                    while (live) {
+                        const uint32_t t = (uint32_t)__builtin_ctz(live);
+                        live &= live - 1u;
+                        s = lcg_jump(s, 2u * (t - at));
+                        at = t + 1u;
+                        uint32_t idx = light_index_from(noise_next(s), N);
+                        load_light(A.scene.lights, idx, L);
+                        uint32_t cc;
+                        f3 col = eval_ltc_cp<X>(px, L, cc);
+                        float p_hat = sqrt_<X>(dot3(col, col));
+                        float w = div_<X>(p_hat, pN);
+                        float r = noise_next(s);
+                        w_sum += w;
+                        // exact: the reference's r < w / w_sum; fast: the
+                        // same test without the division (w_sum >= w > 0)
+                        bool take = X ? (r < w / w_sum) : (r * w_sum < w);
+                        if (w > 0.0f && take) {
+                            chosen = (int32_t)idx;
+                            sample_value = p_hat;
+                        }
+                        if constexpr (DEBUG) {
+                            if (A.debug_cands) {
+                                uint32_t* cr = A.debug_cands + (((size_t)pixel * spp + j) * M + base + t) *
+                                                                   RISLTC_CANDIDATE_RECORD_WORDS;
+                                cr[0] = idx;
+                                cr[1] = cc;
+                                cr[2] = __float_as_uint(p_hat);
+                                cr[3] = __float_as_uint(w);
+                            }
                         }
                     }
                 }
@@ -1200,12 +1334,14 @@ hipError_t launch_shade(const ShadeArgs& a, bool exact, hipStream_t s) {
     uint32_t W = a.c.viewport_size[0], H = a.c.viewport_size[1];
     dim3 grid(((W + 15u) / 16u) * ((H + 15u) / 16u)), block(256);
     bool dbg = a.debug_samples != nullptr || a.debug_cands != nullptr;
+    size_t lds = a.lds_spheres ? (size_t)a.scene.light_count * sizeof(float4) : 0;
+    if (a.lds_spheres && a.scene.light_count > kLdsSphereMax) return hipErrorInvalidValue;
     if (exact) {
-        if (dbg) hipLaunchKernelGGL((shade_kernel<true, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((shade_kernel<true, false>), grid, block, 0, s, a);
+        if (dbg) hipLaunchKernelGGL((shade_kernel<true, true>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((shade_kernel<true, false>), grid, block, lds, s, a);
     } else {
-        if (dbg) hipLaunchKernelGGL((shade_kernel<false, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((shade_kernel<false, false>), grid, block, 0, s, a);
+        if (dbg) hipLaunchKernelGGL((shade_kernel<false, true>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((shade_kernel<false, false>), grid, block, lds, s, a);
     }
     return hipGetLastError();
 }
