@@ -257,10 +257,29 @@ int risltc_set_ltc_table(risltc_context_t* context, const uint16_t* rgba16, cons
                          uint32_t roughness_count, uint32_t inclination_count,
                          uint32_t fresnel_count);
 
+/* Shadow-ray hierarchy builders. Both lay the tree out in depth-first
+ * pre-order for the same stackless traversal; hit and miss flags are the
+ * same for any tree (the oracle restates both builders bit for bit).
+ *   RISLTC_BVH_SAH  -- top-down binned surface area heuristic (default);
+ *   RISLTC_BVH_LBVH -- 30-bit Morton codes of triangle centroids, sorted,
+ *                      Karras hierarchy. */
+#define RISLTC_BVH_SAH 0u
+#define RISLTC_BVH_LBVH 1u
+/* Selects the builder used by the next risltc_set_scene. No reference
+ * counterpart: the reference's acceleration structure is built by the
+ * Vulkan driver (src/scene.c:142-406). */
+int risltc_set_bvh_builder(risltc_context_t* context, uint32_t builder);
+/* Copies the current hierarchy (16-byte units in pre-order, see
+ * risltc_set_scene) to host memory: at most `capacity` units; *units
+ * receives the total. Test and tooling entry point. */
+int risltc_bvh_records(risltc_context_t* context, float* out, uint64_t capacity, uint32_t* units);
+
 /* Mesh in the *.vks layout (src/scene.h:47-70) plus a constant material per
- * material index. Uploads it and builds the LBVH used for shadow rays
- * (30-bit Morton codes of triangle centroids, sorted, Karras hierarchy,
- * nodes stored in depth-first pre-order with escape links). Light triangles
+ * material index. Uploads it and builds the hierarchy used for shadow rays
+ * with the selected builder (risltc_set_bvh_builder), its nodes stored in
+ * depth-first pre-order with escape links: internal node = 2 units
+ * (min.xyz | escape unit, max.xyz | 0), leaf = 3 units (v0.xyz | -1,
+ * v1.xyz | triangle id, v2.xyz | light flag bit 31). Light triangles
  * (bit 31 of the second position word) are part of the hierarchy, as they
  * are in the reference BLAS (src/scene.c:176-187). Host pointers. At most
  * 400,000,000 triangles (RISLTC_E_UNSUPPORTED beyond). */

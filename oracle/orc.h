@@ -92,6 +92,13 @@ typedef struct orc_bvh_s {
 } orc_bvh_t;
 /* Node ids: internal nodes 0 .. n-2, leaves n-1 .. 2n-2 (leaf k = n-1+k). */
 int orc_bvh_build(orc_bvh_t* bvh, const float* triangles, uint32_t triangle_count);
+/* binned-SAH tree (the product's default builder, risltc_amd/csrc/sah.hip):
+ * internal nodes numbered in pre-order, leaves left to right, no keys */
+int orc_bvh_build_sah(orc_bvh_t* bvh, const float* triangles, uint32_t triangle_count);
+/* the pre-order records of the device layout (5 n - 2 units of 4 floats):
+ * internal (min.xyz | escape unit, max.xyz | 0), leaf (v0 | -1, v1 | id,
+ * v2 | light flag from the *.vks positions) */
+void orc_bvh_records(const orc_bvh_t* bvh, const uint32_t* positions, float* out);
 void orc_bvh_free(orc_bvh_t* bvh);
 uint32_t orc_morton_part_1_by_2(uint32_t x);
 /* 1 = hit in [t_min, t_max] */

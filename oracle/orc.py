@@ -80,6 +80,8 @@ def lib():
             "orc_clip_rule": (C.c_int, [C.c_int, C.c_uint32, C.POINTER(C.c_int)]),
             "orc_clip_rotation": (C.c_int, [C.c_int, C.c_uint32]),
             "orc_bvh_build": (C.c_int, [C.POINTER(OrcBvh), f32p, C.c_uint32]),
+            "orc_bvh_build_sah": (C.c_int, [C.POINTER(OrcBvh), f32p, C.c_uint32]),
+            "orc_bvh_records": (None, [C.POINTER(OrcBvh), u32p, f32p]),
             "orc_bvh_free": (None, [C.POINTER(OrcBvh)]),
             "orc_morton_part_1_by_2": (C.c_uint32, [C.c_uint32]),
             "orc_bvh_any_hit": (C.c_int, [C.POINTER(OrcBvh), f32p, f32p, C.c_float, C.c_float, u32p]),
@@ -139,7 +141,7 @@ def ptr(a, t):
 class Oracle:
     """The oracle bound to one synthetic scene (risltc_amd.synth.Scene)."""
 
-    def __init__(self, scene, ltc_rgba16, ltc_rg16, bvh=True):
+    def __init__(self, scene, ltc_rgba16, ltc_rg16, bvh=True, builder=abi.BVH_SAH):
         self.L = lib()
         self.scene = scene
         m = scene.mesh
@@ -173,7 +175,8 @@ class Oracle:
         if bvh:
             self.triangles = self.dequantized_triangles()
             self.bvh = OrcBvh()
-            self.L.orc_bvh_build(C.byref(self.bvh), ptr(self.triangles, f32p), m.triangle_count)
+            build = self.L.orc_bvh_build_sah if builder == abi.BVH_SAH else self.L.orc_bvh_build
+            build(C.byref(self.bvh), ptr(self.triangles, f32p), m.triangle_count)
             s.bvh = C.pointer(self.bvh)
         self.cscene = s
 
@@ -183,6 +186,13 @@ class Oracle:
                 self.L.orc_bvh_free(C.byref(self.bvh))
         except Exception:
             pass
+
+    def bvh_records(self):
+        """Pre-order records of the oracle's tree, as the device lays it out."""
+        n = self.bvh.leaf_count
+        out = np.zeros((5 * n - 2, 4), dtype=np.float32)
+        self.L.orc_bvh_records(C.byref(self.bvh), ptr(self.positions, u32p), ptr(out, f32p))
+        return out
 
     def dequantized_triangles(self):
         m = self.scene.mesh

@@ -631,6 +631,288 @@ int orc_bvh_build(orc_bvh_t* b, const float* tris, uint32_t n) {
     return 0;
 }
 
+/* ---- binned-SAH builder (the product's default, risltc_amd/csrc/sah.hip) --
+ * Same decisions as the device builder: key-ordered min/max (so -0 < +0),
+ * centroid = box centre, 16 bins per axis of non-zero centroid extent,
+ * bin = min(15, (int)((c - lo) * (16 / ext))), cost n_l area_l + n_r area_r
+ * with area = dx dy + dy dz + dz dx, first minimum over (axis, bin), halving
+ * in index order when no axis has extent, stable partition. Internal nodes
+ * are numbered in pre-order, leaves left to right. parity: the device's
+ * pre-order records equal orc_bvh_records of this tree word for word
+ * (tests/test_gpu_parity.py). */
+#define ORC_SAH_BINS 16
+static int orc_key(float f) {
+    int32_t b;
+    memcpy(&b, &f, 4);
+    return b >= 0 ? b : (b ^ 0x7FFFFFFF);
+}
+static float orc_unkey(int k) {
+    int32_t b = k >= 0 ? k : (k ^ 0x7FFFFFFF);
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+#define ORC_KEY_POS_INF 0x7F800000
+#define ORC_KEY_NEG_INF ((int)(0xFF800000u ^ 0x7FFFFFFFu))
+static int imin(int a, int b) { return a < b ? a : b; }
+static int imax(int a, int b) { return a > b ? a : b; }
+static int sah_bin(float c, float lo, float scale) {
+    int b = (int)((c - lo) * scale);
+    return b < ORC_SAH_BINS - 1 ? b : ORC_SAH_BINS - 1;
+}
+static float sah_area(const int k[6]) {
+    float dx = orc_unkey(k[3]) - orc_unkey(k[0]), dy = orc_unkey(k[4]) - orc_unkey(k[1]);
+    float dz = orc_unkey(k[5]) - orc_unkey(k[2]);
+    return dx * dy + dy * dz + dz * dx;
+}
+
+typedef struct {
+    uint32_t first, count;
+    int32_t parent;
+    int right; /* 1 = right child of parent */
+} orc_sah_task_t;
+
+int orc_bvh_build_sah(orc_bvh_t* b, const float* tris, uint32_t n) {
+    memset(b, 0, sizeof(*b));
+    if (n == 0) return 1;
+    b->leaf_count = n;
+    uint32_t nodes = 2 * n - 1;
+    b->node_box = (float*)malloc(sizeof(float) * 6 * nodes);
+    b->node_left = (int32_t*)malloc(sizeof(int32_t) * nodes);
+    b->node_right = (int32_t*)malloc(sizeof(int32_t) * nodes);
+    b->node_escape = (int32_t*)malloc(sizeof(int32_t) * nodes);
+    b->node_parent = (int32_t*)malloc(sizeof(int32_t) * nodes);
+    b->leaf_tri = (float*)malloc(sizeof(float) * 9 * n);
+    b->leaf_prim = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    float* tbox = (float*)malloc(sizeof(float) * 6 * n);
+    float* cen = (float*)malloc(sizeof(float) * 3 * n);
+    uint32_t* idx = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t* tmp = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    orc_sah_task_t* stack = (orc_sah_task_t*)malloc(sizeof(orc_sah_task_t) * (n + 1));
+    for (uint32_t t = 0; t != n; ++t) {
+        const float* v = tris + 9 * (size_t)t;
+        for (int a = 0; a != 3; ++a) {
+            float lo = orc_unkey(imin(imin(orc_key(v[a]), orc_key(v[3 + a])), orc_key(v[6 + a])));
+            float hi = orc_unkey(imax(imax(orc_key(v[a]), orc_key(v[3 + a])), orc_key(v[6 + a])));
+            tbox[6 * (size_t)t + a] = lo;
+            tbox[6 * (size_t)t + 3 + a] = hi;
+            cen[3 * (size_t)t + a] = 0.5f * (lo + hi);
+        }
+        idx[t] = t;
+    }
+    int32_t next_internal = 0, next_leaf = 0;
+    int sp = 0;
+    stack[sp].first = 0;
+    stack[sp].count = n;
+    stack[sp].parent = -1;
+    stack[sp].right = 0;
+    ++sp;
+    while (sp) {
+        orc_sah_task_t tk = stack[--sp];
+        int32_t id;
+        if (tk.count == 1) {
+            uint32_t k = (uint32_t)next_leaf++;
+            id = (int32_t)(n - 1 + k);
+            uint32_t t = idx[tk.first];
+            b->leaf_prim[k] = t;
+            memcpy(b->leaf_tri + 9 * (size_t)k, tris + 9 * (size_t)t, sizeof(float) * 9);
+            memcpy(b->node_box + 6 * (size_t)id, tbox + 6 * (size_t)t, sizeof(float) * 6);
+            b->node_left[id] = -1;
+            b->node_right[id] = -1;
+        } else {
+            id = next_internal++;
+            int box[6] = {ORC_KEY_POS_INF, ORC_KEY_POS_INF, ORC_KEY_POS_INF, ORC_KEY_NEG_INF, ORC_KEY_NEG_INF,
+                          ORC_KEY_NEG_INF};
+            int cb[6] = {ORC_KEY_POS_INF, ORC_KEY_POS_INF, ORC_KEY_POS_INF, ORC_KEY_NEG_INF, ORC_KEY_NEG_INF,
+                         ORC_KEY_NEG_INF};
+            for (uint32_t i = tk.first; i != tk.first + tk.count; ++i) {
+                uint32_t t = idx[i];
+                for (int a = 0; a != 3; ++a) {
+                    box[a] = imin(box[a], orc_key(tbox[6 * (size_t)t + a]));
+                    box[3 + a] = imax(box[3 + a], orc_key(tbox[6 * (size_t)t + 3 + a]));
+                    cb[a] = imin(cb[a], orc_key(cen[3 * (size_t)t + a]));
+                    cb[3 + a] = imax(cb[3 + a], orc_key(cen[3 * (size_t)t + a]));
+                }
+            }
+            for (int a = 0; a != 6; ++a) b->node_box[6 * (size_t)id + a] = orc_unkey(box[a]);
+            float clo[3], scale[3];
+            int use[3];
+            for (int a = 0; a != 3; ++a) {
+                clo[a] = orc_unkey(cb[a]);
+                float ext = orc_unkey(cb[3 + a]) - clo[a];
+                use[a] = ext > 0.0f;
+                scale[a] = use[a] ? (float)ORC_SAH_BINS / ext : 0.0f;
+            }
+            int bin_count[3][ORC_SAH_BINS];
+            int bin_box[3][ORC_SAH_BINS][6];
+            for (int a = 0; a != 3; ++a)
+                for (int k = 0; k != ORC_SAH_BINS; ++k) {
+                    bin_count[a][k] = 0;
+                    for (int c = 0; c != 6; ++c) bin_box[a][k][c] = c < 3 ? ORC_KEY_POS_INF : ORC_KEY_NEG_INF;
+                }
+            for (uint32_t i = tk.first; i != tk.first + tk.count; ++i) {
+                uint32_t t = idx[i];
+                for (int a = 0; a != 3; ++a) {
+                    if (!use[a]) continue;
+                    int k = sah_bin(cen[3 * (size_t)t + a], clo[a], scale[a]);
+                    bin_count[a][k] += 1;
+                    for (int c = 0; c != 3; ++c) {
+                        bin_box[a][k][c] = imin(bin_box[a][k][c], orc_key(tbox[6 * (size_t)t + c]));
+                        bin_box[a][k][3 + c] = imax(bin_box[a][k][3 + c], orc_key(tbox[6 * (size_t)t + 3 + c]));
+                    }
+                }
+            }
+            float best = INFINITY;
+            int ba = -1, bb = -1;
+            uint32_t nl = 0;
+            for (int a = 0; a != 3; ++a) {
+                if (!use[a]) continue;
+                for (int sp2 = 1; sp2 != ORC_SAH_BINS; ++sp2) {
+                    int lc = 0, rc = 0;
+                    int l[6] = {ORC_KEY_POS_INF, ORC_KEY_POS_INF, ORC_KEY_POS_INF, ORC_KEY_NEG_INF, ORC_KEY_NEG_INF,
+                                ORC_KEY_NEG_INF};
+                    int r[6] = {ORC_KEY_POS_INF, ORC_KEY_POS_INF, ORC_KEY_POS_INF, ORC_KEY_NEG_INF, ORC_KEY_NEG_INF,
+                                ORC_KEY_NEG_INF};
+                    for (int k = 0; k != ORC_SAH_BINS; ++k) {
+                        int* dst = k < sp2 ? l : r;
+                        if (k < sp2) lc += bin_count[a][k];
+                        else rc += bin_count[a][k];
+                        for (int c = 0; c != 3; ++c) {
+                            dst[c] = imin(dst[c], bin_box[a][k][c]);
+                            dst[3 + c] = imax(dst[3 + c], bin_box[a][k][3 + c]);
+                        }
+                    }
+                    if (lc == 0 || rc == 0) continue;
+                    float cost = (float)lc * sah_area(l) + (float)rc * sah_area(r);
+                    if (cost < best) {
+                        best = cost;
+                        ba = a;
+                        bb = sp2;
+                        nl = (uint32_t)lc;
+                    }
+                }
+            }
+            if (ba < 0) {
+                nl = tk.count / 2u;
+            } else {
+                uint32_t w0 = 0, w1 = nl;
+                for (uint32_t i = tk.first; i != tk.first + tk.count; ++i) {
+                    uint32_t t = idx[i];
+                    if (sah_bin(cen[3 * (size_t)t + ba], clo[ba], scale[ba]) < bb) tmp[w0++] = t;
+                    else tmp[w1++] = t;
+                }
+                memcpy(idx + tk.first, tmp, sizeof(uint32_t) * tk.count);
+            }
+            /* right pushed first: the left subtree is numbered first */
+            stack[sp].first = tk.first + nl;
+            stack[sp].count = tk.count - nl;
+            stack[sp].parent = id;
+            stack[sp].right = 1;
+            ++sp;
+            stack[sp].first = tk.first;
+            stack[sp].count = nl;
+            stack[sp].parent = id;
+            stack[sp].right = 0;
+            ++sp;
+        }
+        b->node_parent[id] = tk.parent;
+        if (tk.parent >= 0) {
+            if (tk.right) b->node_right[tk.parent] = id;
+            else b->node_left[tk.parent] = id;
+        }
+    }
+    free(tbox);
+    free(cen);
+    free(idx);
+    free(tmp);
+    free(stack);
+    /* escape links top-down in pre-order */
+    int32_t* st = (int32_t*)malloc(sizeof(int32_t) * (nodes + 1));
+    int top = 0;
+    b->node_escape[n > 1 ? 0 : (int32_t)(n - 1)] = -1;
+    if (n > 1) {
+        st[top++] = 0;
+        while (top) {
+            int x = st[--top];
+            if (x >= (int)n - 1) continue;
+            int l = b->node_left[x], r = b->node_right[x];
+            b->node_escape[l] = r;
+            b->node_escape[r] = b->node_escape[x];
+            st[top++] = r;
+            st[top++] = l;
+        }
+    }
+    free(st);
+    return 0;
+}
+
+/* root node id: internal node 0, or the single leaf */
+static int32_t orc_bvh_root(const orc_bvh_t* b) { return b->leaf_count > 1 ? 0 : (int32_t)(b->leaf_count - 1); }
+
+void orc_bvh_records(const orc_bvh_t* b, const uint32_t* positions, float* out) {
+    uint32_t n = b->leaf_count, nodes = 2 * n - 1;
+    /* leaves below every node, by a post-order pass */
+    uint32_t* leaves = (uint32_t*)calloc(nodes, sizeof(uint32_t));
+    int32_t* st = (int32_t*)malloc(sizeof(int32_t) * 2 * (nodes + 1));
+    uint8_t* seen = (uint8_t*)calloc(nodes, 1);
+    int top = 0;
+    st[top++] = orc_bvh_root(b);
+    while (top) {
+        int x = st[top - 1];
+        if (x >= (int)n - 1) {
+            leaves[x] = 1;
+            --top;
+            continue;
+        }
+        if (!seen[x]) {
+            seen[x] = 1;
+            st[top++] = b->node_right[x];
+            st[top++] = b->node_left[x];
+            continue;
+        }
+        --top;
+        leaves[x] = leaves[b->node_left[x]] + leaves[b->node_right[x]];
+    }
+    /* pre-order write */
+    uint32_t u = 0;
+    top = 0;
+    st[top++] = orc_bvh_root(b);
+    while (top) {
+        int x = st[--top];
+        float* r = out + 4 * (size_t)u;
+        if (x >= (int)n - 1) {
+            uint32_t k = (uint32_t)x - (n - 1), t = b->leaf_prim[k];
+            const float* v = b->leaf_tri + 9 * (size_t)k;
+            int32_t m1 = -1;
+            uint32_t light = positions[6 * (size_t)t + 1] & 0x80000000u;
+            for (int i = 0; i != 3; ++i) {
+                r[i] = v[i];
+                r[4 + i] = v[3 + i];
+                r[8 + i] = v[6 + i];
+            }
+            memcpy(&r[3], &m1, 4);
+            memcpy(&r[7], &t, 4);
+            memcpy(&r[11], &light, 4);
+            u += 3;
+        } else {
+            int32_t esc = (int32_t)(u + 5 * leaves[x] - 2);
+            const float* bx = b->node_box + 6 * (size_t)x;
+            for (int i = 0; i != 3; ++i) {
+                r[i] = bx[i];
+                r[4 + i] = bx[3 + i];
+            }
+            memcpy(&r[3], &esc, 4);
+            r[7] = 0.0f;
+            u += 2;
+            st[top++] = b->node_right[x];
+            st[top++] = b->node_left[x];
+        }
+    }
+    free(leaves);
+    free(st);
+    free(seen);
+}
+
 void orc_bvh_free(orc_bvh_t* b) {
     free(b->node_box);
     free(b->node_left);

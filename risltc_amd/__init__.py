@@ -51,6 +51,8 @@ SIGNATURES = {
     "risltc_set_ltc_table": (C.c_int, [_vp, _u16p, _u16p, _u32, _u32, _u32]),
     "risltc_set_scene": (C.c_int, [_vp, _u32p, _u16p, _u8p, C.c_uint64, _f32p, _f32p,
                                    C.POINTER(abi.Material), _u32]),
+    "risltc_set_bvh_builder": (C.c_int, [_vp, _u32]),
+    "risltc_bvh_records": (C.c_int, [_vp, _f32p, C.c_uint64, C.POINTER(C.c_uint32)]),
     "risltc_render_visibility": (C.c_int, [_vp, C.POINTER(abi.FrameConstants), _vp, _vp]),
     "risltc_shade": (C.c_int, [_vp, C.POINTER(abi.FrameConstants), C.POINTER(abi.Settings), _vp, _vp, _vp]),
     "risltc_shade_debug": (C.c_int, [_vp, C.POINTER(abi.FrameConstants), C.POINTER(abi.Settings), _vp, _vp,
@@ -176,6 +178,19 @@ class Context:
         self._check(self.L.risltc_set_scene(self.h, _ptr(pos, _u32p), _ptr(nuv, _u16p), _ptr(mi, _u8p),
                                             mesh.triangle_count, _ptr(f, _f32p), _ptr(s, _f32p), arr, len(mats)),
                     "risltc_set_scene")
+
+    def set_bvh_builder(self, builder):
+        """abi.BVH_SAH (default) or abi.BVH_LBVH, for the next set_scene."""
+        self._check(self.L.risltc_set_bvh_builder(self.h, builder), "risltc_set_bvh_builder")
+
+    def bvh_records(self):
+        """The hierarchy's pre-order records as a (units, 4) float32 array."""
+        units = C.c_uint32(0)
+        self._check(self.L.risltc_bvh_records(self.h, None, 0, C.byref(units)), "risltc_bvh_records")
+        out = np.zeros((units.value, 4), dtype=np.float32)
+        self._check(self.L.risltc_bvh_records(self.h, _ptr(out, _f32p), units.value, C.byref(units)),
+                    "risltc_bvh_records")
+        return out
 
     def load_scene(self, scene, ltc_rgba16, ltc_rg16):
         """Uploads a risltc_amd.synth.Scene and an LTC table."""

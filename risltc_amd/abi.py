@@ -18,6 +18,7 @@ MIS_BALANCE, MIS_POWER, MIS_WEIGHTED, MIS_OPTIMAL_CLAMPED, MIS_OPTIMAL = range(5
 LIGHT_UNIFORM, LIGHT_RESERVOIR = 0, 1
 SAMPLE_POLYGON_LTC_CP = 4
 PROPOSAL_UNIFORM, PROPOSAL_LEARNED = 0, 1
+BVH_SAH, BVH_LBVH = 0, 1
 
 # risltc_sample_record_t word offsets
 REC_LIGHT, REC_WSUM, REC_PHAT, REC_W, REC_CLIPD, REC_CLIPS, REC_PSAD, REC_PSAS = range(8)

@@ -128,10 +128,29 @@ struct BvhBuild {
     uint32_t units = 0;
     uint32_t leaf_count = 0;
 };
-// Builds the LBVH of the dequantised triangles of a *.vks mesh on the device.
-hipError_t build_lbvh(const uint2* d_positions, uint64_t triangle_count, const float deq_factor[3],
-                      const float deq_summand[3], BvhBuild* out, hipStream_t s);
-void free_lbvh(BvhBuild* b);
+// Device arrays of a binary tree over n triangles during a build: 2n - 1
+// nodes, node 0 the root. Leaves have left = right = -1 and prim = triangle;
+// count = triangles below a node; box = 6 floats (internal nodes).
+struct SahTree {
+    float* tbox;    // 6 per triangle (lo xyz, hi xyz)
+    float* cent;    // 3 per triangle
+    uint32_t* idx;  // triangle permutation, partitioned in place
+    uint32_t* tmp;  // partition buffer
+    int* left;
+    int* right;
+    int* parent;
+    int* first;     // start of the node's range in idx
+    int* count;
+    uint32_t* prim;
+    float* box;
+    uint32_t* node_counter;
+};
+hipError_t build_sah_tree(const float* tris, uint32_t n, SahTree& T, hipStream_t s);
+// Builds the hierarchy (RISLTC_BVH_SAH or RISLTC_BVH_LBVH) of the dequantised
+// triangles of a *.vks mesh on the device and lays it out in pre-order.
+hipError_t build_bvh(const uint2* d_positions, uint64_t triangle_count, const float deq_factor[3],
+                     const float deq_summand[3], uint32_t builder, BvhBuild* out, hipStream_t s);
+void free_bvh(BvhBuild* b);
 hipError_t launch_trace_rays(const BvhView& bvh, const float* rays8, uint32_t n, uint32_t* hits,
                              hipStream_t s);
 hipError_t launch_primary(const BvhView& bvh, const uint2* positions, const risltc_frame_constants_t& c,

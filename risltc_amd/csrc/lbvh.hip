@@ -124,8 +124,7 @@ __device__ __forceinline__ int kdelta(const uint64_t* keys, int n, int i, int j)
     return __clzll((long long)(keys[i] ^ keys[j]));
 }
 
-__global__ void karras_kernel(const uint64_t* keys, int n, int* left, int* right, int* parent, int* first,
-                              int* count) {
+__global__ void karras_kernel(const uint64_t* keys, int n, int* left, int* right, int* parent, int* count) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -151,21 +150,26 @@ __global__ void karras_kernel(const uint64_t* keys, int n, int* left, int* right
     right[i] = rc;
     parent[lc] = i;
     parent[rc] = i;
-    first[i] = lo;
     count[i] = hi - lo + 1;
 }
 
-__global__ void leaf_kernel(const uint64_t* keys, const float* tris, int n, float* box) {
+__global__ void leaf_kernel(const uint64_t* keys, const float* tris, int n, float* box, int* left, int* right,
+                            int* count, uint32_t* prim) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    uint32_t prim = (uint32_t)(keys[k] & 0xFFFFFFFFull);
-    const float* v = tris + (size_t)prim * 9;
-    float* b = box + (size_t)(n - 1 + k) * 6;
+    uint32_t t = (uint32_t)(keys[k] & 0xFFFFFFFFull);
+    const float* v = tris + (size_t)t * 9;
+    const int x = n - 1 + k;
+    float* b = box + (size_t)x * 6;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         b[a] = fmin_(fmin_(v[a], v[3 + a]), v[6 + a]);
         b[3 + a] = fmax_(fmax_(v[a], v[3 + a]), v[6 + a]);
     }
+    left[x] = -1;
+    right[x] = -1;
+    count[x] = 1;
+    prim[x] = t;
 }
 
 __global__ void refit_kernel(int n, const int* left, const int* right, const int* parent, float* box,
@@ -193,41 +197,37 @@ __global__ void refit_kernel(int n, const int* left, const int* right, const int
     }
 }
 
-// Writes node x (Karras id) at its pre-order unit. Before x in pre-order
-// come its `depth` ancestors, and the subtrees hanging left of the root-to-x
-// path: they hold the a leaves left of x's range and a - r internal nodes,
-// r being the number of right turns on the path. Internal nodes take 2
-// units, leaves 3, so x starts at unit 2 (depth + a - r) + 3 a and its
-// subtree of m leaves spans 5 m - 2 units.
-__global__ void layout_kernel(int n, const int* right, const int* parent, const int* first, const int* count,
-                              const float* box, const uint64_t* keys, const float* tris, const uint2* pos,
+// Writes node x of a binary tree at its pre-order unit (left child first).
+// Before x in pre-order come its ancestors (2 units each) and, for every
+// ancestor whose right subtree holds x, that ancestor's whole left subtree
+// (a subtree of m leaves spans 5 m - 2 units). The escape unit of x is the
+// first unit past its own subtree.
+__global__ void layout_kernel(int nodes, const int* left, const int* right, const int* parent, const int* count,
+                              const uint32_t* prim, const float* box, const float* tris, const uint2* pos,
                               float4* rec) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= 2 * n - 1) return;
-    int depth = 0, turns = 0;
+    if (x >= nodes) return;
+    int depth = 0, skipped = 0;
     int y = x;
-    for (int steps = 0; steps < n; ++steps) {
+    for (int steps = 0; steps < nodes; ++steps) {
         int p = parent[y];
         if (p < 0) break;
         ++depth;
-        if (right[p] == y) ++turns;
+        if (right[p] == y) skipped += 5 * count[left[p]] - 2;
         y = p;
     }
-    if (x < n - 1) {
-        int a = first[x];
-        int u = 2 * (depth + a - turns) + 3 * a;
+    const int u = 2 * depth + skipped;
+    if (left[x] >= 0) {
         int esc = u + 5 * count[x] - 2;
         const float* b = box + (size_t)x * 6;
         rec[u] = make_float4(b[0], b[1], b[2], __int_as_float(esc));
         rec[u + 1] = make_float4(b[3], b[4], b[5], 0.0f);
     } else {
-        int k = x - (n - 1);
-        int u = 2 * (depth + k - turns) + 3 * k;
-        uint32_t prim = (uint32_t)(keys[k] & 0xFFFFFFFFull);
-        const float* v = tris + (size_t)prim * 9;
-        uint32_t light = pos[(size_t)prim * 3].y & 0x80000000u;
+        uint32_t t = prim[x];
+        const float* v = tris + (size_t)t * 9;
+        uint32_t light = pos[(size_t)t * 3].y & 0x80000000u;
         rec[u] = make_float4(v[0], v[1], v[2], __int_as_float(-1));
-        rec[u + 1] = make_float4(v[3], v[4], v[5], __uint_as_float(prim));
+        rec[u + 1] = make_float4(v[3], v[4], v[5], __uint_as_float(t));
         rec[u + 2] = make_float4(v[6], v[7], v[8], __uint_as_float(light));
     }
 }
@@ -237,94 +237,122 @@ __global__ void fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
-#define HIP_TRY(x)                          \
-    do {                                    \
-        hipError_t e_ = (x);                \
-        if (e_ != hipSuccess) {             \
-            err = e_;                       \
-            goto done;                      \
-        }                                   \
+namespace {
+// device allocations of one build, released on every path
+struct BuildScratch {
+    void* p[20] = {};
+    int k = 0;
+    template <class T>
+    hipError_t get(T** out, size_t bytes) {
+        *out = nullptr;
+        hipError_t e = hipMalloc((void**)out, bytes);
+        if (e == hipSuccess) p[k++] = *out;
+        return e;
+    }
+    ~BuildScratch() {
+        for (int i = 0; i < k; ++i) hipFree(p[i]);
+    }
+};
+}  // namespace
+
+#define BVH_TRY(x)                         \
+    do {                                   \
+        hipError_t e_ = (x);               \
+        if (e_ != hipSuccess) return e_;   \
     } while (0)
 
-hipError_t build_lbvh(const uint2* d_pos, uint64_t tri_count, const float deq_factor[3], const float deq_summand[3],
-                      BvhBuild* out, hipStream_t s) {
-    hipError_t err = hipSuccess;
-    int n = (int)tri_count;
-    float *tris = nullptr, *cent = nullptr, *partial = nullptr, *bounds = nullptr, *box = nullptr;
-    uint64_t *keys = nullptr, *keys_sorted = nullptr;
-    int *left = nullptr, *right = nullptr, *parent = nullptr;
-    int *first = nullptr, *count = nullptr;
-    uint32_t* arrivals = nullptr;
-    void* temp = nullptr;
-    size_t temp_bytes = 0;
+hipError_t build_bvh(const uint2* d_pos, uint64_t tri_count, const float deq_factor[3], const float deq_summand[3],
+                     uint32_t builder, BvhBuild* out, hipStream_t s) {
+    if (tri_count == 0 || tri_count > kMaxBvhTriangles) return hipErrorInvalidValue;
+    if (builder != RISLTC_BVH_SAH && builder != RISLTC_BVH_LBVH) return hipErrorInvalidValue;
+    BuildScratch w;
+    const int n = (int)tri_count;
+    const uint64_t nodes = 2 * tri_count - 1;
     const uint32_t tb = 256;
     const uint32_t tri_blocks = (uint32_t)((tri_count + tb - 1) / tb);
-    const uint32_t red_blocks = 1024;
-    float4* rec = nullptr;
-    const uint64_t units = 5 * tri_count - 2;
-    if (n <= 0 || tri_count > kMaxBvhTriangles) return hipErrorInvalidValue;
-    HIP_TRY(hipMalloc(&tris, sizeof(float) * 9 * tri_count));
-    HIP_TRY(hipMalloc(&cent, sizeof(float) * 3 * tri_count));
-    HIP_TRY(hipMalloc(&partial, sizeof(float) * 6 * red_blocks));
-    HIP_TRY(hipMalloc(&bounds, sizeof(float) * 6));
-    HIP_TRY(hipMalloc(&keys, sizeof(uint64_t) * tri_count));
-    HIP_TRY(hipMalloc(&keys_sorted, sizeof(uint64_t) * tri_count));
-    HIP_TRY(hipMalloc(&box, sizeof(float) * 6 * (2 * tri_count - 1)));
-    HIP_TRY(hipMalloc(&left, sizeof(int) * tri_count));
-    HIP_TRY(hipMalloc(&right, sizeof(int) * tri_count));
-    HIP_TRY(hipMalloc(&parent, sizeof(int) * (2 * tri_count - 1)));
-    HIP_TRY(hipMalloc(&arrivals, sizeof(uint32_t) * tri_count));
-    HIP_TRY(hipMalloc(&first, sizeof(int) * tri_count));
-    HIP_TRY(hipMalloc(&count, sizeof(int) * tri_count));
-    HIP_TRY(hipMalloc(&rec, sizeof(float4) * (units + 2)));
-    // the two padding units after the last record read as zeros
-    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(rec + units), 8, 0u);
+    const uint32_t node_blocks = (uint32_t)((nodes + tb - 1) / tb);
+    float *tris, *cent, *box;
+    int *left, *right, *parent, *count;
+    uint32_t* prim;
+    BVH_TRY(w.get(&tris, sizeof(float) * 9 * tri_count));
+    BVH_TRY(w.get(&cent, sizeof(float) * 3 * tri_count));
+    BVH_TRY(w.get(&box, sizeof(float) * 6 * nodes));
+    BVH_TRY(w.get(&left, sizeof(int) * nodes));
+    BVH_TRY(w.get(&right, sizeof(int) * nodes));
+    BVH_TRY(w.get(&parent, sizeof(int) * nodes));
+    BVH_TRY(w.get(&count, sizeof(int) * nodes));
+    BVH_TRY(w.get(&prim, sizeof(uint32_t) * nodes));
     hipLaunchKernelGGL(dequant_kernel, dim3(tri_blocks), dim3(tb), 0, s, d_pos, tri_count,
                        make_float3(deq_factor[0], deq_factor[1], deq_factor[2]),
                        make_float3(deq_summand[0], deq_summand[1], deq_summand[2]), tris, cent);
-    hipLaunchKernelGGL(bounds_kernel, dim3(red_blocks), dim3(256), 0, s, cent, tri_count, partial);
-    hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(64), 0, s, partial, red_blocks, bounds);
-    hipLaunchKernelGGL(morton_kernel, dim3(tri_blocks), dim3(tb), 0, s, cent, tri_count, bounds, keys);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, temp_bytes, keys, keys_sorted, n, 0, 64, s));
-    HIP_TRY(hipMalloc(&temp, temp_bytes));
-    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(temp, temp_bytes, keys, keys_sorted, n, 0, 64, s));
-    hipLaunchKernelGGL(fill_u32, dim3((uint32_t)((2 * tri_count - 1 + tb - 1) / tb)), dim3(tb), 0, s,
-                       reinterpret_cast<uint32_t*>(parent), 2 * tri_count - 1, 0xFFFFFFFFu);
-    hipLaunchKernelGGL(fill_u32, dim3(tri_blocks), dim3(tb), 0, s, arrivals, tri_count, 0u);
-    if (n > 1)
-        hipLaunchKernelGGL(karras_kernel, dim3(tri_blocks), dim3(tb), 0, s, keys_sorted, n, left, right, parent, first,
-                           count);
-    hipLaunchKernelGGL(leaf_kernel, dim3(tri_blocks), dim3(tb), 0, s, keys_sorted, tris, n, box);
-    if (n > 1) hipLaunchKernelGGL(refit_kernel, dim3(tri_blocks), dim3(tb), 0, s, n, left, right, parent, box, arrivals);
-    hipLaunchKernelGGL(layout_kernel, dim3((uint32_t)((2 * tri_count - 1 + tb - 1) / tb)), dim3(tb), 0, s, n, right,
-                       parent, first, count, box, keys_sorted, tris, d_pos, rec);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(s));
+    hipLaunchKernelGGL(fill_u32, dim3(node_blocks), dim3(tb), 0, s, reinterpret_cast<uint32_t*>(parent), nodes,
+                       0xFFFFFFFFu);
+    BVH_TRY(hipGetLastError());
+    if (builder == RISLTC_BVH_SAH) {
+        SahTree T;
+        T.cent = cent;
+        T.left = left;
+        T.right = right;
+        T.parent = parent;
+        T.count = count;
+        T.prim = prim;
+        T.box = box;
+        BVH_TRY(w.get(&T.tbox, sizeof(float) * 6 * tri_count));
+        BVH_TRY(w.get(&T.idx, sizeof(uint32_t) * tri_count));
+        BVH_TRY(w.get(&T.tmp, sizeof(uint32_t) * tri_count));
+        BVH_TRY(w.get(&T.first, sizeof(int) * nodes));
+        BVH_TRY(w.get(&T.node_counter, sizeof(uint32_t)));
+        BVH_TRY(build_sah_tree(tris, (uint32_t)n, T, s));
+    } else {
+        const uint32_t red_blocks = 1024;
+        float *partial, *bounds;
+        uint64_t *keys, *keys_sorted;
+        uint32_t* arrivals;
+        void* temp;
+        size_t temp_bytes = 0;
+        BVH_TRY(w.get(&partial, sizeof(float) * 6 * red_blocks));
+        BVH_TRY(w.get(&bounds, sizeof(float) * 6));
+        BVH_TRY(w.get(&keys, sizeof(uint64_t) * tri_count));
+        BVH_TRY(w.get(&keys_sorted, sizeof(uint64_t) * tri_count));
+        BVH_TRY(w.get(&arrivals, sizeof(uint32_t) * tri_count));
+        hipLaunchKernelGGL(bounds_kernel, dim3(red_blocks), dim3(256), 0, s, cent, tri_count, partial);
+        hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(64), 0, s, partial, red_blocks, bounds);
+        hipLaunchKernelGGL(morton_kernel, dim3(tri_blocks), dim3(tb), 0, s, cent, tri_count, bounds, keys);
+        BVH_TRY(hipGetLastError());
+        BVH_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, temp_bytes, keys, keys_sorted, n, 0, 64, s));
+        BVH_TRY(w.get(&temp, temp_bytes));
+        BVH_TRY(hipcub::DeviceRadixSort::SortKeys(temp, temp_bytes, keys, keys_sorted, n, 0, 64, s));
+        hipLaunchKernelGGL(fill_u32, dim3(tri_blocks), dim3(tb), 0, s, arrivals, tri_count, 0u);
+        if (n > 1)
+            hipLaunchKernelGGL(karras_kernel, dim3(tri_blocks), dim3(tb), 0, s, keys_sorted, n, left, right, parent,
+                               count);
+        hipLaunchKernelGGL(leaf_kernel, dim3(tri_blocks), dim3(tb), 0, s, keys_sorted, tris, n, box, left, right,
+                           count, prim);
+        if (n > 1)
+            hipLaunchKernelGGL(refit_kernel, dim3(tri_blocks), dim3(tb), 0, s, n, left, right, parent, box, arrivals);
+        BVH_TRY(hipGetLastError());
+    }
+    const uint64_t units = 5 * tri_count - 2;
+    float4* rec = nullptr;
+    BVH_TRY(hipMalloc(&rec, sizeof(float4) * (units + 2)));
+    // the two padding units after the last record read as zeros
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(rec + units), 8, 0u);
+    hipLaunchKernelGGL(layout_kernel, dim3(node_blocks), dim3(tb), 0, s, (int)nodes, left, right, parent, count, prim,
+                       box, tris, d_pos, rec);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        hipFree(rec);
+        return e;
+    }
     out->rec = rec;
     out->units = (uint32_t)units;
     out->leaf_count = (uint32_t)n;
-    rec = nullptr;
-done:
-    hipFree(tris);
-    hipFree(cent);
-    hipFree(partial);
-    hipFree(bounds);
-    hipFree(keys);
-    hipFree(keys_sorted);
-    hipFree(box);
-    hipFree(left);
-    hipFree(right);
-    hipFree(parent);
-    hipFree(arrivals);
-    hipFree(first);
-    hipFree(count);
-    hipFree(temp);
-    hipFree(rec);
-    return err;
+    return hipSuccess;
 }
+#undef BVH_TRY
 
-void free_lbvh(BvhBuild* b) {
+void free_bvh(BvhBuild* b) {
     if (b->rec) hipFree(b->rec);
     b->rec = nullptr;
     b->units = 0;

@@ -44,6 +44,14 @@ def test_city_visibility_bitwise(city):
     assert np.array_equal(city["vis_h"], vis_o)
 
 
+def test_city_bvh_records_bitwise(city):
+    """The 1M-triangle binned-SAH hierarchy equals the oracle's word for word."""
+    got = city["ctx"].bvh_records()
+    ref = city["oracle"].bvh_records()
+    assert got.shape == ref.shape == (5 * city["scene"].mesh.triangle_count - 2, 4)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
 def test_city_exact_subset_bitwise(city):
     s = city["scene"].settings(exact_math=1)
     out = _frame(city, s)

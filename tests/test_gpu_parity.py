@@ -94,6 +94,52 @@ def test_shadow_rays_bitwise(small_gpu):
     assert 0.05 < h.mean() < 0.95
 
 
+def test_bvh_records_bitwise_sah(small_gpu):
+    """The device's binned-SAH hierarchy, laid out in pre-order, equals the
+    oracle's restatement word for word (boxes, escapes, leaf order)."""
+    g = small_gpu
+    got = g["ctx"].bvh_records()
+    ref = g["oracle"].bvh_records()
+    assert got.shape == ref.shape
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_lbvh_builder_bitwise(small_scene, ltc_table):
+    """The Morton-code LBVH builder: records equal the oracle's LBVH, primary
+    visibility and random shadow rays bitwise equal to the oracle on it, and
+    any-hit flags equal to those of the SAH tree (the flags do not depend on
+    the tree)."""
+    import risltc_amd
+    torch = _torch()
+    ctx = risltc_amd.Context(0)
+    ctx.set_bvh_builder(abi.BVH_LBVH)
+    ctx.load_scene(small_scene, *ltc_table)
+    oracle = orc.Oracle(small_scene, *ltc_table, builder=abi.BVH_LBVH)
+    sah = orc.Oracle(small_scene, *ltc_table, builder=abi.BVH_SAH)
+    assert np.array_equal(ctx.bvh_records().view(np.uint32), oracle.bvh_records().view(np.uint32))
+    consts = ctx.frame_constants()
+    P = small_scene.width * small_scene.height
+    vis = torch.empty(P, dtype=torch.int32, device="cuda")
+    ctx.render_visibility(consts, vis)
+    torch.cuda.synchronize()
+    assert np.array_equal(vis.cpu().numpy().view(np.uint32), oracle.render_visibility(consts))
+    rng = np.random.default_rng(6)
+    n = 5000
+    o = rng.uniform([-10, -10, 0], [10, 10, 8], (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    tmax = rng.uniform(0.1, 20, n).astype(np.float32)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3], rays[:, 3], rays[:, 4:7], rays[:, 7] = o, 1e-3, d, tmax
+    hits = torch.zeros(n, dtype=torch.int32, device="cuda")
+    ctx.trace_shadow_rays(torch.from_numpy(rays).cuda(), hits)
+    torch.cuda.synchronize()
+    h = hits.cpu().numpy()
+    for orb in (oracle, sah):
+        ref = np.array([orb.L.orc_bvh_any_hit(orc.C.byref(orb.bvh), orc.ptr(o[i], orc.f32p), orc.ptr(d[i], orc.f32p),
+                                              1e-3, float(tmax[i]), None) for i in range(n)])
+        assert np.array_equal(h, ref)
+
+
 @pytest.mark.parametrize("spp,M", [(1, 8), (2, 8)])
 def test_exact_shading_bitwise(small_gpu, spp, M):
     g = small_gpu
