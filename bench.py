@@ -24,6 +24,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Msamples/s @1080p; GPU time to relMSE 0.01, learned vs plain RIS-LTC"
 HBM_PEAK_GBS = 8000.0
+# wave64 FP32 VALU instructions: 256 CUs x 4 SIMDs, 4 cycles each, 2.4 GHz
+# (non-packed: the build disables packed FP32), in lane-instructions/s
+VALU_PEAK_TLANE = 256 * 4 * 16 * 2.4e9 / 1e12
 PMC_FILE = os.path.join(ROOT, "profiles", "shade_city1080_traffic.json")
 
 
@@ -43,7 +46,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="city")
     ap.add_argument("--exact", action="store_true", help="exact-math kernels (parity build)")
-    ap.add_argument("--cpu-pixels", type=int, default=524288)
+    ap.add_argument("--cpu-pixels", type=int, default=0,
+                    help="pixels in the CPU-baseline sample (0 = the whole frame)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (known HBM bytes) for counter scaling")
     args = ap.parse_args()
@@ -104,11 +108,17 @@ def main():
     alg = algorithmic_bytes(P, scene.light_count, scene.light_vmax, res)
     achieved = alg / (shade_ms * 1e-3) / 1e9
     traffic = None
-    if os.path.exists(PMC_FILE):
-        try:
-            traffic = json.load(open(PMC_FILE)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+    valu = None
+    if os.path.exists(PMC_FILE) and args.workload == "city" and not args.exact:
+        pmc = json.load(open(PMC_FILE))
+        traffic = pmc.get("bytes_per_launch")
+        insts = pmc.get("valu_insts_per_launch")
+        if insts:
+            rate = insts * 64 / (shade_ms * 1e-3) / 1e12
+            valu = {"achieved": round(rate, 3), "peak": round(VALU_PEAK_TLANE, 3), "unit": "T lane-inst/s",
+                    "frac": round(rate / VALU_PEAK_TLANE, 4),
+                    "active_lanes": round(pmc["valu_thread_cycles_per_launch"] / insts / 64, 3),
+                    "insts_per_launch": insts, "source": "profiles/city1080_counters.md"}
     vis_h = vis.cpu().numpy().view(np.uint32)
     frame = out.cpu().numpy().reshape(-1, 4)
     print("kernels (ms): shade %.3f trace %.3f resolve %.3f; frame mean %s" %
@@ -120,15 +130,21 @@ def main():
         import orc
         threads = min(16, os.cpu_count() or 1)
         oracle = orc.Oracle(scene, rgba, rg)
-        rng = np.random.default_rng(0)
-        pix = rng.choice(P, size=min(args.cpu_pixels, P), replace=False).astype(np.uint32)
+        if 0 < args.cpu_pixels < P:
+            rng = np.random.default_rng(0)
+            pix = np.sort(rng.choice(P, size=args.cpu_pixels, replace=False)).astype(np.uint32)
+            what = "%d random pixels of the same %dx%d frame" % (pix.size, scene.width, scene.height)
+        else:
+            pix = None
+            what = "the whole %dx%d frame" % (scene.width, scene.height)
+        count = P if pix is None else pix.size
         t1 = time.perf_counter()
         oracle.shade_frame(consts, scene.settings(), vis_h, pixels=pix, threads=threads)
         cpu_s = time.perf_counter() - t1
-        cpu = {"value": round(pix.size * spp / cpu_s / 1e6, 4), "unit": "Msamples/s", "cores": threads,
+        cpu = {"value": round(count * spp / cpu_s / 1e6, 4), "unit": "Msamples/s", "cores": threads,
                "kind": "port",
-               "sample": "%d random pixels of the same 1920x1080 frame, %d spp, M=%d, FP32 oracle, OpenMP "
-                         "(%.1f s)" % (pix.size, spp, scene.candidates, cpu_s)}
+               "sample": "%s, %d spp, M=%d, shadow rays on the oracle's SAH tree; FP32 oracle, OpenMP, "
+                         "%.1f s" % (what, spp, scene.candidates, cpu_s)}
 
     line = {
         "metric": METRIC,
@@ -152,6 +168,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "kernel": "shade_kernel", "algorithmic_bytes": alg},
+        "valu": valu,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

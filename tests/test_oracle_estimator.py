@@ -1,0 +1,121 @@
+"""Pins of the oracle's composed estimator (CPU only).
+
+  * the reservoir (reservoir.glsl:16-40, shading_pass.frag.glsl:723-761)
+    selects each candidate light with probability w / sum(w);
+  * the final sample (shading_pass.frag.glsl:292-397: PSA sampling of the
+    diffuse and the LTC-space polygon, BRDF, LTC density, MIS) is an unbiased
+    estimator of the polygon's reflected radiance: its mean over many noise
+    seeds equals a brute-force quadrature of f * L * cos over the light's area,
+    computed here in FP64 from the closed-form BRDF.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import orc
+from risltc_amd import abi, synth
+
+
+def _brdf(n, o, alb, f0, a, l):
+    """GGX + Lambert as in brdfs.glsl:43-93 (closed form, FP64), for rows of
+    directions l; returns one RGB row per direction."""
+    h = (l + o) / np.linalg.norm(l + o, axis=1, keepdims=True)
+    li, lo, ndh, odh = l @ n, o @ n, h @ n, h @ o
+    D = a * a / (((a * a - 1) * ndh * ndh + 1) ** 2)
+    V = 0.5 / (li * np.sqrt(lo * lo * (1 - a * a) + a * a) + lo * np.sqrt(li * li * (1 - a * a) + a * a))
+    F = f0[None, :] + (1 - f0[None, :]) * ((1 - np.clip(odh, 0, 1)) ** 5)[:, None]
+    lum = F @ np.array([0.21263901, 0.71516868, 0.07219232])
+    return (alb[None, :] + (D * V * lum)[:, None]) / math.pi
+
+
+def test_reservoir_selects_in_proportion_to_weight(small_oracle, small_scene):
+    """Over all samples of the small scene, the light at candidate slot i is
+    the selected one as often as sum of its weights / w_sum predicts, for every
+    slot (so neither early nor late candidates are favoured)."""
+    o = small_oracle
+    s = small_scene.settings(sample_count_light=2, ris_candidate_count=8, shadow_rays=0)
+    consts = o.frame_constants()
+    vis = o.render_visibility(consts)
+    _, rec, cand = o.shade(consts, s, vis, records=True, candidates=True)
+    idx = cand[..., 0].astype(np.int64).reshape(-1, 8)
+    w = cand[..., 3].view(np.float32).astype(np.float64).reshape(-1, 8)
+    chosen = rec[..., abi.REC_LIGHT].view(np.int32).reshape(-1)
+    wsum = w.sum(1)
+    ok = wsum > 0
+    idx, w, chosen, wsum = idx[ok], w[ok], chosen[ok], wsum[ok]
+    assert idx.shape[0] > 20000
+    # probability that the light of slot i is the chosen light
+    same = idx[:, :, None] == idx[:, None, :]
+    p = (same * w[:, None, :]).sum(2) / wsum[:, None]
+    observed = (idx == chosen[:, None]).sum(0)
+    expected = p.sum(0)
+    sigma = np.sqrt((p * (1 - p)).sum(0))
+    assert np.all(np.abs(observed - expected) < 5 * sigma + 1), (observed, expected, sigma)
+    # and no sample with positive weights is left without a light
+    assert np.all(chosen >= 0)
+
+
+def _single_light_scene(roughness, metal):
+    """A 20 m ground plane and one tilted 1 x 0.6 m light 3 m above it."""
+    g = 10.0
+    tris = [[(-g, -g, 0), (g, -g, 0), (g, g, 0)], [(-g, -g, 0), (g, g, 0), (-g, g, 0)]]
+    nrm = [[(0, 0, 1)] * 3] * 2
+    uvs = [[(0, 0), (1, 0), (1, 1)], [(0, 0), (1, 1), (0, 1)]]
+    square = np.array([[-0.5, -0.5], [0.5, -0.5], [0.5, 0.5], [-0.5, 0.5]])
+    light = synth.Light(square, (0.4, 0.3, 0.2), (0.5, 1.0, 3.0), (1.0, 0.6), (6.0, 4.0, 2.5))
+    material = [0.6, 0.5, 0.4, 0.0, math.sqrt(roughness), metal]
+    cam = synth.make_camera((0.0, -9.0, 6.0), math.pi, math.pi / 2 - 0.6)
+    return synth._finish("one", tris, nrm, uvs, [0, 0], [material], [light], cam, 24, 24, 1, 4, 0)
+
+
+@pytest.mark.parametrize("roughness,metal", [(0.3, 0.0), (0.05, 0.7)])
+def test_estimator_is_unbiased(orc_lib, ltc_table, roughness, metal):
+    scene = _single_light_scene(roughness, metal)
+    o = orc.Oracle(scene, *ltc_table)
+    exposure = 1.0
+    consts0 = o.frame_constants(exposure=exposure)
+    vis = o.render_visibility(consts0)
+    ground = np.flatnonzero(vis < 2)
+    pix = ground[np.linspace(0, ground.size - 1, 6).astype(int)].astype(np.uint32)
+    spp = 256
+    s = scene.settings(sample_count_light=spp, ris_candidate_count=2, shadow_rays=0)
+    means = []
+    for seed in range(10):
+        c = o.frame_constants(exposure=exposure, noise_seed=seed, animate=True)
+        out, rec, _ = o.shade(c, s, vis, pixels=pix, fp64=True, records=(seed == 0))
+        means.append(out[:, :3])
+        if seed == 0:
+            pos = rec[:, 0, abi.REC_POS:abi.REC_POS + 3].astype(np.float64)
+    means = np.array(means)
+    est = means.mean(0)
+    se = means.std(0, ddof=1) / math.sqrt(means.shape[0])
+    # quadrature over the light: midpoint rule on a 240 x 240 grid
+    L = scene.lights[0]
+    rad = L.radiance.astype(np.float64)
+    wv = L.world.astype(np.float64)
+    nl = L.plane[:3].astype(np.float64)
+    k = 240
+    u = (np.arange(k) + 0.5) / k
+    uu, vv = np.meshgrid(u, u, indexing="ij")
+    pts = (wv[0] + uu[..., None] * (wv[1] - wv[0]) + vv[..., None] * (wv[3] - wv[0])).reshape(-1, 3)
+    dA = np.linalg.norm(np.cross(wv[1] - wv[0], wv[3] - wv[0])) / (k * k)
+    cam = np.array(scene.camera.position_world_space, np.float64)
+    mat = scene.materials[0].astype(np.float64)
+    base, m = mat[0:3], mat[5]
+    alb = base * (1 - m)
+    f0 = 0.02 * (1 - m) + base * m
+    alpha = min(max(mat[4] ** 2, 0.0064), 1.0)
+    n = np.array([0.0, 0.0, 1.0])
+    for p in range(pix.size):
+        x = pos[p]
+        ov = (cam - x) / np.linalg.norm(cam - x)
+        d = pts - x
+        r2 = (d * d).sum(1)
+        w = d / np.sqrt(r2)[:, None]
+        cos_x = np.maximum(w @ n, 0.0)
+        cos_y = np.abs(w @ nl)
+        f = _brdf(n, ov, alb, f0, alpha, w)
+        f[cos_x <= 0] = 0.0
+        ref = exposure * (f * rad[None, :] * (cos_x * cos_y / r2 * dA)[:, None]).sum(0)
+        assert np.all(np.abs(est[p] - ref) < 5 * se[p] + 2e-3 * ref), (p, est[p], ref, se[p])
