@@ -359,6 +359,52 @@ int risltc_shade_debug(risltc_context_t* context, const risltc_frame_constants_t
                        float* out_rgba32f, float* sample_records, uint32_t* candidate_records,
                        void* stream);
 
+/* ---- learned RIS proposal (project extension, no reference counterpart:
+ * the reference draws RIS candidates uniformly, shading_pass.frag.glsl:730-733).
+ * With settings.proposal = RISLTC_PROPOSAL_LEARNED the candidates of
+ * risltc_shade come from q = 1/(2N) + (MLP cluster proposal)/2, the RIS
+ * weights use that q, and the estimator stays unbiased. Sequence per frame:
+ * risltc_proposal_update (inference), risltc_shade (learned), then
+ * optionally risltc_proposal_train on the frame's own samples. ---------- */
+#define RISLTC_MAX_LIGHT_CLUSTERS 256
+#define RISLTC_PROPOSAL_PARAMS 31168 /* 96-64-64-64-256 MLP, FP32 */
+/* Explicit light clusters: cluster_of_light[i] < cluster_count <= 256. */
+int risltc_set_light_clusters(risltc_context_t* context, const uint32_t* cluster_of_light,
+                              uint32_t cluster_count);
+/* k-means of the light centroids (deterministic start: light floor(k N / K)
+ * for cluster k; empty clusters restart at the farthest light), then
+ * risltc_set_light_clusters. out (host, may be NULL) receives the assignment. */
+int risltc_cluster_lights(risltc_context_t* context, uint32_t cluster_count, uint32_t iterations,
+                          uint32_t* out);
+/* Deterministic initialisation from a seed; resets the Adam state. */
+int risltc_proposal_init(risltc_context_t* context, uint32_t seed);
+/* Host copies of the FP32 parameters (RISLTC_PROPOSAL_PARAMS floats, every
+ * weight matrix stored [out][in] followed by its bias) and of the last
+ * training step's gradient; either pointer may be NULL. set resets Adam. */
+int risltc_proposal_set_params(risltc_context_t* context, const float* params);
+int risltc_proposal_get_params(risltc_context_t* context, float* params, float* gradient);
+/* Inference for every pixel of the frame: writes the per-pixel tables the
+ * learned shade reads (32 bytes: 16 cluster ids, 16 weights). Optional
+ * device outputs: a copy of the tables, the 256 logits and the 96 bf16
+ * features per pixel. */
+int risltc_proposal_update(risltc_context_t* context, const risltc_frame_constants_t* constants,
+                           const uint32_t* visibility, uint8_t* table_out, float* logits_out,
+                           uint16_t* features_out, void* stream);
+/* Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) on batches of pixel-samples of
+ * the last risltc_shade: target = cluster of the selected light, weight =
+ * luminance of its estimate (cross-entropy against the softmax). batch is a
+ * multiple of 256. loss_out (host, may be NULL): the last step's loss. */
+int risltc_proposal_train(risltc_context_t* context, const risltc_frame_constants_t* constants,
+                          const uint32_t* visibility, uint32_t batch, uint32_t steps,
+                          float learning_rate, uint32_t seed, float* loss_out, void* stream);
+/* The training records of the last risltc_shade (device outputs, may be
+ * NULL): per pixel-sample the cluster of the selected light (0xFFFFFFFF =
+ * none) and the luminance of its estimate; *count receives their number. */
+int risltc_proposal_records(risltc_context_t* context, uint32_t* cluster_out, float* y_out,
+                            uint64_t* count, void* stream);
+/* Milliseconds of the last risltc_proposal_update and risltc_proposal_train. */
+int risltc_proposal_last_timings(const risltc_context_t* context, float out_ms[2]);
+
 /* Milliseconds of the last risltc_shade call per kernel, measured with HIP
  * events on the call's stream: [0] shade, [1] shadow-ray trace, [2] resolve,
  * [3] learned-proposal inference. Entries not run are 0. */

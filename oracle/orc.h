@@ -143,11 +143,13 @@ typedef struct orc_scene_s {
     uint32_t ltc_res_roughness, ltc_res_inclination, ltc_fresnel_count;
     /* shadow rays */
     const orc_bvh_t* bvh;
-    /* learned proposal (NULL = uniform source, the reference) */
-    const float* proposal_pdf;      /* per pixel: cluster probabilities       */
+    /* learned proposal (project extension; read when settings.proposal is
+     * RISLTC_PROPOSAL_LEARNED): 32 bytes per pixel, 16 cluster ids then 16
+     * weights, as risltc_proposal_update writes them */
+    const uint8_t* proposal_table;
     const uint32_t* cluster_of_light;
-    const uint32_t* cluster_offsets; /* lights sorted by cluster: offsets     */
-    const uint32_t* cluster_lights;
+    const uint32_t* cluster_offsets; /* cluster_count + 1 prefix offsets      */
+    const uint32_t* cluster_lights;  /* light indices sorted by cluster       */
     uint32_t cluster_count;
 } orc_scene_t;
 

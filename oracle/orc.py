@@ -47,7 +47,7 @@ class OrcScene(C.Structure):
                 ("light_max_vertex_count", C.c_uint32), ("ltc_rgba16", u16p), ("ltc_rg16", u16p),
                 ("ltc_res_roughness", C.c_uint32), ("ltc_res_inclination", C.c_uint32),
                 ("ltc_fresnel_count", C.c_uint32), ("bvh", C.POINTER(OrcBvh)),
-                ("proposal_pdf", f32p), ("cluster_of_light", u32p), ("cluster_offsets", u32p),
+                ("proposal_table", u8p), ("cluster_of_light", u32p), ("cluster_offsets", u32p),
                 ("cluster_lights", u32p), ("cluster_count", C.c_uint32)]
 
 
@@ -186,6 +186,24 @@ class Oracle:
                 self.L.orc_bvh_free(C.byref(self.bvh))
         except Exception:
             pass
+
+    def set_proposal(self, table, cluster_of_light, cluster_count):
+        """Learned-source inputs: per pixel tables (pixels x 32 uint8, as
+        risltc_proposal_update writes them) and the light clusters; lights are
+        ordered by cluster, then by index (the library's rule)."""
+        self.prop_table = np.ascontiguousarray(table, dtype=np.uint8).reshape(-1)
+        col = np.ascontiguousarray(cluster_of_light, dtype=np.uint32)
+        self.prop_of = col
+        counts = np.bincount(col, minlength=cluster_count).astype(np.uint32)
+        self.prop_off = np.zeros(cluster_count + 1, dtype=np.uint32)
+        self.prop_off[1:] = np.cumsum(counts)
+        self.prop_lights = np.argsort(col, kind="stable").astype(np.uint32)
+        s = self.cscene
+        s.proposal_table = ptr(self.prop_table, u8p)
+        s.cluster_of_light = ptr(self.prop_of, u32p)
+        s.cluster_offsets = ptr(self.prop_off, u32p)
+        s.cluster_lights = ptr(self.prop_lights, u32p)
+        s.cluster_count = cluster_count
 
     def bvh_records(self):
         """Pre-order records of the oracle's tree, as the device lays it out."""

@@ -52,6 +52,16 @@ SIGNATURES = {
     "risltc_set_scene": (C.c_int, [_vp, _u32p, _u16p, _u8p, C.c_uint64, _f32p, _f32p,
                                    C.POINTER(abi.Material), _u32]),
     "risltc_set_bvh_builder": (C.c_int, [_vp, _u32]),
+    "risltc_set_light_clusters": (C.c_int, [_vp, _u32p, _u32]),
+    "risltc_cluster_lights": (C.c_int, [_vp, _u32, _u32, _u32p]),
+    "risltc_proposal_init": (C.c_int, [_vp, _u32]),
+    "risltc_proposal_set_params": (C.c_int, [_vp, _f32p]),
+    "risltc_proposal_get_params": (C.c_int, [_vp, _f32p, _f32p]),
+    "risltc_proposal_update": (C.c_int, [_vp, C.POINTER(abi.FrameConstants), _vp, _vp, _vp, _vp, _vp]),
+    "risltc_proposal_train": (C.c_int, [_vp, C.POINTER(abi.FrameConstants), _vp, _u32, _u32, C.c_float, _u32,
+                                        C.POINTER(C.c_float), _vp]),
+    "risltc_proposal_last_timings": (C.c_int, [_vp, C.POINTER(C.c_float * 2)]),
+    "risltc_proposal_records": (C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_uint64), _vp]),
     "risltc_bvh_records": (C.c_int, [_vp, _f32p, C.c_uint64, C.POINTER(C.c_uint32)]),
     "risltc_render_visibility": (C.c_int, [_vp, C.POINTER(abi.FrameConstants), _vp, _vp]),
     "risltc_shade": (C.c_int, [_vp, C.POINTER(abi.FrameConstants), C.POINTER(abi.Settings), _vp, _vp, _vp]),
@@ -191,6 +201,65 @@ class Context:
         self._check(self.L.risltc_bvh_records(self.h, _ptr(out, _f32p), units.value, C.byref(units)),
                     "risltc_bvh_records")
         return out
+
+    # ---- learned proposal ----------------------------------------------------
+    def set_light_clusters(self, cluster_of_light, cluster_count):
+        a = np.ascontiguousarray(cluster_of_light, dtype=np.uint32)
+        self._check(self.L.risltc_set_light_clusters(self.h, _ptr(a, _u32p), cluster_count),
+                    "risltc_set_light_clusters")
+
+    def cluster_lights(self, cluster_count=256, iterations=20):
+        """k-means of the lights; returns the assignment (numpy uint32)."""
+        n = self.scene.light_count if getattr(self, "scene", None) is not None else None
+        out = np.zeros(n if n is not None else 1 << 20, dtype=np.uint32)
+        self._check(self.L.risltc_cluster_lights(self.h, cluster_count, iterations, _ptr(out, _u32p)),
+                    "risltc_cluster_lights")
+        return out[:n] if n is not None else out
+
+    def proposal_init(self, seed=0):
+        self._check(self.L.risltc_proposal_init(self.h, seed), "risltc_proposal_init")
+
+    def proposal_set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == abi.PROPOSAL_PARAMS
+        self._check(self.L.risltc_proposal_set_params(self.h, _ptr(p, _f32p)), "risltc_proposal_set_params")
+
+    def proposal_get_params(self):
+        """(params, gradient of the last training step) as float32 arrays."""
+        p = np.zeros(abi.PROPOSAL_PARAMS, dtype=np.float32)
+        g = np.zeros(abi.PROPOSAL_PARAMS, dtype=np.float32)
+        self._check(self.L.risltc_proposal_get_params(self.h, _ptr(p, _f32p), _ptr(g, _f32p)),
+                    "risltc_proposal_get_params")
+        return p, g
+
+    def proposal_update(self, constants, visibility, table=None, logits=None, features=None, stream=None):
+        self._check(self.L.risltc_proposal_update(self.h, C.byref(constants), _dptr(visibility), _dptr(table),
+                                                  _dptr(logits), _dptr(features), _stream(stream)),
+                    "risltc_proposal_update")
+
+    def proposal_train(self, constants, visibility, batch=65536, steps=1, lr=1e-3, seed=0, stream=None):
+        loss = C.c_float(0.0)
+        self._check(self.L.risltc_proposal_train(self.h, C.byref(constants), _dptr(visibility), batch, steps,
+                                                 lr, seed, C.byref(loss), _stream(stream)),
+                    "risltc_proposal_train")
+        return loss.value
+
+    def proposal_records(self):
+        """(cluster, y) training records of the last shade, as torch tensors."""
+        import torch
+        n = C.c_uint64(0)
+        self._check(self.L.risltc_proposal_records(self.h, None, None, C.byref(n), None), "risltc_proposal_records")
+        cl = torch.zeros(max(n.value, 1), dtype=torch.int32, device="cuda")
+        y = torch.zeros(max(n.value, 1), dtype=torch.float32, device="cuda")
+        self._check(self.L.risltc_proposal_records(self.h, _dptr(cl), _dptr(y), C.byref(n), _stream(None)),
+                    "risltc_proposal_records")
+        torch.cuda.synchronize()
+        return cl[:n.value], y[:n.value]
+
+    def proposal_last_timings(self):
+        ms = (C.c_float * 2)()
+        self._check(self.L.risltc_proposal_last_timings(self.h, C.byref(ms)), "risltc_proposal_last_timings")
+        return list(ms)
 
     def load_scene(self, scene, ltc_rgba16, ltc_rg16):
         """Uploads a risltc_amd.synth.Scene and an LTC table."""

@@ -19,6 +19,8 @@ LIGHT_UNIFORM, LIGHT_RESERVOIR = 0, 1
 SAMPLE_POLYGON_LTC_CP = 4
 PROPOSAL_UNIFORM, PROPOSAL_LEARNED = 0, 1
 BVH_SAH, BVH_LBVH = 0, 1
+MAX_LIGHT_CLUSTERS = 256
+PROPOSAL_PARAMS = 31168
 
 # risltc_sample_record_t word offsets
 REC_LIGHT, REC_WSUM, REC_PHAT, REC_W, REC_CLIPD, REC_CLIPS, REC_PSAD, REC_PSAS = range(8)

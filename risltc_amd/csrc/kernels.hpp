@@ -62,6 +62,72 @@ constexpr uint64_t kMaxBvhTriangles = 400000000ull;
 // light bounding spheres are staged in LDS up to this many lights
 constexpr uint32_t kLdsSphereMax = 2048;
 
+// ---- learned RIS proposal ----------------------------------------------------------
+// MLP 96 -> 64 -> 64 -> 64 -> 256 over light clusters. Parameters are one FP32
+// array; every weight matrix is stored [out][in], followed by its bias.
+constexpr int kMlpIn = 96, kMlpHid = 64, kMlpOut = 256;
+constexpr int kPW1 = 0, kPB1 = kPW1 + kMlpHid * kMlpIn;
+constexpr int kPW2 = kPB1 + kMlpHid, kPB2 = kPW2 + kMlpHid * kMlpHid;
+constexpr int kPW3 = kPB2 + kMlpHid, kPB3 = kPW3 + kMlpHid * kMlpHid;
+constexpr int kPW4 = kPB3 + kMlpHid, kPB4 = kPW4 + kMlpOut * kMlpHid;
+constexpr int kMlpParams = kPB4 + kMlpOut;  // 31,168
+// per pixel proposal table: 16 cluster ids (u8) then 16 weights (u8); the
+// MLP component picks cluster ids[s] with probability w[s] / sum(w)
+constexpr int kPropTop = 16, kPropBytes = 32;
+
+struct ClusterView {
+    const uint8_t* of_light;   // cluster of each light
+    const uint32_t* offsets;   // count + 1 prefix offsets into lights
+    const uint32_t* lights;    // light indices sorted by cluster
+    uint32_t count;
+};
+
+// inputs of the feature encoder
+struct FeatureView {
+    risltc_frame_constants_t c;
+    SceneView scene;
+    const uint32_t* visibility;
+    float center[3], inv_half[3];  // scene box: position -> [-1, 1]^3
+};
+
+struct ProposalArgs {
+    FeatureView f;
+    const uint16_t* wf;   // bf16 weights, master layout
+    const float* params;  // FP32 master (biases)
+    const uint32_t* cluster_offsets;
+    uint32_t cluster_count;
+    uint32_t pixel_count;
+    uint8_t* out;         // kPropBytes per pixel
+    float* logits;        // optional: kMlpOut per pixel
+    uint16_t* features;   // optional: kMlpIn bf16 per pixel
+};
+hipError_t launch_proposal(const ProposalArgs& a, hipStream_t s);
+
+// one optimisation step on a batch of (pixel, cluster, weight) items
+struct TrainArgs {
+    FeatureView f;
+    uint32_t B;                    // batch size, multiple of 256
+    uint32_t sample_count;         // pixel-samples in the training records
+    uint32_t spp;
+    uint32_t seed;                 // item selection
+    const uint32_t* rec_cluster;   // per pixel-sample: cluster of the selected light or ~0
+    const float* rec_y;            // per pixel-sample: luminance of its estimate
+    float lr, beta1, beta2, eps;
+    uint32_t step;                 // 1-based Adam step
+    float* params;                 // FP32 master
+    float* m;
+    float* v;
+    float* grad;                   // out: the step's gradient
+    uint16_t* wf;                  // bf16 [out][in]
+    uint16_t* wb;                  // bf16 [in][out]
+    void* scratch;                 // train_scratch_bytes(B)
+    float* loss;                   // out: batch loss (device, one float)
+};
+size_t train_scratch_bytes(uint32_t B);
+hipError_t launch_train_step(const TrainArgs& a, hipStream_t s);
+// bf16 copies of the master parameters (after set_params)
+hipError_t launch_mlp_sync(const float* params, uint16_t* wf, uint16_t* wb, hipStream_t s);
+
 struct ShadeArgs {
     risltc_frame_constants_t c;
     uint32_t lds_spheres;      // 1: the kernel stages the spheres in LDS
@@ -75,12 +141,9 @@ struct ShadeArgs {
     uint32_t* ray_count;       // device counter
     float* debug_samples;      // RISLTC_SAMPLE_RECORD_FLOATS per pixel-sample or null
     uint32_t* debug_cands;     // RISLTC_CANDIDATE_RECORD_WORDS per candidate or null
-    // learned proposal (null = uniform source)
-    const uint8_t* proposal;   // per pixel: cluster_count quantised probabilities
-    const uint32_t* proposal_sum;  // per pixel: sum of the quantised probabilities
-    const uint32_t* cluster_offsets;
-    const uint32_t* cluster_lights;
-    uint32_t cluster_count;
+    // learned proposal (null = the reference's uniform source)
+    const uint8_t* proposal;   // kPropBytes per pixel
+    ClusterView clusters;
 };
 
 // The shadow-ray list is cut into kTraceQueues equal slices, one per XCD:
@@ -110,6 +173,11 @@ struct ResolveArgs {
     const uint32_t* hits;
     float* out;            // RGBA32F
     float* debug_samples;  // when set: ray outcomes and colours are written back
+    // training records (optional): per pixel-sample cluster of the selected
+    // light (~0 = none) and the luminance of its estimate
+    const uint8_t* cluster_of_light;
+    uint32_t* rec_cluster;
+    float* rec_y;
 };
 
 // host-side launchers (shade.hip, lbvh.hip)
