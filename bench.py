@@ -9,7 +9,11 @@ resident in HBM. value = samples per second of the whole pass.
 Prints one JSON line. The CPU baseline is the FP32 oracle (OpenMP, the
 host's cores) on a bounded sample of the same frame.
 
-  python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu]
+The learned_vs_plain object is the second half of the metric: GPU time for
+the plain pass and for the pass with the learned target to reach relMSE 0.01
+against an 8192-spp plain reference (risltc_amd/convergence.py).
+
+  python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-learned]
 """
 import argparse
 import json
@@ -48,6 +52,8 @@ def main():
     ap.add_argument("--exact", action="store_true", help="exact-math kernels (parity build)")
     ap.add_argument("--cpu-pixels", type=int, default=0,
                     help="pixels in the CPU-baseline sample (0 = the whole frame)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-vs-plain time-to-relMSE experiment")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (known HBM bytes) for counter scaling")
     args = ap.parse_args()
@@ -124,6 +130,13 @@ def main():
     print("kernels (ms): shade %.3f trace %.3f resolve %.3f; frame mean %s" %
           (kernel_ms[0], kernel_ms[1], kernel_ms[2], frame[:, :3].mean(0)), file=sys.stderr, flush=True)
 
+    learned = None
+    if not args.no_learned and args.workload == "city" and not args.exact:
+        from risltc_amd import convergence
+        t1 = time.time()
+        learned = convergence.compare(ctx, scene, vis)
+        learned["wall_s"] = round(time.time() - t1, 1)
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -169,6 +182,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "kernel": "shade_kernel", "algorithmic_bytes": alg},
         "valu": valu,
+        "learned_vs_plain": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

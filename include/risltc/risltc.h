@@ -184,7 +184,9 @@ typedef struct risltc_settings_s {
     uint32_t proposal;                   /* RISLTC_PROPOSAL_*                     */
     uint32_t exact_math;                 /* 1 = IEEE div/sqrt everywhere: results
                                             bit-identical to the FP32 oracle      */
-    uint32_t reserved[6];
+    uint32_t learned_floor_log2;         /* learned target: p_hat (f + (1 - f) v / 255),
+                                            f = 2^-learned_floor_log2 (0 -> 3)    */
+    uint32_t reserved[5];
 } risltc_settings_t;
 
 /* Fills settings with the reference defaults (src/main.c:220-236): SAMPLE_COUNT
@@ -360,10 +362,15 @@ int risltc_shade_debug(risltc_context_t* context, const risltc_frame_constants_t
                        void* stream);
 
 /* ---- learned RIS proposal (project extension, no reference counterpart:
- * the reference draws RIS candidates uniformly, shading_pass.frag.glsl:730-733).
- * With settings.proposal = RISLTC_PROPOSAL_LEARNED the candidates of
- * risltc_shade come from q = 1/(2N) + (MLP cluster proposal)/2, the RIS
- * weights use that q, and the estimator stays unbiased. Sequence per frame:
+ * the reference's RIS target is the unshadowed LTC estimate,
+ * shading_pass.frag.glsl:723-736). An MLP maps each shading point to one
+ * number per light cluster, v in [0, 255]: the learned probability that a
+ * shadow ray toward the cluster is unoccluded. With settings.proposal =
+ * RISLTC_PROPOSAL_LEARNED, risltc_shade draws candidates as the reference
+ * does and uses the target p_hat * (f + (1 - f) * v / 255), f = 1/8 by
+ * default (settings.learned_floor_log2): the LTC target mixed defensively
+ * with the learned visibility, positive wherever the LTC target is, so the
+ * estimator stays unbiased. Sequence per frame:
  * risltc_proposal_update (inference), risltc_shade (learned), then
  * optionally risltc_proposal_train on the frame's own samples. ---------- */
 #define RISLTC_MAX_LIGHT_CLUSTERS 256
@@ -384,22 +391,24 @@ int risltc_proposal_init(risltc_context_t* context, uint32_t seed);
 int risltc_proposal_set_params(risltc_context_t* context, const float* params);
 int risltc_proposal_get_params(risltc_context_t* context, float* params, float* gradient);
 /* Inference for every pixel of the frame: writes the per-pixel tables the
- * learned shade reads (32 bytes: 16 cluster ids, 16 weights). Optional
- * device outputs: a copy of the tables, the 256 logits and the 96 bf16
- * features per pixel. */
+ * learned shade reads (256 bytes, round(255 sigmoid(logit)) per cluster, 0
+ * for background). Optional device outputs: a copy of the tables, the 256
+ * logits and the 96 bf16 features per pixel. */
 int risltc_proposal_update(risltc_context_t* context, const risltc_frame_constants_t* constants,
                            const uint32_t* visibility, uint8_t* table_out, float* logits_out,
                            uint16_t* features_out, void* stream);
 /* Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) on batches of pixel-samples of
- * the last risltc_shade: target = cluster of the selected light, weight =
- * luminance of its estimate (cross-entropy against the softmax). batch is a
- * multiple of 256. loss_out (host, may be NULL): the last step's loss. */
+ * the last risltc_shade: binary cross-entropy of the sigmoid of the selected
+ * light's cluster against the fraction of the sample's traced shadow rays
+ * that were unoccluded. batch is a multiple of 256. loss_out (host, may be
+ * NULL): the last step's loss. */
 int risltc_proposal_train(risltc_context_t* context, const risltc_frame_constants_t* constants,
                           const uint32_t* visibility, uint32_t batch, uint32_t steps,
                           float learning_rate, uint32_t seed, float* loss_out, void* stream);
 /* The training records of the last risltc_shade (device outputs, may be
  * NULL): per pixel-sample the cluster of the selected light (0xFFFFFFFF =
- * none) and the luminance of its estimate; *count receives their number. */
+ * no traced shadow ray) and the fraction of its traced shadow rays that were
+ * unoccluded; *count receives their number. */
 int risltc_proposal_records(risltc_context_t* context, uint32_t* cluster_out, float* y_out,
                             uint64_t* count, void* stream);
 /* Milliseconds of the last risltc_proposal_update and risltc_proposal_train. */

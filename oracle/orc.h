@@ -143,9 +143,9 @@ typedef struct orc_scene_s {
     uint32_t ltc_res_roughness, ltc_res_inclination, ltc_fresnel_count;
     /* shadow rays */
     const orc_bvh_t* bvh;
-    /* learned proposal (project extension; read when settings.proposal is
-     * RISLTC_PROPOSAL_LEARNED): 32 bytes per pixel, 16 cluster ids then 16
-     * weights, as risltc_proposal_update writes them */
+    /* learned RIS target (project extension; read when settings.proposal is
+     * RISLTC_PROPOSAL_LEARNED): 256 bytes per pixel, one per cluster, as
+     * risltc_proposal_update writes them */
     const uint8_t* proposal_table;
     const uint32_t* cluster_of_light;
     const uint32_t* cluster_offsets; /* cluster_count + 1 prefix offsets      */

@@ -75,7 +75,8 @@ class Settings(C.Structure):
                 ("polygon_sampling_technique", C.c_uint32),
                 ("ris_candidate_count", C.c_uint32), ("fast_atan", C.c_uint32),
                 ("shadow_rays", C.c_uint32), ("proposal", C.c_uint32),
-                ("exact_math", C.c_uint32), ("reserved", C.c_uint32 * 6)]
+                ("exact_math", C.c_uint32), ("learned_floor_log2", C.c_uint32),
+                ("reserved", C.c_uint32 * 5)]
 
 
 def default_settings(**kw):

@@ -71,9 +71,15 @@ constexpr int kPW2 = kPB1 + kMlpHid, kPB2 = kPW2 + kMlpHid * kMlpHid;
 constexpr int kPW3 = kPB2 + kMlpHid, kPB3 = kPW3 + kMlpHid * kMlpHid;
 constexpr int kPW4 = kPB3 + kMlpHid, kPB4 = kPW4 + kMlpOut * kMlpHid;
 constexpr int kMlpParams = kPB4 + kMlpOut;  // 31,168
-// per pixel proposal table: 16 cluster ids (u8) then 16 weights (u8); the
-// MLP component picks cluster ids[s] with probability w[s] / sum(w)
-constexpr int kPropTop = 16, kPropBytes = 32;
+// per pixel proposal table: one byte per cluster, v = round(255 sigmoid(z)),
+// the learned probability that a shadow ray toward the cluster is unoccluded.
+// The learned RIS target is p_hat * (floor + v * scale), scale = (1 - floor)
+// / 255: the LTC target mixed with the visibility estimate, never below
+// p_hat * floor.
+constexpr int kPropBytes = kMlpOut;
+// default floor 1/8: scale = (1 - 0.125) / 255 in FP32
+constexpr float kLearnedFloor = 0.125f;
+constexpr float kLearnedScale = 0x1.c1c1c2p-9f;
 
 struct ClusterView {
     const uint8_t* of_light;   // cluster of each light
@@ -111,7 +117,7 @@ struct TrainArgs {
     uint32_t spp;
     uint32_t seed;                 // item selection
     const uint32_t* rec_cluster;   // per pixel-sample: cluster of the selected light or ~0
-    const float* rec_y;            // per pixel-sample: luminance of its estimate
+    const float* rec_y;            // per pixel-sample: fraction of its shadow rays unoccluded
     float lr, beta1, beta2, eps;
     uint32_t step;                 // 1-based Adam step
     float* params;                 // FP32 master
@@ -141,9 +147,10 @@ struct ShadeArgs {
     uint32_t* ray_count;       // device counter
     float* debug_samples;      // RISLTC_SAMPLE_RECORD_FLOATS per pixel-sample or null
     uint32_t* debug_cands;     // RISLTC_CANDIDATE_RECORD_WORDS per candidate or null
-    // learned proposal (null = the reference's uniform source)
+    // learned target (null = the reference's LTC target)
     const uint8_t* proposal;   // kPropBytes per pixel
     ClusterView clusters;
+    float learned_floor, learned_scale;  // f and (1 - f) / 255 (FP32)
 };
 
 // The shadow-ray list is cut into kTraceQueues equal slices, one per XCD:
@@ -174,7 +181,8 @@ struct ResolveArgs {
     float* out;            // RGBA32F
     float* debug_samples;  // when set: ray outcomes and colours are written back
     // training records (optional): per pixel-sample cluster of the selected
-    // light (~0 = none) and the luminance of its estimate
+    // light (~0 = no traced shadow ray) and the fraction of its traced shadow
+    // rays that were unoccluded
     const uint8_t* cluster_of_light;
     uint32_t* rec_cluster;
     float* rec_y;

@@ -1049,76 +1049,15 @@ __device__ __forceinline__ uint32_t lcg_jump(uint32_t s, uint32_t d) {
     return s;
 }
 
-// ---- learned proposal: candidate draws ------------------------------------------
-// Per pixel table (kPropBytes): cluster ids in bytes 0..15, weights in 16..31,
-// T = sum of the weights. A candidate takes two draws a, b:
-//   T == 0        -> light_index_from(a, N);
-//   a < 1/2       -> light_index_from(2 a, N)          (the uniform half);
-//   otherwise     -> t = min(T - 1, int((2 a - 1) T)), slot s = first with
-//                    t < w_0 + .. + w_s, cluster c = id_s, light = the
-//                    min(size - 1, int(b size))-th light of c (an empty
-//                    cluster, or an id past the clusters, falls back to
-//                    light_index_from(b, N)).
-// Its source density is q = 1/(2N) + w_c / (2 T |c|) (w_c = 0 for clusters
-// not in the table), computed in the same operation order by the oracle.
-struct PropTable {
-    uint32_t ids[4], ws[4];
-    uint32_t T;
-};
-
-__device__ __forceinline__ uint32_t byte_of(const uint32_t* v, int s) { return (v[s >> 2] >> (8 * (s & 3))) & 0xFFu; }
-
-struct ClusterLds {
-    const uint8_t* of_light;
-    const uint32_t* offsets;
-    const uint32_t* lights;
-    uint32_t count;
-};
-
-__device__ __forceinline__ uint32_t draw_learned(uint32_t& seed, const PropTable& P, const ClusterLds& C, uint32_t N) {
-    float a = noise_next(seed);
-    float b = noise_next(seed);
-    if (P.T == 0u) return light_index_from(a, N);
-    if (a < 0.5f) return light_index_from(a + a, N);
-    float u = (a + a) - 1.0f;
-    uint32_t t = (uint32_t)(u * (float)P.T);
-    if (t > P.T - 1u) t = P.T - 1u;
-    uint32_t acc = 0, c = 0;
-    bool found = false;
-#pragma unroll
-    for (int s = 0; s < kPropTop; ++s) {
-        uint32_t w = byte_of(P.ws, s);
-        if (!found && t < acc + w) {
-            c = byte_of(P.ids, s);
-            found = true;
-        }
-        acc += w;
-    }
-    // a cluster id outside the clusters or an empty cluster: uniform
-    if (c >= C.count) return light_index_from(b, N);
-    uint32_t off = C.offsets[c], size = C.offsets[c + 1] - off;
-    if (size == 0u) return light_index_from(b, N);
-    uint32_t j = (uint32_t)(b * (float)size);
-    if (j > size - 1u) j = size - 1u;
-    return C.lights[off + j];
-}
-
-template <bool X>
-__device__ __forceinline__ float learned_density(uint32_t idx, const PropTable& P, const ClusterLds& C, uint32_t N) {
-    float q = 0.5f / (float)N;
-    if (P.T == 0u) return 1.0f / (float)N;
-    uint32_t c = C.of_light[idx];
-    uint32_t wc = 0;
-#pragma unroll
-    for (int s = kPropTop - 1; s >= 0; --s) {
-        uint32_t w = byte_of(P.ws, s);
-        if (w != 0u && byte_of(P.ids, s) == c) wc = w;
-    }
-    if (wc != 0u) {
-        uint32_t size = C.offsets[c + 1] - C.offsets[c];
-        q = q + div_<X>(0.5f * (float)wc, (float)P.T * (float)size);
-    }
-    return q;
+// ---- learned RIS target ----------------------------------------------------------
+// With a proposal table the candidates still come from the reference's
+// uniform source; the target of candidate light l becomes
+// p_hat * (floor + v * scale), v = the table's byte for the cluster of l,
+// scale = (1 - floor) / 255. The oracle evaluates the same expression.
+__device__ __forceinline__ float learned_factor(const uint8_t* table, const uint8_t* of_light, uint32_t idx,
+                                                float floor, float scale) {
+    float v = (float)table[of_light[idx]];
+    return floor + v * scale;
 }
 
 // ---- the kernel ----------------------------------------------------------------
@@ -1126,21 +1065,15 @@ template <bool X, bool DEBUG, bool LEARNED>
 __global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
     extern __shared__ float4 sh_spheres[];
     constexpr bool learned = LEARNED;
-    ClusterLds cl{A.clusters.of_light, A.clusters.offsets, A.clusters.lights, A.clusters.count};
+    const uint8_t* of_light = A.clusters.of_light;
     if (A.lds_spheres) {
         const uint32_t n = A.scene.light_count;
         for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) sh_spheres[k] = A.scene.spheres[k];
         if constexpr (learned) {
-            // after the spheres: lights by cluster (u32), offsets (u32), cluster of light (u8)
-            uint32_t* l = reinterpret_cast<uint32_t*>(sh_spheres + n);
-            uint32_t* o = l + n;
-            uint8_t* c = reinterpret_cast<uint8_t*>(o + A.clusters.count + 1u);
-            for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
-                l[k] = A.clusters.lights[k];
-                c[k] = A.clusters.of_light[k];
-            }
-            for (uint32_t k = threadIdx.x; k <= A.clusters.count; k += blockDim.x) o[k] = A.clusters.offsets[k];
-            cl = ClusterLds{c, o, l, A.clusters.count};
+            // after the spheres: the cluster of every light
+            uint8_t* c = reinterpret_cast<uint8_t*>(sh_spheres + n);
+            for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) c[k] = A.clusters.of_light[k];
+            of_light = c;
         }
         __syncthreads();
     }
@@ -1174,23 +1107,7 @@ __global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
     const uint32_t N = A.scene.light_count;
     const float pN = 1.0f / (float)N;
     const uint32_t M = A.M;
-    // noise draws per candidate: light (1, or 2 for the learned source) + reservoir
-    const uint32_t stride = learned ? 3u : 2u;
     for (uint32_t j = 0; j < spp; ++j) {
-        // the pixel's proposal table, loaded per sample so that it is not
-        // live across the final sample
-        PropTable pt;
-        pt.T = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) pt.ids[k] = pt.ws[k] = 0u;
-        if (learned && shade) {
-            const uint4* t = reinterpret_cast<const uint4*>(A.proposal + (size_t)pixel * kPropBytes);
-            uint4 t0 = t[0], t1 = t[1];
-            pt.ids[0] = t0.x, pt.ids[1] = t0.y, pt.ids[2] = t0.z, pt.ids[3] = t0.w;
-            pt.ws[0] = t1.x, pt.ws[1] = t1.y, pt.ws[2] = t1.z, pt.ws[3] = t1.w;
-#pragma unroll
-            for (int s2 = 0; s2 < kPropTop; ++s2) pt.T += byte_of(pt.ws, s2);
-        }
         int32_t chosen = -1;
         float w_sum = 0.0f, sample_value = 0.0f;
         Light L;
@@ -1208,7 +1125,7 @@ __global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
                     const uint32_t seed_chunk = seed;
                     uint32_t live = 0;
                     for (uint32_t t = 0; t < cnt; ++t) {
-                        uint32_t idx = learned ? draw_learned(seed, pt, cl, N) : light_index_from(noise_next(seed), N);
+                        uint32_t idx = light_index_from(noise_next(seed), N);
                         (void)noise_next(seed);
                         float4 sp = A.lds_spheres ? sh_spheres[idx] : A.scene.spheres[idx];
                         bool cut = culled(cull, sp);
@@ -1228,14 +1145,17 @@ __global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
                     while (live) {
                         const uint32_t t = (uint32_t)__builtin_ctz(live);
                         live &= live - 1u;
-                        s = lcg_jump(s, stride * (t - at));
+                        s = lcg_jump(s, 2u * (t - at));
                         at = t + 1u;
-                        uint32_t idx = learned ? draw_learned(s, pt, cl, N) : light_index_from(noise_next(s), N);
+                        uint32_t idx = light_index_from(noise_next(s), N);
                         load_light(A.scene.lights, idx, L);
                         uint32_t cc;
                         f3 col = eval_ltc_cp<X>(px, L, cc);
                         float p_hat = sqrt_<X>(dot3(col, col));
-                        float w = div_<X>(p_hat, learned ? learned_density<X>(idx, pt, cl, N) : pN);
+                        if constexpr (learned)
+                            p_hat = p_hat * learned_factor(A.proposal + (size_t)pixel * kPropBytes, of_light, idx,
+                                                           A.learned_floor, A.learned_scale);
+                        float w = div_<X>(p_hat, pN);
                         float r = noise_next(s);
                         w_sum += w;
                         // exact: the reference's r < w / w_sum; fast: the
@@ -1378,11 +1298,14 @@ __global__ void __launch_bounds__(256) resolve_kernel(ResolveArgs A) {
                 f3 eo[2] = {mk3(r2.x, r2.y, r2.z), mk3(r2.w, r3.x, r3.y)};
                 f3 result = mk3(0.0f, 0.0f, 0.0f);
                 float* dbg = A.debug_samples ? A.debug_samples + sidx * RISLTC_SAMPLE_RECORD_FLOATS : nullptr;
+                uint32_t traced = 0, unoccluded = 0;
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
                     uint32_t st = (flags >> (4 * t)) & 15u;
                     if (st == 0u) continue;
                     bool hit = st == 5u && A.hits[sidx * 2 + t] != 0u;
+                    traced += st == 5u ? 1u : 0u;
+                    unoccluded += (st == 5u && !hit) ? 1u : 0u;
                     bool vis = st == 4u || (st == 5u && !hit);
                     result = add3(result, vis ? ev[t] : eo[t]);
                     if (dbg && st == 5u) dbg[16 + t] = __uint_as_float(hit ? 2u : 3u);
@@ -1398,10 +1321,9 @@ __global__ void __launch_bounds__(256) resolve_kernel(ResolveArgs A) {
                 fin = add3(fin, mk3((col.x * Wt) / s, (col.y * Wt) / s, (col.z * Wt) / s));
                 if (A.rec_cluster) {
                     int32_t li = __float_as_int(r3.z);
-                    float y = (0.2126f * col.x + 0.7152f * col.y + 0.0722f * col.z) * Wt;
-                    bool ok = li >= 0 && y > 0.0f && y < 3.0e38f;
+                    bool ok = li >= 0 && traced > 0u;
                     A.rec_cluster[sidx] = ok ? (uint32_t)A.cluster_of_light[li] : 0xFFFFFFFFu;
-                    A.rec_y[sidx] = ok ? y : 0.0f;
+                    A.rec_y[sidx] = ok ? (float)unoccluded / (float)traced : 0.0f;
                 }
             }
         }
@@ -1427,8 +1349,7 @@ hipError_t launch_shade(const ShadeArgs& a, bool exact, hipStream_t s) {
     dim3 grid(((W + 15u) / 16u) * ((H + 15u) / 16u)), block(256);
     bool dbg = a.debug_samples != nullptr || a.debug_cands != nullptr;
     size_t lds = a.lds_spheres ? (size_t)a.scene.light_count * sizeof(float4) : 0;
-    if (a.lds_spheres && a.proposal)
-        lds += (size_t)a.scene.light_count * 5 + (size_t)(a.clusters.count + 1) * 4;
+    if (a.lds_spheres && a.proposal) lds += (size_t)a.scene.light_count;
     if (a.lds_spheres && a.scene.light_count > kLdsSphereMax) return hipErrorInvalidValue;
     if (a.proposal) {
         if (exact) {

@@ -121,85 +121,58 @@ def test_estimator_is_unbiased(orc_lib, ltc_table, roughness, metal):
         assert np.all(np.abs(est[p] - ref) < 5 * se[p] + 2e-3 * ref), (p, est[p], ref, se[p])
 
 
-# ---------------------------------------------------------------- learned source
-def _random_tables(rng, P, K, sizes):
-    """Per-pixel tables: 16 distinct non-empty clusters with random 8-bit weights
-    (some zero, as the proposal kernel writes them), on some pixels all zero."""
-    nonempty = np.flatnonzero(sizes > 0)
-    tab = np.zeros((P, 32), np.uint8)
-    for p in range(P):
-        if p % 7 == 3:
-            continue  # T = 0: uniform fallback
-        ids = rng.choice(nonempty, 16, replace=False)
-        w = rng.integers(0, 256, 16)
-        w[rng.random(16) < 0.2] = 0
-        w[0] = max(w[0], 1)
-        ids[w == 0] = 0
-        tab[p, :16] = ids
-        tab[p, 16:] = w
-    return tab
+# ---------------------------------------------------------------- learned target
+SCALE = np.float32(float.fromhex("0x1.c1c1c2p-9"))
 
 
-def _q_numpy(tab, idx, col, sizes, N):
-    """q = 1/(2N) + w_c / (2 T |c|) in float32 with the kernel's operation order."""
-    f = np.float32
-    T = int(tab[16:].astype(np.int64).sum())
-    if T == 0:
-        return f(1.0) / f(N)
-    q = f(0.5) / f(N)
-    c = col[idx]
-    for s in range(16):
-        if tab[16 + s] != 0 and tab[s] == c:
-            return q + (f(0.5) * f(tab[16 + s])) / (f(T) * f(sizes[c]))
-    return q
+def _factor(v):
+    return np.float32(0.125) + np.float32(v) * SCALE
 
 
-def test_learned_draw_density(small_oracle, small_scene):
-    """With the learned source: every candidate's weight is exactly p_hat / q
-    with q from the table, and the candidate lights are distributed as q."""
+def test_learned_target_factor_constant():
+    assert SCALE == np.float32(1.0 - 0.125) / np.float32(255.0)
+    assert _factor(255) <= np.float32(1.0) and _factor(0) == np.float32(0.125)
+
+
+def test_learned_target_scales_p_hat(small_oracle, small_scene):
+    """With a table, the candidates are the plain ones (same noise stream) and
+    each candidate's target is exactly p_hat * (1/8 + v * 7/8 / 255) with v
+    the table byte of the light's cluster; w = target / (1/N)."""
     o = small_oracle
     rng = np.random.default_rng(31)
     N = small_scene.light_count
     K = 16
     col = (np.arange(N) * 7 % K).astype(np.uint32)
-    sizes = np.bincount(col, minlength=K)
     P = small_scene.width * small_scene.height
-    tab = _random_tables(rng, P, K, sizes)
+    tab = rng.integers(0, 256, (P, 256)).astype(np.uint8)
     o.set_proposal(tab, col, K)
     consts = o.frame_constants()
     vis = o.render_visibility(consts)
-    shaded = np.flatnonzero((vis >> 31) == 0)
-    pix = shaded[::23].astype(np.uint32)
-    s = small_scene.settings(sample_count_light=4, ris_candidate_count=8, shadow_rays=0,
-                             proposal=abi.PROPOSAL_LEARNED)
-    _, _, cand = o.shade(consts, s, vis, pixels=pix, candidates=True)
-    idx = cand[..., 0]
-    phat = cand[..., 2].view(np.float32)
-    w = cand[..., 3].view(np.float32)
-    counts = np.zeros(N)
-    expect = np.zeros(N)
-    for k, p in enumerate(pix):
-        qs = np.array([_q_numpy(tab[p], l, col, sizes, N) for l in range(N)], np.float32)
-        assert abs(qs.astype(np.float64).sum() - 1.0) < 1e-5
-        q = qs[idx[k]]
-        assert np.array_equal(w[k], (phat[k] / q).astype(np.float32)), k
-        counts += np.bincount(idx[k].ravel(), minlength=N)
-        expect += qs.astype(np.float64) * idx[k].size
-    chi2 = ((counts - expect) ** 2 / expect).sum()
-    assert chi2 < N + 6 * math.sqrt(2 * N), chi2
+    pix = np.flatnonzero((vis >> 31) == 0)[::17].astype(np.uint32)
+    # one sample per pixel: after the first final sample, the noise streams of
+    # the two runs may diverge (its draw count depends on the chosen light)
+    base = small_scene.settings(sample_count_light=1, ris_candidate_count=8, shadow_rays=0)
+    _, _, plain = o.shade(consts, base, vis, pixels=pix, candidates=True)
+    base.proposal = abi.PROPOSAL_LEARNED
+    _, _, learned = o.shade(consts, base, vis, pixels=pix, candidates=True)
+    assert np.array_equal(plain[..., 0], learned[..., 0]) and np.array_equal(plain[..., 1], learned[..., 1])
+    ph = plain[..., 2].view(np.float32)
+    fac = _factor(tab[pix[:, None, None], col[plain[..., 0]]].astype(np.float32))
+    want = (ph * fac).astype(np.float32)
+    assert np.array_equal(learned[..., 2].view(np.float32), want)
+    assert np.array_equal(learned[..., 3].view(np.float32), (want / (np.float32(1) / np.float32(N))).astype(np.float32))
 
 
 def test_learned_estimator_is_unbiased(orc_lib, ltc_table):
-    """Mean of the learned-source estimator over many noise seeds equals the
-    plain (uniform-source) mean on the same pixels, shadows off."""
+    """Mean of the estimator with a learned target over many noise seeds equals
+    the plain mean on the same pixels (any positive target is unbiased)."""
     scene = synth.small_scene(seed=3, width=48, height=48, light_count=24)
     o = orc.Oracle(scene, *ltc_table, bvh=False)
     rng = np.random.default_rng(32)
     N, K = scene.light_count, 6
     col = (np.arange(N) % K).astype(np.uint32)
-    sizes = np.bincount(col, minlength=K)
     P = scene.width * scene.height
-    o.set_proposal(_random_tables_small(rng, P, K, sizes), col, K)
+    o.set_proposal(rng.integers(0, 256, (P, 256)).astype(np.uint8), col, K)
     consts = o.frame_constants()
     o2 = orc.Oracle(scene, *ltc_table)
     vis = o2.render_visibility(consts)
@@ -218,13 +191,3 @@ def test_learned_estimator_is_unbiased(orc_lib, ltc_table):
     (mu_u, se_u), (mu_l, se_l) = means[abi.PROPOSAL_UNIFORM], means[abi.PROPOSAL_LEARNED]
     se = np.sqrt(se_u ** 2 + se_l ** 2)
     assert np.all(np.abs(mu_u - mu_l) < 5 * se + 1e-4 * np.abs(mu_u) + 1e-7), (mu_u, mu_l, se)
-
-
-def _random_tables_small(rng, P, K, sizes):
-    tab = np.zeros((P, 32), np.uint8)
-    for p in range(P):
-        ids = rng.choice(np.flatnonzero(sizes > 0), min(K, 16), replace=False)
-        w = rng.integers(1, 256, ids.size)
-        tab[p, :ids.size] = ids
-        tab[p, 16:16 + ids.size] = w
-    return tab
