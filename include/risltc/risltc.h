@@ -419,6 +419,73 @@ int risltc_proposal_last_timings(const risltc_context_t* context, float out_ms[2
  * [3] learned-proposal inference. Entries not run are 0. */
 int risltc_last_timings(const risltc_context_t* context, float out_ms[4]);
 
+/* ------------------------------------------------------------------------ */
+/* Batches of views. No counterpart in the reference, which shades one view
+ * per frame (main.c:2050-2055); a batch shades several views (cameras,
+ * resolutions, settings, light sets) of one scene in one launch of each
+ * kernel, and a graph replays such a batch as one hipGraph. Every view's
+ * frame is bitwise equal to a risltc_shade call with the same inputs. */
+
+/* A further light set (records as for risltc_set_lights) that views may
+ * name; *out_id receives its id (>= 1). Id 0 names the lights of
+ * risltc_set_lights. Light sets live until risltc_destroy. */
+int risltc_add_light_set(risltc_context_t* context, const void* records, uint32_t count,
+                         uint32_t max_vertex_count, uint32_t* out_id);
+
+typedef struct risltc_view_s {
+    risltc_frame_constants_t constants;
+    risltc_settings_t settings;   /* proposal must be RISLTC_PROPOSAL_UNIFORM;
+                                     exact_math equal across a batch */
+    const uint32_t* visibility;   /* device, R32_UINT per pixel */
+    float* out;                   /* device, RGBA32F per pixel */
+    uint32_t light_set;           /* 0 or an id from risltc_add_light_set */
+    uint32_t reserved[3];
+} risltc_view_t;
+
+/* Shades count views (device pointers in each view) on stream. */
+int risltc_shade_batch(risltc_context_t* context, const risltc_view_t* views, uint32_t count, void* stream);
+
+/* Launch shape of the kernels. None of these changes any result bit.
+ *   wave_width   8, 16, 32 or 64: pixels per row of a wave's tile (a wave
+ *                shades wave_width x 64/wave_width pixels);
+ *   cull_chunk   1..32: candidates tested against the bounding spheres per
+ *                step of the cull loop;
+ *   lds_lights   0..2048: bounding spheres staged in LDS per work-group
+ *                (the rest are read from global memory);
+ *   trace_refill 1..64: idle lanes at which a wave of the persistent
+ *                shadow-ray kernel fetches new rays. */
+typedef struct risltc_tuning_s {
+    uint32_t wave_width;
+    uint32_t cull_chunk;
+    uint32_t lds_lights;
+    uint32_t trace_refill;
+} risltc_tuning_t;
+
+int risltc_get_tuning(const risltc_context_t* context, risltc_tuning_t* out);
+int risltc_set_tuning(risltc_context_t* context, const risltc_tuning_t* tuning);
+/* Coordinate descent over the tuning space on a batch: each parameter in turn
+ * takes the value with the least median GPU time over `repeats` runs of the
+ * batch, the others held; two sweeps. Leaves the best tuning set on the
+ * context and copies it to *out (may be NULL). Synchronous: the views' output
+ * buffers are overwritten. */
+int risltc_autotune(risltc_context_t* context, const risltc_view_t* views, uint32_t count, uint32_t repeats,
+                    risltc_tuning_t* out);
+
+/* A batch captured as a hipGraph with buffers of its own. risltc_graph_update
+ * takes new views of the same shapes (resolutions, sample counts, shadow-ray
+ * and exact_math flags; other fields, pointers and light sets may change)
+ * and waits for the graph's previous launch. A graph holds device pointers of
+ * the context: after risltc_set_lights, risltc_set_ltc_table, risltc_set_scene
+ * or a tuning change, risltc_graph_launch fails with RISLTC_E_STATE until
+ * the graph is created again. Destroy graphs before their context. Launches
+ * of one graph are ordered by their streams only: do not overlap them. */
+typedef struct risltc_graph_s risltc_graph_t;
+int risltc_graph_create(risltc_context_t* context, const risltc_view_t* views, uint32_t count,
+                        risltc_graph_t** out_graph);
+int risltc_graph_update(risltc_graph_t* graph, const risltc_view_t* views, uint32_t count);
+int risltc_graph_launch(risltc_graph_t* graph, void* stream);
+void risltc_graph_destroy(risltc_graph_t* graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,12 +11,13 @@ call raises.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
 from . import abi
 
-__all__ = ["Context", "lib", "LIB_PATH", "RisltcError", "abi"]
+__all__ = ["Context", "Graph", "lib", "LIB_PATH", "RisltcError", "abi"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librisltc.so")
 _lib = None
@@ -70,6 +71,15 @@ SIGNATURES = {
     "risltc_accumulate": (C.c_int, [_vp, _vp, _vp, _u32, _u32, _vp]),
     "risltc_trace_shadow_rays": (C.c_int, [_vp, _vp, _u32, _vp, _vp]),
     "risltc_last_timings": (C.c_int, [_vp, _f32p]),
+    "risltc_add_light_set": (C.c_int, [_vp, _vp, _u32, _u32, _u32p]),
+    "risltc_shade_batch": (C.c_int, [_vp, C.POINTER(abi.View), _u32, _vp]),
+    "risltc_get_tuning": (C.c_int, [_vp, C.POINTER(abi.Tuning)]),
+    "risltc_set_tuning": (C.c_int, [_vp, C.POINTER(abi.Tuning)]),
+    "risltc_autotune": (C.c_int, [_vp, C.POINTER(abi.View), _u32, _u32, C.POINTER(abi.Tuning)]),
+    "risltc_graph_create": (C.c_int, [_vp, C.POINTER(abi.View), _u32, C.POINTER(_vp)]),
+    "risltc_graph_update": (C.c_int, [_vp, C.POINTER(abi.View), _u32]),
+    "risltc_graph_launch": (C.c_int, [_vp, _vp]),
+    "risltc_graph_destroy": (None, [_vp]),
 }
 
 
@@ -131,6 +141,37 @@ def ltc_constants_for(res_roughness, res_inclination, fresnel_count):
     return k
 
 
+class Graph:
+    """risltc_graph_t: a batch of views captured as one hipGraph with buffers
+    of its own. update() takes views of the same shapes; launch() replays."""
+
+    def __init__(self, ctx, views):
+        self.ctx = ctx
+        arr, n = Context._views(views)
+        h = _vp()
+        ctx._check(ctx.L.risltc_graph_create(ctx.h, arr, n, C.byref(h)), "risltc_graph_create")
+        self.h = h
+        ctx._graphs.add(self)
+
+    def update(self, views):
+        arr, n = Context._views(views)
+        self.ctx._check(self.ctx.L.risltc_graph_update(self.h, arr, n), "risltc_graph_update")
+
+    def launch(self, stream=None):
+        self.ctx._check(self.ctx.L.risltc_graph_launch(self.h, _stream(stream)), "risltc_graph_launch")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.risltc_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """A shading context on one HIP device (risltc_context_t)."""
 
@@ -144,9 +185,13 @@ class Context:
         self.device = device
         self.ltc_constants = None
         self._keep = []
+        self._graphs = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "h", None):
+            # graphs hold the context's device pointers: destroy them first
+            for g in list(self._graphs):
+                g.close()
             self.L.risltc_destroy(self.h)
             self.h = None
 
@@ -302,3 +347,51 @@ class Context:
         ms = (C.c_float * 4)()
         self._check(self.L.risltc_last_timings(self.h, ms), "risltc_last_timings")
         return list(ms)
+
+    # ---- batches of views ----------------------------------------------------
+    def add_light_set(self, records, count, max_vertex_count):
+        """records: bytes / uint8 array as write_lights packs them; returns the id."""
+        buf = np.ascontiguousarray(np.frombuffer(bytes(records), dtype=np.uint8))
+        out = C.c_uint32(0)
+        self._check(self.L.risltc_add_light_set(self.h, buf.ctypes.data_as(_vp), count, max_vertex_count,
+                                                C.byref(out)), "risltc_add_light_set")
+        return out.value
+
+    @staticmethod
+    def view(constants, settings, visibility, out, light_set=0):
+        """A risltc_view_t over torch device tensors (keep them alive)."""
+        v = abi.View()
+        v.constants = constants
+        v.settings = settings
+        v.visibility = visibility.data_ptr()
+        v.out = out.data_ptr()
+        v.light_set = light_set
+        return v
+
+    @staticmethod
+    def _views(views):
+        arr = (abi.View * len(views))(*views)
+        return arr, len(views)
+
+    def shade_batch(self, views, stream=None):
+        arr, n = self._views(views)
+        self._check(self.L.risltc_shade_batch(self.h, arr, n, _stream(stream)), "risltc_shade_batch")
+
+    def get_tuning(self):
+        t = abi.Tuning()
+        self._check(self.L.risltc_get_tuning(self.h, C.byref(t)), "risltc_get_tuning")
+        return t
+
+    def set_tuning(self, wave_width=16, cull_chunk=32, lds_lights=2048, trace_refill=32):
+        t = abi.Tuning(wave_width, cull_chunk, lds_lights, trace_refill)
+        self._check(self.L.risltc_set_tuning(self.h, C.byref(t)), "risltc_set_tuning")
+
+    def autotune(self, views, repeats=3):
+        arr, n = self._views(views)
+        t = abi.Tuning()
+        self._check(self.L.risltc_autotune(self.h, arr, n, repeats, C.byref(t)), "risltc_autotune")
+        return t
+
+    def graph(self, views):
+        """A hipGraph of a batch (risltc_graph_t); see Graph."""
+        return Graph(self, views)

@@ -97,6 +97,21 @@ def default_settings(**kw):
     return s
 
 
+class Tuning(C.Structure):
+    """risltc_tuning_t: launch shape of the kernels (changes no result bit)."""
+    _fields_ = [("wave_width", C.c_uint32), ("cull_chunk", C.c_uint32), ("lds_lights", C.c_uint32),
+                ("trace_refill", C.c_uint32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class View(C.Structure):
+    """risltc_view_t: one view of a batch."""
+    _fields_ = [("constants", FrameConstants), ("settings", Settings), ("visibility", C.c_void_p),
+                ("out", C.c_void_p), ("light_set", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
 class PolygonalLight(C.Structure):
     """polygonal_light_t (src/polygonal_light.h:73-99)."""
     _fields_ = [("rotation_angles", C.c_float * 3), ("scaling_x", C.c_float),
@@ -115,3 +130,4 @@ class PolygonalLight(C.Structure):
 assert C.sizeof(FrameConstants) == 256
 assert C.sizeof(LtcConstants) == 32
 assert C.sizeof(Settings) == 64
+assert C.sizeof(View) == 256 + 64 + 16 + 16

@@ -136,9 +136,13 @@ hipError_t launch_mlp_sync(const float* params, uint16_t* wf, uint16_t* wb, hipS
 
 struct ShadeArgs {
     risltc_frame_constants_t c;
-    uint32_t lds_spheres;      // 1: the kernel stages the spheres in LDS
     uint32_t spp, M, mis, light_sampling, fast_atan, shadow_rays;
-    uint32_t tile_w, tile_h;  // pixel tile of one work-group
+    // launch shape (tuned; none changes a result bit): a wave covers
+    // 2^tile_w_log2 x (64 >> tile_w_log2) pixels, a work-group 4 waves stacked
+    // vertically; candidates are culled in chunks of `chunk` (<= 32); the
+    // first lds_lights light spheres (<= kLdsSphereMax) are staged in LDS
+    uint32_t tile_w_log2, chunk, lds_lights;
+    uint32_t slot_base;        // added to every shadow-ray slot (batches)
     SceneView scene;
     const uint32_t* visibility;
     float* samples;            // kSampleWords per pixel-sample
@@ -190,7 +194,18 @@ struct ResolveArgs {
 
 // host-side launchers (shade.hip, lbvh.hip)
 hipError_t launch_shade(const ShadeArgs& a, bool exact, hipStream_t s);
+uint32_t shade_tiles(const ShadeArgs& a);
+size_t shade_lds_bytes(const ShadeArgs& a);
+// views and their tile offsets (count + 1 entries) in device memory
+hipError_t launch_shade_batch(const ShadeArgs* d_views, const uint32_t* d_tile_offsets, uint32_t count,
+                              uint32_t total_tiles, size_t lds, bool exact, hipStream_t s);
+hipError_t launch_resolve_batch(const ResolveArgs* d_views, const uint32_t* d_block_offsets, uint32_t count,
+                                uint32_t total_blocks, hipStream_t s);
 hipError_t launch_trace(const TraceArgs& a, hipStream_t s);
+// the resident grid size of the trace kernel, and a launch with that size
+// computed beforehand (no device queries: usable under stream capture)
+hipError_t trace_blocks(uint32_t* blocks);
+hipError_t launch_trace(const TraceArgs& a, hipStream_t s, uint32_t blocks);
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s);
 hipError_t launch_accumulate(float* accum, const float* frame, uint32_t pixel_count, uint32_t n,
                              hipStream_t s);

@@ -545,16 +545,28 @@ __global__ void __launch_bounds__(256) primary_kernel(BvhView B, risltc_frame_co
 
 // One resident grid: as many 256-lane work-groups as fit on the device at
 // once, a multiple of the XCD count.
-hipError_t launch_trace(const TraceArgs& a, hipStream_t s) {
+hipError_t trace_blocks(uint32_t* blocks) {
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_kernel, 256, 0);
     if (e != hipSuccess) return e;
-    uint32_t blocks = (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)(per_cu > 0 ? per_cu : 1);
-    blocks = (blocks + kTraceQueues - 1u) / kTraceQueues * kTraceQueues;
+    uint32_t b = (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)(per_cu > 0 ? per_cu : 1);
+    *blocks = (b + kTraceQueues - 1u) / kTraceQueues * kTraceQueues;
+    return hipSuccess;
+}
+
+hipError_t launch_trace(const TraceArgs& a, hipStream_t s, uint32_t blocks) {
+    if (blocks == 0 || blocks % kTraceQueues) return hipErrorInvalidValue;
     hipLaunchKernelGGL(trace_kernel, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
+}
+
+hipError_t launch_trace(const TraceArgs& a, hipStream_t s) {
+    uint32_t blocks = 0;
+    hipError_t e = trace_blocks(&blocks);
+    if (e != hipSuccess) return e;
+    return launch_trace(a, s, blocks);
 }
 
 hipError_t launch_trace_rays(const BvhView& bvh, const float* rays8, uint32_t n, uint32_t* hits, hipStream_t s) {

@@ -1061,28 +1061,35 @@ __device__ __forceinline__ float learned_factor(const uint8_t* table, const uint
 }
 
 // ---- the kernel ----------------------------------------------------------------
+// One 4-wave work-group shades one tile of one view. A wave covers
+// 2^tile_w_log2 x (64 >> tile_w_log2) pixels; the work-group stacks its 4
+// waves vertically. The first lds_lights light spheres (all of them when
+// they fit) are staged in LDS; the cluster table too for the learned target
+// when every sphere is staged.
 template <bool X, bool DEBUG, bool LEARNED>
-__global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
-    extern __shared__ float4 sh_spheres[];
+__device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, float4* sh_spheres) {
     constexpr bool learned = LEARNED;
     const uint8_t* of_light = A.clusters.of_light;
-    if (A.lds_spheres) {
-        const uint32_t n = A.scene.light_count;
-        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) sh_spheres[k] = A.scene.spheres[k];
+    const uint32_t n_lds = A.scene.light_count < A.lds_lights ? A.scene.light_count : A.lds_lights;
+    const bool all_lds = n_lds == A.scene.light_count;
+    if (n_lds > 0u) {
+        for (uint32_t k = threadIdx.x; k < n_lds; k += blockDim.x) sh_spheres[k] = A.scene.spheres[k];
         if constexpr (learned) {
-            // after the spheres: the cluster of every light
-            uint8_t* c = reinterpret_cast<uint8_t*>(sh_spheres + n);
-            for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) c[k] = A.clusters.of_light[k];
-            of_light = c;
+            if (all_lds) {
+                // after the spheres: the cluster of every light
+                uint8_t* c = reinterpret_cast<uint8_t*>(sh_spheres + n_lds);
+                for (uint32_t k = threadIdx.x; k < n_lds; k += blockDim.x) c[k] = A.clusters.of_light[k];
+                of_light = c;
+            }
         }
         __syncthreads();
     }
     const uint32_t W = A.c.viewport_size[0], H = A.c.viewport_size[1];
-    const uint32_t tiles_x = (W + 15u) / 16u;
-    const uint32_t tile = blockIdx.x;
+    const uint32_t ww_log2 = A.tile_w_log2, ww = 1u << ww_log2, wh = 64u >> ww_log2;
+    const uint32_t tiles_x = (W + ww - 1u) >> ww_log2;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t x = (tile % tiles_x) * 16u + (lane & 15u);
-    const uint32_t y = (tile / tiles_x) * 16u + wave * 4u + (lane >> 4);
+    const uint32_t x = (tile % tiles_x) * ww + (lane & (ww - 1u));
+    const uint32_t y = (tile / tiles_x) * (4u * wh) + wave * wh + (lane >> ww_log2);
     const bool inside = x < W && y < H;
     const uint32_t pixel = y * W + x;
     const uint32_t spp = A.spp;
@@ -1120,14 +1127,15 @@ __global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
                 // target of the survivors in their original order, the noise
                 // state jumped to each survivor. Culled candidates have
                 // w = 0 and leave the reservoir exactly as it was.
-                for (uint32_t base = 0; base < M; base += 32u) {
-                    const uint32_t cnt = (M - base < 32u) ? M - base : 32u;
+                const uint32_t chunk = A.chunk;
+                for (uint32_t base = 0; base < M; base += chunk) {
+                    const uint32_t cnt = (M - base < chunk) ? M - base : chunk;
                     const uint32_t seed_chunk = seed;
                     uint32_t live = 0;
                     for (uint32_t t = 0; t < cnt; ++t) {
                         uint32_t idx = light_index_from(noise_next(seed), N);
                         (void)noise_next(seed);
-                        float4 sp = A.lds_spheres ? sh_spheres[idx] : A.scene.spheres[idx];
+                        float4 sp = all_lds ? sh_spheres[idx] : (idx < n_lds ? sh_spheres[idx] : A.scene.spheres[idx]);
                         bool cut = culled(cull, sp);
                         live |= cut ? 0u : (1u << t);
                         if constexpr (DEBUG) {
@@ -1219,7 +1227,7 @@ __global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
                 uint32_t off = base + __popcll(ballot & ((1ull << lane) - 1ull));
                 A.rays[off].org = make_float4(px.pos.x, px.pos.y, px.pos.z, 1.0e-3f);
                 A.rays[off].dir = make_float4(F.dir[t].x, F.dir[t].y, F.dir[t].z, F.tmax[t]);
-                A.ray_slots[off] = (uint32_t)(sidx * 2 + t);
+                A.ray_slots[off] = A.slot_base + (uint32_t)(sidx * 2 + t);
             }
         }
         if (inside) {
@@ -1269,9 +1277,36 @@ __global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
     }
 }
 
+template <bool X, bool DEBUG, bool LEARNED>
+__global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
+    extern __shared__ float4 sh_spheres[];
+    shade_tile<X, DEBUG, LEARNED>(A, blockIdx.x, sh_spheres);
+}
+
+// the last v with offsets[v] <= i (offsets[0] = 0, n + 1 entries)
+__device__ __forceinline__ uint32_t view_of(const uint32_t* offsets, uint32_t n, uint32_t i) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (offsets[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Many views in one launch: work-group b shades tile b - tile_offsets[v] of
+// view v, with v's descriptor read from device memory (so a captured graph
+// can be replayed with new cameras and noise seeds).
+template <bool X>
+__global__ void __launch_bounds__(256) shade_batch_kernel(const ShadeArgs* views, const uint32_t* tile_offsets,
+                                                          uint32_t count) {
+    extern __shared__ float4 sh_spheres[];
+    const uint32_t v = view_of(tile_offsets, count, blockIdx.x);
+    shade_tile<X, false, false>(views[v], blockIdx.x - tile_offsets[v], sh_spheres);
+}
+
 // ---- resolve: sum the estimates in the reference's order -------------------
-__global__ void __launch_bounds__(256) resolve_kernel(ResolveArgs A) {
-    uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pixel) {
     if (pixel >= A.width * A.height) return;
     uint32_t prim = A.visibility[pixel];
     float4 out;
@@ -1335,6 +1370,16 @@ __global__ void __launch_bounds__(256) resolve_kernel(ResolveArgs A) {
     reinterpret_cast<float4*>(A.out)[pixel] = out;
 }
 
+__global__ void __launch_bounds__(256) resolve_kernel(ResolveArgs A) {
+    resolve_pixel(A, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) resolve_batch_kernel(const ResolveArgs* views, const uint32_t* block_offsets,
+                                                            uint32_t count) {
+    const uint32_t v = view_of(block_offsets, count, blockIdx.x);
+    resolve_pixel(views[v], (blockIdx.x - block_offsets[v]) * blockDim.x + threadIdx.x);
+}
+
 // accum_pass.frag.glsl:45-53: out = fma(prev, n, cur) * (1 / (n + 1))
 __global__ void accumulate_kernel(float* accum, const float* frame, uint32_t count, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1344,13 +1389,29 @@ __global__ void accumulate_kernel(float* accum, const float* frame, uint32_t cou
     accum[i] = fma_(r, 1.0f / (float)(n + 1u), 0.0f);
 }
 
+uint32_t shade_tiles(const ShadeArgs& a) {
+    const uint32_t W = a.c.viewport_size[0], H = a.c.viewport_size[1];
+    const uint32_t ww = 1u << a.tile_w_log2, th = 4u * (64u >> a.tile_w_log2);
+    return ((W + ww - 1u) / ww) * ((H + th - 1u) / th);
+}
+
+size_t shade_lds_bytes(const ShadeArgs& a) {
+    const uint32_t n = a.scene.light_count < a.lds_lights ? a.scene.light_count : a.lds_lights;
+    size_t b = (size_t)n * sizeof(float4);
+    if (a.proposal && n == a.scene.light_count) b += n;
+    return b;
+}
+
+static bool shade_args_ok(const ShadeArgs& a) {
+    return a.tile_w_log2 >= 3u && a.tile_w_log2 <= 6u && a.chunk >= 1u && a.chunk <= 32u &&
+           a.lds_lights <= kLdsSphereMax;
+}
+
 hipError_t launch_shade(const ShadeArgs& a, bool exact, hipStream_t s) {
-    uint32_t W = a.c.viewport_size[0], H = a.c.viewport_size[1];
-    dim3 grid(((W + 15u) / 16u) * ((H + 15u) / 16u)), block(256);
+    if (!shade_args_ok(a)) return hipErrorInvalidValue;
+    dim3 grid(shade_tiles(a)), block(256);
     bool dbg = a.debug_samples != nullptr || a.debug_cands != nullptr;
-    size_t lds = a.lds_spheres ? (size_t)a.scene.light_count * sizeof(float4) : 0;
-    if (a.lds_spheres && a.proposal) lds += (size_t)a.scene.light_count;
-    if (a.lds_spheres && a.scene.light_count > kLdsSphereMax) return hipErrorInvalidValue;
+    size_t lds = shade_lds_bytes(a);
     if (a.proposal) {
         if (exact) {
             if (dbg) hipLaunchKernelGGL((shade_kernel<true, true, true>), grid, block, lds, s, a);
@@ -1366,6 +1427,25 @@ hipError_t launch_shade(const ShadeArgs& a, bool exact, hipStream_t s) {
         if (dbg) hipLaunchKernelGGL((shade_kernel<false, true, false>), grid, block, lds, s, a);
         else hipLaunchKernelGGL((shade_kernel<false, false, false>), grid, block, lds, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_shade_batch(const ShadeArgs* d_views, const uint32_t* d_tile_offsets, uint32_t count,
+                              uint32_t total_tiles, size_t lds, bool exact, hipStream_t s) {
+    if (count == 0 || total_tiles == 0) return hipSuccess;
+    if (exact)
+        hipLaunchKernelGGL(shade_batch_kernel<true>, dim3(total_tiles), dim3(256), lds, s, d_views, d_tile_offsets,
+                           count);
+    else
+        hipLaunchKernelGGL(shade_batch_kernel<false>, dim3(total_tiles), dim3(256), lds, s, d_views, d_tile_offsets,
+                           count);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve_batch(const ResolveArgs* d_views, const uint32_t* d_block_offsets, uint32_t count,
+                                uint32_t total_blocks, hipStream_t s) {
+    if (count == 0 || total_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(resolve_batch_kernel, dim3(total_blocks), dim3(256), 0, s, d_views, d_block_offsets, count);
     return hipGetLastError();
 }
 

@@ -1,0 +1,226 @@
+"""GPU tests of batches of views, their hipGraph replays and the launch-shape
+tuning.
+
+  * a batch of views with different resolutions (ragged against every tile
+    shape), cameras, settings and light sets gives, per view, the frame that
+    risltc_shade gives for that view alone, bit for bit;
+  * a graph replays the batch: every replay, after an update with new noise
+    seeds and with other work on the context's own buffers in between, equals
+    the eager batch of the same views;
+  * no tuning (tile shape, cull chunk, LDS staging, trace refill) changes a
+    bit, and the exact build under a non-default tuning still equals the
+    oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import orc
+from risltc_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+SUBSET = 23  # lights of the second light set
+
+
+def constants(ctx, scene, camera, w, h, seed=0):
+    import risltc_amd
+    return risltc_amd.write_frame_constants(camera, w, h, scene.mesh.dequantization_factor,
+                                            scene.mesh.dequantization_summand, ctx.ltc_constants, noise_seed=seed,
+                                            animate=seed != 0)
+
+
+def view_specs(scene):
+    """(camera, width, height, settings, light set) per view; light set 1 is
+    the first SUBSET lights."""
+    cam0 = scene.camera
+    cam1 = synth.make_camera((6.0, -12.0, 5.0), math.pi * 0.8, math.pi / 2 - 0.3)
+    cam2 = synth.make_camera((-4.0, 3.0, 9.0), math.pi * 1.7, math.pi / 2 - 0.9, fov=1.2)
+    return [
+        (cam0, 256, 256, scene.settings(sample_count_light=2, ris_candidate_count=8), 0),
+        (cam1, 200, 120, scene.settings(sample_count_light=1, ris_candidate_count=16), 1),
+        (cam2, 97, 61, scene.settings(sample_count_light=3, ris_candidate_count=4, shadow_rays=0,
+                                      mis_heuristic=abi.MIS_BALANCE), 0),
+        (cam0, 64, 72, scene.settings(sample_count_light=1, ris_candidate_count=8, fast_atan=1), 1),
+        (cam1, 33, 5, scene.settings(sample_count_light=2, light_sampling=abi.LIGHT_UNIFORM), 0),
+    ]
+
+
+@pytest.fixture(scope="module")
+def batch(small_scene, ltc_table):
+    import risltc_amd
+    import torch
+    sc = small_scene
+    ctx = risltc_amd.Context(0)
+    ctx.load_scene(sc, *ltc_table)
+    recs, vmax = synth.pack_lights(sc.lights[:SUBSET])
+    set_id = ctx.add_light_set(recs, SUBSET, vmax)
+    assert set_id == 1
+    # a context whose main lights are the subset: single-view references
+    ctx_sub = risltc_amd.Context(0)
+    ctx_sub.load_scene(sc, *ltc_table)
+    ctx_sub.set_lights(recs, SUBSET, vmax)
+    views, refs, bufs = [], [], []
+    for i, (cam, w, h, s, ls) in enumerate(view_specs(sc)):
+        c = constants(ctx, sc, cam, w, h, seed=i)
+        vis = torch.empty(w * h, dtype=torch.int32, device="cuda")
+        ctx.render_visibility(c, vis)
+        out = torch.full((w * h * 4,), float("nan"), dtype=torch.float32, device="cuda")
+        ref = torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
+        (ctx_sub if ls else ctx).shade(c, s, vis, ref)
+        views.append(ctx.view(c, s, vis, out, ls))
+        refs.append(ref)
+        bufs.append((c, s, vis, out, ls, cam, w, h))
+    torch.cuda.synchronize()
+    yield dict(ctx=ctx, ctx_sub=ctx_sub, views=views, refs=refs, bufs=bufs, scene=sc, ltc=ltc_table)
+    ctx.close()
+    ctx_sub.close()
+
+
+def bits(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+def test_batch_equals_single_views(batch):
+    import torch
+    ctx = batch["ctx"]
+    ctx.shade_batch(batch["views"])
+    torch.cuda.synchronize()
+    for i, ((_, _, _, out, *_), ref) in enumerate(zip(batch["bufs"], batch["refs"])):
+        assert np.array_equal(bits(out), bits(ref)), i
+    # the frames are not trivial: every view has lit pixels
+    for _, _, _, out, *_ in batch["bufs"]:
+        o = out.cpu().numpy().reshape(-1, 4)[:, :3]
+        assert np.isfinite(o).all() and (o > 0).mean() > 0.05
+
+
+def test_batch_rejects_bad_views(batch):
+    import risltc_amd
+    import torch
+    ctx = batch["ctx"]
+    c, s, vis, out, *_ = batch["bufs"][0]
+    bad = ctx.view(c, s, vis, out, light_set=7)
+    with pytest.raises(risltc_amd.RisltcError, match="E_ARG"):
+        ctx.shade_batch([bad])
+    learned = ctx.view(c, batch["scene"].settings(proposal=abi.PROPOSAL_LEARNED), vis, out)
+    with pytest.raises(risltc_amd.RisltcError, match="E_UNSUPPORTED"):
+        ctx.shade_batch([learned])
+    exact = ctx.view(c, batch["scene"].settings(exact_math=1), vis, out)
+    with pytest.raises(risltc_amd.RisltcError, match="E_UNSUPPORTED"):
+        ctx.shade_batch([batch["views"][1], exact])
+    torch.cuda.synchronize()
+
+
+def _reseeded(batch, seed0):
+    """The batch's views with new noise seeds, and single-view references."""
+    import torch
+    ctx, sub = batch["ctx"], batch["ctx_sub"]
+    views, refs = [], []
+    for i, (_, s, vis, out, ls, cam, w, h) in enumerate(batch["bufs"]):
+        c = constants(ctx, batch["scene"], cam, w, h, seed=seed0 + i)
+        ref = torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
+        (sub if ls else ctx).shade(c, s, vis, ref)
+        views.append(ctx.view(c, s, vis, out, ls))
+        refs.append(ref)
+    return views, refs
+
+
+def test_graph_replays_equal_eager(batch):
+    import risltc_amd
+    import torch
+    ctx = batch["ctx"]
+    sc = batch["scene"]
+    g = ctx.graph(batch["views"])
+    try:
+        extra = torch.empty(sc.width * sc.height * 4, dtype=torch.float32, device="cuda")
+        for rep in range(4):
+            views, refs = _reseeded(batch, 100 * rep + 1)
+            torch.cuda.synchronize()
+            g.update(views)
+            for _, _, _, out, *_ in batch["bufs"]:
+                out.fill_(float("nan"))
+            g.launch()
+            torch.cuda.synchronize()
+            for i, ((_, _, _, out, *_), ref) in enumerate(zip(batch["bufs"], refs)):
+                assert np.array_equal(bits(out), bits(ref)), (rep, i)
+            # eager work on the context's own buffers between replays
+            c0, s0, vis0, *_ = batch["bufs"][0]
+            ctx.shade(c0, s0, vis0, extra)
+            ctx.shade_batch(batch["views"][2:])
+            torch.cuda.synchronize()
+        # a view of another shape does not fit the captured launch
+        c, s, vis, out, ls, cam, w, h = batch["bufs"][1]
+        wrong = list(batch["views"])
+        wrong[1] = ctx.view(c, sc.settings(sample_count_light=2, ris_candidate_count=16), vis, out, ls)
+        with pytest.raises(risltc_amd.RisltcError, match="E_ARG"):
+            g.update(wrong)
+        # a tuning change invalidates the captured descriptors
+        t = ctx.get_tuning()
+        ctx.set_tuning(**t.as_dict())
+        with pytest.raises(risltc_amd.RisltcError, match="E_STATE"):
+            g.launch()
+    finally:
+        g.close()
+
+
+TUNINGS = [
+    dict(wave_width=8, cull_chunk=1, lds_lights=0, trace_refill=1),
+    dict(wave_width=32, cull_chunk=7, lds_lights=13, trace_refill=17),
+    dict(wave_width=64, cull_chunk=32, lds_lights=2048, trace_refill=64),
+    dict(wave_width=16, cull_chunk=16, lds_lights=1024, trace_refill=48),
+]
+
+
+def test_tunings_change_no_bit(batch):
+    import torch
+    ctx = batch["ctx"]
+    default = ctx.get_tuning().as_dict()
+    try:
+        for t in TUNINGS:
+            ctx.set_tuning(**t)
+            for _, _, _, out, *_ in batch["bufs"]:
+                out.fill_(float("nan"))
+            ctx.shade_batch(batch["views"])
+            torch.cuda.synchronize()
+            for i, ((_, _, _, out, *_), ref) in enumerate(zip(batch["bufs"], batch["refs"])):
+                assert np.array_equal(bits(out), bits(ref)), (t, i)
+    finally:
+        ctx.set_tuning(**default)
+
+
+def test_exact_build_under_tuning_equals_oracle(batch):
+    import torch
+    ctx = batch["ctx"]
+    sc = batch["scene"]
+    default = ctx.get_tuning().as_dict()
+    c, _, vis, *_ = batch["bufs"][1]
+    cam, w, h = batch["bufs"][1][5:8]
+    s = sc.settings(exact_math=1, sample_count_light=1, ris_candidate_count=8)
+    out = torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
+    try:
+        ctx.set_tuning(wave_width=64, cull_chunk=5, lds_lights=0, trace_refill=24)
+        ctx.shade(c, s, vis, out)
+        torch.cuda.synchronize()
+    finally:
+        ctx.set_tuning(**default)
+    o = orc.Oracle(sc, *batch["ltc"])
+    ref, _, _ = o.shade(c, s, vis.cpu().numpy().view(np.uint32))
+    assert np.array_equal(bits(out).reshape(-1, 4), ref.view(np.uint32))
+
+
+def test_autotune_keeps_results(batch):
+    import torch
+    ctx = batch["ctx"]
+    default = ctx.get_tuning().as_dict()
+    try:
+        t = ctx.autotune(batch["views"], repeats=2)
+        assert t.wave_width in (8, 16, 32, 64) and 1 <= t.cull_chunk <= 32
+        assert t.lds_lights <= 2048 and 1 <= t.trace_refill <= 64
+        assert ctx.get_tuning().as_dict() == t.as_dict()
+        ctx.shade_batch(batch["views"])
+        torch.cuda.synchronize()
+        for i, ((_, _, _, out, *_), ref) in enumerate(zip(batch["bufs"], batch["refs"])):
+            assert np.array_equal(bits(out), bits(ref)), i
+    finally:
+        ctx.set_tuning(**default)
