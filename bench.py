@@ -13,7 +13,13 @@ The learned_vs_plain object is the second half of the metric: GPU time for
 the plain pass and for the pass with the learned target to reach relMSE 0.01
 against an 8192-spp plain reference (risltc_amd/convergence.py).
 
-  python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-learned]
+The batched_views object: 16 views of 480x270 around the city (one 1080p
+frame of pixels in all), shaded by one risltc_shade call per view, by one
+risltc_shade_batch, and by a hipGraph replay of that batch.
+
+The launch shape is autotuned (risltc_autotune) on the frame before timing.
+
+  python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-learned] [--no-batch]
 """
 import argparse
 import json
@@ -42,6 +48,68 @@ def algorithmic_bytes(pixels, lights, vmax, ltc_res):
     return pixels * (4 + 48 + 1 + 16) + lights * (48 + 16 * vmax) + ltc_res * ltc_res * 12
 
 
+def orbit_views(ctx, scene, vis_list, outs, count, width, height, settings):
+    """count cameras on a circle of radius 80 m, 35 m up, around the light
+    field's centre (0, 40), each looking at it (forward = (-sin rz, -cos rz))."""
+    import math
+    import risltc_amd
+    from risltc_amd import synth
+    views = []
+    for i in range(count):
+        phi = 2.0 * math.pi * i / count
+        cam = synth.make_camera((80.0 * math.cos(phi), 40.0 + 80.0 * math.sin(phi), 35.0), math.pi / 2 - phi,
+                                math.pi / 2 - 0.45)
+        c = risltc_amd.write_frame_constants(cam, width, height, scene.mesh.dequantization_factor,
+                                             scene.mesh.dequantization_summand, ctx.ltc_constants)
+        if vis_list[i] is None:
+            import torch
+            vis_list[i] = torch.empty(width * height, dtype=torch.int32, device="cuda")
+            ctx.render_visibility(c, vis_list[i])
+        views.append(ctx.view(c, settings, vis_list[i], outs[i]))
+    return views
+
+
+def bench_batch(ctx, scene, steps, warmup, count=16, width=480, height=270):
+    """count views of width x height (together one 1080p frame of pixels),
+    same settings as the main workload: one risltc_shade per view, one
+    risltc_shade_batch, and a graph replay of the batch, each after its own
+    warm-up; wall time per step over `steps` steps."""
+    import torch
+    settings = scene.settings()
+    outs = [torch.empty(width * height * 4, dtype=torch.float32, device="cuda") for _ in range(count)]
+    vis = [None] * count
+    views = orbit_views(ctx, scene, vis, outs, count, width, height, settings)
+    tuning = ctx.autotune(views, repeats=3).as_dict()
+    samples = count * width * height * settings.sample_count_light
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) * 1e3 / steps
+
+    def per_view():
+        for i, v in enumerate(views):
+            ctx.shade(v.constants, v.settings, vis[i], outs[i])
+
+    ms_views = timed(per_view)
+    ms_batch = timed(lambda: ctx.shade_batch(views))
+    g = ctx.graph(views)
+    try:
+        ms_graph = timed(g.launch)
+    finally:
+        g.close()
+    rate = lambda ms: round(samples / (ms * 1e-3) / 1e6, 3)
+    return {"views": count, "resolution": [width, height], "samples_per_step": samples, "tuning": tuning,
+            "ms_per_step": {"per_view_calls": round(ms_views, 4), "batch": round(ms_batch, 4),
+                            "graph": round(ms_graph, 4)},
+            "msamples_per_s": {"per_view_calls": rate(ms_views), "batch": rate(ms_batch), "graph": rate(ms_graph)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +122,8 @@ def main():
                     help="pixels in the CPU-baseline sample (0 = the whole frame)")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-plain time-to-relMSE experiment")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view measurement")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (known HBM bytes) for counter scaling")
     args = ap.parse_args()
@@ -95,6 +165,11 @@ def main():
           (setup_s, scene.mesh.triangle_count, scene.light_count, scene.width, scene.height, spp,
            scene.candidates), file=sys.stderr, flush=True)
 
+    tuning = None
+    if not args.no_autotune:
+        t1 = time.time()
+        tuning = ctx.autotune([ctx.view(consts, settings, vis, out)], repeats=5).as_dict()
+        print("autotune %.1f s: %s" % (time.time() - t1, tuning), file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         ctx.shade(consts, settings, vis, out)
     torch.cuda.synchronize()
@@ -129,6 +204,11 @@ def main():
     frame = out.cpu().numpy().reshape(-1, 4)
     print("kernels (ms): shade %.3f trace %.3f resolve %.3f; frame mean %s" %
           (kernel_ms[0], kernel_ms[1], kernel_ms[2], frame[:, :3].mean(0)), file=sys.stderr, flush=True)
+
+    batch = None
+    if not args.no_batch and args.workload == "city":
+        batch = bench_batch(ctx, scene, args.steps, args.warmup)
+        print("batch: %s" % batch, file=sys.stderr, flush=True)
 
     learned = None
     if not args.no_learned and args.workload == "city" and not args.exact:
@@ -175,13 +255,14 @@ def main():
         "config": {"workload": "plain RIS-LTC, %dx%d, %d lights (3-8 vertices), ~%.2fM triangles, M=%d, %d spp"
                                % (scene.width, scene.height, scene.light_count, scene.mesh.triangle_count / 1e6,
                                   scene.candidates, spp),
-                   "samples_per_step": samples, "exact_math": bool(args.exact),
+                   "samples_per_step": samples, "exact_math": bool(args.exact), "tuning": tuning,
                    "kernel_ms": {"shade": round(kernel_ms[0], 4), "trace": round(kernel_ms[1], 4),
                                  "resolve": round(kernel_ms[2], 4)}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "kernel": "shade_kernel", "algorithmic_bytes": alg},
         "valu": valu,
+        "batched_views": batch,
         "learned_vs_plain": learned,
         "cpu_baseline": cpu,
     }
