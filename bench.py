@@ -10,8 +10,10 @@ Prints one JSON line. The CPU baseline is the FP32 oracle (OpenMP, the
 host's cores) on a bounded sample of the same frame.
 
 The learned_vs_plain object is the second half of the metric: GPU time for
-the plain pass and for the pass with the learned target to reach relMSE 0.01
-against an 8192-spp plain reference (risltc_amd/convergence.py).
+the plain pass and for the learned pass (MLP + online visibility cache in
+the RIS target) to reach relMSE 0.01 against a 65536-spp plain reference,
+each pass's Msamples/s and per-kernel milliseconds per frame, and two
+ablations (risltc_amd/convergence.py).
 
 The batched_views object: 16 views of 480x270 around the city (one 1080p
 frame of pixels in all), shaded by one risltc_shade call per view, by one
@@ -79,6 +81,7 @@ def bench_batch(ctx, scene, steps, warmup, count=16, width=480, height=270):
     outs = [torch.empty(width * height * 4, dtype=torch.float32, device="cuda") for _ in range(count)]
     vis = [None] * count
     views = orbit_views(ctx, scene, vis, outs, count, width, height, settings)
+    before = ctx.get_tuning().as_dict()
     tuning = ctx.autotune(views, repeats=3).as_dict()
     samples = count * width * height * settings.sample_count_light
 
@@ -103,6 +106,7 @@ def bench_batch(ctx, scene, steps, warmup, count=16, width=480, height=270):
         ms_graph = timed(g.launch)
     finally:
         g.close()
+    ctx.set_tuning(**before)
     rate = lambda ms: round(samples / (ms * 1e-3) / 1e6, 3)
     return {"views": count, "resolution": [width, height], "samples_per_step": samples, "tuning": tuning,
             "ms_per_step": {"per_view_calls": round(ms_views, 4), "batch": round(ms_batch, 4),
@@ -214,8 +218,12 @@ def main():
     if not args.no_learned and args.workload == "city" and not args.exact:
         from risltc_amd import convergence
         t1 = time.time()
-        learned = convergence.compare(ctx, scene, vis)
+        learned = convergence.compare(ctx, scene, vis, log=lambda m: print(m, file=sys.stderr, flush=True))
         learned["wall_s"] = round(time.time() - t1, 1)
+        # each pass's shade kernel against the same algorithmic bytes
+        for k in ("plain", "learned"):
+            ms = learned[k]["ms_per_frame"]["shade"]
+            learned[k]["shade_roofline_frac"] = round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 6)
 
     cpu = None
     if not args.no_cpu:

@@ -186,7 +186,10 @@ typedef struct risltc_settings_s {
                                             bit-identical to the FP32 oracle      */
     uint32_t learned_floor_log2;         /* learned target: p_hat (f + (1 - f) v / 255),
                                             f = 2^-learned_floor_log2 (0 -> 3)    */
-    uint32_t reserved[5];
+    uint32_t skip_training_records;      /* 1: no proposal training records
+                                            (risltc_proposal_train then needs
+                                            a later call without it)          */
+    uint32_t reserved[4];
 } risltc_settings_t;
 
 /* Fills settings with the reference defaults (src/main.c:220-236): SAMPLE_COUNT
@@ -347,7 +350,8 @@ typedef struct risltc_sample_record_s {
     float color[3];          /* evaluate_polygonal_light_shading_peters()    */
     uint32_t noise_seed_end; /* LCG state after this sample                  */
     float position[3];       /* shading position (shadow ray origin)         */
-    float pad[3];
+    uint32_t cache_key;      /* visibility-cache cell of the pixel, 0 without */
+    float pad[2];
 } risltc_sample_record_t;
 typedef struct risltc_candidate_record_s {
     int32_t light_index;
@@ -411,6 +415,31 @@ int risltc_proposal_train(risltc_context_t* context, const risltc_frame_constant
  * unoccluded; *count receives their number. */
 int risltc_proposal_records(risltc_context_t* context, uint32_t* cluster_out, float* y_out,
                             uint64_t* count, void* stream);
+/* Online visibility cache (project extension). 2^log2_entries hashed
+ * world-space cells (cell = floor(position / cell_size) per axis, split by
+ * the dominant axis and sign of the shading normal) x slots, where a slot is
+ * a light cluster (256 slots) or, with per_light = 1, a light (light-count
+ * slots). Each entry holds two uint32 counts (a unoccluded, b occluded) and
+ * one byte, 9 * slots * 2^log2_entries bytes of device memory in all
+ * (log2_entries <= 20, counts at most 4 GiB). While a cache and light
+ * clusters are set, every risltc_shade counts each traced shadow ray of its
+ * final samples at (pixel's cell, slot of the sample's light) and, before
+ * shading, recomputes the bytes of the cells counted in since:
+ * 255 for an empty entry, else round(254 (a + prior_strength / 2) /
+ * (a + b + prior_strength)). With settings.proposal =
+ * RISLTC_PROPOSAL_LEARNED the learned visibility v of a light is its entry's
+ * byte, or the MLP's byte for its cluster while the entry is empty.
+ * Allocates and clears; log2_entries = 0 removes the cache. New lights,
+ * clusters or meshes clear it. */
+int risltc_proposal_cache(risltc_context_t* context, uint32_t log2_entries, float cell_size,
+                          float prior_strength, uint32_t per_light);
+/* Host copies of the cache: counts as 2 words per (cell, slot), cell-major
+ * (read needs capacity >= that many words, write exactly that many; a write
+ * has every cell recomputed by the next shading call); bytes as one per
+ * (cell, slot), as of the last shading call. */
+int risltc_proposal_cache_read(risltc_context_t* context, uint32_t* counts_out, uint64_t capacity);
+int risltc_proposal_cache_write(risltc_context_t* context, const uint32_t* counts, uint64_t words);
+int risltc_proposal_cache_table(risltc_context_t* context, uint8_t* bytes_out, uint64_t capacity);
 /* Milliseconds of the last risltc_proposal_update and risltc_proposal_train. */
 int risltc_proposal_last_timings(const risltc_context_t* context, float out_ms[2]);
 

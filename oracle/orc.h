@@ -151,7 +151,21 @@ typedef struct orc_scene_s {
     const uint32_t* cluster_offsets; /* cluster_count + 1 prefix offsets      */
     const uint32_t* cluster_lights;  /* light indices sorted by cluster       */
     uint32_t cluster_count;
+    /* visibility cache (project extension; NULL = none): one byte per
+     * (cell, slot), cell-major, as orc_cache_bytes makes them from the
+     * counts; 2^k cells, mask = 2^k - 1; a slot is a light's cluster, or the
+     * light with cache_per_light */
+    const uint8_t* cache_vis;
+    uint32_t cache_mask;
+    float cache_inv_cell;
+    uint32_t cache_slots;
+    uint32_t cache_per_light;
 } orc_scene_t;
+
+/* Bytes of the visibility cache from its counts (2 words per entry:
+ * unoccluded a, occluded b): 255 for an empty entry, else
+ * round(254 (a + prior / 2) / (a + b + prior)) in float. */
+void orc_cache_bytes(const uint32_t* counts, uint64_t entries, float prior, uint8_t* out);
 
 /* Shades the listed pixels. out_rgba: 4 floats per listed pixel. samples
  * (may be NULL): spp records of RISLTC_SAMPLE_RECORD_FLOATS per listed pixel.
@@ -215,6 +229,7 @@ float orc_ltc_fetch_f32(const orc_scene_t* scene, int table, int channel, float 
 #define ORC_REC_COLOR 30
 #define ORC_REC_SEED 33
 #define ORC_REC_POS 34
+#define ORC_REC_CACHE_KEY 37
 
 #ifdef __cplusplus
 }

@@ -48,7 +48,9 @@ class OrcScene(C.Structure):
                 ("ltc_res_roughness", C.c_uint32), ("ltc_res_inclination", C.c_uint32),
                 ("ltc_fresnel_count", C.c_uint32), ("bvh", C.POINTER(OrcBvh)),
                 ("proposal_table", u8p), ("cluster_of_light", u32p), ("cluster_offsets", u32p),
-                ("cluster_lights", u32p), ("cluster_count", C.c_uint32)]
+                ("cluster_lights", u32p), ("cluster_count", C.c_uint32), ("cache_vis", u8p),
+                ("cache_mask", C.c_uint32), ("cache_inv_cell", C.c_float), ("cache_slots", C.c_uint32),
+                ("cache_per_light", C.c_uint32)]
 
 
 def lib():
@@ -83,6 +85,7 @@ def lib():
             "orc_bvh_build_sah": (C.c_int, [C.POINTER(OrcBvh), f32p, C.c_uint32]),
             "orc_bvh_records": (None, [C.POINTER(OrcBvh), u32p, f32p]),
             "orc_bvh_free": (None, [C.POINTER(OrcBvh)]),
+            "orc_cache_bytes": (None, [u32p, C.c_uint64, C.c_float, u8p]),
             "orc_morton_part_1_by_2": (C.c_uint32, [C.c_uint32]),
             "orc_bvh_any_hit": (C.c_int, [C.POINTER(OrcBvh), f32p, f32p, C.c_float, C.c_float, u32p]),
             "orc_bvh_closest_hit": (C.c_uint32, [C.POINTER(OrcBvh), f32p, f32p, C.c_float, C.c_float, f32p]),
@@ -136,6 +139,14 @@ def ref_lib():
 
 def ptr(a, t):
     return a.ctypes.data_as(t)
+
+
+def cache_bytes(counts, prior_strength):
+    """orc_cache_bytes over a uint32 array of (unoccluded, occluded) pairs."""
+    c = np.ascontiguousarray(counts, dtype=np.uint32).reshape(-1)
+    out = np.zeros(c.size // 2, dtype=np.uint8)
+    lib().orc_cache_bytes(ptr(c, u32p), out.size, float(np.float32(prior_strength)), ptr(out, u8p))
+    return out
 
 
 class Oracle:
@@ -204,6 +215,29 @@ class Oracle:
         s.cluster_offsets = ptr(self.prop_off, u32p)
         s.cluster_lights = ptr(self.prop_lights, u32p)
         s.cluster_count = cluster_count
+
+    def set_cache(self, counts, cell_size, prior_strength, per_light=False):
+        """Visibility cache from its counts [2^k, slots, 2] uint32 (unoccluded,
+        occluded), as the library's risltc_proposal_cache_read returns them
+        (slots = 256 clusters, or the light count with per_light): the bytes
+        are made by orc_cache_bytes; None removes the cache. Returns the
+        bytes [2^k, slots]."""
+        s = self.cscene
+        if counts is None:
+            self.cache = None
+            s.cache_vis = None
+            return None
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        slots = self.scene.light_count if per_light else 256
+        cells = counts.size // (2 * slots)
+        assert cells * 2 * slots == counts.size and cells & (cells - 1) == 0
+        self.cache = cache_bytes(counts, prior_strength)
+        s.cache_vis = ptr(self.cache, u8p)
+        s.cache_mask = cells - 1
+        s.cache_inv_cell = float(np.float32(1.0) / np.float32(cell_size))
+        s.cache_slots = slots
+        s.cache_per_light = 1 if per_light else 0
+        return self.cache.reshape(cells, slots)
 
     def bvh_records(self):
         """Pre-order records of the oracle's tree, as the device lays it out."""

@@ -1031,3 +1031,20 @@ void orc_render_visibility(const orc_bvh_t* b, const uint32_t* positions, const 
         }
     }
 }
+
+/* Visibility-cache bytes (project extension; no reference code). parity:
+ * bitwise against the library's refresh kernel (tests/test_gpu_learned.py);
+ * the formula's end points by construction in tests/test_oracle_estimator.py. */
+void orc_cache_bytes(const uint32_t* counts, uint64_t entries, float prior, uint8_t* out) {
+    for (uint64_t i = 0; i != entries; ++i) {
+        uint32_t ua = counts[2 * i], ub = counts[2 * i + 1];
+        if (ua == 0u && ub == 0u) {
+            out[i] = 255u;
+            continue;
+        }
+        float a = (float)ua, b = (float)ub;
+        float v = ((a + prior * 0.5f) * 254.0f) / ((a + b) + prior) + 0.5f;
+        uint32_t q = (uint32_t)v;
+        out[i] = (uint8_t)(q > 254u ? 254u : q);
+    }
+}

@@ -894,16 +894,45 @@ static uint32_t light_index_from(R u, uint32_t n) {
 /* Learned RIS target (project extension; no reference code): the candidates
  * come from the reference's uniform source and the target of light l is
  * p_hat * (f + v * (1 - f) / 255), f = 2^-settings.learned_floor_log2
- * (0 -> 1/8), v = the pixel table's byte for the cluster of l (the learned
- * probability that a shadow ray toward it is unoccluded). The factor is
- * formed in float in both builds. parity: bitwise
- * against the kernel's learned shading given the same tables
- * (tests/test_gpu_learned.py); unbiasedness against the plain estimator
- * (tests/test_oracle_estimator.py). */
+ * (0 -> 1/8). v is the visibility cache's byte for the pixel's cell and the
+ * light's slot (its cluster, or the light itself) or, while that entry is
+ * empty (255) or without a cache, the pixel table's byte m for the light's
+ * cluster (the MLP's probability that a shadow ray toward it is
+ * unoccluded). The factor is formed in float in both builds. parity:
+ * bitwise against the kernel's learned shading given the same tables and
+ * cache (tests/test_gpu_learned.py); unbiasedness against the plain
+ * estimator (tests/test_oracle_estimator.py). */
 static float learned_factor(const uint8_t* table, const uint32_t* of_light, uint32_t idx, float floor_,
-                            float scale) {
-    float v = (float)table[of_light[idx]];
-    return floor_ + v * scale;
+                            float scale, const uint8_t* cache_row, uint32_t per_light) {
+    const uint32_t cl = of_light[idx];
+    uint32_t v = table[cl];
+    if (cache_row) {
+        uint32_t c = cache_row[per_light ? idx : cl];
+        if (c != 255u) v = c;
+    }
+    return floor_ + (float)v * scale;
+}
+
+/* Visibility-cache key of a shading point (project extension): the cell
+ * floor(p / cell) per axis (clamped to +-1e9 first) and the dominant axis
+ * and sign of the shading normal (the first axis on ties: 0/1 = +x/-x,
+ * 2/3 = y, 4/5 = z), combined by MurmurHash3's mix and finaliser and masked
+ * to the table. Formed in float in both builds. parity: bitwise against the
+ * kernel's debug records (tests/test_gpu_learned.py); the cell and axis
+ * rules by construction in tests/test_oracle_estimator.py. */
+static uint32_t cache_key(const orc_scene_t* sc, V3 pos, V3 nrm) {
+    const float lim = 1.0e9f, inv = sc->cache_inv_cell;
+    float p[3] = {(float)pos.x, (float)pos.y, (float)pos.z};
+    int32_t ix[3];
+    for (int a = 0; a != 3; ++a) ix[a] = (int32_t)floorf(fminf(fmaxf(p[a] * inv, -lim), lim));
+    float nx = (float)nrm.x, ny = (float)nrm.y, nz = (float)nrm.z;
+    float ax = fabsf(nx), ay = fabsf(ny), az = fabsf(nz);
+    uint32_t code = (ax >= ay && ax >= az) ? (nx < 0.0f ? 1u : 0u)
+                                           : (ay >= az ? (ny < 0.0f ? 3u : 2u) : (nz < 0.0f ? 5u : 4u));
+    uint32_t h = orc_murmur_mix(orc_murmur_mix(orc_murmur_mix(orc_murmur_mix(0u, (uint32_t)ix[0]), (uint32_t)ix[1]),
+                                               (uint32_t)ix[2]),
+                                code);
+    return orc_murmur_finalize(h) & sc->cache_mask;
 }
 
 static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, const risltc_settings_t* s,
@@ -951,6 +980,10 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
         for (int k = 0; k != 4; ++k) px.w2s_flip.m[1][k] = -px.w2s_flip.m[1][k];
         px.w2c_flip = mul33_34(&px.ltc.shading_to_cosine, &px.w2s_flip);
         px.seed = orc_noise_seed(x, y, W, c->noise_random_numbers[0]);
+        const uint32_t ckey = sc->cache_vis ? cache_key(sc, px.sd.position, px.sd.normal) : 0u;
+        if (samples)
+            for (uint32_t j = 0; j != spp; ++j) memcpy(&samples[(size_t)j * RISLTC_SAMPLE_RECORD_FLOATS + ORC_REC_CACHE_KEY],
+                                                      &ckey, 4);
         uint32_t N = sc->light_count;
         light_t light;
         if (s->light_sampling == RISLTC_LIGHT_UNIFORM) {
@@ -989,7 +1022,11 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
                     uint32_t cc = 0;
                     V3 col = eval_ltc_cp(&px, &light, &cc);
                     R p_hat = SQRT(dot3(col, col));
-                    if (learned) p_hat = p_hat * (R)learned_factor(tab, sc->cluster_of_light, idx, lf, ls);
+                    if (learned)
+                        p_hat = p_hat * (R)learned_factor(tab, sc->cluster_of_light, idx, lf, ls,
+                                                          sc->cache_vis ? sc->cache_vis + (size_t)ckey * sc->cache_slots
+                                                                        : NULL,
+                                                          sc->cache_per_light);
                     R p = K(1.0) / (R)N;
                     R w = p_hat / p;
                     R r = (R)orc_noise_next(&px.seed);

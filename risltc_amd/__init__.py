@@ -71,6 +71,10 @@ SIGNATURES = {
     "risltc_accumulate": (C.c_int, [_vp, _vp, _vp, _u32, _u32, _vp]),
     "risltc_trace_shadow_rays": (C.c_int, [_vp, _vp, _u32, _vp, _vp]),
     "risltc_last_timings": (C.c_int, [_vp, _f32p]),
+    "risltc_proposal_cache": (C.c_int, [_vp, _u32, C.c_float, C.c_float, _u32]),
+    "risltc_proposal_cache_read": (C.c_int, [_vp, _u32p, C.c_uint64]),
+    "risltc_proposal_cache_write": (C.c_int, [_vp, _u32p, C.c_uint64]),
+    "risltc_proposal_cache_table": (C.c_int, [_vp, _u8p, C.c_uint64]),
     "risltc_add_light_set": (C.c_int, [_vp, _vp, _u32, _u32, _u32p]),
     "risltc_shade_batch": (C.c_int, [_vp, C.POINTER(abi.View), _u32, _vp]),
     "risltc_get_tuning": (C.c_int, [_vp, C.POINTER(abi.Tuning)]),
@@ -210,6 +214,7 @@ class Context:
         buf = np.frombuffer(records, dtype=np.uint8)
         self._check(self.L.risltc_set_lights(self.h, buf.ctypes.data_as(_vp), count, max_vertex_count),
                     "risltc_set_lights")
+        self._light_count = count
 
     def set_ltc_table(self, rgba16, rg16):
         rgba = np.ascontiguousarray(rgba16, dtype=np.uint16)
@@ -300,6 +305,31 @@ class Context:
                     "risltc_proposal_records")
         torch.cuda.synchronize()
         return cl[:n.value], y[:n.value]
+
+    def proposal_cache(self, log2_entries, cell_size=0.5, prior_strength=2.0, per_light=False):
+        """Allocates and clears the visibility cache (log2_entries = 0 removes
+        it); slots are light clusters, or lights with per_light."""
+        self._check(self.L.risltc_proposal_cache(self.h, log2_entries, cell_size, prior_strength,
+                                                 1 if per_light else 0), "risltc_proposal_cache")
+        self._cache_shape = ((1 << log2_entries), self._light_count if per_light else 256, 2)
+
+    def proposal_cache_read(self):
+        """The cache as a uint32 array [cells, slots, 2] (unoccluded, occluded)."""
+        out = np.zeros(int(np.prod(self._cache_shape)), dtype=np.uint32)
+        self._check(self.L.risltc_proposal_cache_read(self.h, _ptr(out, _u32p), out.size),
+                    "risltc_proposal_cache_read")
+        return out.reshape(self._cache_shape)
+
+    def proposal_cache_table(self):
+        """The cache's bytes [cells, slots] as of the last shading call."""
+        out = np.zeros(self._cache_shape[0] * self._cache_shape[1], dtype=np.uint8)
+        self._check(self.L.risltc_proposal_cache_table(self.h, _ptr(out, _u8p), out.size),
+                    "risltc_proposal_cache_table")
+        return out.reshape(self._cache_shape[:2])
+
+    def proposal_cache_write(self, counts):
+        c = np.ascontiguousarray(counts, dtype=np.uint32).ravel()
+        self._check(self.L.risltc_proposal_cache_write(self.h, _ptr(c, _u32p), c.size), "risltc_proposal_cache_write")
 
     def proposal_last_timings(self):
         ms = (C.c_float * 2)()

@@ -25,6 +25,7 @@ PROPOSAL_PARAMS = 31168
 # risltc_sample_record_t word offsets
 REC_LIGHT, REC_WSUM, REC_PHAT, REC_W, REC_CLIPD, REC_CLIPS, REC_PSAD, REC_PSAS = range(8)
 REC_DIR, REC_TMAX, REC_STATE, REC_EVIS, REC_EOCC, REC_COLOR, REC_SEED, REC_POS = 8, 14, 16, 18, 24, 30, 33, 34
+REC_CACHE_KEY = 37
 
 
 def light_record_size(max_vertex_count):
@@ -76,7 +77,7 @@ class Settings(C.Structure):
                 ("ris_candidate_count", C.c_uint32), ("fast_atan", C.c_uint32),
                 ("shadow_rays", C.c_uint32), ("proposal", C.c_uint32),
                 ("exact_math", C.c_uint32), ("learned_floor_log2", C.c_uint32),
-                ("reserved", C.c_uint32 * 5)]
+                ("skip_training_records", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
 def default_settings(**kw):

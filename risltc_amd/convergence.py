@@ -24,36 +24,58 @@ def rel_mse(x, ref):
     return float((((xs - r) ** 2) / (r * r + eps)).mean())
 
 
-def accumulate(ctx, scene, vis, frames, seed0, learned=None, checkpoints=(), ref=None):
+def accumulate(ctx, scene, vis, frames, seed0, learned=None, checkpoints=(), ref=None, candidates=None):
     """Accumulates `frames` frames. learned = None (plain) or a dict with
-    floor_log2, infer_every, batch, lr, clusters, seed. Returns (image,
-    gpu_ms, curve) where curve maps spp -> (relmse, gpu_ms) at checkpoints."""
+    floor_log2, infer_every, batch, lr, clusters, seed, and optionally
+    cache_log2, cache_cell, cache_prior (the visibility cache) and candidates
+    (M). Returns (image, gpu_ms, curve) where curve maps spp -> (relmse,
+    gpu_ms) at checkpoints."""
     import torch
     P = scene.width * scene.height
     out = torch.empty(P * 4, dtype=torch.float32, device="cuda")
     acc = torch.zeros(P * 4, dtype=torch.float32, device="cuda")
+    lk = learned or {}
+    m = lk.get("candidates", candidates)
     s = scene.settings(proposal=abi.PROPOSAL_LEARNED if learned else abi.PROPOSAL_UNIFORM,
-                       learned_floor_log2=(learned or {}).get("floor_log2", 0))
+                       learned_floor_log2=lk.get("floor_log2", 0),
+                       **({"ris_candidate_count": m} if m else {}))
+    ctx.proposal_cache(0)
     if learned:
         ctx.cluster_lights(learned.get("clusters", 256), 20)
         ctx.proposal_init(learned.get("seed", 1))
+        if learned.get("cache_log2"):
+            ctx.proposal_cache(learned["cache_log2"], learned.get("cache_cell", 0.5), learned.get("cache_prior", 2.0),
+                               learned.get("cache_per_light", False))
+    try:
+        return _frames(ctx, scene, vis, frames, seed0, learned, checkpoints, ref, s, out, acc, P)
+    finally:
+        ctx.proposal_cache(0)
+
+
+def _frames(ctx, scene, vis, frames, seed0, learned, checkpoints, ref, s, out, acc, P):
     gpu = 0.0
     curve = {}
+    kernel_ms = [0.0, 0.0, 0.0, 0.0, 0.0]  # shade, trace, resolve, inference, training
     for f in range(frames):
         c = ctx.frame_constants(noise_seed=seed0 + f, animate=True)
         if learned and f % learned.get("infer_every", 1) == 0:
             ctx.proposal_update(c, vis)
-            gpu += ctx.proposal_last_timings()[0]
+            kernel_ms[3] += ctx.proposal_last_timings()[0]
+        train = bool(learned) and f % learned.get("train_every", 1) == 0
+        s.skip_training_records = 0 if train else 1
         ctx.shade(c, s, vis, out)
         t = ctx.last_timings()
-        gpu += t[0] + t[1] + t[2]
+        for k in range(3):
+            kernel_ms[k] += t[k]
         ctx.accumulate(acc, out, P, f)
-        if learned:
+        if train:
             ctx.proposal_train(c, vis, batch=learned.get("batch", 16384), steps=1, lr=learned.get("lr", 2e-3),
                                seed=seed0 + f)
-            gpu += ctx.proposal_last_timings()[1]
+            kernel_ms[4] += ctx.proposal_last_timings()[1]
+        gpu = sum(kernel_ms)
         if ref is not None and (f + 1) in checkpoints:
             curve[(f + 1) * scene.spp] = (rel_mse(acc, ref), gpu)
+    _frames.kernel_ms = [k / max(frames, 1) for k in kernel_ms]
     return acc, gpu, curve
 
 
@@ -74,24 +96,73 @@ def time_to(curve, target):
     return ms * rel / target, True
 
 
-def compare(ctx, scene, vis, target=0.01, ref_frames=2048, frames=256, learned=None):
+# The learned configuration the experiment uses: target floor 1/128, per-light
+# visibility cache of 2^16 one-metre cells, MLP inference every 16 frames and
+# one training step every 4, M = 48 candidates.
+DEFAULT_LEARNED = dict(floor_log2=7, infer_every=16, train_every=4, batch=16384, lr=2e-3, clusters=256, seed=1,
+                       cache_log2=16, cache_cell=1.0, cache_prior=1.0, cache_per_light=True, candidates=48)
+
+
+def reference(ctx, scene, vis, frames, chunk=2048, seed0=1000000, log=None):
+    """Plain-pass reference: the mean of frames frames (chunks of `chunk`
+    accumulated on the device, their means averaged). log(msg) is called
+    after each chunk."""
+    import torch
+    ref = None
+    n = 0
+    while n < frames:
+        k = min(chunk, frames - n)
+        img, _, _ = accumulate(ctx, scene, vis, k, seed0 + n)
+        ref = img * (k / frames) if ref is None else ref + img * (k / frames)
+        n += k
+        torch.cuda.synchronize()
+        if log:
+            log("reference: %d of %d frames" % (n, frames))
+    return ref
+
+
+def compare(ctx, scene, vis, target=0.01, ref_frames=16384, plain_frames=320, frames=256, learned=None,
+            ablations=True, log=None):
     """The learned-vs-plain experiment on one scene; returns a dict for the
-    bench line."""
-    learned = learned or dict(floor_log2=5, infer_every=4, batch=16384, lr=2e-3, clusters=256, seed=1)
-    ref, ref_ms, _ = accumulate(ctx, scene, vis, ref_frames, 1000000)
-    cps = tuple(2 ** k for k in range(4, 20) if 2 ** k <= frames)
-    _, _, plain = accumulate(ctx, scene, vis, frames, 0, checkpoints=cps, ref=ref)
-    _, _, learn = accumulate(ctx, scene, vis, frames, 0, learned=learned, checkpoints=cps, ref=ref)
-    tp, ep = time_to(plain, target)
-    tl, el = time_to(learn, target)
-    last = max(plain)
-    return {
+    bench line. The plain pass is the scene's own configuration (M and spp);
+    the learned pass uses `learned` (DEFAULT_LEARNED by default). Ablations:
+    the learned pass without the visibility cache (MLP only), and with the MLP
+    left untrained (learning rate 0)."""
+    learned = dict(DEFAULT_LEARNED, **(learned or {}))
+    ref = reference(ctx, scene, vis, ref_frames, log=log)
+    cps = tuple(k for k in (16, 32, 64, 128, 256, 320, 512) if k <= max(frames, plain_frames))
+    spf = scene.width * scene.height * scene.spp
+
+    def run(lk, n):
+        _, _, curve = accumulate(ctx, scene, vis, n, 0, learned=lk, checkpoints=cps, ref=ref)
+        km = list(_frames.kernel_ms)
+        t, ex = time_to(curve, target)
+        return {"time_to_target_ms": round(t, 1), "extrapolated": ex,
+                "curve": {str(k): [round(v[0], 6), round(v[1], 1)] for k, v in sorted(curve.items())},
+                "ms_per_frame": {"shade": round(km[0], 4), "trace": round(km[1], 4), "resolve": round(km[2], 4),
+                                 "inference": round(km[3], 4), "training": round(km[4], 4)},
+                "msamples_per_s": round(spf / (sum(km) * 1e-3) / 1e6, 2)}
+
+    plain = run(None, plain_frames)
+    learn = run(learned, frames)
+    if log:
+        log("plain %.1f ms, learned %.1f ms to relMSE %g" % (plain["time_to_target_ms"], learn["time_to_target_ms"],
+                                                             target))
+    out = {
         "target_relmse": target,
         "reference_spp": ref_frames * scene.spp,
-        "time_to_target_ms": {"plain": round(tp, 1), "learned": round(tl, 1)},
-        "extrapolated": {"plain": ep, "learned": el},
-        "learned_over_plain": round(tl / tp, 4),
-        "relmse_at_%dspp" % last: {"plain": round(plain[last][0], 5), "learned": round(learn[last][0], 5)},
-        "gpu_ms_at_%dspp" % last: {"plain": round(plain[last][1], 1), "learned": round(learn[last][1], 1)},
+        "learned_over_plain": round(learn["time_to_target_ms"] / plain["time_to_target_ms"], 4),
+        "plain": plain,
+        "learned": learn,
         "learned_config": learned,
     }
+    if ablations:
+        no_cache = dict(learned, cache_log2=0)
+        untrained = dict(learned, lr=0.0)
+        out["ablations"] = {}
+        for name, lk in (("mlp_only", no_cache), ("cache_with_untrained_mlp", untrained)):
+            r = run(lk, frames)
+            out["ablations"][name] = {"time_to_target_ms": r["time_to_target_ms"],
+                                      "over_plain": round(r["time_to_target_ms"] / plain["time_to_target_ms"], 4),
+                                      "relmse_at_1024spp": r["curve"].get("1024", [None])[0]}
+    return out

@@ -98,6 +98,11 @@ struct risltc_context_s {
     size_t rec_samples = 0;  // pixel-samples covered by the training records
     uint32_t rec_spp = 0;
     float prop_ms[2] = {0, 0};  // last inference, last training call
+    // visibility cache: 2^vcache_log2 cells x vcache_slots x (uint2 counts,
+    // byte), a dirty byte per cell
+    DevBuf vcache, vcache_vis, vcache_dirty;
+    uint32_t vcache_log2 = 0, vcache_slots = 0, vcache_per_light = 0;
+    float vcache_cell = 1.0f, vcache_prior = 2.0f;
     // launch shape (autotuned; RISLTC_TRACE_REFILL overrides the refill at create)
     risltc_tuning_t tuning = {16, 32, rl::kLdsSphereMax, 32};
     // extra light sets for batched views (id = index + 1)
@@ -488,7 +493,8 @@ void risltc_destroy(risltc_context_t* c) {
     c->batch.release();
     DevBuf* learned_bufs[] = {&c->cluster_of_light, &c->cluster_offsets, &c->cluster_lights, &c->mlp_params,
                               &c->mlp_m, &c->mlp_v, &c->mlp_grad, &c->mlp_wf, &c->mlp_wb, &c->mlp_loss,
-                              &c->train_scratch, &c->prop_table, &c->rec_cluster, &c->rec_y};
+                              &c->train_scratch, &c->prop_table, &c->rec_cluster, &c->rec_y, &c->vcache, &c->vcache_vis,
+                              &c->vcache_dirty};
     for (DevBuf* b : learned_bufs) b->release();
     DevBuf* bufs[] = {&c->lights, &c->spheres, &c->ltc0, &c->ltc1, &c->positions, &c->normals_uv, &c->material_idx,
                       &c->materials, &c->samples, &c->rays, &c->ray_slots, &c->hits, &c->counter};
@@ -583,6 +589,36 @@ static int pack_lights(risltc_context_t* c, const void* records, uint32_t count,
     return RISLTC_OK;
 }
 
+static size_t vcache_words(const risltc_context_t* c) {
+    return c->vcache.p ? ((size_t)2 * c->vcache_slots) << c->vcache_log2 : 0;
+}
+
+// empties the visibility cache (its cells or slots changed meaning): zero
+// counts, bytes 255 (empty), no dirty cell
+static int clear_vcache(risltc_context_t* c) {
+    if (!c->vcache.p) return RISLTC_OK;
+    HIPC(c, hipSetDevice(c->device));
+    const size_t entries = vcache_words(c) / 2, cells = (size_t)1 << c->vcache_log2;
+    HIPC(c, rl::launch_zero_u32(c->vcache.as<uint32_t>(), (uint32_t)vcache_words(c), nullptr));
+    HIPC(c, rl::launch_fill_u32(c->vcache_vis.as<uint32_t>(), (uint32_t)((entries + 3) / 4), 0xFFFFFFFFu, nullptr));
+    HIPC(c, rl::launch_zero_u32(c->vcache_dirty.as<uint32_t>(), (uint32_t)((cells + 3) / 4), nullptr));
+    HIPC(c, hipDeviceSynchronize());
+    return RISLTC_OK;
+}
+
+static rl::VisCacheView vcache_view(const risltc_context_t* c) {
+    rl::VisCacheView v;
+    v.counts = c->vcache.as<uint2>();
+    v.vis = c->vcache_vis.as<uint8_t>();
+    v.dirty = c->vcache_dirty.as<uint8_t>();
+    v.mask = (1u << c->vcache_log2) - 1u;
+    v.inv_cell = 1.0f / c->vcache_cell;
+    v.prior = c->vcache_prior;
+    v.slots = c->vcache_slots;
+    v.per_light = c->vcache_per_light;
+    return v;
+}
+
 int risltc_set_lights(risltc_context_t* c, const void* records, uint32_t count, uint32_t vmax) {
     if (!c) return RISLTC_E_ARG;
     PackedLights pl;
@@ -599,7 +635,7 @@ int risltc_set_lights(risltc_context_t* c, const void* records, uint32_t count, 
     c->light_centroids = std::move(pl.centroids);
     // clusters refer to the previous light set
     c->cluster_count = 0;
-    return RISLTC_OK;
+    return clear_vcache(c);
 }
 
 int risltc_add_light_set(risltc_context_t* c, const void* records, uint32_t count, uint32_t vmax, uint32_t* id) {
@@ -672,6 +708,70 @@ int risltc_set_light_clusters(risltc_context_t* c, const uint32_t* cluster_of_li
     HIPC(c, hipMemcpy(c->cluster_offsets.p, offsets.data(), offsets.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIPC(c, hipMemcpy(c->cluster_lights.p, sorted.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
     c->cluster_count = cluster_count;
+    return clear_vcache(c);
+}
+
+int risltc_proposal_cache(risltc_context_t* c, uint32_t log2_entries, float cell_size, float prior,
+                          uint32_t per_light) {
+    if (!c) return RISLTC_E_ARG;
+    if (log2_entries == 0) {
+        c->vcache.release();
+        c->vcache_vis.release();
+        c->vcache_dirty.release();
+        c->vcache_log2 = 0;
+        return RISLTC_OK;
+    }
+    if (log2_entries > 20) return fail(c, RISLTC_E_UNSUPPORTED, "at most 2^20 cache cells");
+    if (!(cell_size > 0.0f) || !std::isfinite(1.0f / cell_size))
+        return fail(c, RISLTC_E_ARG, "cell size %g", (double)cell_size);
+    if (!(prior > 0.0f) || !std::isfinite(prior)) return fail(c, RISLTC_E_ARG, "prior strength %g", (double)prior);
+    if (per_light > 1) return fail(c, RISLTC_E_ARG, "per_light %u", per_light);
+    if (per_light && c->light_count == 0) return fail(c, RISLTC_E_STATE, "risltc_set_lights has not been called");
+    const uint32_t slots = per_light ? c->light_count : (uint32_t)rl::kMlpOut;
+    const size_t words = ((size_t)2 * slots) << log2_entries;
+    if (words > ((size_t)1 << 30)) return fail(c, RISLTC_E_UNSUPPORTED, "cache larger than 4 GiB");
+    HIPC(c, hipSetDevice(c->device));
+    c->vcache_log2 = 0;
+    HIPC(c, c->vcache.ensure(words * sizeof(uint32_t)));
+    HIPC(c, c->vcache_vis.ensure((words / 2 + 3) / 4 * 4));
+    HIPC(c, c->vcache_dirty.ensure((((size_t)1 << log2_entries) + 3) / 4 * 4));
+    c->vcache_log2 = log2_entries;
+    c->vcache_slots = slots;
+    c->vcache_per_light = per_light;
+    c->vcache_cell = cell_size;
+    c->vcache_prior = prior;
+    return clear_vcache(c);
+}
+
+int risltc_proposal_cache_read(risltc_context_t* c, uint32_t* out, uint64_t capacity) {
+    if (!c) return RISLTC_E_ARG;
+    if (!c->vcache.p) return fail(c, RISLTC_E_STATE, "no visibility cache");
+    if (!out || capacity < vcache_words(c)) return fail(c, RISLTC_E_ARG, "output smaller than the cache");
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipMemcpy(out, c->vcache.p, vcache_words(c) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return RISLTC_OK;
+}
+
+int risltc_proposal_cache_write(risltc_context_t* c, const uint32_t* counts, uint64_t words) {
+    if (!c) return RISLTC_E_ARG;
+    if (!c->vcache.p) return fail(c, RISLTC_E_STATE, "no visibility cache");
+    if (!counts || words != vcache_words(c)) return fail(c, RISLTC_E_ARG, "expected %zu words", vcache_words(c));
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipMemcpy(c->vcache.p, counts, words * sizeof(uint32_t), hipMemcpyHostToDevice));
+    // every cell is recomputed by the next shading call
+    const size_t cells = (size_t)1 << c->vcache_log2;
+    HIPC(c, rl::launch_fill_u32(c->vcache_dirty.as<uint32_t>(), (uint32_t)((cells + 3) / 4), 0x01010101u, nullptr));
+    HIPC(c, hipDeviceSynchronize());
+    return RISLTC_OK;
+}
+
+int risltc_proposal_cache_table(risltc_context_t* c, uint8_t* out, uint64_t capacity) {
+    if (!c) return RISLTC_E_ARG;
+    if (!c->vcache.p) return fail(c, RISLTC_E_STATE, "no visibility cache");
+    const size_t entries = vcache_words(c) / 2;
+    if (!out || capacity < entries) return fail(c, RISLTC_E_ARG, "output smaller than the cache");
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipMemcpy(out, c->vcache_vis.p, entries, hipMemcpyDeviceToHost));
     return RISLTC_OK;
 }
 
@@ -990,7 +1090,7 @@ int risltc_set_scene(risltc_context_t* c, const uint32_t* positions, const uint1
     rl::free_bvh(&c->bvh);
     HIPC(c, rl::build_bvh(c->positions.as<uint2>(), tri_count, deq_factor, deq_summand, c->bvh_builder, &c->bvh,
                           nullptr));
-    return RISLTC_OK;
+    return clear_vcache(c);
 }
 
 static rl::BvhView bvh_view(const risltc_context_t* c) {
@@ -1038,6 +1138,8 @@ static int check_settings(risltc_context_t* c, const risltc_frame_constants_t* k
         return fail(c, RISLTC_E_ARG, "ris_candidate_count %u", s->ris_candidate_count);
     if (s->proposal > RISLTC_PROPOSAL_LEARNED) return fail(c, RISLTC_E_ARG, "proposal %u", s->proposal);
     if (s->learned_floor_log2 > 16) return fail(c, RISLTC_E_ARG, "learned_floor_log2 %u", s->learned_floor_log2);
+    if (s->skip_training_records > 1)
+        return fail(c, RISLTC_E_ARG, "skip_training_records %u", s->skip_training_records);
     if (s->shadow_rays && !c->bvh.rec) return fail(c, RISLTC_E_STATE, "no hierarchy for shadow rays");
     if (k->viewport_size[0] == 0 || k->viewport_size[1] == 0) return fail(c, RISLTC_E_ARG, "empty viewport");
     const size_t S = (size_t)k->viewport_size[0] * k->viewport_size[1] * s->sample_count_light;
@@ -1181,8 +1283,17 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     a.debug_samples = dbg_s;
     a.debug_cands = dbg_c;
     a.proposal = learned ? c->prop_table.as<uint8_t>() : nullptr;
+    const bool cached = c->vcache.p && c->cluster_count > 0;
+    if (cached) {
+        if (c->vcache_per_light && c->vcache_slots != c->light_count)
+            return fail(c, RISLTC_E_STATE, "the visibility cache was made for %u lights", c->vcache_slots);
+        a.cache = vcache_view(c);
+    }
     HIPC(c, rl::launch_zero_u32(c->counter.as<uint32_t>(), kCounterWords, st));
     HIPC(c, hipEventRecord(c->ev[0], st));
+    // the cells the previous resolve counted in get their new bytes (timed
+    // with the shade kernel)
+    if (cached) HIPC(c, rl::launch_refresh_cache(a.cache, st));
     HIPC(c, rl::launch_shade(a, s->exact_math != 0, st));
     HIPC(c, hipEventRecord(c->ev[1], st));
     if (s->shadow_rays)
@@ -1191,15 +1302,19 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     rl::ResolveArgs r;
     fill_resolve(c, k, s, vis, a.samples, c->hits.as<uint32_t>(), out, r);
     r.debug_samples = dbg_s;
+    c->rec_samples = 0;
     if (c->cluster_count > 0) {
-        // training records for the proposal: one per pixel-sample
-        HIPC(c, c->rec_cluster.ensure(S * sizeof(uint32_t)));
-        HIPC(c, c->rec_y.ensure(S * sizeof(float)));
         r.cluster_of_light = c->cluster_of_light.as<uint8_t>();
-        r.rec_cluster = c->rec_cluster.as<uint32_t>();
-        r.rec_y = c->rec_y.as<float>();
-        c->rec_samples = S;
-        c->rec_spp = s->sample_count_light;
+        if (!s->skip_training_records) {
+            // training records for the proposal: one per pixel-sample
+            HIPC(c, c->rec_cluster.ensure(S * sizeof(uint32_t)));
+            HIPC(c, c->rec_y.ensure(S * sizeof(float)));
+            r.rec_cluster = c->rec_cluster.as<uint32_t>();
+            r.rec_y = c->rec_y.as<float>();
+            c->rec_samples = S;
+            c->rec_spp = s->sample_count_light;
+        }
+        if (cached) r.cache = a.cache;
     }
     HIPC(c, rl::launch_resolve(r, st));
     HIPC(c, hipEventRecord(c->ev[3], st));

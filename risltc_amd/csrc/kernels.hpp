@@ -109,6 +109,29 @@ struct ProposalArgs {
 };
 hipError_t launch_proposal(const ProposalArgs& a, hipStream_t s);
 
+// Online visibility cache: per hashed world-space cell (cell size `cell`,
+// split by the dominant axis and sign of the shading normal) and slot, the
+// counts of unoccluded and occluded shadow rays of the final samples whose
+// light maps to that slot: its cluster (slots = kMlpOut) or the light itself
+// (per_light, slots = light count). Entry (key, slot) is
+// counts[key * slots + slot] = (a unoccluded, b occluded). The shade kernel
+// reads a byte per entry, vis[key * slots + slot]: 255 while the entry is
+// empty (the MLP's byte is used instead), else
+// round(254 (a + prior / 2) / (a + b + prior)). Resolve marks the cells it
+// counts in `dirty`; refresh_cache recomputes the bytes of dirty cells.
+struct VisCacheView {
+    uint2* counts;      // null = no cache
+    uint8_t* vis;       // bytes, as above
+    uint8_t* dirty;     // one byte per cell
+    uint32_t mask;      // cells - 1 (a power of two)
+    float inv_cell;     // 1 / cell size
+    float prior;        // pseudo-counts of the symmetric Beta prior
+    uint32_t slots;     // per cell
+    uint32_t per_light; // slot = light index (1) or its cluster (0)
+};
+// recomputes the bytes of every dirty cell and clears its flag
+hipError_t launch_refresh_cache(const VisCacheView& c, hipStream_t s);
+
 // one optimisation step on a batch of (pixel, cluster, weight) items
 struct TrainArgs {
     FeatureView f;
@@ -155,6 +178,9 @@ struct ShadeArgs {
     const uint8_t* proposal;   // kPropBytes per pixel
     ClusterView clusters;
     float learned_floor, learned_scale;  // f and (1 - f) / 255 (FP32)
+    // cache keys are computed (and written to the sample records) whenever
+    // counts is set; the learned target reads it
+    VisCacheView cache;
 };
 
 // The shadow-ray list is cut into kTraceQueues equal slices, one per XCD:
@@ -190,6 +216,9 @@ struct ResolveArgs {
     const uint8_t* cluster_of_light;
     uint32_t* rec_cluster;
     float* rec_y;
+    // visibility cache (optional): every traced shadow ray of a sample is
+    // counted at (the sample's cache key, its light's slot)
+    VisCacheView cache;
 };
 
 // host-side launchers (shade.hip, lbvh.hip)
@@ -213,6 +242,7 @@ hipError_t upload_clip_table(const uint8_t* table, size_t bytes);
 // clears device words with a kernel (no memset nodes, so it is safe under
 // stream capture)
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
 
 struct BvhBuild {
     float4* rec = nullptr;

@@ -1052,12 +1052,64 @@ __device__ __forceinline__ uint32_t lcg_jump(uint32_t s, uint32_t d) {
 // ---- learned RIS target ----------------------------------------------------------
 // With a proposal table the candidates still come from the reference's
 // uniform source; the target of candidate light l becomes
-// p_hat * (floor + v * scale), v = the table's byte for the cluster of l,
-// scale = (1 - floor) / 255. The oracle evaluates the same expression.
+// p_hat * (floor + v * scale), scale = (1 - floor) / 255, where v is the
+// visibility cache's byte for the pixel's cell and l's slot or, while that
+// entry is empty (255) or without a cache, the table's byte for l's cluster.
+// The oracle evaluates the same expression.
 __device__ __forceinline__ float learned_factor(const uint8_t* table, const uint8_t* of_light, uint32_t idx,
-                                                float floor, float scale) {
-    float v = (float)table[of_light[idx]];
-    return floor + v * scale;
+                                                float floor, float scale, const uint8_t* cache_row,
+                                                const uint8_t* lds, int32_t lds_row, uint32_t per_light) {
+    const uint32_t cl = of_light[idx];
+    const uint32_t slot = per_light ? idx : cl;
+    uint32_t v = 255u;
+    if (lds_row >= 0) v = lds[(uint32_t)lds_row + slot];
+    else if (cache_row) v = cache_row[slot];
+    // the table row is per pixel (a scattered load): read only on a miss
+    if (v == 255u) v = table[cl];
+    return floor + (float)v * scale;
+}
+
+// The cache byte of counts (a, b): round(254 (a + prior / 2) / (a + b + prior)),
+// 255 for an empty entry. IEEE operations in this order; the oracle
+// restates it.
+__device__ __forceinline__ uint32_t cache_byte(uint2 c, float prior) {
+    if (c.x == 0u && c.y == 0u) return 255u;
+    float a = (float)c.x, b = (float)c.y;
+    float v = div_<true>((a + prior * 0.5f) * 254.0f, (a + b) + prior) + 0.5f;
+    uint32_t q = (uint32_t)v;
+    return q > 254u ? 254u : q;
+}
+
+// The learned target reads the cache rows of a wave's cells from LDS
+// (staged once per wave) instead of by scattered global loads. Up to
+// kRowsPerWave rows of up to kRowLdsMax slots per wave; the row area follows
+// the spheres and the cluster table, 16-byte aligned.
+constexpr uint32_t kRowLdsMax = 1024, kRowsPerWave = 4;
+
+__device__ __host__ __forceinline__ size_t row_lds_offset(uint32_t n_lds, bool cluster_table) {
+    size_t b = (size_t)n_lds * 16u + (cluster_table ? n_lds : 0u);
+    return (b + 15u) & ~(size_t)15u;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Cache key of a shading point: its cell floor(p / cell) per axis and the
+// dominant axis and sign of the shading normal (the first axis on ties),
+// combined with MurmurHash3's mix and finaliser, masked to the table.
+__device__ __forceinline__ uint32_t cache_key(const VisCacheView& C, f3 p, f3 n) {
+    const float lim = 1.0e9f;
+    int32_t ix = (int32_t)__builtin_floorf(clamp_(p.x * C.inv_cell, -lim, lim));
+    int32_t iy = (int32_t)__builtin_floorf(clamp_(p.y * C.inv_cell, -lim, lim));
+    int32_t iz = (int32_t)__builtin_floorf(clamp_(p.z * C.inv_cell, -lim, lim));
+    float ax = fabs_(n.x), ay = fabs_(n.y), az = fabs_(n.z);
+    uint32_t code = (ax >= ay && ax >= az) ? (n.x < 0.0f ? 1u : 0u)
+                                           : (ay >= az ? (n.y < 0.0f ? 3u : 2u) : (n.z < 0.0f ? 5u : 4u));
+    uint32_t h = murmur_mix(murmur_mix(murmur_mix(murmur_mix(0u, (uint32_t)ix), (uint32_t)iy), (uint32_t)iz), code);
+    return murmur_finalize(h) & C.mask;
 }
 
 // ---- the kernel ----------------------------------------------------------------
@@ -1111,6 +1163,40 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
         cull = make_cull(px, A.scene.sphere_bound);
         seed = murmur_finalize(murmur_mix(murmur_mix(0u, x + y * W), A.c.noise_random_numbers[0]));
     }
+    const uint32_t ckey = (shade && A.cache.counts) ? cache_key(A.cache, px.pos, px.n) : 0u;
+    // the learned target's cache rows: the wave stages the rows of up to
+    // kRowsPerWave distinct cells of its shaded lanes in LDS; a lane whose
+    // cell did not fit reads its row from global memory
+    const uint8_t* crow = nullptr;
+    int32_t lds_row = -1;  // byte offset of the lane's staged row in the LDS area, or -1
+    if constexpr (learned) {
+        if (A.cache.vis) {
+            crow = A.cache.vis + (size_t)ckey * A.cache.slots;
+            if (A.cache.slots <= kRowLdsMax) {
+                const uint32_t wave_base = (uint32_t)row_lds_offset(n_lds, learned && all_lds) +
+                                           wave * kRowsPerWave * kRowLdsMax;
+                uint8_t* area = reinterpret_cast<uint8_t*>(sh_spheres) + wave_base;
+                const uint32_t slots = A.cache.slots;
+                uint64_t pending = __ballot(shade);
+                for (uint32_t r = 0; r < kRowsPerWave && pending != 0ull; ++r) {
+                    const int first = __ffsll((unsigned long long)pending) - 1;
+                    const uint32_t kr = (uint32_t)__shfl((int)ckey, first);
+                    const bool mine = shade && ckey == kr;
+                    pending &= ~__ballot(mine);
+                    uint8_t* dst = area + (size_t)r * kRowLdsMax;
+                    const uint8_t* src = A.cache.vis + (size_t)kr * slots;
+                    if ((slots & 15u) == 0u) {
+                        for (uint32_t o = lane * 16u; o < slots; o += 1024u)
+                            *reinterpret_cast<uint4*>(dst + o) = *reinterpret_cast<const uint4*>(src + o);
+                    } else {
+                        for (uint32_t o = lane; o < slots; o += 64u) dst[o] = src[o];
+                    }
+                    if (mine) lds_row = (int32_t)(wave_base + r * kRowLdsMax);
+                }
+                wave_lds_sync();
+            }
+        }
+    }
     const uint32_t N = A.scene.light_count;
     const float pN = 1.0f / (float)N;
     const uint32_t M = A.M;
@@ -1157,12 +1243,18 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                         at = t + 1u;
                         uint32_t idx = light_index_from(noise_next(s), N);
                         load_light(A.scene.lights, idx, L);
+                        // the learned factor first: its table and cache loads
+                        // are in flight during the target's evaluation
+                        float factor = 1.0f;
+                        if constexpr (learned)
+                            factor = learned_factor(A.proposal + (size_t)pixel * kPropBytes, of_light, idx,
+                                                    A.learned_floor, A.learned_scale, crow,
+                                                    reinterpret_cast<const uint8_t*>(sh_spheres), lds_row,
+                                                    A.cache.per_light);
                         uint32_t cc;
                         f3 col = eval_ltc_cp<X>(px, L, cc);
                         float p_hat = sqrt_<X>(dot3(col, col));
-                        if constexpr (learned)
-                            p_hat = p_hat * learned_factor(A.proposal + (size_t)pixel * kPropBytes, of_light, idx,
-                                                           A.learned_floor, A.learned_scale);
+                        if constexpr (learned) p_hat = p_hat * factor;
                         float w = div_<X>(p_hat, pN);
                         float r = noise_next(s);
                         w_sum += w;
@@ -1236,7 +1328,8 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
             rec[0] = make_float4(Wt, __uint_as_float(flags), F.e_vis[0].x, F.e_vis[0].y);
             rec[1] = make_float4(F.e_vis[0].z, F.e_vis[1].x, F.e_vis[1].y, F.e_vis[1].z);
             rec[2] = make_float4(F.e_occ[0].x, F.e_occ[0].y, F.e_occ[0].z, F.e_occ[1].x);
-            rec[3] = make_float4(F.e_occ[1].y, F.e_occ[1].z, __int_as_float(contributes ? chosen : -1), 0.0f);
+            rec[3] = make_float4(F.e_occ[1].y, F.e_occ[1].z, __int_as_float(contributes ? chosen : -1),
+                                 __uint_as_float(ckey));
         }
         if constexpr (DEBUG) {
             if (inside && A.debug_samples) {
@@ -1267,6 +1360,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                 r[31] = F.color.y;
                 r[32] = F.color.z;
                 r[33] = __uint_as_float(seed);
+                r[37] = __uint_as_float(ckey);
                 if (contributes) {
                     r[34] = px.pos.x;
                     r[35] = px.pos.y;
@@ -1354,11 +1448,25 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
                 float Wt = r0.x;
                 float s = (float)A.spp;
                 fin = add3(fin, mk3((col.x * Wt) / s, (col.y * Wt) / s, (col.z * Wt) / s));
-                if (A.rec_cluster) {
+                if (A.rec_cluster || A.cache.counts) {
                     int32_t li = __float_as_int(r3.z);
                     bool ok = li >= 0 && traced > 0u;
-                    A.rec_cluster[sidx] = ok ? (uint32_t)A.cluster_of_light[li] : 0xFFFFFFFFu;
-                    A.rec_y[sidx] = ok ? (float)unoccluded / (float)traced : 0.0f;
+                    uint32_t cl = ok ? (uint32_t)A.cluster_of_light[li] : 0xFFFFFFFFu;
+                    if (A.rec_cluster) {
+                        A.rec_cluster[sidx] = cl;
+                        A.rec_y[sidx] = ok ? (float)unoccluded / (float)traced : 0.0f;
+                    }
+                    if (A.cache.counts && ok) {
+                        // (unoccluded, occluded) in one 64-bit add
+                        unsigned long long add =
+                            (unsigned long long)unoccluded | ((unsigned long long)(traced - unoccluded) << 32);
+                        const uint32_t slot = A.cache.per_light ? (uint32_t)li : cl;
+                        const uint32_t key = __float_as_uint(r3.w);
+                        atomicAdd(reinterpret_cast<unsigned long long*>(A.cache.counts + (size_t)key * A.cache.slots +
+                                                                        slot),
+                                  add);
+                        if (A.cache.dirty[key] == 0u) A.cache.dirty[key] = 1u;
+                    }
                 }
             }
         }
@@ -1397,8 +1505,11 @@ uint32_t shade_tiles(const ShadeArgs& a) {
 
 size_t shade_lds_bytes(const ShadeArgs& a) {
     const uint32_t n = a.scene.light_count < a.lds_lights ? a.scene.light_count : a.lds_lights;
-    size_t b = (size_t)n * sizeof(float4);
-    if (a.proposal && n == a.scene.light_count) b += n;
+    const bool table = a.proposal && n == a.scene.light_count;
+    size_t b = (size_t)n * sizeof(float4) + (table ? n : 0u);
+    // learned target with a cache: one staged row per wave
+    if (a.proposal && a.cache.vis && a.cache.slots <= kRowLdsMax)
+        b = row_lds_offset(n, table) + 4u * kRowsPerWave * kRowLdsMax;
     return b;
 }
 
@@ -1460,15 +1571,36 @@ hipError_t launch_accumulate(float* accum, const float* frame, uint32_t count, u
     return hipGetLastError();
 }
 
-__global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
+// one work-group per cell: skips clean cells, recomputes the bytes of a dirty
+// one, then clears its flag
+__global__ void __launch_bounds__(256) refresh_cache_kernel(VisCacheView C) {
+    const uint32_t cell = blockIdx.x;
+    if (C.dirty[cell] == 0u) return;
+    const size_t base = (size_t)cell * C.slots;
+    for (uint32_t k = threadIdx.x; k < C.slots; k += blockDim.x)
+        C.vis[base + k] = (uint8_t)cache_byte(C.counts[base + k], C.prior);
+    __syncthreads();
+    if (threadIdx.x == 0) C.dirty[cell] = 0u;
 }
 
-hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, p, n);
+hipError_t launch_refresh_cache(const VisCacheView& c, hipStream_t s) {
+    if (!c.counts || !c.vis || !c.dirty) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(refresh_cache_kernel, dim3(c.mask + 1u), dim3(256), 0, s, c);
     return hipGetLastError();
 }
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
+    if (n == 0u) return hipSuccess;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) { return launch_fill_u32(p, n, 0u, s); }
 
 hipError_t upload_clip_table(const uint8_t* table, size_t bytes) {
     return hipMemcpyToSymbol(HIP_SYMBOL(c_clip_rot), table, bytes, 0, hipMemcpyHostToDevice);

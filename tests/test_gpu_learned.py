@@ -62,6 +62,16 @@ def hash32(x):
 
 @pytest.fixture(scope="module")
 def learned(small_scene, ltc_table):
+    return _learned_context(small_scene, ltc_table)
+
+
+@pytest.fixture(scope="module")
+def learned_odd(ltc_table):
+    """61 lights: a per-light cache row is not a multiple of 16 bytes."""
+    return _learned_context(synth.small_scene(seed=5, width=96, height=80, light_count=61), ltc_table)
+
+
+def _learned_context(small_scene, ltc_table):
     import risltc_amd
     import torch
     ctx = risltc_amd.Context(0)
@@ -211,3 +221,97 @@ def test_training_lowers_the_loss(learned):
     # the same batch as the first step, after training
     last = ctx.proposal_train(g["consts"], g["vis"], batch=8192, steps=1, lr=0.0, seed=100)
     assert last < 0.9 * first, (first, last)
+
+
+@pytest.mark.parametrize("per_light,odd", [(False, False), (True, False), (True, True)])
+def test_cached_learned_shading_bitwise_vs_oracle(learned, learned_odd, per_light, odd):
+    """Exact build, learned target with a visibility cache of random counts
+    (slots per cluster or per light; 64 or 61 lights): cache keys,
+    candidates, sample records and the frame equal the oracle's given the
+    same tables, clusters and cache."""
+    import torch
+    g = learned_odd if odd else learned
+    sc, ctx, P = g["scene"], g["ctx"], g["P"]
+    log2, cell, prior = 10, 0.5, 2.0
+    rng = np.random.default_rng(77)
+    N = sc.light_count
+    used = N if per_light else g["K"]
+    counts = np.zeros((1 << log2, N if per_light else 256, 2), dtype=np.uint32)
+    counts[:, :used] = rng.integers(0, 60, (1 << log2, used, 2))
+    ctx.proposal_cache(log2, cell, prior, per_light)
+    try:
+        s = sc.settings(exact_math=1, sample_count_light=2, ris_candidate_count=8, proposal=abi.PROPOSAL_LEARNED,
+                        shadow_rays=0)
+        out = torch.empty(P * 4, dtype=torch.float32, device="cuda")
+        rec = torch.zeros(P * 2 * abi.SAMPLE_RECORD_FLOATS, dtype=torch.float32, device="cuda")
+        cand = torch.zeros(P * 2 * 8 * abi.CANDIDATE_RECORD_WORDS, dtype=torch.int32, device="cuda")
+        # a quarter of the entries empty (the MLP's byte is used there)
+        counts[rng.random(counts.shape[:2]) < 0.25] = 0
+        ctx.proposal_cache_write(counts)
+        ctx.shade_debug(g["consts"], s, g["vis"], out, rec, cand)
+        torch.cuda.synchronize()
+        # without shadow rays nothing is traced: the counts are unchanged
+        assert np.array_equal(ctx.proposal_cache_read(), counts)
+        table = ctx.proposal_cache_table()
+    finally:
+        ctx.proposal_cache(0)
+    oracle = orc.Oracle(sc, *g["ltc"])
+    oracle.set_proposal(g["table"], g["col"], g["K"])
+    want = oracle.set_cache(counts, cell, prior, per_light)
+    # the refresh before shading made every byte from the written counts
+    assert np.array_equal(table, want)
+    vis = g["vis"].cpu().numpy().view(np.uint32)
+    ref, rrec, rcand = oracle.shade(g["consts"], s, vis, records=True, candidates=True)
+    r = rec.cpu().numpy().reshape(rrec.shape)
+    assert np.array_equal(r[..., abi.REC_CACHE_KEY].view(np.uint32), rrec[..., abi.REC_CACHE_KEY].view(np.uint32))
+    assert np.unique(rrec[..., abi.REC_CACHE_KEY].view(np.uint32)).size > 50
+    c = cand.cpu().numpy().view(np.uint32).reshape(rcand.shape)
+    assert np.array_equal(c, rcand)
+    for f in (abi.REC_LIGHT, abi.REC_WSUM, abi.REC_PHAT, abi.REC_W):
+        assert np.array_equal(r[..., f].view(np.uint32), rrec[..., f].view(np.uint32)), f
+    o = out.cpu().numpy().reshape(P, 4)
+    assert np.array_equal(o.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("per_light", [False, True])
+def test_resolve_counts_every_traced_ray(learned, per_light):
+    """Two frames with shadow rays: the cache holds, per (key, slot), the
+    unoccluded and occluded shadow rays of the samples the debug records
+    list, summed over both frames."""
+    import torch
+    g = learned
+    sc, ctx, P = g["scene"], g["ctx"], g["P"]
+    spp = 2
+    ctx.proposal_cache(9, 0.75, 2.0, per_light)
+    try:
+        want = np.zeros((1 << 9, sc.light_count if per_light else 256, 2), dtype=np.int64)
+        for frame in range(2):
+            c = ctx.frame_constants(noise_seed=frame + 5, animate=True)
+            s = sc.settings(sample_count_light=spp, ris_candidate_count=8,
+                            proposal=abi.PROPOSAL_LEARNED if frame else abi.PROPOSAL_UNIFORM)
+            out = torch.empty(P * 4, dtype=torch.float32, device="cuda")
+            rec = torch.zeros(P * spp * abi.SAMPLE_RECORD_FLOATS, dtype=torch.float32, device="cuda")
+            ctx.shade_debug(c, s, g["vis"], out, rec)
+            torch.cuda.synchronize()
+            r = rec.cpu().numpy().reshape(-1, abi.SAMPLE_RECORD_FLOATS)
+            li = r[:, abi.REC_LIGHT].view(np.int32)
+            st = r[:, abi.REC_STATE:abi.REC_STATE + 2].view(np.uint32)
+            hit = (st == 2).sum(1)
+            miss = (st == 3).sum(1)
+            ok = (li >= 0) & (hit + miss > 0)
+            key = r[ok, abi.REC_CACHE_KEY].view(np.uint32).astype(np.int64)
+            cl = (li[ok] if per_light else g["col"][li[ok]]).astype(np.int64)
+            np.add.at(want, (key, cl, 0), miss[ok])
+            np.add.at(want, (key, cl, 1), hit[ok])
+            assert ok.sum() > 0.2 * P
+        got = ctx.proposal_cache_read()
+        # a third call refreshes the bytes of every cell counted in
+        ctx.shade(c, sc.settings(sample_count_light=1, shadow_rays=0), g["vis"], out)
+        torch.cuda.synchronize()
+        table = ctx.proposal_cache_table()
+    finally:
+        ctx.proposal_cache(0)
+    assert np.array_equal(got.astype(np.int64), want)
+    assert 0.05 < want[..., 0].sum() / want.sum() < 0.95
+    assert np.array_equal(table, orc.cache_bytes(got, 2.0).reshape(table.shape))
+    assert (table != 255).sum() > 100

@@ -191,3 +191,150 @@ def test_learned_estimator_is_unbiased(orc_lib, ltc_table):
     (mu_u, se_u), (mu_l, se_l) = means[abi.PROPOSAL_UNIFORM], means[abi.PROPOSAL_LEARNED]
     se = np.sqrt(se_u ** 2 + se_l ** 2)
     assert np.all(np.abs(mu_u - mu_l) < 5 * se + 1e-4 * np.abs(mu_u) + 1e-7), (mu_u, mu_l, se)
+
+
+# ---------------------------------------------------------------- visibility cache
+def _fmix(h):
+    """MurmurHash3 x86_32 finaliser (public algorithm), uint64 arrays."""
+    h = h & 0xFFFFFFFF
+    h ^= h >> 16
+    h = (h * 0x85EBCA6B) & 0xFFFFFFFF
+    h ^= h >> 13
+    h = (h * 0xC2B2AE35) & 0xFFFFFFFF
+    h ^= h >> 16
+    return h
+
+
+def _mix(h, k):
+    """One MurmurHash3 x86_32 block."""
+    k = (k * 0xCC9E2D51) & 0xFFFFFFFF
+    k = ((k << 15) | (k >> 17)) & 0xFFFFFFFF
+    k = (k * 0x1B873593) & 0xFFFFFFFF
+    h = (h ^ k) & 0xFFFFFFFF
+    h = ((h << 13) | (h >> 19)) & 0xFFFFFFFF
+    return (h * 5 + 0xE6546B64) & 0xFFFFFFFF
+
+
+def _cache_oracle(scene, ltc_table, log2, cell, prior, rng, per_light=False):
+    o = orc.Oracle(scene, *ltc_table)
+    N = scene.light_count
+    K = 16
+    col = (np.arange(N) * 7 % K).astype(np.uint32)
+    P = scene.width * scene.height
+    tab = rng.integers(0, 256, (P, 256)).astype(np.uint8)
+    o.set_proposal(tab, col, K)
+    slots = N if per_light else 256
+    used = N if per_light else K
+    counts = np.zeros((1 << log2, slots, 2), dtype=np.uint32)
+    counts[:, :used] = rng.integers(0, 40, (1 << log2, used, 2))
+    o.set_cache(counts, cell, prior, per_light)
+    return o, tab, col, counts
+
+
+def test_cache_keys_follow_cells_and_normals(small_scene, ltc_table):
+    """Keys of ground pixels (normal +z, code 4) equal MurmurHash3 of their
+    cell indices floor(p / cell), computed here from the recorded positions;
+    the table mask keeps the low bits."""
+    rng = np.random.default_rng(40)
+    cell, log2 = 0.5, 12
+    o, _, _, _ = _cache_oracle(small_scene, ltc_table, log2, cell, 2.0, rng)
+    consts = o.frame_constants()
+    vis = o.render_visibility(consts)
+    ground = np.flatnonzero(vis < 2)[::23].astype(np.uint32)  # triangles 0, 1: the ground plane
+    s = small_scene.settings(sample_count_light=1, ris_candidate_count=8, shadow_rays=0)
+    _, rec, _ = o.shade(consts, s, vis, pixels=ground, records=True)
+    rec = rec[:, 0]
+    ok = rec[:, abi.REC_LIGHT].view(np.int32) >= 0
+    assert ok.sum() > 100
+    pos = rec[ok, abi.REC_POS:abi.REC_POS + 3].astype(np.float32)
+    assert np.all(np.abs(pos[:, 2]) < 1e-3)  # the plane, up to the position quantisation
+    inv = np.float32(1.0) / np.float32(cell)
+    idx = np.floor(pos * inv).astype(np.int64) & 0xFFFFFFFF
+    h = np.zeros(idx.shape[0], dtype=np.uint64)
+    for a in range(3):
+        h = _mix(h, idx[:, a].astype(np.uint64))
+    h = _mix(h, np.uint64(4))
+    want = _fmix(h) & ((1 << log2) - 1)
+    got = rec[ok, abi.REC_CACHE_KEY].view(np.uint32)
+    assert np.array_equal(got, want.astype(np.uint32))
+    # neighbouring pixels in one cell share a key; the frame spans many cells
+    assert 20 < np.unique(got).size < got.size
+
+
+def test_cache_bytes_end_points():
+    """orc_cache_bytes: 255 marks an empty entry; only unoccluded rays give
+    254 (in the limit), only occluded rays 0 (in the limit); equal counts give
+    127; the prior pulls small counts toward the middle; monotone in a."""
+    cb = orc.cache_bytes
+    c = np.array([[0, 0], [1, 0], [0, 1], [10**6, 0], [0, 10**6], [7, 7], [100, 0], [0, 100]], dtype=np.uint32)
+    b = cb(c, 1.0)
+    assert b[0] == 255 and b[3] == 254 and b[4] == 0 and b[5] == 127
+    assert 127 < b[1] < 254 and 0 < b[2] < 127 and b[1] < b[6] and b[2] > b[7]
+    assert cb(c[1:2], 8.0)[0] < cb(c[1:2], 1.0)[0]
+    a = np.arange(0, 200, dtype=np.uint32)
+    mono = cb(np.stack([a, np.full_like(a, 50)], 1), 1.0)
+    assert np.all(np.diff(mono.astype(int)) >= 0) and mono.max() <= 254
+
+
+@pytest.mark.parametrize("per_light", [False, True])
+def test_cache_target_uses_entry_bytes(small_scene, ltc_table, per_light):
+    """With a cache, each candidate's target is p_hat times f + v (1 - f) / 255
+    with v the cache byte at the pixel's key and the light's slot (its
+    cluster, or the light), or the table's byte for its cluster where that
+    entry is empty; an all-empty cache gives the table's targets bit for
+    bit."""
+    rng = np.random.default_rng(41)
+    log2, cell, prior = 10, 0.5, 2.0
+    o, tab, col, counts = _cache_oracle(small_scene, ltc_table, log2, cell, prior, rng, per_light)
+    # a quarter of the used entries empty
+    counts[rng.random(counts.shape[:2]) < 0.25] = 0
+    vis8 = o.set_cache(counts, cell, prior, per_light)
+    consts = o.frame_constants()
+    vis = o.render_visibility(consts)
+    pix = np.flatnonzero((vis >> 31) == 0)[::19].astype(np.uint32)
+    base = small_scene.settings(sample_count_light=1, ris_candidate_count=8, shadow_rays=0)
+    _, _, plain = o.shade(consts, base, vis, pixels=pix, candidates=True)
+    base.proposal = abi.PROPOSAL_LEARNED
+    _, rec, learned = o.shade(consts, base, vis, pixels=pix, records=True, candidates=True)
+    key = rec[:, 0, abi.REC_CACHE_KEY].view(np.uint32)
+    li = plain[..., 0]
+    cl = col[li]
+    v = vis8[key[:, None, None], li if per_light else cl].astype(np.int64)
+    m = tab[pix[:, None, None], cl].astype(np.int64)
+    assert np.any(v == 255) and np.any(v != 255)
+    v = np.where(v == 255, m, v).astype(np.float32)
+    f = np.float32(0.125)
+    fac = f + v * ((np.float32(1) - f) / np.float32(255))
+    want = (plain[..., 2].view(np.float32) * fac).astype(np.float32)
+    assert np.array_equal(learned[..., 2].view(np.float32), want)
+    # all entries empty: identical to the table-only target
+    o.set_cache(np.zeros_like(counts), cell, prior, per_light)
+    _, _, empty = o.shade(consts, base, vis, pixels=pix, candidates=True)
+    o.set_cache(None, cell, prior)
+    _, _, table_only = o.shade(consts, base, vis, pixels=pix, candidates=True)
+    assert np.array_equal(empty, table_only)
+
+
+def test_cached_learned_estimator_is_unbiased(orc_lib, ltc_table):
+    """A random visibility cache keeps the learned estimator's mean equal to
+    the plain one (the target stays positive wherever the LTC target is)."""
+    scene = synth.small_scene(seed=3, width=48, height=48, light_count=24)
+    rng = np.random.default_rng(42)
+    o, _, _, _ = _cache_oracle(scene, ltc_table, 8, 0.7, 4.0, rng, per_light=True)
+    consts = o.frame_constants()
+    o2 = orc.Oracle(scene, *ltc_table)
+    vis = o2.render_visibility(consts)
+    pix = np.flatnonzero((vis >> 31) == 0)[::41][:10].astype(np.uint32)
+    means = {}
+    for prop in (abi.PROPOSAL_UNIFORM, abi.PROPOSAL_LEARNED):
+        s = scene.settings(sample_count_light=256, ris_candidate_count=4, shadow_rays=0, proposal=prop)
+        frames = []
+        for seed in range(12):
+            c = o.frame_constants(noise_seed=seed, animate=True)
+            out, _, _ = o.shade(c, s, vis, pixels=pix, fp64=True)
+            frames.append(out[:, :3])
+        frames = np.array(frames)
+        means[prop] = (frames.mean(0), frames.std(0, ddof=1) / math.sqrt(len(frames)))
+    (mu_u, se_u), (mu_l, se_l) = means[abi.PROPOSAL_UNIFORM], means[abi.PROPOSAL_LEARNED]
+    se = np.sqrt(se_u ** 2 + se_l ** 2)
+    assert np.all(np.abs(mu_u - mu_l) < 5 * se + 1e-4 * np.abs(mu_u) + 1e-7), (mu_u, mu_l, se)
