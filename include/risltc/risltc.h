@@ -166,7 +166,11 @@ typedef struct risltc_material_s {
 
 /* RIS source distribution */
 #define RISLTC_PROPOSAL_UNIFORM 0 /* reference: p = 1/N (shading_pass.frag.glsl:733) */
-#define RISLTC_PROPOSAL_LEARNED 1 /* 0.5 * MLP cluster proposal + 0.5 / N     */
+#define RISLTC_PROPOSAL_LEARNED 1 /* 0.5 * MLP cluster proposal + 0.5 / N,
+                                     target with the visibility cache if set */
+#define RISLTC_PROPOSAL_CACHE 2   /* uniform source (three draws per
+                                     candidate), target with the visibility
+                                     cache: the learned pass without the MLP */
 
 /* Render settings. The fields mirror render_settings_t (src/main.h:123-156)
  * and the shader defines built from it (src/main.c:962-991); what the
@@ -185,6 +189,7 @@ typedef struct risltc_settings_s {
     uint32_t exact_math;                 /* 1 = IEEE div/sqrt everywhere: results
                                             bit-identical to the FP32 oracle      */
     uint32_t learned_floor_log2;         /* learned target: p_hat (f + (1 - f) v / 255),
+                                            with a visibility cache,
                                             f = 2^-learned_floor_log2 (0 -> 3)    */
     uint32_t skip_training_records;      /* 1: no proposal training records
                                             (risltc_proposal_train then needs
@@ -366,17 +371,24 @@ int risltc_shade_debug(risltc_context_t* context, const risltc_frame_constants_t
                        void* stream);
 
 /* ---- learned RIS proposal (project extension, no reference counterpart:
- * the reference's RIS target is the unshadowed LTC estimate,
- * shading_pass.frag.glsl:723-736). An MLP maps each shading point to one
- * number per light cluster, v in [0, 255]: the learned probability that a
- * shadow ray toward the cluster is unoccluded. With settings.proposal =
- * RISLTC_PROPOSAL_LEARNED, risltc_shade draws candidates as the reference
- * does and uses the target p_hat * (f + (1 - f) * v / 255), f = 1/8 by
- * default (settings.learned_floor_log2): the LTC target mixed defensively
- * with the learned visibility, positive wherever the LTC target is, so the
- * estimator stays unbiased. Sequence per frame:
- * risltc_proposal_update (inference), risltc_shade (learned), then
- * optionally risltc_proposal_train on the frame's own samples. ---------- */
+ * the reference draws RIS candidates uniformly and uses the unshadowed LTC
+ * estimate as the target, shading_pass.frag.glsl:723-736). Two learned
+ * parts, both trained online on the pass's own output:
+ *   - source: an MLP maps each shading point to a distribution over light
+ *     clusters (softmax of 256 logits; per pixel the 16 kept clusters of
+ *     risltc_proposal_update's table). With settings.proposal =
+ *     RISLTC_PROPOSAL_LEARNED, risltc_shade draws each candidate from the
+ *     mixture q = 1/(2N) + (1/2) pi_c / |c| of the uniform source and the
+ *     table, so every light keeps q >= 1/(2N) and the estimator stays
+ *     unbiased; the MLP is trained by softmax cross-entropy on the clusters
+ *     of the lights the RIS selects.
+ *   - target: with a visibility cache (risltc_proposal_cache) the target is
+ *     p_hat * (f + (1 - f) v / 255), v the cache's visibility byte for the
+ *     pixel's cell and the light, f = 1/8 by default
+ *     (settings.learned_floor_log2): positive wherever p_hat is.
+ * Sequence per frame: risltc_proposal_update (inference, any cadence),
+ * risltc_shade (learned), then optionally risltc_proposal_train on the
+ * frame's own samples. ------------------------------------------------- */
 #define RISLTC_MAX_LIGHT_CLUSTERS 256
 #define RISLTC_PROPOSAL_PARAMS 31168 /* 96-64-64-64-256 MLP, FP32 */
 /* Explicit light clusters: cluster_of_light[i] < cluster_count <= 256. */
@@ -395,24 +407,27 @@ int risltc_proposal_init(risltc_context_t* context, uint32_t seed);
 int risltc_proposal_set_params(risltc_context_t* context, const float* params);
 int risltc_proposal_get_params(risltc_context_t* context, float* params, float* gradient);
 /* Inference for every pixel of the frame: writes the per-pixel tables the
- * learned shade reads (256 bytes, round(255 sigmoid(logit)) per cluster, 0
- * for background). Optional device outputs: a copy of the tables, the 256
+ * learned shade reads, 32 bytes: cluster ids in bytes 0..15 and weights in
+ * 16..31; in each group g = (c % 16) / 4 of non-empty clusters the four with
+ * the largest logits z (descending, ties to the smaller id) with weight
+ * round(255 exp(z - z_max)), z_max over all non-empty clusters; weights 0
+ * for background. Optional device outputs: a copy of the tables, the 256
  * logits and the 96 bf16 features per pixel. */
 int risltc_proposal_update(risltc_context_t* context, const risltc_frame_constants_t* constants,
                            const uint32_t* visibility, uint8_t* table_out, float* logits_out,
                            uint16_t* features_out, void* stream);
 /* Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) on batches of pixel-samples of
- * the last risltc_shade: binary cross-entropy of the sigmoid of the selected
- * light's cluster against the fraction of the sample's traced shadow rays
- * that were unoccluded. batch is a multiple of 256. loss_out (host, may be
- * NULL): the last step's loss. */
+ * the last risltc_shade: softmax cross-entropy of the logits against the
+ * cluster of the sample's selected light (samples without one are left
+ * out). batch is a multiple of 256. loss_out (host, may be NULL): the last
+ * step's loss. */
 int risltc_proposal_train(risltc_context_t* context, const risltc_frame_constants_t* constants,
                           const uint32_t* visibility, uint32_t batch, uint32_t steps,
                           float learning_rate, uint32_t seed, float* loss_out, void* stream);
 /* The training records of the last risltc_shade (device outputs, may be
  * NULL): per pixel-sample the cluster of the selected light (0xFFFFFFFF =
- * no traced shadow ray) and the fraction of its traced shadow rays that were
- * unoccluded; *count receives their number. */
+ * none) and the fraction of its traced shadow rays that were unoccluded (0
+ * without one); *count receives their number. */
 int risltc_proposal_records(risltc_context_t* context, uint32_t* cluster_out, float* y_out,
                             uint64_t* count, void* stream);
 /* Online visibility cache (project extension). 2^log2_entries hashed
@@ -427,8 +442,8 @@ int risltc_proposal_records(risltc_context_t* context, uint32_t* cluster_out, fl
  * shading, recomputes the bytes of the cells counted in since:
  * 255 for an empty entry, else round(254 (a + prior_strength / 2) /
  * (a + b + prior_strength)). With settings.proposal =
- * RISLTC_PROPOSAL_LEARNED the learned visibility v of a light is its entry's
- * byte, or the MLP's byte for its cluster while the entry is empty.
+ * RISLTC_PROPOSAL_LEARNED the target uses its entry's byte v (255 while
+ * the entry is empty).
  * Allocates and clears; log2_entries = 0 removes the cache. New lights,
  * clusters or meshes clear it. */
 int risltc_proposal_cache(risltc_context_t* context, uint32_t log2_entries, float cell_size,

@@ -891,25 +891,70 @@ static uint32_t light_index_from(R u, uint32_t n) {
     return (uint32_t)idx;
 }
 
-/* Learned RIS target (project extension; no reference code): the candidates
- * come from the reference's uniform source and the target of light l is
- * p_hat * (f + v * (1 - f) / 255), f = 2^-settings.learned_floor_log2
- * (0 -> 1/8). v is the visibility cache's byte for the pixel's cell and the
- * light's slot (its cluster, or the light itself) or, while that entry is
- * empty (255) or without a cache, the pixel table's byte m for the light's
- * cluster (the MLP's probability that a shadow ray toward it is
- * unoccluded). The factor is formed in float in both builds. parity:
- * bitwise against the kernel's learned shading given the same tables and
- * cache (tests/test_gpu_learned.py); unbiasedness against the plain
- * estimator (tests/test_oracle_estimator.py). */
-static float learned_factor(const uint8_t* table, const uint32_t* of_light, uint32_t idx, float floor_,
-                            float scale, const uint8_t* cache_row, uint32_t per_light) {
-    const uint32_t cl = of_light[idx];
-    uint32_t v = table[cl];
-    if (cache_row) {
-        uint32_t c = cache_row[per_light ? idx : cl];
-        if (c != 255u) v = c;
+/* Learned RIS source (project extension; no reference code). Table of the
+ * pixel: ids t[0..15], weights t[16..31], T = their sum. Two draws a, b per
+ * candidate; every decision in float, as light_index_from:
+ *   T == 0 -> light_index_from(a); a < 1/2 -> light_index_from(2a);
+ *   else slot s = first with (int)((2a - 1) T) (clamped to T - 1) <
+ *   w_0 + .. + w_s, cluster id_s, light number min(size - 1, (int)(b size))
+ *   of that cluster (an empty cluster or an id past the clusters:
+ *   light_index_from(b)).
+ * parity: bitwise against the kernel's learned shading given the same tables
+ * (tests/test_gpu_learned.py); the mixture is an exact pdf of this draw
+ * (tests/test_oracle_estimator.py::test_learned_draw_density). */
+static uint32_t learned_draw(const orc_scene_t* sc, const uint8_t* t, uint32_t T, uint32_t* seed, uint32_t N) {
+    float a = orc_noise_next(seed);
+    float b = orc_noise_next(seed);
+    if (T == 0u) return light_index_from((R)a, N);
+    if (a < 0.5f) return light_index_from((R)(a + a), N);
+    float u = (a + a) - 1.0f;
+    uint32_t k = (uint32_t)(u * (float)T);
+    if (k > T - 1u) k = T - 1u;
+    uint32_t acc = 0, c = 0;
+    for (int s = 0; s != 16; ++s) {
+        if (k < acc + t[16 + s]) {
+            c = t[s];
+            break;
+        }
+        acc += t[16 + s];
     }
+    if (c >= sc->cluster_count) return light_index_from((R)b, N);
+    uint32_t off = sc->cluster_offsets[c], size = sc->cluster_offsets[c + 1] - off;
+    if (size == 0u) return light_index_from((R)b, N);
+    uint32_t j = (uint32_t)(b * (float)size);
+    if (j > size - 1u) j = size - 1u;
+    return sc->cluster_lights[off + j];
+}
+
+/* q = 1/(2N) + w_c / (2 T |c|), w_c = 0 for clusters not in the table */
+static R learned_density(const orc_scene_t* sc, const uint8_t* t, uint32_t T, uint32_t idx, uint32_t N) {
+    if (T == 0u) return K(1.0) / (R)N;
+    R q = K(0.5) / (R)N;
+    uint32_t c = sc->cluster_of_light[idx], wc = 0;
+    for (int s = 0; s != 16; ++s)
+        if (t[16 + s] != 0u && t[s] == c) {
+            wc = t[16 + s];
+            break;
+        }
+    if (wc != 0u) {
+        uint32_t size = sc->cluster_offsets[c + 1] - sc->cluster_offsets[c];
+        q = q + (K(0.5) * (R)wc) / ((R)T * (R)size);
+    }
+    return q;
+}
+
+/* Learned RIS target (project extension; no reference code): with a
+ * visibility cache, the learned pass's target of light l is
+ * p_hat * (f + v * (1 - f) / 255), f = 2^-settings.learned_floor_log2
+ * (0 -> 1/8), v = the cache's byte for the pixel's cell and the light's slot
+ * (its cluster, or the light itself; 255 while the entry is empty). The
+ * factor is formed in float in both builds. parity: bitwise against the
+ * kernel's learned shading given the same tables and cache
+ * (tests/test_gpu_learned.py); unbiasedness against the plain estimator
+ * (tests/test_oracle_estimator.py). */
+static float cache_factor(const uint32_t* of_light, uint32_t idx, float floor_, float scale, const uint8_t* cache_row,
+                          uint32_t per_light) {
+    uint32_t v = cache_row[per_light ? idx : of_light[idx]];
     return floor_ + (float)v * scale;
 }
 
@@ -1011,23 +1056,30 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
                 float* rec = samples ? samples + (size_t)j * RISLTC_SAMPLE_RECORD_FLOATS : NULL;
                 R w_sum = K(0.0), sample_value = K(0.0);
                 int32_t chosen = -1;
-                const int learned = s->proposal == RISLTC_PROPOSAL_LEARNED;
-                const uint8_t* tab = learned ? sc->proposal_table + (size_t)pixel * 256 : NULL;
+                const int learned = s->proposal != RISLTC_PROPOSAL_UNIFORM;
+                const uint8_t* tab =
+                    s->proposal == RISLTC_PROPOSAL_LEARNED ? sc->proposal_table + (size_t)pixel * 32 : NULL;
+                uint32_t T = 0;
+                if (tab)
+                    for (int k2 = 0; k2 != 16; ++k2) T += tab[16 + k2];
+                const uint8_t* crow = learned && sc->cache_vis ? sc->cache_vis + (size_t)ckey * sc->cache_slots : NULL;
                 const float lf = ldexpf(1.0f, -(int)(s->learned_floor_log2 ? s->learned_floor_log2 : 3u));
                 const float ls = (1.0f - lf) / 255.0f;
                 for (uint32_t i = 0; i != M; ++i) {
-                    R u = (R)orc_noise_next(&px.seed);
-                    uint32_t idx = light_index_from(u, N);
+                    uint32_t idx;
+                    if (learned) {
+                        idx = learned_draw(sc, tab, T, &px.seed, N);
+                    } else {
+                        R u = (R)orc_noise_next(&px.seed);
+                        idx = light_index_from(u, N);
+                    }
                     load_light(sc, idx, &light);
                     uint32_t cc = 0;
                     V3 col = eval_ltc_cp(&px, &light, &cc);
                     R p_hat = SQRT(dot3(col, col));
-                    if (learned)
-                        p_hat = p_hat * (R)learned_factor(tab, sc->cluster_of_light, idx, lf, ls,
-                                                          sc->cache_vis ? sc->cache_vis + (size_t)ckey * sc->cache_slots
-                                                                        : NULL,
-                                                          sc->cache_per_light);
-                    R p = K(1.0) / (R)N;
+                    if (crow)
+                        p_hat = p_hat * (R)cache_factor(sc->cluster_of_light, idx, lf, ls, crow, sc->cache_per_light);
+                    R p = learned ? learned_density(sc, tab, T, idx, N) : K(1.0) / (R)N;
                     R w = p_hat / p;
                     R r = (R)orc_noise_next(&px.seed);
                     w_sum += w;
@@ -1084,7 +1136,12 @@ int SFX(orc_shade_pixels)(const orc_scene_t* sc, const risltc_frame_constants_t*
         return RISLTC_E_UNSUPPORTED;
     if (s->proposal == RISLTC_PROPOSAL_LEARNED) {
         if (s->light_sampling != RISLTC_LIGHT_RESERVOIR) return RISLTC_E_UNSUPPORTED;
-        if (!sc->proposal_table || !sc->cluster_of_light || sc->cluster_count == 0) return RISLTC_E_STATE;
+        if (!sc->proposal_table || !sc->cluster_of_light || !sc->cluster_offsets || !sc->cluster_lights ||
+            sc->cluster_count == 0)
+            return RISLTC_E_STATE;
+    } else if (s->proposal == RISLTC_PROPOSAL_CACHE) {
+        if (s->light_sampling != RISLTC_LIGHT_RESERVOIR) return RISLTC_E_UNSUPPORTED;
+        if (!sc->cluster_of_light || !sc->cache_vis) return RISLTC_E_STATE;
     } else if (s->proposal != RISLTC_PROPOSAL_UNIFORM) {
         return RISLTC_E_ARG;
     }
@@ -1109,7 +1166,12 @@ int orc_shade_frame_f32(const orc_scene_t* sc, const risltc_frame_constants_t* c
         return RISLTC_E_UNSUPPORTED;
     if (s->proposal == RISLTC_PROPOSAL_LEARNED) {
         if (s->light_sampling != RISLTC_LIGHT_RESERVOIR) return RISLTC_E_UNSUPPORTED;
-        if (!sc->proposal_table || !sc->cluster_of_light || sc->cluster_count == 0) return RISLTC_E_STATE;
+        if (!sc->proposal_table || !sc->cluster_of_light || !sc->cluster_offsets || !sc->cluster_lights ||
+            sc->cluster_count == 0)
+            return RISLTC_E_STATE;
+    } else if (s->proposal == RISLTC_PROPOSAL_CACHE) {
+        if (s->light_sampling != RISLTC_LIGHT_RESERVOIR) return RISLTC_E_UNSUPPORTED;
+        if (!sc->cluster_of_light || !sc->cache_vis) return RISLTC_E_STATE;
     } else if (s->proposal != RISLTC_PROPOSAL_UNIFORM) {
         return RISLTC_E_ARG;
     }

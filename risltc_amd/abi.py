@@ -17,7 +17,7 @@ ERROR_NAMES = {0: "OK", 1: "E_ARG", 2: "E_HIP", 3: "E_FORMAT", 4: "E_UNSUPPORTED
 MIS_BALANCE, MIS_POWER, MIS_WEIGHTED, MIS_OPTIMAL_CLAMPED, MIS_OPTIMAL = range(5)
 LIGHT_UNIFORM, LIGHT_RESERVOIR = 0, 1
 SAMPLE_POLYGON_LTC_CP = 4
-PROPOSAL_UNIFORM, PROPOSAL_LEARNED = 0, 1
+PROPOSAL_UNIFORM, PROPOSAL_LEARNED, PROPOSAL_CACHE = 0, 1, 2
 BVH_SAH, BVH_LBVH = 0, 1
 MAX_LIGHT_CLUSTERS = 256
 PROPOSAL_PARAMS = 31168

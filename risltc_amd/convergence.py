@@ -36,7 +36,7 @@ def accumulate(ctx, scene, vis, frames, seed0, learned=None, checkpoints=(), ref
     acc = torch.zeros(P * 4, dtype=torch.float32, device="cuda")
     lk = learned or {}
     m = lk.get("candidates", candidates)
-    s = scene.settings(proposal=abi.PROPOSAL_LEARNED if learned else abi.PROPOSAL_UNIFORM,
+    s = scene.settings(proposal=lk.get("proposal", abi.PROPOSAL_LEARNED) if learned else abi.PROPOSAL_UNIFORM,
                        learned_floor_log2=lk.get("floor_log2", 0),
                        **({"ris_candidate_count": m} if m else {}))
     ctx.proposal_cache(0)
@@ -58,10 +58,12 @@ def _frames(ctx, scene, vis, frames, seed0, learned, checkpoints, ref, s, out, a
     kernel_ms = [0.0, 0.0, 0.0, 0.0, 0.0]  # shade, trace, resolve, inference, training
     for f in range(frames):
         c = ctx.frame_constants(noise_seed=seed0 + f, animate=True)
-        if learned and f % learned.get("infer_every", 1) == 0:
+        if learned and learned.get("proposal", abi.PROPOSAL_LEARNED) == abi.PROPOSAL_LEARNED and \
+                f % learned.get("infer_every", 1) == 0:
             ctx.proposal_update(c, vis)
             kernel_ms[3] += ctx.proposal_last_timings()[0]
-        train = bool(learned) and f % learned.get("train_every", 1) == 0
+        train = bool(learned) and learned.get("proposal", abi.PROPOSAL_LEARNED) == abi.PROPOSAL_LEARNED and \
+            f % learned.get("train_every", 1) == 0
         s.skip_training_records = 0 if train else 1
         ctx.shade(c, s, vis, out)
         t = ctx.last_timings()
@@ -96,11 +98,12 @@ def time_to(curve, target):
     return ms * rel / target, True
 
 
-# The learned configuration the experiment uses: target floor 1/128, per-light
-# visibility cache of 2^16 one-metre cells, MLP inference every 16 frames and
-# one training step every 4, M = 48 candidates.
-DEFAULT_LEARNED = dict(floor_log2=7, infer_every=16, train_every=4, batch=16384, lr=2e-3, clusters=256, seed=1,
-                       cache_log2=16, cache_cell=1.0, cache_prior=1.0, cache_per_light=True, candidates=48)
+# The learned configuration the experiment uses: learned source (MLP
+# inference every 2 frames, one Adam step of 8192 samples per frame, learning
+# rate 0.02), target floor 1/128 with a per-light visibility cache of 2^16
+# one-metre cells, M = 24 candidates.
+DEFAULT_LEARNED = dict(floor_log2=7, infer_every=2, train_every=1, batch=8192, lr=2e-2, clusters=256, seed=1,
+                       cache_log2=16, cache_cell=1.0, cache_prior=1.0, cache_per_light=True, candidates=24)
 
 
 def reference(ctx, scene, vis, frames, chunk=2048, seed0=1000000, log=None):
@@ -157,10 +160,10 @@ def compare(ctx, scene, vis, target=0.01, ref_frames=16384, plain_frames=320, fr
         "learned_config": learned,
     }
     if ablations:
-        no_cache = dict(learned, cache_log2=0)
-        untrained = dict(learned, lr=0.0)
         out["ablations"] = {}
-        for name, lk in (("mlp_only", no_cache), ("cache_with_untrained_mlp", untrained)):
+        for name, lk in (("source_only", dict(learned, cache_log2=0)),
+                         ("cache_only_uniform_source", dict(learned, proposal=abi.PROPOSAL_CACHE, candidates=48)),
+                         ("untrained_mlp", dict(learned, lr=0.0))):
             r = run(lk, frames)
             out["ablations"][name] = {"time_to_target_ms": r["time_to_target_ms"],
                                       "over_plain": round(r["time_to_target_ms"] / plain["time_to_target_ms"], 4),

@@ -1136,7 +1136,7 @@ static int check_settings(risltc_context_t* c, const risltc_frame_constants_t* k
         return fail(c, RISLTC_E_ARG, "sample_count_light %u", s->sample_count_light);
     if (s->light_sampling == RISLTC_LIGHT_RESERVOIR && (s->ris_candidate_count == 0 || s->ris_candidate_count > 4096))
         return fail(c, RISLTC_E_ARG, "ris_candidate_count %u", s->ris_candidate_count);
-    if (s->proposal > RISLTC_PROPOSAL_LEARNED) return fail(c, RISLTC_E_ARG, "proposal %u", s->proposal);
+    if (s->proposal > RISLTC_PROPOSAL_CACHE) return fail(c, RISLTC_E_ARG, "proposal %u", s->proposal);
     if (s->learned_floor_log2 > 16) return fail(c, RISLTC_E_ARG, "learned_floor_log2 %u", s->learned_floor_log2);
     if (s->skip_training_records > 1)
         return fail(c, RISLTC_E_ARG, "skip_training_records %u", s->skip_training_records);
@@ -1256,14 +1256,17 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     if (c->light_count == 0) return fail(c, RISLTC_E_STATE, "risltc_set_lights has not been called");
     int e = check_settings(c, k, s);
     if (e) return e;
-    const bool learned = s->proposal == RISLTC_PROPOSAL_LEARNED;
+    const bool learned = s->proposal != RISLTC_PROPOSAL_UNIFORM;
     if (learned) {
         if (s->light_sampling != RISLTC_LIGHT_RESERVOIR)
-            return fail(c, RISLTC_E_UNSUPPORTED, "the learned target is a RIS target: light_sampling must be reservoir");
+            return fail(c, RISLTC_E_UNSUPPORTED, "the learned pass is a RIS pass: light_sampling must be reservoir");
         if (c->cluster_count == 0) return fail(c, RISLTC_E_STATE, "no light clusters");
-        if (!c->prop_table.p || c->prop_width != k->viewport_size[0] || c->prop_height != k->viewport_size[1])
-            return fail(c, RISLTC_E_STATE, "risltc_proposal_update has not run for this viewport");
     }
+    if (s->proposal == RISLTC_PROPOSAL_LEARNED &&
+        (!c->prop_table.p || c->prop_width != k->viewport_size[0] || c->prop_height != k->viewport_size[1]))
+        return fail(c, RISLTC_E_STATE, "risltc_proposal_update has not run for this viewport");
+    if (s->proposal == RISLTC_PROPOSAL_CACHE && !c->vcache.p)
+        return fail(c, RISLTC_E_STATE, "no visibility cache");
     const size_t S = (size_t)k->viewport_size[0] * k->viewport_size[1] * s->sample_count_light;
     hipStream_t st = (hipStream_t)stream;
     HIPC(c, hipSetDevice(c->device));
@@ -1282,7 +1285,8 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     a.ray_count = c->counter.as<uint32_t>();
     a.debug_samples = dbg_s;
     a.debug_cands = dbg_c;
-    a.proposal = learned ? c->prop_table.as<uint8_t>() : nullptr;
+    a.learned_pass = learned ? 1u : 0u;
+    a.proposal = s->proposal == RISLTC_PROPOSAL_LEARNED ? c->prop_table.as<uint8_t>() : nullptr;
     const bool cached = c->vcache.p && c->cluster_count > 0;
     if (cached) {
         if (c->vcache_per_light && c->vcache_slots != c->light_count)

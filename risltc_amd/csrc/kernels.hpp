@@ -76,7 +76,11 @@ constexpr int kMlpParams = kPB4 + kMlpOut;  // 31,168
 // The learned RIS target is p_hat * (floor + v * scale), scale = (1 - floor)
 // / 255: the LTC target mixed with the visibility estimate, never below
 // p_hat * floor.
-constexpr int kPropBytes = kMlpOut;
+// per pixel: the ids of 16 clusters (bytes 0..15) and their weights
+// (bytes 16..31): in each of 4 groups of clusters (c % 16) / 4 = g, the 4
+// with the largest logits, weight round(255 exp(z - z_max))
+constexpr int kPropTop = 16;
+constexpr int kPropBytes = 2 * kPropTop;
 // default floor 1/8: scale = (1 - 0.125) / 255 in FP32
 constexpr float kLearnedFloor = 0.125f;
 constexpr float kLearnedScale = 0x1.c1c1c2p-9f;
@@ -175,11 +179,17 @@ struct ShadeArgs {
     float* debug_samples;      // RISLTC_SAMPLE_RECORD_FLOATS per pixel-sample or null
     uint32_t* debug_cands;     // RISLTC_CANDIDATE_RECORD_WORDS per candidate or null
     // learned target (null = the reference's LTC target)
-    const uint8_t* proposal;   // kPropBytes per pixel
+    // learned pass (1): three draws per candidate, the learned source when
+    // proposal is set, the cache factor in the target when cache.vis is set
+    uint32_t learned_pass;
+    // learned source (null = uniform): kPropBytes per pixel, mixed half and
+    // half with the uniform source
+    const uint8_t* proposal;
     ClusterView clusters;
+    // learned target factor f + v (1 - f) / 255 of the visibility cache
     float learned_floor, learned_scale;  // f and (1 - f) / 255 (FP32)
     // cache keys are computed (and written to the sample records) whenever
-    // counts is set; the learned target reads it
+    // counts is set; with a proposal the target reads the cache's bytes
     VisCacheView cache;
 };
 

@@ -1,13 +1,13 @@
 // mlp.hip -- the learned RIS proposal: feature encoder, MLP inference and
 // training on BF16 MFMA with FP32 accumulation.
 //
-// No reference code: the reference's RIS target is the unshadowed LTC
-// estimate (shading_pass.frag.glsl:723-736). The MLP (96 -> 64 -> 64 -> 64
-// -> 256) maps a shading point to one probability per light cluster: that a
-// shadow ray toward the cluster is unoccluded. It is trained online on the
-// visibility of the plain pass's own final samples. The shade kernel mixes it
-// defensively into the target, p_hat * (1/8 + 7/8 v), so the target stays
-// positive wherever the LTC target is and the RIS estimator stays unbiased.
+// No reference code: the reference draws RIS candidates uniformly
+// (shading_pass.frag.glsl:730-733). The proposal adds a per-pixel
+// distribution over light clusters from a 96 -> 64 -> 64 -> 64 -> 256 MLP
+// (softmax of its logits); the shade kernel mixes it half and half with the
+// uniform source, so every light keeps probability >= 0.5 / N and the RIS
+// estimator stays unbiased. It is trained online, by softmax cross-entropy,
+// on the clusters of the lights that the pass's own RIS selects.
 //
 // Matrix products use v_mfma_f32_16x16x32_bf16. Lane l of a wave holds
 // A[l % 16][8 (l / 16) .. + 7], B[8 (l / 16) .. + 7][l % 16] and returns
@@ -158,6 +158,7 @@ struct PropShared {
     uint16_t w3[kMlpHid * kLdH];
     uint16_t w4[kMlpOut * kLdH];
     float bias[3 * kMlpHid + kMlpOut];
+    uint32_t csize[kMlpOut];
     uint16_t act_in[kPropWaves][16 * kLdW1];
     uint16_t act_h[kPropWaves][2][16 * kLdH];
 };
@@ -200,7 +201,10 @@ __global__ void __launch_bounds__(64 * kPropWaves) proposal_kernel(ProposalArgs 
         sh.bias[kMlpHid + e] = A.params[kPB2 + e];
         sh.bias[2 * kMlpHid + e] = A.params[kPB3 + e];
     }
-    for (int e = tid; e < kMlpOut; e += blockDim.x) sh.bias[3 * kMlpHid + e] = A.params[kPB4 + e];
+    for (int e = tid; e < kMlpOut; e += blockDim.x) {
+        sh.bias[3 * kMlpHid + e] = A.params[kPB4 + e];
+        sh.csize[e] = (uint32_t)e < A.cluster_count ? A.cluster_offsets[e + 1] - A.cluster_offsets[e] : 0u;
+    }
     __syncthreads();
     const int g = lane >> 4, r = lane & 15;
     uint16_t* act = sh.act_in[wave];
@@ -230,9 +234,10 @@ __global__ void __launch_bounds__(64 * kPropWaves) proposal_kernel(ProposalArgs 
         wave_sync();
         hidden_layer<kMlpHid, kLdH, kLdH>(sh.w3, sh.bias + 2 * kMlpHid, h1, h0, lane);
         wave_sync();
-        // output layer: this lane holds the logits of clusters 16 t + 4 g + i
-        // of pixel r; the table byte is round(255 sigmoid(z)), 0 for
-        // background pixels
+        // output layer: this lane keeps logits of clusters 16 t + 4 g + i of pixel r,
+        // and the 4 largest of them
+        float bv[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+        int bi[4] = {0, 0, 0, 0};
 #pragma unroll 2
         for (int t = 0; t < kMlpOut / 16; ++t) {
             f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -240,17 +245,49 @@ __global__ void __launch_bounds__(64 * kPropWaves) proposal_kernel(ProposalArgs 
             for (int ks = 0; ks < kMlpHid / 32; ++ks)
                 acc = mfma(frag(sh.w4 + (t * 16 + r) * kLdH + ks * 32 + 8 * g), frag(h0 + r * kLdH + ks * 32 + 8 * g),
                            acc);
-            uint32_t packed = 0;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 int cl = t * 16 + 4 * g + i;
                 float z = acc[i] + sh.bias[3 * kMlpHid + cl];
                 if (A.logits && in) A.logits[(size_t)pixel * kMlpOut + cl] = z;
-                float v = 255.0f / (1.0f + __expf(-z)) + 0.5f;
-                uint32_t b = shaded ? (v >= 255.0f ? 255u : (uint32_t)v) : 0u;
-                packed |= b << (8 * i);
+                if (sh.csize[cl] == 0u) continue;
+                // insertion into the sorted best-4 (ties keep the smaller id)
+                if (z > bv[3]) {
+                    bv[3] = z;
+                    bi[3] = cl;
+#pragma unroll
+                    for (int s = 3; s > 0; --s)
+                        if (bv[s] > bv[s - 1]) {
+                            float tv = bv[s];
+                            bv[s] = bv[s - 1];
+                            bv[s - 1] = tv;
+                            int ti = bi[s];
+                            bi[s] = bi[s - 1];
+                            bi[s - 1] = ti;
+                        }
+                }
             }
-            if (in) *reinterpret_cast<uint32_t*>(A.out + (size_t)pixel * kPropBytes + t * 16 + 4 * g) = packed;
+        }
+        // the pixel's maximum over its four lanes (lanes r, r+16, r+32, r+48)
+        float zmax = bv[0];
+        zmax = fmax_(zmax, __shfl_xor(zmax, 16));
+        zmax = fmax_(zmax, __shfl_xor(zmax, 32));
+        uint32_t ids = 0, ws = 0;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            uint32_t w = 0;
+            if (shaded && bv[s] > -__builtin_inff()) {
+                float e = __expf(bv[s] - zmax) * 255.0f + 0.5f;
+                w = e >= 255.0f ? 255u : (uint32_t)e;
+            }
+            if (w == 0u) bi[s] = 0;
+            ids |= (uint32_t)bi[s] << (8 * s);
+            ws |= w << (8 * s);
+        }
+        if (in) {
+            uint32_t* o = reinterpret_cast<uint32_t*>(A.out + (size_t)pixel * kPropBytes);
+            o[g] = ids;
+            o[4 + g] = ws;
         }
         wave_sync();
     }
@@ -346,9 +383,8 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
 }
 
 // batch item b: a pseudo-random pixel-sample; features (both layouts), the
-// cluster of its light, the fraction of its shadow rays that were unoccluded
-// and a weight (1, or 0 without a traced ray)
-__global__ void train_gather_kernel(TrainArgs A, uint16_t* X, uint16_t* Xt, uint32_t* target, float* vis, float* y) {
+// cluster of its selected light and a weight (1, or 0 without a selection)
+__global__ void train_gather_kernel(TrainArgs A, uint16_t* X, uint16_t* Xt, uint32_t* target, float* y) {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= A.B) return;
     const uint32_t item = hash32(A.seed * 0x9E3779B9u + b) % A.sample_count;
@@ -356,15 +392,12 @@ __global__ void train_gather_kernel(TrainArgs A, uint16_t* X, uint16_t* Xt, uint
     float f[kMlpIn];
     bool ok = encode_features(A.f, pixel, f);
     uint32_t c = A.rec_cluster[item];
-    float v = A.rec_y[item];
     float w = 1.0f;
-    if (!ok || c >= (uint32_t)kMlpOut || !(v >= 0.0f && v <= 1.0f)) {
+    if (!ok || c >= (uint32_t)kMlpOut) {
         c = 0u;
-        v = 0.0f;
         w = 0.0f;
     }
     target[b] = c;
-    vis[b] = v;
     y[b] = w;
     for (int k = 0; k < kMlpIn; ++k) {
         uint16_t h = bf16_bits(f[k]);
@@ -387,26 +420,27 @@ __global__ void batch_sum_kernel(const float* y, uint32_t B, float* out) {
     if (threadIdx.x == 0) *out = part[0];
 }
 
-// binary cross-entropy of the target cluster's logit against the observed
-// visibility: dL/dz_c = (w_b / W) (sigmoid(z_c) - v_b), zero for the other
-// clusters; per-row loss (w_b / W) (softplus(z_c) - v_b z_c)
-__global__ void bce_grad_kernel(const float* Z, const uint32_t* target, const float* vis, const float* y,
-                                const float* ysum, uint32_t B, uint16_t* G, uint16_t* Gt, float* row_loss) {
+// dL/dz = (y_b / Y) (softmax(z_b) - onehot(c_b)); per-row loss terms
+__global__ void softmax_grad_kernel(const float* Z, const uint32_t* target, const float* y, const float* ysum,
+                                    uint32_t B, uint16_t* G, uint16_t* Gt, float* row_loss) {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
-    const uint32_t c = target[b];
-    const float z = Z[(size_t)b * kMlpOut + c];
+    const float* z = Z + (size_t)b * kMlpOut;
+    float zm = -__builtin_inff();
+    for (int n = 0; n < kMlpOut; ++n) zm = fmax_(zm, z[n]);
+    float s = 0.0f;
+    for (int n = 0; n < kMlpOut; ++n) s += __expf(z[n] - zm);
     const float Y = *ysum;
     const float wb = Y > 0.0f ? y[b] / Y : 0.0f;
-    const float sg = 1.0f / (1.0f + __expf(-z));
-    const float d = wb * (sg - vis[b]);
+    const uint32_t c = target[b];
     for (int n = 0; n < kMlpOut; ++n) {
-        uint16_t h = (uint32_t)n == c ? bf16_bits(d) : (uint16_t)0;
+        float p = __expf(z[n] - zm) / s;
+        float d = wb * (p - ((uint32_t)n == c ? 1.0f : 0.0f));
+        uint16_t h = bf16_bits(d);
         G[(size_t)b * kMlpOut + n] = h;
         Gt[(size_t)n * B + b] = h;
     }
-    const float softplus = z > 0.0f ? z + __logf(1.0f + __expf(-z)) : __logf(1.0f + __expf(z));
-    row_loss[b] = wb > 0.0f ? wb * (softplus - vis[b] * z) : 0.0f;
+    row_loss[b] = wb > 0.0f ? -wb * (z[c] - zm - __logf(s)) : 0.0f;
 }
 
 // grad[w] = sum over slices (fixed order); bias grads = column sums of G
@@ -504,7 +538,7 @@ namespace {
 constexpr uint32_t kTrainSlice = 2048;  // split-K slice of the weight-gradient products
 struct TrainScratch {
     uint16_t *X, *Xt, *H1, *H1t, *H2, *H2t, *H3, *H3t, *G4, *G4t, *G3, *G3t, *G2, *G2t, *G1, *G1t;
-    float *Z, *y, *vis, *ysum, *row_loss, *partial;
+    float *Z, *y, *ysum, *row_loss, *partial;
     uint32_t* target;
 };
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -535,7 +569,6 @@ TrainScratch carve(void* base, uint32_t B, size_t* total) {
     t.G1t = (uint16_t*)take(2ull * B * kMlpHid);
     t.Z = (float*)take(4ull * B * kMlpOut);
     t.y = (float*)take(4ull * B);
-    t.vis = (float*)take(4ull * B);
     t.ysum = (float*)take(4);
     t.row_loss = (float*)take(4ull * B);
     const uint32_t slices = (B + kTrainSlice - 1) / kTrainSlice;
@@ -556,7 +589,7 @@ hipError_t launch_train_step(const TrainArgs& a, hipStream_t s) {
     if (a.B == 0 || a.B % 256u != 0u || a.sample_count == 0) return hipErrorInvalidValue;
     TrainScratch t = carve(a.scratch, a.B, nullptr);
     const uint32_t B = a.B, slices = (B + kTrainSlice - 1) / kTrainSlice;
-    hipLaunchKernelGGL(train_gather_kernel, dim3(B / 256u), dim3(256), 0, s, a, t.X, t.Xt, t.target, t.vis, t.y);
+    hipLaunchKernelGGL(train_gather_kernel, dim3(B / 256u), dim3(256), 0, s, a, t.X, t.Xt, t.target, t.y);
     hipLaunchKernelGGL(batch_sum_kernel, dim3(1), dim3(256), 0, s, t.y, B, t.ysum);
     // forward
     GemmArgs g{};
@@ -568,7 +601,7 @@ hipError_t launch_train_step(const TrainArgs& a, hipStream_t s) {
     gemm<kEpiHidden>(g, 1, s);
     g = GemmArgs{t.H3, a.wf + kPW4, B, kMlpOut, kMlpHid, kMlpHid, kMlpHid, kMlpHid, a.params + kPB4, nullptr, nullptr, t.Z, nullptr};
     gemm<kEpiLogits>(g, 1, s);
-    hipLaunchKernelGGL(bce_grad_kernel, dim3(B / 256u), dim3(256), 0, s, t.Z, t.target, t.vis, t.y, t.ysum, B, t.G4,
+    hipLaunchKernelGGL(softmax_grad_kernel, dim3(B / 256u), dim3(256), 0, s, t.Z, t.target, t.y, t.ysum, B, t.G4,
                        t.G4t, t.row_loss);
     hipLaunchKernelGGL(loss_sum_kernel, dim3(1), dim3(256), 0, s, t.row_loss, B, a.loss);
     // layer 4: dW4[o][i] = sum_b G4[b][o] H3[b][i]; dH3 = G4 . W4, masked by H3 > 0

@@ -1049,23 +1049,90 @@ __device__ __forceinline__ uint32_t lcg_jump(uint32_t s, uint32_t d) {
     return s;
 }
 
+// ---- learned proposal: candidate draws ------------------------------------------
+// Per pixel table (kPropBytes): cluster ids in bytes 0..15, weights in 16..31,
+// T = sum of the weights. A candidate takes two draws a, b:
+//   T == 0        -> light_index_from(a, N);
+//   a < 1/2       -> light_index_from(2 a, N)          (the uniform half);
+//   otherwise     -> t = min(T - 1, int((2 a - 1) T)), slot s = first with
+//                    t < w_0 + .. + w_s, cluster c = id_s, light = the
+//                    min(size - 1, int(b size))-th light of c (an empty
+//                    cluster, or an id past the clusters, falls back to
+//                    light_index_from(b, N)).
+// Its source density is q = 1/(2N) + w_c / (2 T |c|) (w_c = 0 for clusters
+// not in the table), computed in the same operation order by the oracle.
+struct PropTable {
+    uint32_t ids[4], ws[4];
+    uint32_t T;
+};
+
+__device__ __forceinline__ uint32_t byte_of(const uint32_t* v, int s) { return (v[s >> 2] >> (8 * (s & 3))) & 0xFFu; }
+
+struct ClusterLds {
+    const uint8_t* of_light;
+    const uint32_t* offsets;
+    const uint32_t* lights;
+    uint32_t count;
+};
+
+__device__ __forceinline__ uint32_t draw_learned(uint32_t& seed, const PropTable& P, const ClusterLds& C, uint32_t N) {
+    float a = noise_next(seed);
+    float b = noise_next(seed);
+    if (P.T == 0u) return light_index_from(a, N);
+    if (a < 0.5f) return light_index_from(a + a, N);
+    float u = (a + a) - 1.0f;
+    uint32_t t = (uint32_t)(u * (float)P.T);
+    if (t > P.T - 1u) t = P.T - 1u;
+    uint32_t acc = 0, c = 0;
+    bool found = false;
+#pragma unroll
+    for (int s = 0; s < kPropTop; ++s) {
+        uint32_t w = byte_of(P.ws, s);
+        if (!found && t < acc + w) {
+            c = byte_of(P.ids, s);
+            found = true;
+        }
+        acc += w;
+    }
+    // a cluster id outside the clusters or an empty cluster: uniform
+    if (c >= C.count) return light_index_from(b, N);
+    uint32_t off = C.offsets[c], size = C.offsets[c + 1] - off;
+    if (size == 0u) return light_index_from(b, N);
+    uint32_t j = (uint32_t)(b * (float)size);
+    if (j > size - 1u) j = size - 1u;
+    return C.lights[off + j];
+}
+
+template <bool X>
+__device__ __forceinline__ float learned_density(uint32_t idx, const PropTable& P, const ClusterLds& C, uint32_t N) {
+    float q = 0.5f / (float)N;
+    if (P.T == 0u) return 1.0f / (float)N;
+    uint32_t c = C.of_light[idx];
+    uint32_t wc = 0;
+#pragma unroll
+    for (int s = kPropTop - 1; s >= 0; --s) {
+        uint32_t w = byte_of(P.ws, s);
+        if (w != 0u && byte_of(P.ids, s) == c) wc = w;
+    }
+    if (wc != 0u) {
+        uint32_t size = C.offsets[c + 1] - C.offsets[c];
+        q = q + div_<X>(0.5f * (float)wc, (float)P.T * (float)size);
+    }
+    return q;
+}
+
 // ---- learned RIS target ----------------------------------------------------------
-// With a proposal table the candidates still come from the reference's
-// uniform source; the target of candidate light l becomes
-// p_hat * (floor + v * scale), scale = (1 - floor) / 255, where v is the
-// visibility cache's byte for the pixel's cell and l's slot or, while that
-// entry is empty (255) or without a cache, the table's byte for l's cluster.
-// The oracle evaluates the same expression.
-__device__ __forceinline__ float learned_factor(const uint8_t* table, const uint8_t* of_light, uint32_t idx,
-                                                float floor, float scale, const uint8_t* cache_row,
-                                                const uint8_t* lds, int32_t lds_row, uint32_t per_light) {
-    const uint32_t cl = of_light[idx];
-    const uint32_t slot = per_light ? idx : cl;
+// With a visibility cache the learned pass's target of candidate light l is
+// p_hat * (floor + v * scale), scale = (1 - floor) / 255, v = the cache's
+// byte for the pixel's cell and l's slot (255, so a factor of about 1, while
+// the entry is empty). The oracle evaluates the same expression.
+__device__ __forceinline__ float cache_factor(const uint8_t* of_light, uint32_t idx, float floor, float scale,
+                                              const uint8_t* cache_row, const uint8_t* lds, int32_t lds_row,
+                                              uint32_t per_light) {
+    const uint32_t slot = per_light ? idx : (uint32_t)of_light[idx];
     uint32_t v = 255u;
     if (lds_row >= 0) v = lds[(uint32_t)lds_row + slot];
     else if (cache_row) v = cache_row[slot];
-    // the table row is per pixel (a scattered load): read only on a miss
-    if (v == 255u) v = table[cl];
     return floor + (float)v * scale;
 }
 
@@ -1086,9 +1153,14 @@ __device__ __forceinline__ uint32_t cache_byte(uint2 c, float prior) {
 // the spheres and the cluster table, 16-byte aligned.
 constexpr uint32_t kRowLdsMax = 1024, kRowsPerWave = 4;
 
-__device__ __host__ __forceinline__ size_t row_lds_offset(uint32_t n_lds, bool cluster_table) {
-    size_t b = (size_t)n_lds * 16u + (cluster_table ? n_lds : 0u);
-    return (b + 15u) & ~(size_t)15u;
+// bytes of the spheres and, for the learned source with every light staged,
+// the cluster tables (lights by cluster, offsets, cluster of each light)
+__device__ __host__ __forceinline__ size_t cluster_lds_end(uint32_t n_lds, bool cluster_tables, uint32_t clusters) {
+    return (size_t)n_lds * 16u + (cluster_tables ? (size_t)n_lds * 5u + ((size_t)clusters + 1u) * 4u : 0u);
+}
+
+__device__ __host__ __forceinline__ size_t row_lds_offset(uint32_t n_lds, bool cluster_tables, uint32_t clusters) {
+    return (cluster_lds_end(n_lds, cluster_tables, clusters) + 15u) & ~(size_t)15u;
 }
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -1121,17 +1193,24 @@ __device__ __forceinline__ uint32_t cache_key(const VisCacheView& C, f3 p, f3 n)
 template <bool X, bool DEBUG, bool LEARNED>
 __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, float4* sh_spheres) {
     constexpr bool learned = LEARNED;
-    const uint8_t* of_light = A.clusters.of_light;
+    ClusterLds cl{A.clusters.of_light, A.clusters.offsets, A.clusters.lights, A.clusters.count};
     const uint32_t n_lds = A.scene.light_count < A.lds_lights ? A.scene.light_count : A.lds_lights;
     const bool all_lds = n_lds == A.scene.light_count;
     if (n_lds > 0u) {
         for (uint32_t k = threadIdx.x; k < n_lds; k += blockDim.x) sh_spheres[k] = A.scene.spheres[k];
         if constexpr (learned) {
             if (all_lds) {
-                // after the spheres: the cluster of every light
-                uint8_t* c = reinterpret_cast<uint8_t*>(sh_spheres + n_lds);
-                for (uint32_t k = threadIdx.x; k < n_lds; k += blockDim.x) c[k] = A.clusters.of_light[k];
-                of_light = c;
+                // after the spheres: lights by cluster (u32), offsets (u32),
+                // cluster of light (u8)
+                uint32_t* l = reinterpret_cast<uint32_t*>(sh_spheres + n_lds);
+                uint32_t* o = l + n_lds;
+                uint8_t* c = reinterpret_cast<uint8_t*>(o + A.clusters.count + 1u);
+                for (uint32_t k = threadIdx.x; k < n_lds; k += blockDim.x) {
+                    l[k] = A.clusters.lights[k];
+                    c[k] = A.clusters.of_light[k];
+                }
+                for (uint32_t k = threadIdx.x; k <= A.clusters.count; k += blockDim.x) o[k] = A.clusters.offsets[k];
+                cl = ClusterLds{c, o, l, A.clusters.count};
             }
         }
         __syncthreads();
@@ -1173,7 +1252,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
         if (A.cache.vis) {
             crow = A.cache.vis + (size_t)ckey * A.cache.slots;
             if (A.cache.slots <= kRowLdsMax) {
-                const uint32_t wave_base = (uint32_t)row_lds_offset(n_lds, learned && all_lds) +
+                const uint32_t wave_base = (uint32_t)row_lds_offset(n_lds, learned && all_lds, A.clusters.count) +
                                            wave * kRowsPerWave * kRowLdsMax;
                 uint8_t* area = reinterpret_cast<uint8_t*>(sh_spheres) + wave_base;
                 const uint32_t slots = A.cache.slots;
@@ -1200,10 +1279,27 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
     const uint32_t N = A.scene.light_count;
     const float pN = 1.0f / (float)N;
     const uint32_t M = A.M;
+    // noise draws per candidate: light (1, or 2 for the learned source) + reservoir
+    const uint32_t stride = learned ? 3u : 2u;
+    const bool cached = learned && A.cache.vis != nullptr;
     for (uint32_t j = 0; j < spp; ++j) {
         int32_t chosen = -1;
         float w_sum = 0.0f, sample_value = 0.0f;
         Light L;
+        // the pixel's proposal table, loaded per sample so that it is not
+        // live across the final sample
+        PropTable pt;
+        pt.T = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) pt.ids[k] = pt.ws[k] = 0u;
+        if (learned && shade && A.proposal) {
+            const uint4* t = reinterpret_cast<const uint4*>(A.proposal + (size_t)pixel * kPropBytes);
+            uint4 t0 = t[0], t1 = t[1];
+            pt.ids[0] = t0.x, pt.ids[1] = t0.y, pt.ids[2] = t0.z, pt.ids[3] = t0.w;
+            pt.ws[0] = t1.x, pt.ws[1] = t1.y, pt.ws[2] = t1.z, pt.ws[3] = t1.w;
+#pragma unroll
+            for (int s2 = 0; s2 < kPropTop; ++s2) pt.T += byte_of(pt.ws, s2);
+        }
         if (shade) {
             if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
                 chosen = (int32_t)light_index_from(noise_next(seed), N);
@@ -1219,7 +1315,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                     const uint32_t seed_chunk = seed;
                     uint32_t live = 0;
                     for (uint32_t t = 0; t < cnt; ++t) {
-                        uint32_t idx = light_index_from(noise_next(seed), N);
+                        uint32_t idx = learned ? draw_learned(seed, pt, cl, N) : light_index_from(noise_next(seed), N);
                         (void)noise_next(seed);
                         float4 sp = all_lds ? sh_spheres[idx] : (idx < n_lds ? sh_spheres[idx] : A.scene.spheres[idx]);
                         bool cut = culled(cull, sp);
@@ -1239,23 +1335,22 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                     while (live) {
                         const uint32_t t = (uint32_t)__builtin_ctz(live);
                         live &= live - 1u;
-                        s = lcg_jump(s, 2u * (t - at));
+                        s = lcg_jump(s, stride * (t - at));
                         at = t + 1u;
-                        uint32_t idx = light_index_from(noise_next(s), N);
+                        uint32_t idx = learned ? draw_learned(s, pt, cl, N) : light_index_from(noise_next(s), N);
                         load_light(A.scene.lights, idx, L);
-                        // the learned factor first: its table and cache loads
-                        // are in flight during the target's evaluation
+                        // the cache factor first: its load is in flight
+                        // during the target's evaluation
                         float factor = 1.0f;
-                        if constexpr (learned)
-                            factor = learned_factor(A.proposal + (size_t)pixel * kPropBytes, of_light, idx,
-                                                    A.learned_floor, A.learned_scale, crow,
-                                                    reinterpret_cast<const uint8_t*>(sh_spheres), lds_row,
-                                                    A.cache.per_light);
+                        if (cached)
+                            factor = cache_factor(cl.of_light, idx, A.learned_floor, A.learned_scale, crow,
+                                                  reinterpret_cast<const uint8_t*>(sh_spheres), lds_row,
+                                                  A.cache.per_light);
                         uint32_t cc;
                         f3 col = eval_ltc_cp<X>(px, L, cc);
                         float p_hat = sqrt_<X>(dot3(col, col));
-                        if constexpr (learned) p_hat = p_hat * factor;
-                        float w = div_<X>(p_hat, pN);
+                        if (cached) p_hat = p_hat * factor;
+                        float w = div_<X>(p_hat, learned ? learned_density<X>(idx, pt, cl, N) : pN);
                         float r = noise_next(s);
                         w_sum += w;
                         // exact: the reference's r < w / w_sum; fast: the
@@ -1451,7 +1546,7 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
                 if (A.rec_cluster || A.cache.counts) {
                     int32_t li = __float_as_int(r3.z);
                     bool ok = li >= 0 && traced > 0u;
-                    uint32_t cl = ok ? (uint32_t)A.cluster_of_light[li] : 0xFFFFFFFFu;
+                    uint32_t cl = li >= 0 ? (uint32_t)A.cluster_of_light[li] : 0xFFFFFFFFu;
                     if (A.rec_cluster) {
                         A.rec_cluster[sidx] = cl;
                         A.rec_y[sidx] = ok ? (float)unoccluded / (float)traced : 0.0f;
@@ -1505,11 +1600,11 @@ uint32_t shade_tiles(const ShadeArgs& a) {
 
 size_t shade_lds_bytes(const ShadeArgs& a) {
     const uint32_t n = a.scene.light_count < a.lds_lights ? a.scene.light_count : a.lds_lights;
-    const bool table = a.proposal && n == a.scene.light_count;
-    size_t b = (size_t)n * sizeof(float4) + (table ? n : 0u);
-    // learned target with a cache: one staged row per wave
-    if (a.proposal && a.cache.vis && a.cache.slots <= kRowLdsMax)
-        b = row_lds_offset(n, table) + 4u * kRowsPerWave * kRowLdsMax;
+    const bool tables = a.learned_pass && n == a.scene.light_count;
+    size_t b = cluster_lds_end(n, tables, a.clusters.count);
+    // learned pass with a cache: staged rows, kRowsPerWave per wave
+    if (a.learned_pass && a.cache.vis && a.cache.slots <= kRowLdsMax)
+        b = row_lds_offset(n, tables, a.clusters.count) + 4u * kRowsPerWave * kRowLdsMax;
     return b;
 }
 
@@ -1523,7 +1618,7 @@ hipError_t launch_shade(const ShadeArgs& a, bool exact, hipStream_t s) {
     dim3 grid(shade_tiles(a)), block(256);
     bool dbg = a.debug_samples != nullptr || a.debug_cands != nullptr;
     size_t lds = shade_lds_bytes(a);
-    if (a.proposal) {
+    if (a.learned_pass) {
         if (exact) {
             if (dbg) hipLaunchKernelGGL((shade_kernel<true, true, true>), grid, block, lds, s, a);
             else hipLaunchKernelGGL((shade_kernel<true, false, true>), grid, block, lds, s, a);

@@ -121,63 +121,98 @@ def test_estimator_is_unbiased(orc_lib, ltc_table, roughness, metal):
         assert np.all(np.abs(est[p] - ref) < 5 * se[p] + 2e-3 * ref), (p, est[p], ref, se[p])
 
 
-# ---------------------------------------------------------------- learned target
-SCALE = np.float32(float.fromhex("0x1.c1c1c2p-9"))
+# ---------------------------------------------------------------- learned source
+def _random_tables(rng, P, K, sizes):
+    """Per-pixel tables: 16 distinct non-empty clusters with random 8-bit weights
+    (some zero, as the proposal kernel writes them), on some pixels all zero."""
+    nonempty = np.flatnonzero(sizes > 0)
+    tab = np.zeros((P, 32), np.uint8)
+    for p in range(P):
+        if p % 7 == 3:
+            continue  # T = 0: uniform fallback
+        ids = rng.choice(nonempty, min(16, nonempty.size), replace=False)
+        w = rng.integers(0, 256, ids.size)
+        w[rng.random(ids.size) < 0.2] = 0
+        w[0] = max(w[0], 1)
+        ids[w == 0] = 0
+        tab[p, :ids.size] = ids
+        tab[p, 16:16 + ids.size] = w
+    return tab
 
 
-def _factor(v):
-    return np.float32(0.125) + np.float32(v) * SCALE
+def _q_numpy(tab, idx, col, sizes, N):
+    """q = 1/(2N) + w_c / (2 T |c|) in float32 with the kernel's operation order."""
+    f = np.float32
+    T = int(tab[16:].astype(np.int64).sum())
+    if T == 0:
+        return f(1.0) / f(N)
+    q = f(0.5) / f(N)
+    c = col[idx]
+    for s in range(16):
+        if tab[16 + s] != 0 and tab[s] == c:
+            return q + (f(0.5) * f(tab[16 + s])) / (f(T) * f(sizes[c]))
+    return q
+
+
+def _source_oracle(scene, ltc_table, rng, K=16, bvh=True):
+    o = orc.Oracle(scene, *ltc_table, bvh=bvh)
+    N = scene.light_count
+    col = (np.arange(N) * 7 % K).astype(np.uint32)
+    sizes = np.bincount(col, minlength=K)
+    P = scene.width * scene.height
+    tab = _random_tables(rng, P, K, sizes)
+    o.set_proposal(tab, col, K)
+    return o, tab, col, sizes
+
+
+def test_learned_draw_density(small_scene, ltc_table):
+    """With the learned source: every candidate's weight is exactly p_hat / q
+    with q from the table (a pdf over the lights), and the candidate lights
+    are distributed as q (chi-square)."""
+    rng = np.random.default_rng(31)
+    o, tab, col, sizes = _source_oracle(small_scene, ltc_table, rng)
+    N = small_scene.light_count
+    consts = o.frame_constants()
+    vis = o.render_visibility(consts)
+    shaded = np.flatnonzero((vis >> 31) == 0)
+    pix = shaded[::23].astype(np.uint32)
+    s = small_scene.settings(sample_count_light=4, ris_candidate_count=8, shadow_rays=0,
+                             proposal=abi.PROPOSAL_LEARNED)
+    _, _, cand = o.shade(consts, s, vis, pixels=pix, candidates=True)
+    idx = cand[..., 0]
+    phat = cand[..., 2].view(np.float32)
+    w = cand[..., 3].view(np.float32)
+    counts = np.zeros(N)
+    expect = np.zeros(N)
+    for k, p in enumerate(pix):
+        qs = np.array([_q_numpy(tab[p], l, col, sizes, N) for l in range(N)], np.float32)
+        assert abs(qs.astype(np.float64).sum() - 1.0) < 1e-5
+        q = qs[idx[k]]
+        assert np.array_equal(w[k], (phat[k] / q).astype(np.float32)), k
+        counts += np.bincount(idx[k].ravel(), minlength=N)
+        expect += qs.astype(np.float64) * idx[k].size
+    chi2 = ((counts - expect) ** 2 / expect).sum()
+    assert chi2 < N + 6 * math.sqrt(2 * N), chi2
 
 
 def test_learned_target_factor_constant():
-    assert SCALE == np.float32(1.0 - 0.125) / np.float32(255.0)
-    assert _factor(255) <= np.float32(1.0) and _factor(0) == np.float32(0.125)
-
-
-def test_learned_target_scales_p_hat(small_oracle, small_scene):
-    """With a table, the candidates are the plain ones (same noise stream) and
-    each candidate's target is exactly p_hat * (1/8 + v * 7/8 / 255) with v
-    the table byte of the light's cluster; w = target / (1/N)."""
-    o = small_oracle
-    rng = np.random.default_rng(31)
-    N = small_scene.light_count
-    K = 16
-    col = (np.arange(N) * 7 % K).astype(np.uint32)
-    P = small_scene.width * small_scene.height
-    tab = rng.integers(0, 256, (P, 256)).astype(np.uint8)
-    o.set_proposal(tab, col, K)
-    consts = o.frame_constants()
-    vis = o.render_visibility(consts)
-    pix = np.flatnonzero((vis >> 31) == 0)[::17].astype(np.uint32)
-    # one sample per pixel: after the first final sample, the noise streams of
-    # the two runs may diverge (its draw count depends on the chosen light)
-    base = small_scene.settings(sample_count_light=1, ris_candidate_count=8, shadow_rays=0)
-    _, _, plain = o.shade(consts, base, vis, pixels=pix, candidates=True)
-    base.proposal = abi.PROPOSAL_LEARNED
-    _, _, learned = o.shade(consts, base, vis, pixels=pix, candidates=True)
-    assert np.array_equal(plain[..., 0], learned[..., 0]) and np.array_equal(plain[..., 1], learned[..., 1])
-    ph = plain[..., 2].view(np.float32)
-    fac = _factor(tab[pix[:, None, None], col[plain[..., 0]]].astype(np.float32))
-    want = (ph * fac).astype(np.float32)
-    assert np.array_equal(learned[..., 2].view(np.float32), want)
-    assert np.array_equal(learned[..., 3].view(np.float32), (want / (np.float32(1) / np.float32(N))).astype(np.float32))
+    """The cache factor of an empty entry (byte 255) is 1 to within 2 ulp."""
+    f = np.float32(1.0 / 128)
+    scale = (np.float32(1) - f) / np.float32(255)
+    assert abs(float(f + np.float32(255) * scale) - 1.0) < 2.5e-7
+    assert f + np.float32(0) * scale == f
 
 
 def test_learned_estimator_is_unbiased(orc_lib, ltc_table):
-    """Mean of the estimator with a learned target over many noise seeds equals
-    the plain mean on the same pixels (any positive target is unbiased)."""
+    """Mean of the learned-source estimator over many noise seeds equals the
+    plain (uniform-source) mean on the same pixels, shadows off."""
     scene = synth.small_scene(seed=3, width=48, height=48, light_count=24)
-    o = orc.Oracle(scene, *ltc_table, bvh=False)
     rng = np.random.default_rng(32)
-    N, K = scene.light_count, 6
-    col = (np.arange(N) % K).astype(np.uint32)
-    P = scene.width * scene.height
-    o.set_proposal(rng.integers(0, 256, (P, 256)).astype(np.uint8), col, K)
+    o, _, _, _ = _source_oracle(scene, ltc_table, rng, K=6, bvh=False)
     consts = o.frame_constants()
     o2 = orc.Oracle(scene, *ltc_table)
     vis = o2.render_visibility(consts)
-    shaded = np.flatnonzero((vis >> 31) == 0)
-    pix = shaded[::41][:10].astype(np.uint32)
+    pix = np.flatnonzero((vis >> 31) == 0)[::41][:10].astype(np.uint32)
     means = {}
     for prop in (abi.PROPOSAL_UNIFORM, abi.PROPOSAL_LEARNED):
         s = scene.settings(sample_count_light=256, ris_candidate_count=4, shadow_rays=0, proposal=prop)
@@ -215,20 +250,15 @@ def _mix(h, k):
     return (h * 5 + 0xE6546B64) & 0xFFFFFFFF
 
 
-def _cache_oracle(scene, ltc_table, log2, cell, prior, rng, per_light=False):
-    o = orc.Oracle(scene, *ltc_table)
+def _cache_oracle(scene, ltc_table, log2, cell, prior, rng, per_light=False, bvh=True):
+    o, tab, col, sizes = _source_oracle(scene, ltc_table, rng, bvh=bvh)
     N = scene.light_count
-    K = 16
-    col = (np.arange(N) * 7 % K).astype(np.uint32)
-    P = scene.width * scene.height
-    tab = rng.integers(0, 256, (P, 256)).astype(np.uint8)
-    o.set_proposal(tab, col, K)
     slots = N if per_light else 256
-    used = N if per_light else K
+    used = N if per_light else 16
     counts = np.zeros((1 << log2, slots, 2), dtype=np.uint32)
     counts[:, :used] = rng.integers(0, 40, (1 << log2, used, 2))
     o.set_cache(counts, cell, prior, per_light)
-    return o, tab, col, counts
+    return o, tab, col, sizes, counts
 
 
 def test_cache_keys_follow_cells_and_normals(small_scene, ltc_table):
@@ -237,7 +267,7 @@ def test_cache_keys_follow_cells_and_normals(small_scene, ltc_table):
     the table mask keeps the low bits."""
     rng = np.random.default_rng(40)
     cell, log2 = 0.5, 12
-    o, _, _, _ = _cache_oracle(small_scene, ltc_table, log2, cell, 2.0, rng)
+    o = _cache_oracle(small_scene, ltc_table, log2, cell, 2.0, rng)[0]
     consts = o.frame_constants()
     vis = o.render_visibility(consts)
     ground = np.flatnonzero(vis < 2)[::23].astype(np.uint32)  # triangles 0, 1: the ground plane
@@ -280,47 +310,43 @@ def test_cache_bytes_end_points():
 def test_cache_target_uses_entry_bytes(small_scene, ltc_table, per_light):
     """With a cache, each candidate's target is p_hat times f + v (1 - f) / 255
     with v the cache byte at the pixel's key and the light's slot (its
-    cluster, or the light), or the table's byte for its cluster where that
-    entry is empty; an all-empty cache gives the table's targets bit for
-    bit."""
+    cluster, or the light); 255 (about 1) for empty entries. Candidates and
+    source densities do not depend on the cache."""
     rng = np.random.default_rng(41)
     log2, cell, prior = 10, 0.5, 2.0
-    o, tab, col, counts = _cache_oracle(small_scene, ltc_table, log2, cell, prior, rng, per_light)
-    # a quarter of the used entries empty
+    o, tab, col, sizes, counts = _cache_oracle(small_scene, ltc_table, log2, cell, prior, rng, per_light)
     counts[rng.random(counts.shape[:2]) < 0.25] = 0
     vis8 = o.set_cache(counts, cell, prior, per_light)
     consts = o.frame_constants()
     vis = o.render_visibility(consts)
     pix = np.flatnonzero((vis >> 31) == 0)[::19].astype(np.uint32)
-    base = small_scene.settings(sample_count_light=1, ris_candidate_count=8, shadow_rays=0)
-    _, _, plain = o.shade(consts, base, vis, pixels=pix, candidates=True)
-    base.proposal = abi.PROPOSAL_LEARNED
-    _, rec, learned = o.shade(consts, base, vis, pixels=pix, records=True, candidates=True)
+    s = small_scene.settings(sample_count_light=1, ris_candidate_count=8, shadow_rays=0,
+                             proposal=abi.PROPOSAL_LEARNED)
+    _, rec, cached = o.shade(consts, s, vis, pixels=pix, records=True, candidates=True)
+    o.set_cache(None, cell, prior)
+    _, _, plain = o.shade(consts, s, vis, pixels=pix, candidates=True)
+    assert np.array_equal(cached[..., 0], plain[..., 0]) and np.array_equal(cached[..., 1], plain[..., 1])
     key = rec[:, 0, abi.REC_CACHE_KEY].view(np.uint32)
     li = plain[..., 0]
-    cl = col[li]
-    v = vis8[key[:, None, None], li if per_light else cl].astype(np.int64)
-    m = tab[pix[:, None, None], cl].astype(np.int64)
+    v = vis8[key[:, None, None], li if per_light else col[li]].astype(np.float32)
     assert np.any(v == 255) and np.any(v != 255)
-    v = np.where(v == 255, m, v).astype(np.float32)
     f = np.float32(0.125)
     fac = f + v * ((np.float32(1) - f) / np.float32(255))
     want = (plain[..., 2].view(np.float32) * fac).astype(np.float32)
-    assert np.array_equal(learned[..., 2].view(np.float32), want)
-    # all entries empty: identical to the table-only target
-    o.set_cache(np.zeros_like(counts), cell, prior, per_light)
-    _, _, empty = o.shade(consts, base, vis, pixels=pix, candidates=True)
-    o.set_cache(None, cell, prior)
-    _, _, table_only = o.shade(consts, base, vis, pixels=pix, candidates=True)
-    assert np.array_equal(empty, table_only)
+    assert np.array_equal(cached[..., 2].view(np.float32), want)
+    # w = target / q with the same q
+    ok = plain[..., 3].view(np.float32) > 0
+    q = plain[..., 2].view(np.float32)[ok] / plain[..., 3].view(np.float32)[ok]
+    assert np.allclose(want[ok] / cached[..., 3].view(np.float32)[ok], q, rtol=1e-6)
 
 
 def test_cached_learned_estimator_is_unbiased(orc_lib, ltc_table):
-    """A random visibility cache keeps the learned estimator's mean equal to
-    the plain one (the target stays positive wherever the LTC target is)."""
+    """A random visibility cache with the learned source keeps the
+    estimator's mean equal to the plain one (the target stays positive
+    wherever the LTC target is)."""
     scene = synth.small_scene(seed=3, width=48, height=48, light_count=24)
     rng = np.random.default_rng(42)
-    o, _, _, _ = _cache_oracle(scene, ltc_table, 8, 0.7, 4.0, rng, per_light=True)
+    o = _cache_oracle(scene, ltc_table, 8, 0.7, 4.0, rng, per_light=True, bvh=False)[0]
     consts = o.frame_constants()
     o2 = orc.Oracle(scene, *ltc_table)
     vis = o2.render_visibility(consts)
