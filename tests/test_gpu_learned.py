@@ -335,3 +335,30 @@ def test_resolve_counts_every_traced_ray(learned, per_light):
     assert 0.05 < want[..., 0].sum() / want.sum() < 0.95
     assert np.array_equal(table, orc.cache_bytes(got, 2.0).reshape(table.shape))
     assert (table != 255).sum() > 100
+
+
+def test_kmeans_reaches_a_lloyd_fixed_point(small_scene, ltc_table):
+    """risltc_cluster_lights after many iterations: every cluster is
+    non-empty and every light's nearest cluster mean (the means of the
+    returned assignment, over the lights' vertex centroids) is its own
+    cluster, the fixed point of Lloyd's iteration; the within-cluster sum of
+    squares does not grow with more iterations."""
+    import risltc_amd
+    ctx = risltc_amd.Context(0)
+    try:
+        ctx.load_scene(small_scene, *ltc_table)
+        cen = np.array([np.asarray(L.world, np.float64)[:L.vertex_count].mean(0) for L in small_scene.lights])
+        K = 12
+
+        def wcss(a):
+            return sum(((cen[a == k] - cen[a == k].mean(0)) ** 2).sum() for k in range(K))
+
+        a2 = ctx.cluster_lights(K, 2).astype(np.int64)
+        a200 = ctx.cluster_lights(K, 200).astype(np.int64)
+        assert np.bincount(a200, minlength=K).min() > 0
+        means = np.stack([cen[a200 == k].mean(0) for k in range(K)])
+        d = ((cen[:, None, :] - means[None]) ** 2).sum(2)
+        assert np.array_equal(d.argmin(1), a200)
+        assert wcss(a200) <= wcss(a2) + 1e-9
+    finally:
+        ctx.close()
