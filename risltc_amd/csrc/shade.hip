@@ -1498,6 +1498,14 @@ __global__ void __launch_bounds__(256) shade_batch_kernel(const ShadeArgs* views
 __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pixel) {
     if (pixel >= A.width * A.height) return;
     uint32_t prim = A.visibility[pixel];
+    const bool shaded = prim != 0xFFFFFFFFu && (prim >> 31) == 0u && (uint64_t)prim < A.triangle_count;
+    if (!shaded && A.rec_cluster) {
+        // no sample: empty training records
+        for (uint32_t j = 0; j < A.spp; ++j) {
+            A.rec_cluster[(size_t)pixel * A.spp + j] = 0xFFFFFFFFu;
+            A.rec_y[(size_t)pixel * A.spp + j] = 0.0f;
+        }
+    }
     float4 out;
     if (prim == 0xFFFFFFFFu || ((prim >> 31) == 0u && (uint64_t)prim >= A.triangle_count)) {
         out = make_float4(0.0f, 0.0f, 0.0f, 1.0f);

@@ -189,6 +189,9 @@ def test_training_step_gradient_and_adam(learned):
     y = y.cpu().numpy()
     assert cl.size == P * spp and (cl != 0xFFFFFFFF).sum() > 0.2 * cl.size
     valid = cl != 0xFFFFFFFF
+    # background and light pixels have empty records (not stale ones)
+    vis_px = g["vis"].cpu().numpy().view(np.uint32)
+    assert np.all(cl.reshape(P, spp)[(vis_px >> 31) != 0] == 0xFFFFFFFF)
     assert np.all(cl[valid] < g["K"]) and np.all((y[valid] >= 0) & (y[valid] <= 1))
     assert 0.05 < y[valid].mean() < 0.95
     before, _ = ctx.proposal_get_params()
