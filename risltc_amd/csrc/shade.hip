@@ -1169,6 +1169,82 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ---- survivor compaction -----------------------------------------------------------
+// The non-debug kernels evaluate the candidates that survive the cull
+// compacted over the wave: item (k, o) is the k-th survivor of lane o, the
+// items are dealt to the 64 lanes in order of k then o, and each round's
+// results go back to their owners, which run the reservoir in candidate
+// order. An evaluator reads its owner's pixel from LDS (one float per field
+// and lane, field-major). Every value is computed as in the owner lane.
+// The fields of a Pixel that eval_ltc_cp reads: position, albedo, the world
+// to shading matrix, the LTC matrix and its albedo.
+constexpr uint32_t kPxFloats = 28;
+constexpr uint32_t kPxLdsBytes = 4u * kPxFloats * 64u * 4u;  // 4 waves
+
+__device__ __forceinline__ void px_lds_store(float* b, uint32_t l, const Pixel& p) {
+    b[0u * 64u + l] = p.pos.x;
+    b[1u * 64u + l] = p.pos.y;
+    b[2u * 64u + l] = p.pos.z;
+    b[3u * 64u + l] = p.albedo.x;
+    b[4u * 64u + l] = p.albedo.y;
+    b[5u * 64u + l] = p.albedo.z;
+#pragma unroll
+    for (uint32_t i = 0; i < 12u; ++i) b[(6u + i) * 64u + l] = p.w2s[i];
+#pragma unroll
+    for (uint32_t i = 0; i < 9u; ++i) b[(18u + i) * 64u + l] = p.s2c[i];
+    b[27u * 64u + l] = p.ltc_albedo;
+}
+
+// (fast build: the five non-zero LTC entries)
+template <bool X>
+__device__ __forceinline__ void px_lds_load_eval(const float* b, uint32_t l, Pixel& p) {
+    p.pos.x = b[0u * 64u + l];
+    p.pos.y = b[1u * 64u + l];
+    p.pos.z = b[2u * 64u + l];
+    p.n = p.o = mk3(0.0f, 0.0f, 0.0f);
+    p.lo = p.rough = p.ltc_det = 0.0f;
+    p.f0 = mk3(0.0f, 0.0f, 0.0f);
+    p.albedo.x = b[3u * 64u + l];
+    p.albedo.y = b[4u * 64u + l];
+    p.albedo.z = b[5u * 64u + l];
+#pragma unroll
+    for (uint32_t i = 0; i < 12u; ++i) p.w2s[i] = b[(6u + i) * 64u + l];
+#pragma unroll
+    for (uint32_t i = 0; i < 9u; ++i) p.s2c[i] = (X || (i & 1u) == 0u) ? b[(18u + i) * 64u + l] : 0.0f;
+    p.ltc_albedo = b[27u * 64u + l];
+}
+
+// position of the n-th (from 0) set bit of m; m has more than n set bits
+__device__ __forceinline__ uint32_t nth_set_bit32(uint32_t m, uint32_t n) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t w = 16; w >= 1u; w >>= 1) {
+        const uint32_t lo = m & ((1u << w) - 1u);
+        const uint32_t c = (uint32_t)__popc(lo);
+        if (n >= c) {
+            n -= c;
+            m >>= w;
+            pos += w;
+        } else {
+            m = lo;
+        }
+    }
+    return pos;
+}
+
+__device__ __forceinline__ uint32_t nth_set_bit64(uint64_t m, uint32_t n) {
+    const uint32_t lo = (uint32_t)m, c = (uint32_t)__popc(lo);
+    if (n >= c) return 32u + nth_set_bit32((uint32_t)(m >> 32), n - c);
+    return nth_set_bit32(lo, n);
+}
+
+// LDS offset of the pixel area: after the spheres, cluster tables and rows
+__device__ __host__ __forceinline__ size_t px_lds_offset(uint32_t n_lds, bool tables, uint32_t clusters, bool rows) {
+    size_t b = rows ? row_lds_offset(n_lds, tables, clusters) + 4u * kRowsPerWave * kRowLdsMax
+                    : cluster_lds_end(n_lds, tables, clusters);
+    return (b + 15u) & ~(size_t)15u;
+}
+
 // Cache key of a shading point: its cell floor(p / cell) per axis and the
 // dominant axis and sign of the shading normal (the first axis on ties),
 // combined with MurmurHash3's mix and finaliser, masked to the table.
@@ -1229,7 +1305,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
     // out of bounds)
     bool shade = inside && (prim >> 31) == 0u && (uint64_t)prim < A.scene.triangle_count;
 
-    Pixel px;
+    Pixel px = {};
     Cull cull;
     uint32_t seed = 0;
     if (shade) {
@@ -1282,6 +1358,16 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
     // noise draws per candidate: light (1, or 2 for the learned source) + reservoir
     const uint32_t stride = learned ? 3u : 2u;
     const bool cached = learned && A.cache.vis != nullptr;
+    // the wave's pixels in LDS for the compacted evaluation (non-debug)
+    float* pxa = nullptr;
+    if constexpr (!DEBUG) {
+        const bool rows = learned && A.cache.vis != nullptr && A.cache.slots <= kRowLdsMax;
+        pxa = reinterpret_cast<float*>(reinterpret_cast<uint8_t*>(sh_spheres) +
+                                       px_lds_offset(n_lds, learned && all_lds, A.clusters.count, rows)) +
+              wave * kPxFloats * 64u;
+        px_lds_store(pxa, lane, px);
+        wave_lds_sync();
+    }
     for (uint32_t j = 0; j < spp; ++j) {
         int32_t chosen = -1;
         float w_sum = 0.0f, sample_value = 0.0f;
@@ -1300,10 +1386,10 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
 #pragma unroll
             for (int s2 = 0; s2 < kPropTop; ++s2) pt.T += byte_of(pt.ws, s2);
         }
-        if (shade) {
-            if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
-                chosen = (int32_t)light_index_from(noise_next(seed), N);
-            } else {
+        if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
+            if (shade) chosen = (int32_t)light_index_from(noise_next(seed), N);
+        } else if constexpr (DEBUG) {
+            if (shade) {
                 // Candidates in chunks of 32: first every candidate's light
                 // index and cull test (cheap, wave-uniform), then the full
                 // target of the survivors in their original order, the noise
@@ -1370,6 +1456,133 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                                 cr[3] = __float_as_uint(w);
                             }
                         }
+                    }
+                }
+            }
+        } else {
+            // Candidates in chunks: every lane draws its chunk's light
+            // indices and cull tests, then the wave evaluates the survivors
+            // compacted (see above) and each owner runs its reservoir over
+            // its survivors in their original order, the noise state jumped
+            // to each. Culled candidates have w = 0 and leave the reservoir
+            // as it was.
+            const uint32_t chunk = A.chunk;
+            for (uint32_t base = 0; base < M; base += chunk) {
+                const uint32_t cnt = (M - base < chunk) ? M - base : chunk;
+                const uint32_t seed_chunk = seed;
+                uint32_t live = 0;
+                if (shade) {
+                    for (uint32_t t = 0; t < cnt; ++t) {
+                        uint32_t idx = learned ? draw_learned(seed, pt, cl, N) : light_index_from(noise_next(seed), N);
+                        (void)noise_next(seed);
+                        float4 sp = all_lds ? sh_spheres[idx] : (idx < n_lds ? sh_spheres[idx] : A.scene.spheres[idx]);
+                        live |= culled(cull, sp) ? 0u : (1u << t);
+                    }
+                }
+                const uint32_t mycnt = (uint32_t)__popc(live);
+                // owner cursor over its own survivors
+                uint32_t s = seed_chunk, at = 0, live_rem = live;
+                // wave cursor: the next item is rank lvl_off of level lvl
+                uint32_t lvl = 0, lvl_off = 0;
+                uint64_t lvl_mask = __ballot(mycnt > 0u);
+                while (lvl_mask != 0ull) {
+                    // deal this round's items: lane e takes the e-th remaining item
+                    uint32_t my_k = 0xFFFFFFFFu, my_o = lane;
+                    {
+                        uint32_t kk = lvl, oo = lvl_off, ls = 0;
+                        uint64_t m = lvl_mask;
+                        while (ls < 64u && m != 0ull) {
+                            const uint32_t n = (uint32_t)__popcll(m) - oo;
+                            if (lane >= ls && lane - ls < n) {
+                                my_k = kk;
+                                my_o = nth_set_bit64(m, oo + (lane - ls));
+                            }
+                            ls += n;
+                            kk += 1u;
+                            oo = 0u;
+                            m = __ballot(mycnt > kk);
+                        }
+                    }
+                    const bool have = my_k != 0xFFFFFFFFu;
+                    const uint32_t live_o = (uint32_t)__shfl((int)live, (int)my_o);
+                    const uint32_t seed_o = (uint32_t)__shfl((int)seed_chunk, (int)my_o);
+                    PropTable pt_o;
+                    pt_o.T = 0u;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) pt_o.ids[k] = pt_o.ws[k] = 0u;
+                    if constexpr (learned) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            pt_o.ids[k] = (uint32_t)__shfl((int)pt.ids[k], (int)my_o);
+                            pt_o.ws[k] = (uint32_t)__shfl((int)pt.ws[k], (int)my_o);
+                        }
+                        pt_o.T = (uint32_t)__shfl((int)pt.T, (int)my_o);
+                    }
+                    float p_raw = 0.0f;
+                    uint32_t idx_e = 0u;
+                    if (have) {
+                        const uint32_t t = nth_set_bit32(live_o, my_k);
+                        uint32_t se = lcg_jump(seed_o, stride * t);
+                        idx_e = learned ? draw_learned(se, pt_o, cl, N) : light_index_from(noise_next(se), N);
+                        Pixel q;
+                        px_lds_load_eval<X>(pxa, my_o, q);
+                        load_light(A.scene.lights, idx_e, L);
+                        uint32_t cc;
+                        f3 col = eval_ltc_cp<X>(q, L, cc);
+                        p_raw = sqrt_<X>(dot3(col, col));
+                    }
+                    // owners take their items of this round in level order
+                    {
+                        uint32_t kk = lvl, oo = lvl_off, ls = 0;
+                        uint64_t m = lvl_mask;
+                        while (ls < 64u && m != 0ull) {
+                            const uint32_t n = (uint32_t)__popcll(m) - oo;
+                            const bool mine = ((m >> lane) & 1ull) != 0ull;
+                            const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                            const bool here = mine && rank >= oo && rank - oo < n && ls + (rank - oo) < 64u;
+                            const int src = here ? (int)(ls + rank - oo) : (int)lane;
+                            const float ph = __shfl(p_raw, src);
+                            const uint32_t idx = (uint32_t)__shfl((int)idx_e, src);
+                            if (here) {
+                                const uint32_t t = (uint32_t)__builtin_ctz(live_rem);
+                                live_rem &= live_rem - 1u;
+                                s = lcg_jump(s, stride * (t - at));
+                                at = t + 1u;
+                                // past the light draws
+                                (void)noise_next(s);
+                                if (learned) (void)noise_next(s);
+                                float p_hat = ph;
+                                if (cached)
+                                    p_hat = p_hat * cache_factor(cl.of_light, idx, A.learned_floor, A.learned_scale,
+                                                                 crow, reinterpret_cast<const uint8_t*>(sh_spheres),
+                                                                 lds_row, A.cache.per_light);
+                                float w = div_<X>(p_hat, learned ? learned_density<X>(idx, pt, cl, N) : pN);
+                                float r = noise_next(s);
+                                w_sum += w;
+                                bool take = X ? (r < w / w_sum) : (r * w_sum < w);
+                                if (w > 0.0f && take) {
+                                    chosen = (int32_t)idx;
+                                    sample_value = p_hat;
+                                }
+                            }
+                            ls += n;
+                            kk += 1u;
+                            oo = 0u;
+                            m = __ballot(mycnt > kk);
+                        }
+                    }
+                    // advance the wave cursor past the round's 64 items
+                    uint32_t left = 64u;
+                    while (lvl_mask != 0ull) {
+                        const uint32_t n = (uint32_t)__popcll(lvl_mask) - lvl_off;
+                        if (left < n) {
+                            lvl_off += left;
+                            break;
+                        }
+                        left -= n;
+                        lvl += 1u;
+                        lvl_off = 0u;
+                        lvl_mask = __ballot(mycnt > lvl);
                     }
                 }
             }
@@ -1466,8 +1679,11 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
     }
 }
 
+// Three waves per SIMD: the register budget is 168 VGPRs (the final-sample
+// code spills); the measured gain over two waves comes from hiding the
+// latency of the candidate evaluation.
 template <bool X, bool DEBUG, bool LEARNED>
-__global__ void __launch_bounds__(256) shade_kernel(ShadeArgs A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) shade_kernel(ShadeArgs A) {
     extern __shared__ float4 sh_spheres[];
     shade_tile<X, DEBUG, LEARNED>(A, blockIdx.x, sh_spheres);
 }
@@ -1487,7 +1703,8 @@ __device__ __forceinline__ uint32_t view_of(const uint32_t* offsets, uint32_t n,
 // view v, with v's descriptor read from device memory (so a captured graph
 // can be replayed with new cameras and noise seeds).
 template <bool X>
-__global__ void __launch_bounds__(256) shade_batch_kernel(const ShadeArgs* views, const uint32_t* tile_offsets,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+shade_batch_kernel(const ShadeArgs* views, const uint32_t* tile_offsets,
                                                           uint32_t count) {
     extern __shared__ float4 sh_spheres[];
     const uint32_t v = view_of(tile_offsets, count, blockIdx.x);
@@ -1609,11 +1826,10 @@ uint32_t shade_tiles(const ShadeArgs& a) {
 size_t shade_lds_bytes(const ShadeArgs& a) {
     const uint32_t n = a.scene.light_count < a.lds_lights ? a.scene.light_count : a.lds_lights;
     const bool tables = a.learned_pass && n == a.scene.light_count;
-    size_t b = cluster_lds_end(n, tables, a.clusters.count);
     // learned pass with a cache: staged rows, kRowsPerWave per wave
-    if (a.learned_pass && a.cache.vis && a.cache.slots <= kRowLdsMax)
-        b = row_lds_offset(n, tables, a.clusters.count) + 4u * kRowsPerWave * kRowLdsMax;
-    return b;
+    const bool rows = a.learned_pass && a.cache.vis && a.cache.slots <= kRowLdsMax;
+    // then the wave's pixels for the compacted evaluation
+    return px_lds_offset(n, tables, a.clusters.count, rows) + kPxLdsBytes;
 }
 
 static bool shade_args_ok(const ShadeArgs& a) {

@@ -365,3 +365,28 @@ def test_kmeans_reaches_a_lloyd_fixed_point(small_scene, ltc_table):
         assert wcss(a200) <= wcss(a2) + 1e-9
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+def test_learned_compacted_equals_debug(learned, exact):
+    """Learned source with a visibility cache: the non-debug (compacted)
+    frame equals the debug (per-lane) frame bit for bit."""
+    import torch
+    g = learned
+    sc, ctx, P = g["scene"], g["ctx"], g["P"]
+    rng = np.random.default_rng(5)
+    ctx.proposal_cache(8, 0.5, 1.0, True)
+    try:
+        counts = rng.integers(0, 30, (1 << 8, sc.light_count, 2)).astype(np.uint32)
+        ctx.proposal_cache_write(counts)
+        s = sc.settings(exact_math=exact, sample_count_light=2, ris_candidate_count=24,
+                        proposal=abi.PROPOSAL_LEARNED, shadow_rays=0)
+        a = torch.empty(P * 4, dtype=torch.float32, device="cuda")
+        b = torch.empty(P * 4, dtype=torch.float32, device="cuda")
+        rec = torch.zeros(P * 2 * abi.SAMPLE_RECORD_FLOATS, dtype=torch.float32, device="cuda")
+        ctx.shade(g["consts"], s, g["vis"], a)
+        ctx.shade_debug(g["consts"], s, g["vis"], b, rec)
+        torch.cuda.synchronize()
+    finally:
+        ctx.proposal_cache(0)
+    assert np.array_equal(a.cpu().numpy().view(np.uint32), b.cpu().numpy().view(np.uint32))

@@ -235,6 +235,34 @@ def test_fast_shading_parity(small_gpu):
     assert np.percentile(rel, 99) < 1e-3, _quantiles(rel)
 
 
+COMPACT_CASES = [dict(exact_math=1, sample_count_light=2, ris_candidate_count=8),
+                 dict(exact_math=1, sample_count_light=1, ris_candidate_count=32, mis_heuristic=abi.MIS_BALANCE),
+                 dict(exact_math=0, sample_count_light=3, ris_candidate_count=32),
+                 dict(exact_math=0, sample_count_light=1, ris_candidate_count=77, shadow_rays=0)]
+
+
+@pytest.mark.parametrize("case", range(len(COMPACT_CASES)))
+@pytest.mark.parametrize("tuning", [None, dict(wave_width=64, cull_chunk=5, lds_lights=0, trace_refill=16)])
+def test_compacted_evaluation_equals_debug(small_gpu, case, tuning):
+    """The non-debug kernels evaluate the surviving candidates compacted over
+    the wave; the debug kernels (bitwise equal to the oracle in the exact
+    build) evaluate them per lane. The frames are equal bit for bit, in both
+    builds and under a non-default launch shape."""
+    g = small_gpu
+    s = g["scene"].settings(**COMPACT_CASES[case])
+    ctx = g["ctx"]
+    default = ctx.get_tuning().as_dict()
+    try:
+        if tuning:
+            ctx.set_tuning(**tuning)
+        a, _, _ = _shade(g, s, debug=False)
+        b, _, _ = _shade(g, s, debug=True)
+    finally:
+        ctx.set_tuning(**default)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert (a[:, :3] > 0).any()
+
+
 def test_deterministic_repeat(small_gpu):
     """Two passes on the same inputs give the same bits (ray compaction order
     varies, results must not)."""
