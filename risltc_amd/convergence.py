@@ -40,6 +40,9 @@ def accumulate(ctx, scene, vis, frames, seed0, learned=None, checkpoints=(), ref
                        learned_floor_log2=lk.get("floor_log2", 0),
                        **({"ris_candidate_count": m} if m else {}))
     ctx.proposal_cache(0)
+    tuning = ctx.get_tuning().as_dict()
+    if lk.get("lds_lights") is not None:
+        ctx.set_tuning(**dict(tuning, lds_lights=lk["lds_lights"]))
     if learned:
         ctx.cluster_lights(learned.get("clusters", 256), 20)
         ctx.proposal_init(learned.get("seed", 1))
@@ -50,6 +53,7 @@ def accumulate(ctx, scene, vis, frames, seed0, learned=None, checkpoints=(), ref
         return _frames(ctx, scene, vis, frames, seed0, learned, checkpoints, ref, s, out, acc, P)
     finally:
         ctx.proposal_cache(0)
+        ctx.set_tuning(**tuning)
 
 
 def _frames(ctx, scene, vis, frames, seed0, learned, checkpoints, ref, s, out, acc, P):
@@ -101,9 +105,12 @@ def time_to(curve, target):
 # The learned configuration the experiment uses: learned source (MLP
 # inference every 2 frames, one Adam step of 8192 samples per frame, learning
 # rate 0.02), target floor 1/128 with a per-light visibility cache of 2^16
-# one-metre cells, M = 24 candidates.
+# one-metre cells, M = 24 candidates, 512 light spheres staged in LDS (the
+# learned pass's cluster tables and cache rows need LDS too; with all 1024
+# spheres a CU holds two work-groups instead of three).
 DEFAULT_LEARNED = dict(floor_log2=7, infer_every=2, train_every=1, batch=8192, lr=2e-2, clusters=256, seed=1,
-                       cache_log2=16, cache_cell=1.0, cache_prior=1.0, cache_per_light=True, candidates=24)
+                       cache_log2=16, cache_cell=1.0, cache_prior=1.0, cache_per_light=True, candidates=24,
+                       lds_lights=512)
 
 
 def reference(ctx, scene, vis, frames, chunk=2048, seed0=1000000, log=None):

@@ -40,6 +40,14 @@ __device__ __forceinline__ uint16_t bf16_bits(float f) {
 }
 __device__ __forceinline__ float bf16_float(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// two values rounded to nearest even by v_cvt_pk_bf16_f32: a in the low half
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t bf16_pack2(float a, float b) {
+    f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+
 __device__ __forceinline__ bf16x8 frag(const uint16_t* p) {
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
 }
@@ -174,15 +182,11 @@ __device__ __forceinline__ void hidden_layer(const uint16_t* W, const float* bia
 #pragma unroll
         for (int ks = 0; ks < K / 32; ++ks)
             acc = mfma(frag(W + (ot * 16 + r) * LDW + ks * 32 + 8 * g), frag(act + r * LDA + ks * 32 + 8 * g), acc);
-        uint32_t lo = 0, hi = 0;
+        float v[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            float v = fmax_(acc[i] + bias[ot * 16 + 4 * g + i], 0.0f);
-            uint32_t b = bf16_bits(v);
-            if (i < 2) lo |= b << (16 * i);
-            else hi |= b << (16 * (i - 2));
-        }
-        *reinterpret_cast<uint2*>(out + r * kLdH + ot * 16 + 4 * g) = make_uint2(lo, hi);
+        for (int i = 0; i < 4; ++i) v[i] = fmax_(acc[i] + bias[ot * 16 + 4 * g + i], 0.0f);
+        *reinterpret_cast<uint2*>(out + r * kLdH + ot * 16 + 4 * g) =
+            make_uint2(bf16_pack2(v[0], v[1]), bf16_pack2(v[2], v[3]));
     }
 }
 
@@ -220,8 +224,7 @@ __global__ void __launch_bounds__(64 * kPropWaves) proposal_kernel(ProposalArgs 
             shaded = in && encode_features(A.f, pixel, f);
 #pragma unroll
             for (int k = 0; k < kMlpIn; k += 2)
-                *reinterpret_cast<uint32_t*>(act + r * kLdW1 + k) =
-                    (uint32_t)bf16_bits(f[k]) | ((uint32_t)bf16_bits(f[k + 1]) << 16);
+                *reinterpret_cast<uint32_t*>(act + r * kLdW1 + k) = bf16_pack2(f[k], f[k + 1]);
             if (A.features && in)
                 for (int k = 0; k < kMlpIn; ++k) A.features[(size_t)pixel * kMlpIn + k] = act[r * kLdW1 + k];
         }
