@@ -128,6 +128,8 @@ def main():
                     help="skip the learned-vs-plain time-to-relMSE experiment")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view measurement")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--learned-frames", type=int, default=0,
+                    help="only run N frames of the learned pass (for profiling its kernels)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (known HBM bytes) for counter scaling")
     args = ap.parse_args()
@@ -168,6 +170,18 @@ def main():
     print("setup %.1f s: %d triangles, %d lights, %dx%d, spp %d, M %d" %
           (setup_s, scene.mesh.triangle_count, scene.light_count, scene.width, scene.height, spp,
            scene.candidates), file=sys.stderr, flush=True)
+
+    if args.learned_frames > 0:
+        # the learned pass's kernels as the experiment runs them, without the
+        # reference and the plain pass
+        from risltc_amd import convergence
+        t1 = time.time()
+        _, gpu_ms, _ = convergence.accumulate(ctx, scene, vis, args.learned_frames, 0,
+                                              learned=dict(convergence.DEFAULT_LEARNED))
+        print(json.dumps({"learned_frames": args.learned_frames, "gpu_ms": round(gpu_ms, 2),
+                          "ms_per_frame": [round(x, 4) for x in convergence._frames.kernel_ms],
+                          "wall_s": round(time.time() - t1, 1)}), flush=True)
+        return
 
     tuning = None
     if not args.no_autotune:
