@@ -466,15 +466,31 @@ int risltc_last_timings(const risltc_context_t* context, float out_ms[4]);
 /* ------------------------------------------------------------------------ */
 /* Batches of views. No counterpart in the reference, which shades one view
  * per frame (main.c:2050-2055); a batch shades several views (cameras,
- * resolutions, settings, light sets) of one scene in one launch of each
- * kernel, and a graph replays such a batch as one hipGraph. Every view's
- * frame is bitwise equal to a risltc_shade call with the same inputs. */
+ * resolutions, settings, light sets, meshes) in one launch of each kernel,
+ * the shadow rays of every view in one persistent trace over the views'
+ * hierarchies, and a graph replays such a batch as one hipGraph. Every
+ * view's frame is bitwise equal to a risltc_shade call with the same inputs
+ * on a context holding its lights and mesh. */
 
 /* A further light set (records as for risltc_set_lights) that views may
  * name; *out_id receives its id (>= 1). Id 0 names the lights of
  * risltc_set_lights. Light sets live until risltc_destroy. */
 int risltc_add_light_set(risltc_context_t* context, const void* records, uint32_t count,
                          uint32_t max_vertex_count, uint32_t* out_id);
+
+/* A further mesh (arguments as for risltc_set_scene, its hierarchy built
+ * with the context's builder) that views may name; *out_id receives its id
+ * (>= 1). Id 0 names the mesh of risltc_set_scene. Meshes live until
+ * risltc_destroy. */
+int risltc_add_scene(risltc_context_t* context, const uint32_t* positions,
+                     const uint16_t* normals_and_tex_coords, const uint8_t* material_indices,
+                     uint64_t triangle_count, const float dequantization_factor[3],
+                     const float dequantization_summand[3], const risltc_material_t* materials,
+                     uint32_t material_count, uint32_t* out_id);
+/* risltc_render_visibility on the mesh with the given id. */
+int risltc_render_visibility_scene(risltc_context_t* context, uint32_t scene,
+                                   const risltc_frame_constants_t* constants, uint32_t* visibility_out,
+                                   void* stream);
 
 typedef struct risltc_view_s {
     risltc_frame_constants_t constants;
@@ -483,7 +499,8 @@ typedef struct risltc_view_s {
     const uint32_t* visibility;   /* device, R32_UINT per pixel */
     float* out;                   /* device, RGBA32F per pixel */
     uint32_t light_set;           /* 0 or an id from risltc_add_light_set */
-    uint32_t reserved[3];
+    uint32_t scene;               /* 0 or an id from risltc_add_scene */
+    uint32_t reserved[2];
 } risltc_view_t;
 
 /* Shades count views (device pointers in each view) on stream. */

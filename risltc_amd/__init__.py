@@ -76,6 +76,9 @@ SIGNATURES = {
     "risltc_proposal_cache_write": (C.c_int, [_vp, _u32p, C.c_uint64]),
     "risltc_proposal_cache_table": (C.c_int, [_vp, _u8p, C.c_uint64]),
     "risltc_add_light_set": (C.c_int, [_vp, _vp, _u32, _u32, _u32p]),
+    "risltc_add_scene": (C.c_int, [_vp, _u32p, _u16p, _u8p, C.c_uint64, _f32p, _f32p, C.POINTER(abi.Material), _u32,
+                                   _u32p]),
+    "risltc_render_visibility_scene": (C.c_int, [_vp, _u32, C.POINTER(abi.FrameConstants), _vp, _vp]),
     "risltc_shade_batch": (C.c_int, [_vp, C.POINTER(abi.View), _u32, _vp]),
     "risltc_get_tuning": (C.c_int, [_vp, C.POINTER(abi.Tuning)]),
     "risltc_set_tuning": (C.c_int, [_vp, C.POINTER(abi.Tuning)]),
@@ -387,8 +390,28 @@ class Context:
                                                 C.byref(out)), "risltc_add_light_set")
         return out.value
 
+    def add_scene(self, scene):
+        """A further mesh (a risltc_amd.synth.Scene's mesh and materials); returns its id."""
+        m = scene.mesh
+        pos = np.ascontiguousarray(m.positions, dtype=np.uint32)
+        nuv = np.ascontiguousarray(m.normals_uv, dtype=np.uint16)
+        mi = np.ascontiguousarray(m.material_index, dtype=np.uint8)
+        f = np.ascontiguousarray(m.dequantization_factor, dtype=np.float32)
+        a = np.ascontiguousarray(m.dequantization_summand, dtype=np.float32)
+        mats = scene.materials_ctypes()
+        out = C.c_uint32(0)
+        self._check(self.L.risltc_add_scene(self.h, _ptr(pos, _u32p), _ptr(nuv, _u16p), _ptr(mi, _u8p),
+                                            m.triangle_count, _ptr(f, _f32p), _ptr(a, _f32p),
+                                            C.cast(mats, C.POINTER(abi.Material)), len(mats), C.byref(out)),
+                    "risltc_add_scene")
+        return out.value
+
+    def render_visibility_scene(self, scene_id, constants, out, stream=None):
+        self._check(self.L.risltc_render_visibility_scene(self.h, scene_id, C.byref(constants), _dptr(out),
+                                                          _stream(stream)), "risltc_render_visibility_scene")
+
     @staticmethod
-    def view(constants, settings, visibility, out, light_set=0):
+    def view(constants, settings, visibility, out, light_set=0, scene=0):
         """A risltc_view_t over torch device tensors (keep them alive)."""
         v = abi.View()
         v.constants = constants
@@ -396,6 +419,7 @@ class Context:
         v.visibility = visibility.data_ptr()
         v.out = out.data_ptr()
         v.light_set = light_set
+        v.scene = scene
         return v
 
     @staticmethod

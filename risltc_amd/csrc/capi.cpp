@@ -55,14 +55,23 @@ struct LightSet {
     float bound = 0.0f;
 };
 
-// everything a batch of views writes: per-view descriptors, samples, the
-// shared shadow-ray list, hits and counters
+// a further mesh with its hierarchy (id = index + 1)
+struct SceneSet {
+    DevBuf positions, normals_uv, material_idx, materials;
+    uint64_t triangle_count = 0;
+    uint32_t material_count = 0;
+    rl::BvhBuild bvh;
+};
+
+// everything a batch of views writes: per-view descriptors, the hierarchy
+// table of their scenes, samples, the shared shadow-ray list, hits and
+// counters
 struct BatchBuffers {
-    DevBuf shade_args, tile_offsets, resolve_args, block_offsets;
+    DevBuf shade_args, tile_offsets, resolve_args, block_offsets, bvh_table;
     DevBuf samples, rays, ray_slots, hits, counter;
     void release() {
-        DevBuf* b[] = {&shade_args, &tile_offsets, &resolve_args, &block_offsets, &samples, &rays, &ray_slots, &hits,
-                       &counter};
+        DevBuf* b[] = {&shade_args, &tile_offsets, &resolve_args, &block_offsets, &bvh_table, &samples, &rays,
+                       &ray_slots, &hits, &counter};
         for (DevBuf* x : b) x->release();
     }
 };
@@ -105,8 +114,11 @@ struct risltc_context_s {
     float vcache_cell = 1.0f, vcache_prior = 2.0f;
     // launch shape (autotuned; RISLTC_TRACE_REFILL overrides the refill at create)
     risltc_tuning_t tuning = {16, 32, rl::kLdsSphereMax, 32};
-    // extra light sets for batched views (id = index + 1)
+    // extra light sets and meshes for batched views (id = index + 1)
     std::vector<LightSet*> light_sets;
+    std::vector<SceneSet*> scene_sets;
+    // the context mesh's hierarchy as a one-entry table for the trace kernel
+    DevBuf bvh_table;
     // buffers of risltc_shade_batch (graphs own theirs)
     BatchBuffers batch;
     // bumped whenever device data that view descriptors point at, or the
@@ -490,6 +502,14 @@ void risltc_destroy(risltc_context_t* c) {
         delete ls;
     }
     c->light_sets.clear();
+    for (SceneSet* ss : c->scene_sets) {
+        DevBuf* b[] = {&ss->positions, &ss->normals_uv, &ss->material_idx, &ss->materials};
+        for (DevBuf* x : b) x->release();
+        rl::free_bvh(&ss->bvh);
+        delete ss;
+    }
+    c->scene_sets.clear();
+    c->bvh_table.release();
     c->batch.release();
     DevBuf* learned_bufs[] = {&c->cluster_of_light, &c->cluster_offsets, &c->cluster_lights, &c->mlp_params,
                               &c->mlp_m, &c->mlp_v, &c->mlp_grad, &c->mlp_wf, &c->mlp_wb, &c->mlp_loss,
@@ -1057,29 +1077,50 @@ int risltc_bvh_records(risltc_context_t* c, float* out, uint64_t capacity, uint3
     return RISLTC_OK;
 }
 
-int risltc_set_scene(risltc_context_t* c, const uint32_t* positions, const uint16_t* nuv, const uint8_t* mat_idx,
-                     uint64_t tri_count, const float deq_factor[3], const float deq_summand[3],
-                     const risltc_material_t* materials, uint32_t material_count) {
-    if (!c) return RISLTC_E_ARG;
+// uploads a mesh in the *.vks layout and builds its hierarchy
+static int upload_mesh(risltc_context_t* c, const uint32_t* positions, const uint16_t* nuv, const uint8_t* mat_idx,
+                       uint64_t tri_count, const float deq_factor[3], const float deq_summand[3],
+                       const risltc_material_t* materials, uint32_t material_count, DevBuf& P, DevBuf& N, DevBuf& I,
+                       DevBuf& Mt, rl::BvhBuild* bvh) {
     if (!positions || !nuv || !mat_idx || !deq_factor || !deq_summand || !materials || tri_count == 0 ||
         material_count == 0)
         return fail(c, RISLTC_E_ARG, "bad scene arguments");
     if (tri_count > rl::kMaxBvhTriangles) return fail(c, RISLTC_E_UNSUPPORTED, "too many triangles for the LBVH");
-    ++c->generation;
     HIPC(c, hipSetDevice(c->device));
-    HIPC(c, c->positions.ensure(tri_count * 3 * 8));
-    HIPC(c, c->normals_uv.ensure(tri_count * 3 * 8));
-    HIPC(c, c->material_idx.ensure(tri_count));
-    HIPC(c, c->materials.ensure((size_t)material_count * 6 * sizeof(float)));
-    HIPC(c, hipMemcpy(c->positions.p, positions, tri_count * 3 * 8, hipMemcpyHostToDevice));
-    HIPC(c, hipMemcpy(c->normals_uv.p, nuv, tri_count * 3 * 8, hipMemcpyHostToDevice));
-    HIPC(c, hipMemcpy(c->material_idx.p, mat_idx, tri_count, hipMemcpyHostToDevice));
+    HIPC(c, P.ensure(tri_count * 3 * 8));
+    HIPC(c, N.ensure(tri_count * 3 * 8));
+    HIPC(c, I.ensure(tri_count));
+    HIPC(c, Mt.ensure((size_t)material_count * 6 * sizeof(float)));
+    HIPC(c, hipMemcpy(P.p, positions, tri_count * 3 * 8, hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(N.p, nuv, tri_count * 3 * 8, hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(I.p, mat_idx, tri_count, hipMemcpyHostToDevice));
     std::vector<float> m((size_t)material_count * 6);
     for (uint32_t i = 0; i < material_count; ++i) {
         for (int k = 0; k < 3; ++k) m[6 * i + k] = materials[i].base_color[k];
         for (int k = 0; k < 3; ++k) m[6 * i + 3 + k] = materials[i].specular[k];
     }
-    HIPC(c, hipMemcpy(c->materials.p, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(Mt.p, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice));
+    rl::free_bvh(bvh);
+    HIPC(c, rl::build_bvh(P.as<uint2>(), tri_count, deq_factor, deq_summand, c->bvh_builder, bvh, nullptr));
+    return RISLTC_OK;
+}
+
+static rl::BvhView bvh_view_of(const rl::BvhBuild& b) {
+    rl::BvhView v;
+    v.rec = b.rec;
+    v.units = b.units;
+    v.leaf_count = b.leaf_count;
+    return v;
+}
+
+int risltc_set_scene(risltc_context_t* c, const uint32_t* positions, const uint16_t* nuv, const uint8_t* mat_idx,
+                     uint64_t tri_count, const float deq_factor[3], const float deq_summand[3],
+                     const risltc_material_t* materials, uint32_t material_count) {
+    if (!c) return RISLTC_E_ARG;
+    ++c->generation;
+    int e = upload_mesh(c, positions, nuv, mat_idx, tri_count, deq_factor, deq_summand, materials, material_count,
+                        c->positions, c->normals_uv, c->material_idx, c->materials, &c->bvh);
+    if (e) return e;
     c->triangle_count = tri_count;
     c->material_count = material_count;
     // the quantisation grid spans the scene: 21 bits per axis
@@ -1087,28 +1128,75 @@ int risltc_set_scene(risltc_context_t* c, const uint32_t* positions, const uint1
         c->scene_lo[a] = deq_summand[a];
         c->scene_hi[a] = deq_summand[a] + deq_factor[a] * 2097151.0f;
     }
-    rl::free_bvh(&c->bvh);
-    HIPC(c, rl::build_bvh(c->positions.as<uint2>(), tri_count, deq_factor, deq_summand, c->bvh_builder, &c->bvh,
-                          nullptr));
+    const rl::BvhView v = bvh_view_of(c->bvh);
+    HIPC(c, c->bvh_table.ensure(sizeof(rl::BvhView)));
+    HIPC(c, hipMemcpy(c->bvh_table.p, &v, sizeof(v), hipMemcpyHostToDevice));
     return clear_vcache(c);
 }
 
-static rl::BvhView bvh_view(const risltc_context_t* c) {
-    rl::BvhView v;
-    v.rec = c->bvh.rec;
-    v.units = c->bvh.units;
-    v.leaf_count = c->bvh.leaf_count;
-    return v;
+int risltc_add_scene(risltc_context_t* c, const uint32_t* positions, const uint16_t* nuv, const uint8_t* mat_idx,
+                     uint64_t tri_count, const float deq_factor[3], const float deq_summand[3],
+                     const risltc_material_t* materials, uint32_t material_count, uint32_t* id) {
+    if (!c) return RISLTC_E_ARG;
+    if (!id) return fail(c, RISLTC_E_ARG, "null id");
+    auto* ss = new SceneSet();
+    int e = upload_mesh(c, positions, nuv, mat_idx, tri_count, deq_factor, deq_summand, materials, material_count,
+                        ss->positions, ss->normals_uv, ss->material_idx, ss->materials, &ss->bvh);
+    if (e) {
+        DevBuf* b[] = {&ss->positions, &ss->normals_uv, &ss->material_idx, &ss->materials};
+        for (DevBuf* x : b) x->release();
+        rl::free_bvh(&ss->bvh);
+        delete ss;
+        return e;
+    }
+    ss->triangle_count = tri_count;
+    ss->material_count = material_count;
+    c->scene_sets.push_back(ss);
+    *id = (uint32_t)c->scene_sets.size();
+    return RISLTC_OK;
+}
+
+// the mesh and hierarchy a view shades: id 0 = the context's own
+struct SceneRef {
+    const uint2* positions;
+    const ushort4* normals_uv;
+    const uint8_t* material_idx;
+    const float* materials;
+    uint32_t material_count;
+    uint64_t triangle_count;
+    rl::BvhView bvh;
+};
+
+static int scene_ref(risltc_context_t* c, uint32_t id, SceneRef& r) {
+    if (id == 0) {
+        if (!c->positions.p) return fail(c, RISLTC_E_STATE, "risltc_set_scene has not been called");
+        r = SceneRef{c->positions.as<uint2>(), c->normals_uv.as<ushort4>(), c->material_idx.as<uint8_t>(),
+                     c->materials.as<float>(), c->material_count, c->triangle_count, bvh_view_of(c->bvh)};
+        return RISLTC_OK;
+    }
+    if (id > c->scene_sets.size()) return fail(c, RISLTC_E_ARG, "scene %u does not exist", id);
+    const SceneSet* ss = c->scene_sets[id - 1];
+    r = SceneRef{ss->positions.as<uint2>(), ss->normals_uv.as<ushort4>(), ss->material_idx.as<uint8_t>(),
+                 ss->materials.as<float>(), ss->material_count, ss->triangle_count, bvh_view_of(ss->bvh)};
+    return RISLTC_OK;
+}
+
+int risltc_render_visibility_scene(risltc_context_t* c, uint32_t scene, const risltc_frame_constants_t* k,
+                                   uint32_t* out, void* stream) {
+    if (!c) return RISLTC_E_ARG;
+    if (!k || !out) return fail(c, RISLTC_E_ARG, "null argument");
+    SceneRef S;
+    int e = scene_ref(c, scene, S);
+    if (e) return e;
+    if (!S.bvh.rec) return fail(c, RISLTC_E_STATE, "risltc_set_scene has not been called");
+    if (k->viewport_size[0] == 0 || k->viewport_size[1] == 0) return fail(c, RISLTC_E_ARG, "empty viewport");
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, rl::launch_primary(S.bvh, S.positions, *k, out, (hipStream_t)stream));
+    return RISLTC_OK;
 }
 
 int risltc_render_visibility(risltc_context_t* c, const risltc_frame_constants_t* k, uint32_t* out, void* stream) {
-    if (!c) return RISLTC_E_ARG;
-    if (!k || !out) return fail(c, RISLTC_E_ARG, "null argument");
-    if (!c->bvh.rec) return fail(c, RISLTC_E_STATE, "risltc_set_scene has not been called");
-    if (k->viewport_size[0] == 0 || k->viewport_size[1] == 0) return fail(c, RISLTC_E_ARG, "empty viewport");
-    HIPC(c, hipSetDevice(c->device));
-    HIPC(c, rl::launch_primary(bvh_view(c), c->positions.as<uint2>(), *k, out, (hipStream_t)stream));
-    return RISLTC_OK;
+    return risltc_render_visibility_scene(c, 0, k, out, stream);
 }
 
 int risltc_trace_shadow_rays(risltc_context_t* c, const float* rays, uint32_t n, uint32_t* hits, void* stream) {
@@ -1117,7 +1205,7 @@ int risltc_trace_shadow_rays(risltc_context_t* c, const float* rays, uint32_t n,
     if (!c->bvh.rec) return fail(c, RISLTC_E_STATE, "risltc_set_scene has not been called");
     if (n == 0) return RISLTC_OK;
     HIPC(c, hipSetDevice(c->device));
-    HIPC(c, rl::launch_trace_rays(bvh_view(c), rays, n, hits, (hipStream_t)stream));
+    HIPC(c, rl::launch_trace_rays(bvh_view_of(c->bvh), rays, n, hits, (hipStream_t)stream));
     return RISLTC_OK;
 }
 
@@ -1181,7 +1269,7 @@ static int check_tuning(risltc_context_t* c, const risltc_tuning_t& t) {
 
 // the descriptor of one view, without its output buffers
 static void fill_shade(risltc_context_t* c, const risltc_frame_constants_t* k, const risltc_settings_t* s,
-                       const uint32_t* vis, const LightRef& L, rl::ShadeArgs& a) {
+                       const uint32_t* vis, const LightRef& L, const SceneRef& S, rl::ShadeArgs& a) {
     memset(&a, 0, sizeof(a));
     a.c = *k;
     a.spp = s->sample_count_light;
@@ -1193,12 +1281,12 @@ static void fill_shade(risltc_context_t* c, const risltc_frame_constants_t* k, c
     a.tile_w_log2 = log2_of(c->tuning.wave_width);
     a.chunk = c->tuning.cull_chunk;
     a.lds_lights = c->tuning.lds_lights;
-    a.scene.positions = c->positions.as<uint2>();
-    a.scene.normals_uv = c->normals_uv.as<ushort4>();
-    a.scene.material_idx = c->material_idx.as<uint8_t>();
-    a.scene.materials = c->materials.as<float>();
-    a.scene.material_count = c->material_count;
-    a.scene.triangle_count = c->triangle_count;
+    a.scene.positions = S.positions;
+    a.scene.normals_uv = S.normals_uv;
+    a.scene.material_idx = S.material_idx;
+    a.scene.materials = S.materials;
+    a.scene.material_count = S.material_count;
+    a.scene.triangle_count = S.triangle_count;
     a.scene.lights = L.lights;
     a.scene.spheres = L.spheres;
     a.scene.sphere_bound = L.bound;
@@ -1218,14 +1306,14 @@ static void fill_shade(risltc_context_t* c, const risltc_frame_constants_t* k, c
     a.clusters.count = c->cluster_count;
 }
 
-static void fill_resolve(risltc_context_t* c, const risltc_frame_constants_t* k, const risltc_settings_t* s,
-                         const uint32_t* vis, const float* samples, const uint32_t* hits, float* out,
+static void fill_resolve(const risltc_frame_constants_t* k, const risltc_settings_t* s, const uint32_t* vis,
+                         const float* samples, const uint32_t* hits, float* out, uint64_t triangle_count,
                          rl::ResolveArgs& r) {
     memset(&r, 0, sizeof(r));
     r.width = k->viewport_size[0];
     r.height = k->viewport_size[1];
     r.spp = s->sample_count_light;
-    r.triangle_count = c->triangle_count;
+    r.triangle_count = triangle_count;
     r.exposure = k->exposure_factor;
     r.visibility = vis;
     r.samples = samples;
@@ -1235,10 +1323,10 @@ static void fill_resolve(risltc_context_t* c, const risltc_frame_constants_t* k,
 
 static const uint32_t kCounterWords = rl::kQueueStride * (1u + rl::kTraceQueues);
 
-static rl::TraceArgs trace_args(risltc_context_t* c, rl::ShadowRay* rays, uint32_t* slots, uint32_t* counter,
-                                uint32_t* hits, size_t max_rays) {
+static rl::TraceArgs trace_args(risltc_context_t* c, const rl::BvhView* bvhs, rl::ShadowRay* rays, uint32_t* slots,
+                                uint32_t* counter, uint32_t* hits, size_t max_rays) {
     rl::TraceArgs t;
-    t.bvh = bvh_view(c);
+    t.bvhs = bvhs;
     t.rays = rays;
     t.ray_slots = slots;
     t.ray_count = counter;
@@ -1277,8 +1365,11 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     HIPC(c, c->counter.ensure(kCounterWords * sizeof(uint32_t)));
     LightRef L;
     light_ref(c, 0, L);
+    SceneRef SR;
+    e = scene_ref(c, 0, SR);
+    if (e) return e;
     rl::ShadeArgs a;
-    fill_shade(c, k, s, vis, L, a);
+    fill_shade(c, k, s, vis, L, SR, a);
     a.samples = c->samples.as<float>();
     a.rays = c->rays.as<rl::ShadowRay>();
     a.ray_slots = c->ray_slots.as<uint32_t>();
@@ -1301,10 +1392,12 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     HIPC(c, rl::launch_shade(a, s->exact_math != 0, st));
     HIPC(c, hipEventRecord(c->ev[1], st));
     if (s->shadow_rays)
-        HIPC(c, rl::launch_trace(trace_args(c, a.rays, a.ray_slots, a.ray_count, c->hits.as<uint32_t>(), S * 2), st));
+        HIPC(c, rl::launch_trace(trace_args(c, c->bvh_table.as<rl::BvhView>(), a.rays, a.ray_slots, a.ray_count,
+                                            c->hits.as<uint32_t>(), S * 2),
+                                 st));
     HIPC(c, hipEventRecord(c->ev[2], st));
     rl::ResolveArgs r;
-    fill_resolve(c, k, s, vis, a.samples, c->hits.as<uint32_t>(), out, r);
+    fill_resolve(k, s, vis, a.samples, c->hits.as<uint32_t>(), out, SR.triangle_count, r);
     r.debug_samples = dbg_s;
     c->rec_samples = 0;
     if (c->cluster_count > 0) {
@@ -1331,15 +1424,17 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
 struct BatchPlan {
     std::vector<rl::ShadeArgs> shade;
     std::vector<rl::ResolveArgs> resolve;
+    std::vector<rl::BvhView> bvhs;  // the hierarchies of the batch's scenes
     std::vector<uint32_t> tile_off, block_off;
     size_t samples = 0;
     uint32_t tiles = 0, blocks = 0, trace_blocks = 0;
+    size_t bvh_capacity = 0;
     size_t lds = 0;
     bool exact = false, shadows = false;
     // whether a graph captured with plan `o` can run this plan's descriptors
     bool fits(const BatchPlan& o) const {
         return tile_off == o.tile_off && block_off == o.block_off && samples == o.samples && lds <= o.lds &&
-               exact == o.exact && shadows == o.shadows;
+               exact == o.exact && shadows == o.shadows && bvhs.size() <= o.bvh_capacity;
     }
 };
 
@@ -1372,13 +1467,22 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
     }
     p.tile_off.push_back(0);
     p.block_off.push_back(0);
+    std::vector<int64_t> table_of(c->scene_sets.size() + 1, -1);  // scene id -> hierarchy table index
     for (uint32_t v = 0; v < n; ++v) {
         const risltc_view_t& w = views[v];
         LightRef L;
         int e = light_ref(c, w.light_set, L);
         if (e) return e;
+        SceneRef SR;
+        e = scene_ref(c, w.scene, SR);
+        if (e) return e;
+        if (table_of[w.scene] < 0) {
+            table_of[w.scene] = (int64_t)p.bvhs.size();
+            p.bvhs.push_back(SR.bvh);
+        }
         rl::ShadeArgs a;
-        fill_shade(c, &w.constants, &w.settings, w.visibility, L, a);
+        fill_shade(c, &w.constants, &w.settings, w.visibility, L, SR, a);
+        a.bvh_index = (uint32_t)table_of[w.scene];
         a.samples = B.samples.as<float>() + off[v] * rl::kSampleWords;
         a.rays = B.rays.as<rl::ShadowRay>();
         a.ray_slots = B.ray_slots.as<uint32_t>();
@@ -1388,14 +1492,16 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
         p.tile_off.push_back(p.tile_off.back() + rl::shade_tiles(a));
         p.shade.push_back(a);
         rl::ResolveArgs r;
-        fill_resolve(c, &w.constants, &w.settings, w.visibility, a.samples, B.hits.as<uint32_t>() + off[v] * 2, w.out,
-                     r);
+        fill_resolve(&w.constants, &w.settings, w.visibility, a.samples, B.hits.as<uint32_t>() + off[v] * 2, w.out,
+                     SR.triangle_count, r);
         const uint32_t P = w.constants.viewport_size[0] * w.constants.viewport_size[1];
         p.block_off.push_back(p.block_off.back() + (P + 255u) / 256u);
         p.resolve.push_back(r);
     }
     p.tiles = p.tile_off.back();
     p.blocks = p.block_off.back();
+    p.bvh_capacity = std::max<size_t>(p.bvhs.size(), 1u + c->scene_sets.size());
+    if (allocate) HIPC(c, B.bvh_table.ensure(p.bvh_capacity * sizeof(rl::BvhView)));
     HIPC(c, rl::trace_blocks(&p.trace_blocks));
     return RISLTC_OK;
 }
@@ -1411,6 +1517,7 @@ static int upload_plan(risltc_context_t* c, BatchBuffers& B, const BatchPlan& p)
     HIPC(c, hipMemcpy(B.tile_offsets.p, p.tile_off.data(), p.tile_off.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIPC(c, hipMemcpy(B.block_offsets.p, p.block_off.data(), p.block_off.size() * sizeof(uint32_t),
                       hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(B.bvh_table.p, p.bvhs.data(), p.bvhs.size() * sizeof(rl::BvhView), hipMemcpyHostToDevice));
     return RISLTC_OK;
 }
 
@@ -1423,8 +1530,9 @@ static int launch_plan(risltc_context_t* c, BatchBuffers& B, const BatchPlan& p,
                                    p.exact, st));
     if (timed) HIPC(c, hipEventRecord(c->ev[1], st));
     if (p.shadows)
-        HIPC(c, rl::launch_trace(trace_args(c, B.rays.as<rl::ShadowRay>(), B.ray_slots.as<uint32_t>(),
-                                            B.counter.as<uint32_t>(), B.hits.as<uint32_t>(), p.samples * 2),
+        HIPC(c, rl::launch_trace(trace_args(c, B.bvh_table.as<rl::BvhView>(), B.rays.as<rl::ShadowRay>(),
+                                            B.ray_slots.as<uint32_t>(), B.counter.as<uint32_t>(),
+                                            B.hits.as<uint32_t>(), p.samples * 2),
                                  st, p.trace_blocks));
     if (timed) HIPC(c, hipEventRecord(c->ev[2], st));
     HIPC(c, rl::launch_resolve_batch(B.resolve_args.as<rl::ResolveArgs>(), B.block_offsets.as<uint32_t>(), n, p.blocks,
@@ -1607,6 +1715,7 @@ int risltc_graph_update(risltc_graph_t* g, const risltc_view_t* views, uint32_t 
     e = upload_plan(c, g->buf, p);
     if (e) return e;
     p.lds = g->plan.lds;
+    p.bvh_capacity = g->plan.bvh_capacity;
     g->plan = std::move(p);
     return RISLTC_OK;
 }

@@ -170,6 +170,7 @@ struct ShadeArgs {
     // first lds_lights light spheres (<= kLdsSphereMax) are staged in LDS
     uint32_t tile_w_log2, chunk, lds_lights;
     uint32_t slot_base;        // added to every shadow-ray slot (batches)
+    uint32_t bvh_index;        // hierarchy of the view's scene in the trace table
     SceneView scene;
     const uint32_t* visibility;
     float* samples;            // kSampleWords per pixel-sample
@@ -200,9 +201,13 @@ struct ShadeArgs {
 constexpr uint32_t kTraceQueues = 8;
 constexpr uint32_t kQueueStride = 32;  // words: one 128-byte line per counter
 
+// shadow rays start 1e-3 from the shading point (scene.c:317-327 via
+// get_polygon_visibility, shading_pass.frag.glsl:112-129)
+constexpr float kShadowTMin = 1.0e-3f;
+
 struct TraceArgs {
-    BvhView bvh;
-    const ShadowRay* rays;
+    const BvhView* bvhs;       // per ray: bvhs[the ray's hierarchy index]
+    const ShadowRay* rays;     // org.w holds the hierarchy index (bits)
     const uint32_t* ray_slots;
     const uint32_t* ray_count;
     uint32_t max_rays;

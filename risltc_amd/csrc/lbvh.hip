@@ -450,8 +450,8 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs A) {
     if (count > A.max_rays) count = A.max_rays;
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (1ull << lane) - 1ull;
-    const float4* rec = A.bvh.rec;
-    const int end = (int)A.bvh.units;
+    const float4* rec = nullptr;
+    int end = 0;
     uint32_t q = blockIdx.x % kTraceQueues;
     uint32_t tried = 0;
     bool active = false;
@@ -479,7 +479,10 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs A) {
                     o = make_float3(r.org.x, r.org.y, r.org.z);
                     d = make_float3(r.dir.x, r.dir.y, r.dir.z);
                     inv = make_float3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-                    tmin = r.org.w;
+                    const BvhView bv = A.bvhs[__float_as_uint(r.org.w)];
+                    rec = bv.rec;
+                    end = (int)bv.units;
+                    tmin = kShadowTMin;
                     tmax = r.dir.w;
                     x = 0;
                     steps = 0;

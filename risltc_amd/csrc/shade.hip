@@ -1625,7 +1625,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
             }
             if (emit) {
                 uint32_t off = base + __popcll(ballot & ((1ull << lane) - 1ull));
-                A.rays[off].org = make_float4(px.pos.x, px.pos.y, px.pos.z, 1.0e-3f);
+                A.rays[off].org = make_float4(px.pos.x, px.pos.y, px.pos.z, __uint_as_float(A.bvh_index));
                 A.rays[off].dir = make_float4(F.dir[t].x, F.dir[t].y, F.dir[t].z, F.tmax[t]);
                 A.ray_slots[off] = A.slot_base + (uint32_t)(sidx * 2 + t);
             }

@@ -2,8 +2,8 @@
 tuning.
 
   * a batch of views with different resolutions (ragged against every tile
-    shape), cameras, settings and light sets gives, per view, the frame that
-    risltc_shade gives for that view alone, bit for bit;
+    shape), cameras, settings, light sets and meshes gives, per view, the
+    frame that risltc_shade gives for that view alone, bit for bit;
   * a graph replays the batch: every replay, after an update with new noise
     seeds and with other work on the context's own buffers in between, equals
     the eager batch of the same views;
@@ -224,3 +224,64 @@ def test_autotune_keeps_results(batch):
             assert np.array_equal(bits(out), bits(ref)), i
     finally:
         ctx.set_tuning(**default)
+
+
+def test_batch_with_several_meshes(small_scene, ltc_table):
+    """Views on two meshes (with their own hierarchies and light sets) in one
+    batch, and replayed as a graph: each view equals risltc_shade on a
+    context holding that view's mesh and lights, bit for bit."""
+    import risltc_amd
+    import torch
+    a_sc = small_scene
+    b_sc = synth.small_scene(seed=9, width=120, height=90, light_count=40)
+    ctx = risltc_amd.Context(0)
+    ref_b = risltc_amd.Context(0)
+    ref_ab = risltc_amd.Context(0)
+    try:
+        ctx.load_scene(a_sc, *ltc_table)
+        sid = ctx.add_scene(b_sc)
+        lid = ctx.add_light_set(b_sc.light_records, b_sc.light_count, b_sc.light_vmax)
+        assert sid == 1 and lid == 1
+        ref_b.load_scene(b_sc, *ltc_table)
+        ref_ab.load_scene(a_sc, *ltc_table)
+        ref_ab.set_lights(b_sc.light_records, b_sc.light_count, b_sc.light_vmax)
+        cam2 = synth.make_camera((5.0, -11.0, 6.0), math.pi * 0.85, math.pi / 2 - 0.4)
+        specs = [(a_sc, 0, 0, a_sc.camera, 128, 96, ctx),
+                 (b_sc, sid, lid, b_sc.camera, 150, 70, ref_b),
+                 (a_sc, 0, lid, cam2, 77, 64, ref_ab),
+                 (b_sc, sid, lid, cam2, 96, 96, ref_b)]
+        views, outs, refs, keep = [], [], [], []
+        for i, (sc, scene_id, light_set, cam, w, h, ref_ctx) in enumerate(specs):
+            c = risltc_amd.write_frame_constants(cam, w, h, sc.mesh.dequantization_factor,
+                                                 sc.mesh.dequantization_summand, ctx.ltc_constants, noise_seed=i + 1,
+                                                 animate=True)
+            vis = torch.empty(w * h, dtype=torch.int32, device="cuda")
+            ctx.render_visibility_scene(scene_id, c, vis)
+            s = sc.settings(sample_count_light=2, ris_candidate_count=8 + 8 * i)
+            out = torch.full((w * h * 4,), float("nan"), dtype=torch.float32, device="cuda")
+            ref = torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
+            ref_ctx.shade(c, s, vis, ref)
+            views.append(ctx.view(c, s, vis, out, light_set=light_set, scene=scene_id))
+            outs.append(out)
+            refs.append(ref)
+            keep.append(vis)
+        torch.cuda.synchronize()
+        ctx.shade_batch(views)
+        torch.cuda.synchronize()
+        for i, (o, r) in enumerate(zip(outs, refs)):
+            assert np.array_equal(bits(o), bits(r)), i
+            assert (o.cpu().numpy().reshape(-1, 4)[:, :3] > 0).mean() > 0.05, i
+        g = ctx.graph(views)
+        try:
+            for _ in range(3):
+                for o in outs:
+                    o.fill_(float("nan"))
+                g.launch()
+                torch.cuda.synchronize()
+                for i, (o, r) in enumerate(zip(outs, refs)):
+                    assert np.array_equal(bits(o), bits(r)), i
+        finally:
+            g.close()
+    finally:
+        for c_ in (ctx, ref_b, ref_ab):
+            c_.close()
