@@ -15,9 +15,11 @@ the RIS target) to reach relMSE 0.01 against a 65536-spp plain reference,
 each pass's Msamples/s and per-kernel milliseconds per frame, and two
 ablations (risltc_amd/convergence.py).
 
-The batched_views object: 16 views of 480x270 around the city (one 1080p
-frame of pixels in all), shaded by one risltc_shade call per view, by one
-risltc_shade_batch, and by a hipGraph replay of that batch.
+The batched_views object: views of the batched configuration (each its own
+city mesh of ~50k-500k triangles and its own light set of 16-8192 polygons,
+resolutions from 320x240 to 1920x1080, M in {8, 16, 32}, 2 spp; 16 views by
+default, --batch-views 64 for the full configuration), shaded by one call per
+view, by one risltc_shade_batch, and by a hipGraph replay of that batch.
 
 The launch shape is autotuned (risltc_autotune) on the frame before timing.
 
@@ -50,40 +52,50 @@ def algorithmic_bytes(pixels, lights, vmax, ltc_res):
     return pixels * (4 + 48 + 1 + 16) + lights * (48 + 16 * vmax) + ltc_res * ltc_res * 12
 
 
-def orbit_views(ctx, scene, vis_list, outs, count, width, height, settings):
-    """count cameras on a circle of radius 80 m, 35 m up, around the light
-    field's centre (0, 40), each looking at it (forward = (-sin rz, -cos rz))."""
+def batch_views(ctx, ltc_rgba, ltc_rg, count, seed=7):
+    """Views shaped as the batched configuration: each view its own
+    procedural city of ~50k-500k triangles and its own light set of 16-8192
+    polygons (3-8 vertices, count log-uniform), resolution uniform in
+    320x240 .. 1920x1080, M from {8, 16, 32}, 2 spp. Returns the views and
+    the buffers they point at."""
     import math
+    import torch
     import risltc_amd
     from risltc_amd import synth
-    views = []
-    for i in range(count):
-        phi = 2.0 * math.pi * i / count
-        cam = synth.make_camera((80.0 * math.cos(phi), 40.0 + 80.0 * math.sin(phi), 35.0), math.pi / 2 - phi,
-                                math.pi / 2 - 0.45)
-        c = risltc_amd.write_frame_constants(cam, width, height, scene.mesh.dequantization_factor,
-                                             scene.mesh.dequantization_summand, ctx.ltc_constants)
-        if vis_list[i] is None:
-            import torch
-            vis_list[i] = torch.empty(width * height, dtype=torch.int32, device="cuda")
-            ctx.render_visibility(c, vis_list[i])
-        views.append(ctx.view(c, settings, vis_list[i], outs[i]))
-    return views
+    rng = np.random.default_rng(seed)
+    views, keep = [], []
+    samples = 0
+    for v in range(count):
+        grid = int(rng.integers(60, 191))
+        lights = int(round(math.exp(rng.uniform(math.log(16), math.log(8192)))))
+        w, h = int(rng.integers(320, 1921)), int(rng.integers(240, 1081))
+        sc = synth.city_scene(seed=100 + v, width=w, height=h, light_count=lights, grid=grid, spp=2,
+                              candidates=int(rng.choice([8, 16, 32])))
+        sid = ctx.add_scene(sc)
+        lid = ctx.add_light_set(sc.light_records, sc.light_count, sc.light_vmax)
+        c = risltc_amd.write_frame_constants(sc.camera, w, h, sc.mesh.dequantization_factor,
+                                             sc.mesh.dequantization_summand, ctx.ltc_constants)
+        vis = torch.empty(w * h, dtype=torch.int32, device="cuda")
+        ctx.render_visibility_scene(sid, c, vis)
+        out = torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
+        st = sc.settings()
+        views.append(ctx.view(c, st, vis, out, light_set=lid, scene=sid))
+        keep.append((vis, out))
+        samples += w * h * st.sample_count_light
+    return views, keep, samples
 
 
-def bench_batch(ctx, scene, steps, warmup, count=16, width=480, height=270):
-    """count views of width x height (together one 1080p frame of pixels),
-    same settings as the main workload: one risltc_shade per view, one
-    risltc_shade_batch, and a graph replay of the batch, each after its own
-    warm-up; wall time per step over `steps` steps."""
+def bench_batch(ctx, ltc_rgba, ltc_rg, steps, warmup, count):
+    """count views of the batched configuration, shaded by one risltc_shade
+    call per view, by one risltc_shade_batch, and by a graph replay of that
+    batch, each after its own warm-up; wall time per step over `steps`
+    steps. The launch shape is autotuned on the batch."""
     import torch
-    settings = scene.settings()
-    outs = [torch.empty(width * height * 4, dtype=torch.float32, device="cuda") for _ in range(count)]
-    vis = [None] * count
-    views = orbit_views(ctx, scene, vis, outs, count, width, height, settings)
+    t0 = time.time()
+    views, keep, samples = batch_views(ctx, ltc_rgba, ltc_rg, count)
+    setup_s = time.time() - t0
     before = ctx.get_tuning().as_dict()
     tuning = ctx.autotune(views, repeats=3).as_dict()
-    samples = count * width * height * settings.sample_count_light
 
     def timed(fn):
         for _ in range(warmup):
@@ -97,10 +109,11 @@ def bench_batch(ctx, scene, steps, warmup, count=16, width=480, height=270):
 
     def per_view():
         for i, v in enumerate(views):
-            ctx.shade(v.constants, v.settings, vis[i], outs[i])
+            ctx.shade_batch([v])
 
     ms_views = timed(per_view)
     ms_batch = timed(lambda: ctx.shade_batch(views))
+    kernel = ctx.last_timings()
     g = ctx.graph(views)
     try:
         ms_graph = timed(g.launch)
@@ -108,7 +121,11 @@ def bench_batch(ctx, scene, steps, warmup, count=16, width=480, height=270):
         g.close()
     ctx.set_tuning(**before)
     rate = lambda ms: round(samples / (ms * 1e-3) / 1e6, 3)
-    return {"views": count, "resolution": [width, height], "samples_per_step": samples, "tuning": tuning,
+    res = [[int(v.constants.viewport_size[0]), int(v.constants.viewport_size[1])] for v in views]
+    return {"views": count, "samples_per_step": samples, "setup_s": round(setup_s, 1), "tuning": tuning,
+            "resolutions": res, "candidates": sorted(set(int(v.settings.ris_candidate_count) for v in views)),
+            "batch_kernel_ms": {"shade": round(kernel[0], 3), "trace": round(kernel[1], 3),
+                                "resolve": round(kernel[2], 3)},
             "ms_per_step": {"per_view_calls": round(ms_views, 4), "batch": round(ms_batch, 4),
                             "graph": round(ms_graph, 4)},
             "msamples_per_s": {"per_view_calls": rate(ms_views), "batch": rate(ms_batch), "graph": rate(ms_graph)}}
@@ -127,6 +144,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-plain time-to-relMSE experiment")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view measurement")
+    ap.add_argument("--batch-views", type=int, default=16,
+                    help="views in the batched measurement (the batched configuration has 64)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--learned-frames", type=int, default=0,
                     help="only run N frames of the learned pass (for profiling its kernels)")
@@ -225,7 +244,7 @@ def main():
 
     batch = None
     if not args.no_batch and args.workload == "city":
-        batch = bench_batch(ctx, scene, args.steps, args.warmup)
+        batch = bench_batch(ctx, rgba, rg, args.steps, args.warmup, args.batch_views)
         print("batch: %s" % batch, file=sys.stderr, flush=True)
 
     learned = None
