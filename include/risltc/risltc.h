@@ -244,6 +244,80 @@ int risltc_load_ltc_fit(const char* directory, uint32_t fresnel_count, uint32_t*
                         uint16_t* rgba16, uint16_t* rg16, size_t capacity_values);
 
 /* ------------------------------------------------------------------------ */
+/* On-disk formats                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* Header of a *.vks scene file (src/scene.c:419-433). */
+typedef struct risltc_vks_info_s {
+    uint64_t material_count;
+    uint64_t triangle_count;
+    float dequantization_factor[3];
+    float dequantization_summand[3];
+    uint64_t material_name_bytes; /* sum over materials of (name length + 1) */
+} risltc_vks_info_t;
+
+/* The file reading of load_scene (src/scene.c:409-483; written by
+ * io_export_vulkan_blender28.py:481-541): uint32 marker 0xabcabc and version
+ * 1, uint64 material and triangle counts, 3 + 3 float dequantisation
+ * constants, per material a uint64 length, the name and a NUL, then the mesh
+ * buffers in the layout risltc_set_scene takes (positions 24 B, normals and
+ * texture coordinates 24 B, material index 1 B per triangle) and the uint32
+ * end marker 0xE0FE0F. With positions, normals_and_tex_coords and
+ * material_indices all NULL only *info is read (the whole file is still
+ * checked); otherwise all three must hold triangle_capacity >= triangle_count
+ * triangles. names (optional, names_capacity bytes) receives the
+ * NUL-terminated names back to back. Returns RISLTC_E_IO when the file cannot
+ * be opened, RISLTC_E_FORMAT on a wrong marker or version, zero triangles, a
+ * short file or a missing end marker, RISLTC_E_ARG when a buffer is too
+ * small. */
+int risltc_load_vks(const char* path, risltc_vks_info_t* info, uint32_t* positions,
+                    uint16_t* normals_and_tex_coords, uint8_t* material_indices,
+                    uint64_t triangle_capacity, char* names, uint64_t names_capacity);
+
+/* Totals of a quicksave file. */
+typedef struct risltc_quicksave_info_s {
+    uint32_t light_count;
+    uint32_t total_vertex_count;  /* sum of vertex_count over the lights */
+    uint64_t texture_path_bytes;  /* sum of the texture path sizes (with NUL) */
+} risltc_quicksave_info_t;
+
+/* quick_load (src/main.c:78-125): the camera (48 bytes), a uint32 legacy
+ * count, the uint32 light count, then per light the first 88 bytes of
+ * risltc_polygonal_light_t (rotation_angles .. texturing_technique), a
+ * uint64 texture path size and path, 16 bytes of former pointers and
+ * 4 floats per plane-space vertex. A light with scaling_y <= 0 gets
+ * scaling_y = scaling_x, as in the reference. With lights == NULL only *info
+ * (and *camera if not NULL) is filled. Otherwise light i's plane-space and
+ * (zeroed) world-space vertex arrays are carved out of vertex_storage (8
+ * floats per vertex: the plane-space vertices of a light, then its
+ * world-space ones), its texture_file_path points into path_storage (or is
+ * NULL) and the fields after texturing_technique are zero; run
+ * risltc_write_lights (which calls risltc_update_polygonal_light) next.
+ * Errors as for risltc_load_vks; a vertex count of 0 or above 65536 is
+ * RISLTC_E_FORMAT. */
+int risltc_load_quicksave(const char* path, risltc_camera_t* camera, risltc_quicksave_info_t* info,
+                          risltc_polygonal_light_t* lights, uint32_t light_capacity,
+                          float* vertex_storage, uint64_t vertex_storage_floats, char* path_storage,
+                          uint64_t path_capacity);
+
+/* quick_save (src/main.c:44-75): writes the file risltc_load_quicksave reads. */
+int risltc_save_quicksave(const char* path, const risltc_camera_t* camera,
+                          const risltc_polygonal_light_t* lights, uint32_t light_count);
+
+/* Radiance RGBE image, as the reference's HDR screenshots are written with
+ * stbi_write_hdr (src/main.c:2392-2401): rows top to bottom ("-Y h +X w"),
+ * scanlines of 8..32767 pixels run-length coded per component, others flat.
+ * The pixel bytes equal stbi_write_hdr's for finite non-negative input; the
+ * header comment and EXPOSURE lines are left out. pixels holds
+ * width*height*channels floats (channels 3 or 4; alpha is ignored). With
+ * half_round = 1 every value is rounded to binary16 (to nearest even) first,
+ * as the screenshot path stores halves (copy_pass.frag.glsl:36-46,
+ * src/main.c:2339-2349). Negative and NaN components are written as 0,
+ * values of 2^127 and above as the largest value below it. */
+int risltc_write_hdr(const char* path, const float* pixels, uint32_t width, uint32_t height,
+                     uint32_t channels, int half_round);
+
+/* ------------------------------------------------------------------------ */
 /* Context                                                                   */
 /* ------------------------------------------------------------------------ */
 typedef struct risltc_context_s risltc_context_t;

@@ -133,8 +133,22 @@ def ref_lib():
         R.ref_matrix_inverse.restype = None
         R.ref_wang_random_number.argtypes = [C.c_uint32]
         R.ref_wang_random_number.restype = C.c_uint32
+        if hasattr(R, "ref_half_to_float"):
+            R.ref_half_to_float.argtypes = [C.c_uint16]
+            R.ref_half_to_float.restype = C.c_float
         _ref = R
     return _ref
+
+
+def ref_stbiw_lib():
+    """The reference's stb_image_write.h compiled by oracle/Makefile, or None."""
+    path = os.path.join(os.path.dirname(REF_LIB_PATH), "libref_stbiw.so")
+    if not os.path.exists(path):
+        return None
+    R = C.CDLL(path)
+    R.stbi_write_hdr.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    R.stbi_write_hdr.restype = C.c_int
+    return R
 
 
 def ptr(a, t):

@@ -6,3 +6,4 @@
 
 void ref_matrix_inverse(float inverse[4][4], const float matrix[4][4]) { matrix_inverse(inverse, matrix); }
 uint32_t ref_wang_random_number(uint32_t seed) { return wang_random_number(seed); }
+float ref_half_to_float(uint16_t half) { return half_to_float(half); }

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librisltc.so")
-SOURCES = ["capi.cpp", "shade.hip", "lbvh.hip", "sah.hip", "mlp.hip"]
+SOURCES = ["capi.cpp", "formats.cpp", "shade.hip", "lbvh.hip", "sah.hip", "mlp.hip"]
 ARCH = os.environ.get("RISLTC_ARCH", "gfx950")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
