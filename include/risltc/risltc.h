@@ -160,8 +160,23 @@ typedef struct risltc_material_s {
 #define RISLTC_LIGHT_UNIFORM 0
 #define RISLTC_LIGHT_RESERVOIR 1
 
-/* sample_polygon_technique_t (src/polygonal_light.h:30-45); only
- * sample_polygon_ltc_cp (the reference default, src/main.c:227) is built. */
+/* sample_polygon_technique_t (src/polygonal_light.h:30-45). LTC_CP is the
+ * reference default (src/main.c:227) and the production path. The two
+ * baselines of the reference's comparisons (src/experiment_list.c:313-396)
+ * are built with the uniform source only:
+ *   AREA_TURK             -- each candidate is a uniform point on the
+ *                            triangle of vertices 0, 1, 2 (Turk), target =
+ *                            its unshadowed area-sampling estimate; the final
+ *                            sample traces one shadow ray to that point
+ *                            (shading_pass.frag.glsl:410-418, 757-759);
+ *   PROJECTED_SOLID_ANGLE -- each candidate's target is the unshadowed
+ *                            estimate of one projected-solid-angle sample per
+ *                            technique with MIS; the final sample is the
+ *                            LTC_CP final sample (shading_pass.frag.glsl:
+ *                            458-551).
+ * They run in a simpler per-pixel kernel and not in batches or graphs. */
+#define RISLTC_SAMPLE_POLYGON_AREA_TURK 1
+#define RISLTC_SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE 2
 #define RISLTC_SAMPLE_POLYGON_LTC_CP 4
 
 /* RIS source distribution */

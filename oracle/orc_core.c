@@ -772,8 +772,17 @@ static V2 noise2(uint32_t* seed) {
 
 /* evaluate_polygonal_light_shading_peters (shading_pass.frag.glsl:292-397),
  * SAMPLE_COUNT = 1, with visibility through the CPU LBVH. rec (may be NULL)
- * receives the per-technique intermediate values. */
-static V3 eval_peters(pix_t* px, const light_t* l, float* rec) {
+ * receives the per-technique intermediate values.
+ *
+ * The same code is the SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE branch of
+ * evaluate_polygonal_light_shading (shading_pass.frag.glsl:458-551): with
+ * trace = 1 (visibility = true, the final sample) an occluded sample's
+ * integrand is 0, exactly as above; with trace = 0 (the RIS candidates,
+ * dummy_vis = false) no ray is traced and a sample above the horizon keeps
+ * its unshadowed integrand (get_polygon_radiance_brdf_product, 200-208).
+ * Its final "result * (1.0f / SAMPLE_COUNT)" is a multiplication by 1.
+ * clip_counts (may be NULL) receives vcd | vcs << 8. */
+static V3 eval_peters_ex(pix_t* px, const light_t* l, float* rec, int trace, uint32_t* clip_counts) {
     V3 result = v3(K(0.0), K(0.0), K(0.0));
     R side = plane_side(px->sd.position, l->plane);
     const A34* w2s = (side < K(0.0)) ? &px->w2s_flip : &px->ltc.world_to_shading;
@@ -788,11 +797,13 @@ static V3 eval_peters(pix_t* px, const light_t* l, float* rec) {
     for (uint32_t i = 0; i != l->vertex_count; ++i) vs[i] = xform(w2s, l->v[i]);
     uint32_t vcd = clip_polygon(l->vertex_count, vs);
     if (rec) memcpy(&rec[ORC_REC_CLIPD], &vcd, 4);
+    if (clip_counts) *clip_counts = vcd;
     if (vcd == 0) return result;
     prepare_psa(&pd, vcd, vs);
     for (uint32_t i = 0; i != l->vertex_count; ++i) vs[i] = xform(w2c, l->v[i]);
     uint32_t vcs = clip_polygon(l->vertex_count, vs);
     if (rec) memcpy(&rec[ORC_REC_CLIPS], &vcs, 4);
+    if (clip_counts) *clip_counts = vcd | (vcs << 8);
     if (vcs == 0) {
         memset(&ps, 0, sizeof(ps));
         ps.psa = K(0.0);
@@ -839,7 +850,7 @@ static V3 eval_peters(pix_t* px, const light_t* l, float* rec) {
         if (lambert_ok) {
             tmax = -plane_side(px->sd.position, l->plane) / dot3(wdir, v3(l->plane[0], l->plane[1], l->plane[2]));
             tmax -= K(1e-3);
-            if (px->s->shadow_rays && px->sc->bvh) {
+            if (trace && px->s->shadow_rays && px->sc->bvh) {
                 float o[3] = {(float)px->sd.position.x, (float)px->sd.position.y, (float)px->sd.position.z};
                 float dir[3] = {(float)wdir.x, (float)wdir.y, (float)wdir.z};
                 int hit = orc_bvh_any_hit(px->sc->bvh, o, dir, 1.0e-3f, (float)tmax, NULL);
@@ -881,6 +892,89 @@ static V3 eval_peters(pix_t* px, const light_t* l, float* rec) {
         }
     }
     return scl3(result, K(1.0) / K(1.0));
+}
+
+static V3 eval_peters(pix_t* px, const light_t* l, float* rec) { return eval_peters_ex(px, l, rec, 1, NULL); }
+
+/* sample_area_polygon_turk (polygon_sampling_related_work.glsl:34-43): a
+ * uniform point on the triangle of vertices 0, 1, 2 (the reference samples
+ * only this triangle of the polygon). */
+static V3 turk_point(const light_t* l, V2 u) {
+    R s = SQRT(u.x);
+    R b0 = K(1.0) - s, b1 = s * u.y, b2 = FMA(-s, u.y, s);
+    return v3(b0 * l->v[0].x + b1 * l->v[1].x + b2 * l->v[2].x, b0 * l->v[0].y + b1 * l->v[1].y + b2 * l->v[2].y,
+              b0 * l->v[0].z + b1 * l->v[1].z + b2 * l->v[2].z);
+}
+
+/* get_polygon_area (shading_pass.frag.glsl:148-152): the area of the
+ * triangle of vertices 0, 1, 2. */
+static R turk_area(const light_t* l) {
+    V3 c = cross_stable(sub3(l->v[1], l->v[0]), sub3(l->v[2], l->v[0]));
+    return SQRT(dot3(c, c)) / K(2.0);
+}
+
+/* The SAMPLE_POLYGON_AREA_TURK branch of evaluate_polygonal_light_shading
+ * (shading_pass.frag.glsl:410-418) for SAMPLE_COUNT = 1 at the point
+ * `sample`: get_area_sample_density (polygon_sampling_related_work.glsl:
+ * 57-64) and get_polygonal_light_mis_estimate (shading_pass.frag.glsl:
+ * 281-287), without visibility. Returns the unshadowed estimate; *dir and
+ * *tmax receive the shadow ray of get_polygon_visibility (112-129) and
+ * *above whether the sample is above the shading horizon. */
+static V3 eval_turk(const pix_t* px, const light_t* l, V3 sample, V3* dir, R* tmax, int* above) {
+    V3 d = sub3(sample, px->sd.position);
+    R d2 = dot3(d, d);
+    d = scl3(d, rsq(d2));
+    V3 pn = v3(l->plane[0], l->plane[1], l->plane[2]);
+    R density = d2 / (FABS(dot3(pn, d)) * turk_area(l));
+    R lambert = dot3(px->sd.normal, d);
+    *dir = d;
+    *above = lambert > K(0.0);
+    *tmax = K(0.0);
+    V3 rb = v3(K(0.0), K(0.0), K(0.0));
+    if (*above) {
+        *tmax = -plane_side(px->sd.position, l->plane) / dot3(d, pn);
+        *tmax -= K(1e-3);
+        rb = mul3(l->radiance, evaluate_brdf(&px->sd, d));
+    }
+    return (density > K(0.0)) ? scl3(rb, lambert / density) : v3(K(0.0), K(0.0), K(0.0));
+}
+
+/* Final sample of the Turk technique: the estimate at the stored point and
+ * one shadow ray. Returns the estimate times visibility; rec as for
+ * eval_peters (technique slot 0). */
+static V3 final_turk(pix_t* px, const light_t* l, V3 sample, float* rec) {
+    V3 dir;
+    R tmax;
+    int above;
+    V3 col = eval_turk(px, l, sample, &dir, &tmax, &above);
+    uint32_t state = 1;
+    int vis = 0;
+    if (above) {
+        if (px->s->shadow_rays && px->sc->bvh) {
+            float o[3] = {(float)px->sd.position.x, (float)px->sd.position.y, (float)px->sd.position.z};
+            float fd[3] = {(float)dir.x, (float)dir.y, (float)dir.z};
+            int hit = orc_bvh_any_hit(px->sc->bvh, o, fd, 1.0e-3f, (float)tmax, NULL);
+            vis = !hit;
+            state = hit ? 2u : 3u;
+        } else {
+            vis = 1;
+            state = 4u;
+        }
+    }
+    if (rec) {
+        rec[ORC_REC_POS + 0] = (float)px->sd.position.x;
+        rec[ORC_REC_POS + 1] = (float)px->sd.position.y;
+        rec[ORC_REC_POS + 2] = (float)px->sd.position.z;
+        rec[ORC_REC_DIR + 0] = (float)dir.x;
+        rec[ORC_REC_DIR + 1] = (float)dir.y;
+        rec[ORC_REC_DIR + 2] = (float)dir.z;
+        rec[ORC_REC_TMAX] = (float)tmax;
+        memcpy(&rec[ORC_REC_STATE], &state, 4);
+        rec[ORC_REC_EVIS + 0] = (float)col.x;
+        rec[ORC_REC_EVIS + 1] = (float)col.y;
+        rec[ORC_REC_EVIS + 2] = (float)col.z;
+    }
+    return vis ? col : v3(K(0.0), K(0.0), K(0.0));
 }
 
 /* int(u * N) in float arithmetic in both builds (u is a float draw), so the
@@ -980,6 +1074,15 @@ static uint32_t cache_key(const orc_scene_t* sc, V3 pos, V3 nrm) {
     return orc_murmur_finalize(h) & sc->cache_mask;
 }
 
+/* LTC_CP (the default) and, as baselines, AREA_TURK and PROJECTED_SOLID_ANGLE
+ * with the uniform source; the learned and cached targets are LTC_CP only. */
+static int orc_technique_supported(const risltc_settings_t* s) {
+    uint32_t t = s->polygon_sampling_technique;
+    if (t == RISLTC_SAMPLE_POLYGON_LTC_CP) return 1;
+    return (t == RISLTC_SAMPLE_POLYGON_AREA_TURK || t == RISLTC_SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE) &&
+           s->proposal == RISLTC_PROPOSAL_UNIFORM;
+}
+
 static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, const risltc_settings_t* s,
                       uint32_t pixel, uint32_t prim, R out[4], float* samples, uint32_t* cands) {
     uint32_t W = c->viewport_size[0];
@@ -1031,13 +1134,20 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
                                                       &ckey, 4);
         uint32_t N = sc->light_count;
         light_t light;
+        const uint32_t tech = s->polygon_sampling_technique;
         if (s->light_sampling == RISLTC_LIGHT_UNIFORM) {
             for (uint32_t j = 0; j != spp; ++j) {
                 float* rec = samples ? samples + (size_t)j * RISLTC_SAMPLE_RECORD_FLOATS : NULL;
                 R u = (R)orc_noise_next(&px.seed);
                 uint32_t idx = light_index_from(u, N);
                 load_light(sc, idx, &light);
-                V3 col = eval_peters(&px, &light, rec);
+                V3 col;
+                if (tech == RISLTC_SAMPLE_POLYGON_AREA_TURK) {
+                    V3 point = turk_point(&light, noise2(&px.seed));
+                    col = final_turk(&px, &light, point, rec);
+                } else {
+                    col = eval_peters(&px, &light, rec);
+                }
                 V3 res = v3(col.x * (R)N, col.y * (R)N, col.z * (R)N);
                 res = v3(res.x / (R)spp, res.y / (R)spp, res.z / (R)spp);
                 fin = add3(fin, scl3(res, K(1.0)));
@@ -1056,6 +1166,7 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
                 float* rec = samples ? samples + (size_t)j * RISLTC_SAMPLE_RECORD_FLOATS : NULL;
                 R w_sum = K(0.0), sample_value = K(0.0);
                 int32_t chosen = -1;
+                V3 chosen_point = v3(K(0.0), K(0.0), K(0.0));
                 const int learned = s->proposal != RISLTC_PROPOSAL_UNIFORM;
                 const uint8_t* tab =
                     s->proposal == RISLTC_PROPOSAL_LEARNED ? sc->proposal_table + (size_t)pixel * 32 : NULL;
@@ -1075,7 +1186,18 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
                     }
                     load_light(sc, idx, &light);
                     uint32_t cc = 0;
-                    V3 col = eval_ltc_cp(&px, &light, &cc);
+                    V3 col, point = v3(K(0.0), K(0.0), K(0.0));
+                    if (tech == RISLTC_SAMPLE_POLYGON_AREA_TURK) {
+                        V3 dir;
+                        R tmax;
+                        int above;
+                        point = turk_point(&light, noise2(&px.seed));
+                        col = eval_turk(&px, &light, point, &dir, &tmax, &above);
+                    } else if (tech == RISLTC_SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE) {
+                        col = eval_peters_ex(&px, &light, NULL, 0, &cc);
+                    } else {
+                        col = eval_ltc_cp(&px, &light, &cc);
+                    }
                     R p_hat = SQRT(dot3(col, col));
                     if (crow)
                         p_hat = p_hat * (R)cache_factor(sc->cluster_of_light, idx, lf, ls, crow, sc->cache_per_light);
@@ -1086,6 +1208,7 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
                     if (w > K(0.0) && r < (w / w_sum)) {
                         chosen = (int32_t)idx;
                         sample_value = p_hat;
+                        chosen_point = point;
                     }
                     if (cands) {
                         uint32_t* cr = cands + ((size_t)j * M + i) * RISLTC_CANDIDATE_RECORD_WORDS;
@@ -1103,7 +1226,8 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
                 }
                 if (chosen >= 0) {
                     load_light(sc, (uint32_t)chosen, &light);
-                    V3 col = eval_peters(&px, &light, rec);
+                    V3 col = (tech == RISLTC_SAMPLE_POLYGON_AREA_TURK) ? final_turk(&px, &light, chosen_point, rec)
+                                                                      : eval_peters(&px, &light, rec);
                     R p_hat = sample_value;
                     R Wt = w_sum / ((R)M * p_hat);
                     if (p_hat == K(0.0)) Wt = K(0.0);
@@ -1132,7 +1256,7 @@ int SFX(orc_shade_pixels)(const orc_scene_t* sc, const risltc_frame_constants_t*
                           const uint32_t* visibility, const uint32_t* pixels, uint32_t count, R* out,
                           float* samples, uint32_t* cands) {
     if (!sc || !c || !s || !visibility || !out) return RISLTC_E_ARG;
-    if (s->sample_count != 1 || s->polygon_sampling_technique != RISLTC_SAMPLE_POLYGON_LTC_CP)
+    if (s->sample_count != 1 || !orc_technique_supported(s))
         return RISLTC_E_UNSUPPORTED;
     if (s->proposal == RISLTC_PROPOSAL_LEARNED) {
         if (s->light_sampling != RISLTC_LIGHT_RESERVOIR) return RISLTC_E_UNSUPPORTED;
@@ -1162,7 +1286,7 @@ int orc_shade_frame_f32(const orc_scene_t* sc, const risltc_frame_constants_t* c
                         const uint32_t* visibility, const uint32_t* pixels, uint32_t count, float* out,
                         int threads) {
     if (!sc || !c || !s || !visibility || !out) return RISLTC_E_ARG;
-    if (s->sample_count != 1 || s->polygon_sampling_technique != RISLTC_SAMPLE_POLYGON_LTC_CP)
+    if (s->sample_count != 1 || !orc_technique_supported(s))
         return RISLTC_E_UNSUPPORTED;
     if (s->proposal == RISLTC_PROPOSAL_LEARNED) {
         if (s->light_sampling != RISLTC_LIGHT_RESERVOIR) return RISLTC_E_UNSUPPORTED;

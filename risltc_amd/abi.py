@@ -16,7 +16,7 @@ ERROR_NAMES = {0: "OK", 1: "E_ARG", 2: "E_HIP", 3: "E_FORMAT", 4: "E_UNSUPPORTED
 
 MIS_BALANCE, MIS_POWER, MIS_WEIGHTED, MIS_OPTIMAL_CLAMPED, MIS_OPTIMAL = range(5)
 LIGHT_UNIFORM, LIGHT_RESERVOIR = 0, 1
-SAMPLE_POLYGON_LTC_CP = 4
+SAMPLE_POLYGON_AREA_TURK, SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE, SAMPLE_POLYGON_LTC_CP = 1, 2, 4
 PROPOSAL_UNIFORM, PROPOSAL_LEARNED, PROPOSAL_CACHE = 0, 1, 2
 BVH_SAH, BVH_LBVH = 0, 1
 MAX_LIGHT_CLUSTERS = 256

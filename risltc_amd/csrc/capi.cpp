@@ -1215,9 +1215,14 @@ static int check_settings(risltc_context_t* c, const risltc_frame_constants_t* k
     if (s->sample_count != 1)
         return fail(c, RISLTC_E_UNSUPPORTED, "SAMPLE_COUNT %u: the kernels take one sample per technique",
                     s->sample_count);
-    if (s->polygon_sampling_technique != RISLTC_SAMPLE_POLYGON_LTC_CP)
-        return fail(c, RISLTC_E_UNSUPPORTED, "polygon technique %u: only sample_polygon_ltc_cp is built",
-                    s->polygon_sampling_technique);
+    const uint32_t tech = s->polygon_sampling_technique;
+    if (tech != RISLTC_SAMPLE_POLYGON_LTC_CP && tech != RISLTC_SAMPLE_POLYGON_AREA_TURK &&
+        tech != RISLTC_SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE)
+        return fail(c, RISLTC_E_UNSUPPORTED,
+                    "polygon technique %u: sample_polygon_ltc_cp, _area_turk and _projected_solid_angle are built",
+                    tech);
+    if (tech != RISLTC_SAMPLE_POLYGON_LTC_CP && s->proposal != RISLTC_PROPOSAL_UNIFORM)
+        return fail(c, RISLTC_E_UNSUPPORTED, "polygon technique %u runs with the uniform source only", tech);
     if (s->mis_heuristic > RISLTC_MIS_OPTIMAL) return fail(c, RISLTC_E_ARG, "mis_heuristic %u", s->mis_heuristic);
     if (s->light_sampling > RISLTC_LIGHT_RESERVOIR) return fail(c, RISLTC_E_ARG, "light_sampling %u", s->light_sampling);
     if (s->sample_count_light == 0 || s->sample_count_light > 1024)
@@ -1389,7 +1394,10 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     // the cells the previous resolve counted in get their new bytes (timed
     // with the shade kernel)
     if (cached) HIPC(c, rl::launch_refresh_cache(a.cache, st));
-    HIPC(c, rl::launch_shade(a, s->exact_math != 0, st));
+    if (s->polygon_sampling_technique == RISLTC_SAMPLE_POLYGON_LTC_CP)
+        HIPC(c, rl::launch_shade(a, s->exact_math != 0, st));
+    else
+        HIPC(c, rl::launch_shade_baseline(a, s->polygon_sampling_technique, s->exact_math != 0, st));
     HIPC(c, hipEventRecord(c->ev[1], st));
     if (s->shadow_rays)
         HIPC(c, rl::launch_trace(trace_args(c, c->bvh_table.as<rl::BvhView>(), a.rays, a.ray_slots, a.ray_count,
@@ -1450,6 +1458,8 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
         if (e) return e;
         if (w.settings.proposal != RISLTC_PROPOSAL_UNIFORM)
             return fail(c, RISLTC_E_UNSUPPORTED, "view %u: batches use the reference's target", v);
+        if (w.settings.polygon_sampling_technique != RISLTC_SAMPLE_POLYGON_LTC_CP)
+            return fail(c, RISLTC_E_UNSUPPORTED, "view %u: batches run sample_polygon_ltc_cp only", v);
         if ((w.settings.exact_math != 0) != (views[0].settings.exact_math != 0))
             return fail(c, RISLTC_E_UNSUPPORTED, "a batch has one exact_math for all views");
         off[v] = p.samples;

@@ -238,6 +238,10 @@ struct ResolveArgs {
 
 // host-side launchers (shade.hip, lbvh.hip)
 hipError_t launch_shade(const ShadeArgs& a, bool exact, hipStream_t s);
+// the baseline techniques (RISLTC_SAMPLE_POLYGON_AREA_TURK or
+// _PROJECTED_SOLID_ANGLE), uniform source, per-pixel kernel; same outputs as
+// launch_shade (debug records when the debug pointers are set)
+hipError_t launch_shade_baseline(const ShadeArgs& a, uint32_t technique, bool exact, hipStream_t s);
 uint32_t shade_tiles(const ShadeArgs& a);
 size_t shade_lds_bytes(const ShadeArgs& a);
 // views and their tile offsets (count + 1 entries) in device memory

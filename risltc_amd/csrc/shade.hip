@@ -1689,6 +1689,249 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) s
 }
 
 // the last v with offsets[v] <= i (offsets[0] = 0, n + 1 entries)
+// ---- baseline techniques ----------------------------------------------------------
+// The two polygon sampling techniques the reference compares against
+// (src/experiment_list.c:313-396), with the uniform source. One lane per
+// pixel evaluates its candidates one after the other: a projected-solid-angle
+// candidate consumes 0, 2 or 4 noise draws depending on the polygon, which
+// rules out the compacted evaluation of shade_kernel. Lights are read from
+// global memory (they stay in L2). The kernel writes the same sample records
+// and compacted shadow rays as shade_kernel, so trace and resolve are shared.
+
+// sample_area_polygon_turk (polygon_sampling_related_work.glsl:34-43): a
+// uniform point on the triangle of vertices 0, 1, 2 (the reference samples
+// only this triangle of the polygon)
+template <bool X>
+__device__ __forceinline__ f3 turk_point(const Light& L, float u0, float u1) {
+    float s = sqrt_<X>(u0);
+    float b0 = 1.0f - s, b1 = s * u1, b2 = fma_(-s, u1, s);
+    return mk3(b0 * L.v[0] + b1 * L.v[3] + b2 * L.v[6], b0 * L.v[1] + b1 * L.v[4] + b2 * L.v[7],
+               b0 * L.v[2] + b1 * L.v[5] + b2 * L.v[8]);
+}
+
+// get_polygon_area (shading_pass.frag.glsl:148-152): the triangle of
+// vertices 0, 1, 2
+template <bool X>
+__device__ __forceinline__ float turk_area(const Light& L) {
+    f3 v0 = mk3(L.v[0], L.v[1], L.v[2]);
+    f3 c = cross_stable(sub3(mk3(L.v[3], L.v[4], L.v[5]), v0), sub3(mk3(L.v[6], L.v[7], L.v[8]), v0));
+    return sqrt_<X>(dot3(c, c)) / 2.0f;
+}
+
+// The AREA_TURK branch of evaluate_polygonal_light_shading
+// (shading_pass.frag.glsl:410-418) at one point: get_area_sample_density
+// (polygon_sampling_related_work.glsl:57-64) and
+// get_polygonal_light_mis_estimate (shading_pass.frag.glsl:281-287) without
+// visibility. dir and tmax are the shadow ray of get_polygon_visibility
+// (112-129); above = the point is above the shading horizon.
+template <bool X>
+__device__ __forceinline__ f3 eval_turk(const Pixel& px, const Light& L, f3 point, f3& dir, float& tmax,
+                                        bool& above) {
+    f3 d = sub3(point, px.pos);
+    float d2 = dot3(d, d);
+    d = scl3(d, rsq_<X>(d2));
+    f3 pn = mk3(L.plane[0], L.plane[1], L.plane[2]);
+    float density = div_<X>(d2, fabs_(dot3(pn, d)) * turk_area<X>(L));
+    float lambert = dot3(px.n, d);
+    dir = d;
+    above = lambert > 0.0f;
+    tmax = 0.0f;
+    f3 rb = mk3(0.0f, 0.0f, 0.0f);
+    if (above) {
+        tmax = div_<X>(-plane_side(px.pos, L.plane), dot3(d, pn));
+        tmax -= 1e-3f;
+        rb = mul3(L.rad, evaluate_brdf<X>(px, d));
+    }
+    return (density > 0.0f) ? scl3(rb, div_<X>(lambert, density)) : mk3(0.0f, 0.0f, 0.0f);
+}
+
+// the final sample of AREA_TURK at the stored point: technique slot 0 holds
+// the unshadowed estimate and its shadow ray; occluded means 0
+// (shading_pass.frag.glsl:757-759)
+template <bool X>
+__device__ __forceinline__ void final_turk(const ShadeArgs& A, const Pixel& px, const Light& L, f3 point,
+                                           FinalOut& F) {
+    bool above;
+    f3 dir;
+    float tmax;
+    f3 col = eval_turk<X>(px, L, point, dir, tmax, above);
+    F.e_vis[0] = col;
+    F.dir[0] = dir;
+    F.tmax[0] = tmax;
+    F.state[0] = above ? (A.shadow_rays ? 5u : 4u) : 1u;
+}
+
+template <bool X, uint32_t TECH>
+__global__ void __launch_bounds__(256) shade_baseline_kernel(ShadeArgs A) {
+    const uint32_t W = A.c.viewport_size[0], H = A.c.viewport_size[1];
+    const uint32_t ww_log2 = A.tile_w_log2, ww = 1u << ww_log2, wh = 64u >> ww_log2;
+    const uint32_t tiles_x = (W + ww - 1u) >> ww_log2;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t x = (tile % tiles_x) * ww + (lane & (ww - 1u));
+    const uint32_t y = (tile / tiles_x) * (4u * wh) + wave * wh + (lane >> ww_log2);
+    const bool inside = x < W && y < H;
+    const uint32_t pixel = y * W + x;
+    const uint32_t spp = A.spp, M = A.M, N = A.scene.light_count;
+    const float pN = 1.0f / (float)N;
+    uint32_t prim = inside ? A.visibility[pixel] : 0xFFFFFFFFu;
+    const bool shade = inside && (prim >> 31) == 0u && (uint64_t)prim < A.scene.triangle_count;
+    Pixel px = {};
+    uint32_t seed = 0;
+    if (shade) {
+        const float(*P)[4] = A.c.pixel_to_ray_direction_world_space;
+        float fx = (float)x, fy = (float)y;
+        f3 ray = mk3(fma_(P[0][2], 1.0f, fma_(P[0][1], fy, P[0][0] * fx)),
+                     fma_(P[1][2], 1.0f, fma_(P[1][1], fy, P[1][0] * fx)),
+                     fma_(P[2][2], 1.0f, fma_(P[2][1], fy, P[2][0] * fx)));
+        setup_pixel(A, prim, ray, px);
+        seed = murmur_finalize(murmur_mix(murmur_mix(0u, x + y * W), A.c.noise_random_numbers[0]));
+    }
+    const uint32_t ckey = (shade && A.cache.counts) ? cache_key(A.cache, px.pos, px.n) : 0u;
+    for (uint32_t j = 0; j < spp; ++j) {
+        int32_t chosen = -1;
+        float w_sum = 0.0f, sample_value = 0.0f;
+        f3 chosen_point = mk3(0.0f, 0.0f, 0.0f);
+        Light L;
+        if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
+            if (shade) chosen = (int32_t)light_index_from(noise_next(seed), N);
+        } else if (shade) {
+            for (uint32_t i = 0; i < M; ++i) {
+                uint32_t idx = light_index_from(noise_next(seed), N);
+                load_light(A.scene.lights, idx, L);
+                f3 col = mk3(0.0f, 0.0f, 0.0f), point = mk3(0.0f, 0.0f, 0.0f);
+                uint32_t cc = 0;
+                if constexpr (TECH == RISLTC_SAMPLE_POLYGON_AREA_TURK) {
+                    float u0 = noise_next(seed);
+                    float u1 = noise_next(seed);
+                    point = turk_point<X>(L, u0, u1);
+                    f3 d;
+                    float tm;
+                    bool ab;
+                    col = eval_turk<X>(px, L, point, d, tm, ab);
+                } else {
+                    // the final-sample code without tracing: a sample above
+                    // the horizon keeps its unshadowed estimate
+                    FinalOut F;
+                    eval_peters<X>(A, px, L, seed, F);
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                        if (F.state[t] != 0u) col = add3(col, F.state[t] >= 4u ? F.e_vis[t] : F.e_occ[t]);
+                    cc = F.clipd | (F.clips << 8);
+                }
+                float p_hat = sqrt_<X>(dot3(col, col));
+                float w = div_<X>(p_hat, pN);
+                float r = noise_next(seed);
+                w_sum += w;
+                bool take = X ? (r < w / w_sum) : (r * w_sum < w);
+                if (w > 0.0f && take) {
+                    chosen = (int32_t)idx;
+                    sample_value = p_hat;
+                    chosen_point = point;
+                }
+                if (A.debug_cands) {
+                    uint32_t* cr = A.debug_cands + (((size_t)pixel * spp + j) * M + i) * RISLTC_CANDIDATE_RECORD_WORDS;
+                    cr[0] = idx;
+                    cr[1] = cc;
+                    cr[2] = __float_as_uint(p_hat);
+                    cr[3] = __float_as_uint(w);
+                }
+            }
+        }
+        FinalOut F;
+        F.color = mk3(0.0f, 0.0f, 0.0f);
+        F.state[0] = F.state[1] = 0u;
+        F.clipd = F.clips = 0u;
+        F.psad = F.psas = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            F.e_vis[t] = F.e_occ[t] = F.dir[t] = mk3(0.0f, 0.0f, 0.0f);
+            F.tmax[t] = 0.0f;
+        }
+        float Wt = 0.0f;
+        const bool contributes = shade && chosen >= 0;
+        if (contributes) {
+            load_light(A.scene.lights, (uint32_t)chosen, L);
+            if constexpr (TECH == RISLTC_SAMPLE_POLYGON_AREA_TURK) {
+                if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
+                    float u0 = noise_next(seed);
+                    float u1 = noise_next(seed);
+                    chosen_point = turk_point<X>(L, u0, u1);
+                }
+                final_turk<X>(A, px, L, chosen_point, F);
+            } else {
+                eval_peters<X>(A, px, L, seed, F);
+            }
+            if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
+                Wt = (float)N;
+            } else {
+                Wt = w_sum / ((float)M * sample_value);
+                if (sample_value == 0.0f) Wt = 0.0f;
+            }
+        }
+        const size_t sidx = (size_t)pixel * spp + j;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            bool emit = contributes && F.state[t] == 5u;
+            uint64_t ballot = __ballot(emit);
+            uint32_t cnt = __popcll(ballot);
+            uint32_t base = 0;
+            if (cnt) {
+                uint32_t first = (uint32_t)__ffsll((unsigned long long)ballot) - 1u;
+                if (lane == first) base = atomicAdd(A.ray_count, cnt);
+                base = __shfl(base, (int)first);
+            }
+            if (emit) {
+                uint32_t off = base + __popcll(ballot & ((1ull << lane) - 1ull));
+                A.rays[off].org = make_float4(px.pos.x, px.pos.y, px.pos.z, __uint_as_float(A.bvh_index));
+                A.rays[off].dir = make_float4(F.dir[t].x, F.dir[t].y, F.dir[t].z, F.tmax[t]);
+                A.ray_slots[off] = A.slot_base + (uint32_t)(sidx * 2 + t);
+            }
+        }
+        if (inside) {
+            float4* rec = reinterpret_cast<float4*>(A.samples + sidx * kSampleWords);
+            uint32_t flags = (F.state[0] & 15u) | ((F.state[1] & 15u) << 4) | (contributes ? 256u : 0u);
+            rec[0] = make_float4(Wt, __uint_as_float(flags), F.e_vis[0].x, F.e_vis[0].y);
+            rec[1] = make_float4(F.e_vis[0].z, F.e_vis[1].x, F.e_vis[1].y, F.e_vis[1].z);
+            rec[2] = make_float4(F.e_occ[0].x, F.e_occ[0].y, F.e_occ[0].z, F.e_occ[1].x);
+            rec[3] = make_float4(F.e_occ[1].y, F.e_occ[1].z, __int_as_float(contributes ? chosen : -1),
+                                 __uint_as_float(ckey));
+        }
+        if (inside && A.debug_samples) {
+            float* r = A.debug_samples + sidx * RISLTC_SAMPLE_RECORD_FLOATS;
+            for (int k = 0; k < RISLTC_SAMPLE_RECORD_FLOATS; ++k) r[k] = 0.0f;
+            r[0] = __int_as_float(shade ? chosen : -1);
+            r[1] = w_sum;
+            r[2] = sample_value;
+            r[3] = Wt;
+            r[4] = __uint_as_float(F.clipd);
+            r[5] = __uint_as_float(F.clips);
+            r[6] = F.psad;
+            r[7] = F.psas;
+            for (int t = 0; t < 2; ++t) {
+                r[8 + 3 * t] = F.dir[t].x;
+                r[9 + 3 * t] = F.dir[t].y;
+                r[10 + 3 * t] = F.dir[t].z;
+                r[14 + t] = F.tmax[t];
+                r[16 + t] = __uint_as_float(F.state[t]);
+                r[18 + 3 * t] = F.e_vis[t].x;
+                r[19 + 3 * t] = F.e_vis[t].y;
+                r[20 + 3 * t] = F.e_vis[t].z;
+                r[24 + 3 * t] = F.e_occ[t].x;
+                r[25 + 3 * t] = F.e_occ[t].y;
+                r[26 + 3 * t] = F.e_occ[t].z;
+            }
+            r[33] = __uint_as_float(seed);
+            r[37] = __uint_as_float(ckey);
+            if (contributes) {
+                r[34] = px.pos.x;
+                r[35] = px.pos.y;
+                r[36] = px.pos.z;
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ uint32_t view_of(const uint32_t* offsets, uint32_t n, uint32_t i) {
     uint32_t lo = 0, hi = n;
     while (hi - lo > 1u) {
@@ -1856,6 +2099,25 @@ hipError_t launch_shade(const ShadeArgs& a, bool exact, hipStream_t s) {
     } else {
         if (dbg) hipLaunchKernelGGL((shade_kernel<false, true, false>), grid, block, lds, s, a);
         else hipLaunchKernelGGL((shade_kernel<false, false, false>), grid, block, lds, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_shade_baseline(const ShadeArgs& a, uint32_t technique, bool exact, hipStream_t s) {
+    if (!shade_args_ok(a)) return hipErrorInvalidValue;
+    dim3 grid(shade_tiles(a)), block(256);
+    if (technique == RISLTC_SAMPLE_POLYGON_AREA_TURK) {
+        if (exact) hipLaunchKernelGGL((shade_baseline_kernel<true, RISLTC_SAMPLE_POLYGON_AREA_TURK>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((shade_baseline_kernel<false, RISLTC_SAMPLE_POLYGON_AREA_TURK>), grid, block, 0, s, a);
+    } else if (technique == RISLTC_SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE) {
+        if (exact)
+            hipLaunchKernelGGL((shade_baseline_kernel<true, RISLTC_SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE>), grid, block, 0,
+                               s, a);
+        else
+            hipLaunchKernelGGL((shade_baseline_kernel<false, RISLTC_SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE>), grid, block,
+                               0, s, a);
+    } else {
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -310,3 +310,70 @@ def test_accumulate_running_mean(small_gpu):
         g["ctx"].accumulate(acc, frames[i], n, i)
     torch.cuda.synchronize()
     assert torch.allclose(acc, frames.mean(0), rtol=1e-5, atol=1e-6)
+
+
+BASELINE_CASES = [
+    dict(polygon_sampling_technique=abi.SAMPLE_POLYGON_AREA_TURK),
+    dict(polygon_sampling_technique=abi.SAMPLE_POLYGON_AREA_TURK, light_sampling=abi.LIGHT_UNIFORM),
+    dict(polygon_sampling_technique=abi.SAMPLE_POLYGON_AREA_TURK, shadow_rays=0, ris_candidate_count=3),
+    dict(polygon_sampling_technique=abi.SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE),
+    dict(polygon_sampling_technique=abi.SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE, mis_heuristic=abi.MIS_OPTIMAL,
+         ris_candidate_count=5),
+    dict(polygon_sampling_technique=abi.SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE, light_sampling=abi.LIGHT_UNIFORM),
+]
+
+
+@pytest.mark.parametrize("case", range(len(BASELINE_CASES)))
+def test_baseline_techniques_exact_bitwise(small_gpu, case):
+    """The reference's comparison techniques (Turk area sampling and
+    projected-solid-angle RIS) in the per-pixel baseline kernel: every
+    candidate record, every sample record and the frame bitwise equal to the
+    FP32 oracle."""
+    g = small_gpu
+    kw = dict(exact_math=1, sample_count_light=2, ris_candidate_count=8)
+    kw.update(BASELINE_CASES[case])
+    print("case", kw, flush=True)
+    s = g["scene"].settings(**kw)
+    out, rec, cand = _shade(g, s)
+    vis = g["vis"].cpu().numpy().view(np.uint32)
+    ref, rrec, rcand = g["oracle"].shade(g["consts"], s, vis, records=True, candidates=True)
+    assert np.array_equal(cand, rcand)
+    ri, oi = rec.view(np.uint32), rrec.view(np.uint32)
+    bad = np.argwhere(np.any(ri != oi, axis=-1))
+    assert bad.size == 0, ("sample records differ", bad[:5], np.flatnonzero(ri[tuple(bad[0])] != oi[tuple(bad[0])]))
+    assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+    shaded = (vis >> 31) == 0
+    assert (out[shaded, :3].sum(1) > 0).mean() > 0.3
+    st = ri[..., abi.REC_STATE]
+    if kw.get("shadow_rays", 1):
+        assert (st == 2).sum() > 0 and (st == 3).sum() > 0
+
+
+@pytest.mark.parametrize("technique", [abi.SAMPLE_POLYGON_AREA_TURK, abi.SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE])
+def test_baseline_techniques_fast(small_gpu, technique):
+    """Production math: the chosen lights equal the FP32 oracle's except near
+    ties, and the frame agrees where they do."""
+    g = small_gpu
+    s = g["scene"].settings(exact_math=0, sample_count_light=2, ris_candidate_count=16,
+                            polygon_sampling_technique=technique)
+    out, rec, _ = _shade(g, s)
+    vis = g["vis"].cpu().numpy().view(np.uint32)
+    ref, rrec, _ = g["oracle"].shade(g["consts"], s, vis, records=True)
+    li_g = rec[..., abi.REC_LIGHT].view(np.int32)
+    li_o = rrec[..., abi.REC_LIGHT].view(np.int32)
+    assert (li_g == li_o).mean() > 0.99
+    same = np.all(li_g == li_o, axis=1)
+    rel = np.abs(out[same, :3] - ref[same, :3]) / (np.abs(ref[same, :3]) + 1e-3)
+    assert np.percentile(rel, 99) < 1e-3, _quantiles(rel)
+
+
+def test_baseline_techniques_rejected_in_batches(small_gpu):
+    import risltc_amd
+    g = small_gpu
+    torch = _torch()
+    out = torch.empty(g["P"] * 4, dtype=torch.float32, device="cuda")
+    s = g["scene"].settings(polygon_sampling_technique=abi.SAMPLE_POLYGON_AREA_TURK)
+    v = risltc_amd.Context.view(g["consts"], s, g["vis"], out)
+    with pytest.raises(risltc_amd.RisltcError) as e:
+        g["ctx"].shade_batch([v])
+    assert e.value.code == 4

@@ -364,3 +364,100 @@ def test_cached_learned_estimator_is_unbiased(orc_lib, ltc_table):
     (mu_u, se_u), (mu_l, se_l) = means[abi.PROPOSAL_UNIFORM], means[abi.PROPOSAL_LEARNED]
     se = np.sqrt(se_u ** 2 + se_l ** 2)
     assert np.all(np.abs(mu_u - mu_l) < 5 * se + 1e-4 * np.abs(mu_u) + 1e-7), (mu_u, mu_l, se)
+
+
+# ---------------------------------------------------------------- baseline techniques
+
+
+def _quadrature(scene, x, exposure, triangle_only):
+    """f * L * cos_x * cos_y / r^2 over the light of _single_light_scene (a
+    parallelogram v0, v1, v3 spanned), or over its triangle v0, v1, v2 only
+    (u >= v in the parallelogram's coordinates; diagonal cells count half)."""
+    L = scene.lights[0]
+    rad = L.radiance.astype(np.float64)
+    wv = L.world.astype(np.float64)
+    nl = L.plane[:3].astype(np.float64)
+    k = 240
+    u = (np.arange(k) + 0.5) / k
+    uu, vv = np.meshgrid(u, u, indexing="ij")
+    wgt = np.ones_like(uu)
+    if triangle_only:
+        wgt = np.where(uu > vv, 1.0, np.where(uu == vv, 0.5, 0.0))
+    pts = (wv[0] + uu[..., None] * (wv[1] - wv[0]) + vv[..., None] * (wv[3] - wv[0])).reshape(-1, 3)
+    dA = np.linalg.norm(np.cross(wv[1] - wv[0], wv[3] - wv[0])) / (k * k)
+    cam = np.array(scene.camera.position_world_space, np.float64)
+    mat = scene.materials[0].astype(np.float64)
+    base, m = mat[0:3], mat[5]
+    alb = base * (1 - m)
+    f0 = 0.02 * (1 - m) + base * m
+    alpha = min(max(mat[4] ** 2, 0.0064), 1.0)
+    n = np.array([0.0, 0.0, 1.0])
+    ov = (cam - x) / np.linalg.norm(cam - x)
+    d = pts - x
+    r2 = (d * d).sum(1)
+    w = d / np.sqrt(r2)[:, None]
+    cos_x = np.maximum(w @ n, 0.0)
+    cos_y = np.abs(w @ nl)
+    f = _brdf(n, ov, alb, f0, alpha, w)
+    f[cos_x <= 0] = 0.0
+    return exposure * (f * rad[None, :] * (cos_x * cos_y / r2 * dA * wgt.ravel())[:, None]).sum(0)
+
+
+@pytest.mark.parametrize("technique,triangle_only", [(abi.SAMPLE_POLYGON_AREA_TURK, True),
+                                                     (abi.SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE, False)])
+@pytest.mark.parametrize("light_sampling", [abi.LIGHT_UNIFORM, abi.LIGHT_RESERVOIR])
+def test_baseline_techniques_converge(orc_lib, ltc_table, technique, triangle_only, light_sampling):
+    """The reference's baselines (shading_pass.frag.glsl:410-418, 458-551)
+    with shadows off: projected-solid-angle sampling converges to the light's
+    reflected radiance; Turk's area sampling to the part of it due to the
+    triangle of vertices 0, 1, 2 -- the reference samples and measures only
+    that triangle (polygon_sampling_related_work.glsl:34-43,
+    shading_pass.frag.glsl:148-152), half of this rectangular light."""
+    scene = _single_light_scene(0.3, 0.0)
+    o = orc.Oracle(scene, *ltc_table)
+    consts0 = o.frame_constants(exposure=1.0)
+    vis = o.render_visibility(consts0)
+    ground = np.flatnonzero(vis < 2)
+    pix = ground[np.linspace(0, ground.size - 1, 6).astype(int)].astype(np.uint32)
+    s = scene.settings(sample_count_light=256, ris_candidate_count=2, shadow_rays=0, light_sampling=light_sampling,
+                       polygon_sampling_technique=technique)
+    means = []
+    for seed in range(10):
+        c = o.frame_constants(exposure=1.0, noise_seed=seed, animate=True)
+        out, rec, _ = o.shade(c, s, vis, pixels=pix, fp64=True, records=(seed == 0))
+        means.append(out[:, :3])
+        if seed == 0:
+            pos = rec[:, 0, abi.REC_POS:abi.REC_POS + 3].astype(np.float64)
+    means = np.array(means)
+    est, se = means.mean(0), means.std(0, ddof=1) / math.sqrt(means.shape[0])
+    for p in range(pix.size):
+        ref = _quadrature(scene, pos[p], 1.0, triangle_only)
+        assert np.all(np.abs(est[p] - ref) < 5 * se[p] + 4e-3 * ref), (p, est[p], ref, se[p])
+    if triangle_only:
+        full = np.array([_quadrature(scene, pos[p], 1.0, False) for p in range(pix.size)])
+        assert np.all(est < 0.8 * full)
+
+
+def test_psa_uniform_light_equals_ltc_cp(small_oracle, small_scene):
+    """With uniform light selection (shading_pass.frag.glsl:707-721) the
+    projected-solid-angle branch (458-551) and the LTC_CP final sample
+    (292-397) compute the same estimate from the same draws: the frames are
+    equal bit for bit. The techniques differ only in the RIS target."""
+    o = small_oracle
+    consts = o.frame_constants()
+    vis = o.render_visibility(consts)
+    a = small_scene.settings(light_sampling=abi.LIGHT_UNIFORM, sample_count_light=2)
+    b = small_scene.settings(light_sampling=abi.LIGHT_UNIFORM, sample_count_light=2,
+                             polygon_sampling_technique=abi.SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE)
+    oa, _, _ = o.shade(consts, a, vis)
+    ob, _, _ = o.shade(consts, b, vis)
+    assert np.array_equal(oa.view(np.uint32), ob.view(np.uint32))
+
+
+def test_baselines_reject_learned_source(small_oracle, small_scene):
+    o = small_oracle
+    consts = o.frame_constants()
+    vis = o.render_visibility(consts)
+    s = small_scene.settings(polygon_sampling_technique=abi.SAMPLE_POLYGON_AREA_TURK, proposal=abi.PROPOSAL_CACHE)
+    with pytest.raises(Exception):
+        o.shade(consts, s, vis, pixels=np.arange(4, dtype=np.uint32))
