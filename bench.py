@@ -12,8 +12,10 @@ host's cores) on a bounded sample of the same frame.
 The learned_vs_plain object is the second half of the metric: GPU time for
 the plain pass and for the learned pass (MLP + online visibility cache in
 the RIS target) to reach relMSE 0.01 against a 65536-spp plain reference,
-each pass's Msamples/s and per-kernel milliseconds per frame, and two
-ablations (risltc_amd/convergence.py).
+each pass's Msamples/s and per-kernel milliseconds per frame, three
+ablations, and the plain pass with the reference's two comparison
+techniques (Turk area sampling, projected-solid-angle RIS;
+risltc_amd/convergence.py).
 
 The batched_views object: views of the batched configuration (each its own
 city mesh of ~50k-500k triangles and its own light set of 16-8192 polygons,

@@ -24,12 +24,14 @@ def rel_mse(x, ref):
     return float((((xs - r) ** 2) / (r * r + eps)).mean())
 
 
-def accumulate(ctx, scene, vis, frames, seed0, learned=None, checkpoints=(), ref=None, candidates=None):
+def accumulate(ctx, scene, vis, frames, seed0, learned=None, checkpoints=(), ref=None, candidates=None,
+               technique=abi.SAMPLE_POLYGON_LTC_CP):
     """Accumulates `frames` frames. learned = None (plain) or a dict with
     floor_log2, infer_every, batch, lr, clusters, seed, and optionally
     cache_log2, cache_cell, cache_prior (the visibility cache) and candidates
-    (M). Returns (image, gpu_ms, curve) where curve maps spp -> (relmse,
-    gpu_ms) at checkpoints."""
+    (M). technique selects the polygon sampling technique of a plain pass.
+    Returns (image, gpu_ms, curve) where curve maps spp -> (relmse, gpu_ms)
+    at checkpoints."""
     import torch
     P = scene.width * scene.height
     out = torch.empty(P * 4, dtype=torch.float32, device="cuda")
@@ -37,7 +39,7 @@ def accumulate(ctx, scene, vis, frames, seed0, learned=None, checkpoints=(), ref
     lk = learned or {}
     m = lk.get("candidates", candidates)
     s = scene.settings(proposal=lk.get("proposal", abi.PROPOSAL_LEARNED) if learned else abi.PROPOSAL_UNIFORM,
-                       learned_floor_log2=lk.get("floor_log2", 0),
+                       learned_floor_log2=lk.get("floor_log2", 0), polygon_sampling_technique=technique,
                        **({"ris_candidate_count": m} if m else {}))
     ctx.proposal_cache(0)
     tuning = ctx.get_tuning().as_dict()
@@ -132,19 +134,23 @@ def reference(ctx, scene, vis, frames, chunk=2048, seed0=1000000, log=None):
 
 
 def compare(ctx, scene, vis, target=0.01, ref_frames=16384, plain_frames=320, frames=256, learned=None,
-            ablations=True, log=None):
+            ablations=True, baselines=True, log=None):
     """The learned-vs-plain experiment on one scene; returns a dict for the
     bench line. The plain pass is the scene's own configuration (M and spp);
     the learned pass uses `learned` (DEFAULT_LEARNED by default). Ablations:
     the learned pass without the visibility cache (MLP only), and with the MLP
-    left untrained (learning rate 0)."""
+    left untrained (learning rate 0). Baselines: the plain pass with the
+    reference's comparison techniques (Turk area sampling, projected solid
+    angle RIS); their time to the target is null when the curve does not
+    reach it (Turk's estimate covers only each polygon's first triangle, so
+    its error has a floor wherever lights have more than three vertices)."""
     learned = dict(DEFAULT_LEARNED, **(learned or {}))
     ref = reference(ctx, scene, vis, ref_frames, log=log)
     cps = tuple(k for k in (16, 32, 64, 128, 256, 320, 512) if k <= max(frames, plain_frames))
     spf = scene.width * scene.height * scene.spp
 
-    def run(lk, n):
-        _, _, curve = accumulate(ctx, scene, vis, n, 0, learned=lk, checkpoints=cps, ref=ref)
+    def run(lk, n, technique=abi.SAMPLE_POLYGON_LTC_CP):
+        _, _, curve = accumulate(ctx, scene, vis, n, 0, learned=lk, checkpoints=cps, ref=ref, technique=technique)
         km = list(_frames.kernel_ms)
         t, ex = time_to(curve, target)
         return {"time_to_target_ms": round(t, 1), "extrapolated": ex,
@@ -175,4 +181,16 @@ def compare(ctx, scene, vis, target=0.01, ref_frames=16384, plain_frames=320, fr
             out["ablations"][name] = {"time_to_target_ms": r["time_to_target_ms"],
                                       "over_plain": round(r["time_to_target_ms"] / plain["time_to_target_ms"], 4),
                                       "relmse_at_1024spp": r["curve"].get("1024", [None])[0]}
+    if baselines:
+        out["baselines"] = {}
+        for name, tech in (("area_turk", abi.SAMPLE_POLYGON_AREA_TURK),
+                           ("projected_solid_angle", abi.SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE)):
+            r = run(None, plain_frames, technique=tech)
+            t = None if r["extrapolated"] else r["time_to_target_ms"]
+            out["baselines"][name] = {"time_to_target_ms": t,
+                                      "over_plain": None if t is None else round(t / plain["time_to_target_ms"], 4),
+                                      "curve": r["curve"], "ms_per_frame": r["ms_per_frame"],
+                                      "msamples_per_s": r["msamples_per_s"]}
+            if log:
+                log("baseline %s: %s ms to relMSE %g" % (name, t, target))
     return out
