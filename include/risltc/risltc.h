@@ -349,6 +349,19 @@ const char* risltc_last_error(const risltc_context_t* context);
 int risltc_set_lights(risltc_context_t* context, const void* records, uint32_t count,
                       uint32_t max_vertex_count);
 
+/* Convenience over host-side lights: risltc_write_lights (which runs
+ * update_polygonal_light on every light, src/main.c:456-490) followed by
+ * risltc_set_lights. max_vertex_count = 0 takes the largest vertex_count of
+ * the lights. */
+int risltc_set_polygonal_lights(risltc_context_t* context, risltc_polygonal_light_t* lights,
+                                uint32_t count, uint32_t max_vertex_count);
+
+/* Convenience with load_ltc_table's file semantics (src/ltc_table.c:23-194):
+ * reads <directory>/fit<i>.dat for i < fresnel_count with
+ * risltc_load_ltc_fit and uploads the square table with
+ * risltc_set_ltc_table. Errors as for risltc_load_ltc_fit. */
+int risltc_load_ltc_table(risltc_context_t* context, const char* directory, uint32_t fresnel_count);
+
 /* LTC tables as load_ltc_table builds them: rgba16 holds fresnel_count slices
  * of roughness_count x inclination_count texels (roughness fastest), 4
  * channels; rg16 the same with 2 channels. Host pointers. */

@@ -76,6 +76,8 @@ SIGNATURES = {
     "risltc_proposal_cache_write": (C.c_int, [_vp, _u32p, C.c_uint64]),
     "risltc_proposal_cache_table": (C.c_int, [_vp, _u8p, C.c_uint64]),
     "risltc_add_light_set": (C.c_int, [_vp, _vp, _u32, _u32, _u32p]),
+    "risltc_set_polygonal_lights": (C.c_int, [_vp, C.POINTER(abi.PolygonalLight), _u32, _u32]),
+    "risltc_load_ltc_table": (C.c_int, [_vp, C.c_char_p, _u32]),
     "risltc_add_scene": (C.c_int, [_vp, _u32p, _u16p, _u8p, C.c_uint64, _f32p, _f32p, C.POINTER(abi.Material), _u32,
                                    _u32p]),
     "risltc_render_visibility_scene": (C.c_int, [_vp, _u32, C.POINTER(abi.FrameConstants), _vp, _vp]),
@@ -218,6 +220,24 @@ class Context:
         self._check(self.L.risltc_set_lights(self.h, buf.ctypes.data_as(_vp), count, max_vertex_count),
                     "risltc_set_lights")
         self._light_count = count
+
+    def set_polygonal_lights(self, lights, max_vertex_count=0):
+        """risltc_set_polygonal_lights over a ctypes array of
+        abi.PolygonalLight (update_polygonal_light + write_lights + upload)."""
+        n = len(lights)
+        self._check(self.L.risltc_set_polygonal_lights(self.h, lights, n, max_vertex_count),
+                    "risltc_set_polygonal_lights")
+        self._light_count = n
+
+    def load_ltc_table(self, directory, fresnel_count=51):
+        """risltc_load_ltc_table: fit<i>.dat files with load_ltc_table's semantics."""
+        res = C.c_uint32()
+        rc = self.L.risltc_load_ltc_fit(str(directory).encode(), fresnel_count, C.byref(res), None, None, 0)
+        if rc != 0:
+            raise RisltcError(rc, "risltc_load_ltc_fit(%s)" % directory)
+        self._check(self.L.risltc_load_ltc_table(self.h, str(directory).encode(), fresnel_count),
+                    "risltc_load_ltc_table")
+        self.ltc_constants = ltc_constants_for(res.value, res.value, fresnel_count)
 
     def set_ltc_table(self, rgba16, rg16):
         rgba = np.ascontiguousarray(rgba16, dtype=np.uint16)

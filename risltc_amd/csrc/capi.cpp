@@ -658,6 +658,36 @@ int risltc_set_lights(risltc_context_t* c, const void* records, uint32_t count, 
     return clear_vcache(c);
 }
 
+int risltc_set_polygonal_lights(risltc_context_t* c, risltc_polygonal_light_t* lights, uint32_t count,
+                                uint32_t vmax) {
+    if (!c) return RISLTC_E_ARG;
+    if (!lights || count == 0) return fail(c, RISLTC_E_ARG, "no lights");
+    if (vmax == 0)
+        for (uint32_t i = 0; i < count; ++i) vmax = std::max(vmax, lights[i].vertex_count);
+    if (vmax < 3 || vmax > RISLTC_MAX_LIGHT_VERTEX_COUNT)
+        return fail(c, RISLTC_E_UNSUPPORTED, "max_vertex_count %u outside [3, %d]", vmax,
+                    RISLTC_MAX_LIGHT_VERTEX_COUNT);
+    for (uint32_t i = 0; i < count; ++i)
+        if (!lights[i].vertices_plane_space || !lights[i].vertices_world_space)
+            return fail(c, RISLTC_E_ARG, "light %u: vertex arrays not allocated", i);
+    std::vector<uint8_t> records((size_t)count * RISLTC_LIGHT_RECORD_SIZE(vmax));
+    if (risltc_write_lights(lights, count, vmax, records.data(), records.size()) != records.size())
+        return fail(c, RISLTC_E_FORMAT, "a light has a vertex count outside [3, %u]", vmax);
+    return risltc_set_lights(c, records.data(), count, vmax);
+}
+
+int risltc_load_ltc_table(risltc_context_t* c, const char* dir, uint32_t fresnel_count) {
+    if (!c) return RISLTC_E_ARG;
+    uint32_t res = 0;
+    int e = risltc_load_ltc_fit(dir, fresnel_count, &res, nullptr, nullptr, 0);
+    if (e) return fail(c, e, "cannot read the LTC table in %s", dir ? dir : "(null)");
+    const size_t texels = (size_t)res * res * fresnel_count;
+    std::vector<uint16_t> rgba(texels * 4), rg(texels * 2);
+    e = risltc_load_ltc_fit(dir, fresnel_count, &res, rgba.data(), rg.data(), rgba.size());
+    if (e) return fail(c, e, "cannot read the LTC table in %s", dir);
+    return risltc_set_ltc_table(c, rgba.data(), rg.data(), res, res, fresnel_count);
+}
+
 int risltc_add_light_set(risltc_context_t* c, const void* records, uint32_t count, uint32_t vmax, uint32_t* id) {
     if (!c) return RISLTC_E_ARG;
     if (!id) return fail(c, RISLTC_E_ARG, "null id");

@@ -12,10 +12,16 @@ from risltc_amd import abi, formats, synth
 pytestmark = pytest.mark.gpu
 
 
-def _render(ctx, scene, records, vmax, mesh, camera, ltc_table):
+def _render(ctx, scene, records, vmax, mesh, camera, ltc_table, lights=None, ltc_dir=None):
     import torch
-    ctx.set_lights(records, scene.light_count, vmax)
-    ctx.set_ltc_table(*ltc_table)
+    if lights is not None:
+        ctx.set_polygonal_lights(lights)
+    else:
+        ctx.set_lights(records, scene.light_count, vmax)
+    if ltc_dir is not None:
+        ctx.load_ltc_table(ltc_dir, 51)
+    else:
+        ctx.set_ltc_table(*ltc_table)
     ctx.set_scene(mesh, scene.materials)
     ctx.scene = scene
     consts = ctx.frame_constants(camera=camera)
@@ -51,6 +57,14 @@ def test_vks_and_quicksave_render_like_memory(ltc_table, tmp_path):
     assert np.array_equal(vis_a, vis_b)
     a = out_a.cpu().numpy()
     assert np.array_equal(a.view(np.uint32), out_b.cpu().numpy().view(np.uint32))
+    # the conveniences: lights straight from polygonal_light_t, the LTC
+    # table from fit<i>.dat files
+    from risltc_amd import ltcfit
+    ltcfit.write_fit_dir(str(tmp_path / "fit"), 51)
+    q2 = formats.load_quicksave(tmp_path / "s.save")
+    vis_c, out_c = _render(ctx, scene, None, 0, vks, q2.camera, ltc_table, lights=q2.lights,
+                           ltc_dir=tmp_path / "fit")
+    assert np.array_equal(a.view(np.uint32), out_c.cpu().numpy().view(np.uint32))
 
     accum = torch.zeros_like(out_a)
     ctx.accumulate(accum, out_a, scene.width * scene.height, 0)
