@@ -596,16 +596,26 @@ int risltc_render_visibility_scene(risltc_context_t* context, uint32_t scene,
 
 typedef struct risltc_view_s {
     risltc_frame_constants_t constants;
-    risltc_settings_t settings;   /* proposal must be RISLTC_PROPOSAL_UNIFORM;
-                                     exact_math equal across a batch */
+    risltc_settings_t settings;   /* exact_math and (proposal != UNIFORM)
+                                     equal across a batch */
     const uint32_t* visibility;   /* device, R32_UINT per pixel */
     float* out;                   /* device, RGBA32F per pixel */
     uint32_t light_set;           /* 0 or an id from risltc_add_light_set */
     uint32_t scene;               /* 0 or an id from risltc_add_scene */
-    uint32_t reserved[2];
+    const uint8_t* proposal_table; /* device, with settings.proposal =
+                                      RISLTC_PROPOSAL_LEARNED: this view's
+                                      table (width*height*32 bytes) from
+                                      risltc_proposal_update's table_out;
+                                      NULL otherwise */
 } risltc_view_t;
 
-/* Shades count views (device pointers in each view) on stream. */
+/* Shades count views (device pointers in each view) on stream.
+ * Learned views (settings.proposal LEARNED or CACHE) run the learned pass:
+ * they use light set 0 and scene 0 (the clusters, the MLP's features and
+ * the visibility cache refer to them), share the context's visibility cache
+ * (refreshed once before the batch; every view's shadow rays are counted
+ * into it) and write no training records (train on risltc_shade frames).
+ * A batch is either all learned or all plain. */
 int risltc_shade_batch(risltc_context_t* context, const risltc_view_t* views, uint32_t count, void* stream);
 
 /* Launch shape of the kernels. None of these changes any result bit.
@@ -637,8 +647,10 @@ int risltc_autotune(risltc_context_t* context, const risltc_view_t* views, uint3
 /* A batch captured as a hipGraph with buffers of its own. risltc_graph_update
  * takes new views of the same shapes (resolutions, sample counts, shadow-ray
  * and exact_math flags; other fields, pointers and light sets may change)
- * and waits for the graph's previous launch. A graph holds device pointers of
- * the context: after risltc_set_lights, risltc_set_ltc_table, risltc_set_scene
+ * and waits for the graph's previous launch; learned-ness and whether a
+ * visibility cache is used must stay too. A graph holds device pointers of
+ * the context: after risltc_set_lights, risltc_set_ltc_table, risltc_set_scene,
+ * risltc_set_light_clusters (or risltc_cluster_lights), risltc_proposal_cache
  * or a tuning change, risltc_graph_launch fails with RISLTC_E_STATE until
  * the graph is created again. Destroy graphs before their context. Launches
  * of one graph are ordered by their streams only: do not overlap them. */

@@ -431,7 +431,7 @@ class Context:
                                                           _stream(stream)), "risltc_render_visibility_scene")
 
     @staticmethod
-    def view(constants, settings, visibility, out, light_set=0, scene=0):
+    def view(constants, settings, visibility, out, light_set=0, scene=0, proposal_table=None):
         """A risltc_view_t over torch device tensors (keep them alive)."""
         v = abi.View()
         v.constants = constants
@@ -440,6 +440,7 @@ class Context:
         v.out = out.data_ptr()
         v.light_set = light_set
         v.scene = scene
+        v.proposal_table = None if proposal_table is None else proposal_table.data_ptr()
         return v
 
     @staticmethod

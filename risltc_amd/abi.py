@@ -110,7 +110,8 @@ class Tuning(C.Structure):
 class View(C.Structure):
     """risltc_view_t: one view of a batch."""
     _fields_ = [("constants", FrameConstants), ("settings", Settings), ("visibility", C.c_void_p),
-                ("out", C.c_void_p), ("light_set", C.c_uint32), ("scene", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("out", C.c_void_p), ("light_set", C.c_uint32), ("scene", C.c_uint32),
+                ("proposal_table", C.c_void_p)]
 
 
 class PolygonalLight(C.Structure):

@@ -737,6 +737,8 @@ int risltc_set_light_clusters(risltc_context_t* c, const uint32_t* cluster_of_li
     if (!cluster_of_light || cluster_count == 0) return fail(c, RISLTC_E_ARG, "null clusters");
     if (cluster_count > RISLTC_MAX_LIGHT_CLUSTERS)
         return fail(c, RISLTC_E_UNSUPPORTED, "at most %d clusters", RISLTC_MAX_LIGHT_CLUSTERS);
+    // graphs of learned batches hold the cluster tables' device pointers
+    ++c->generation;
     const uint32_t n = c->light_count;
     if (n == 0) return fail(c, RISLTC_E_STATE, "risltc_set_lights has not been called");
     std::vector<uint32_t> offsets(cluster_count + 1, 0), sorted(n);
@@ -764,6 +766,8 @@ int risltc_set_light_clusters(risltc_context_t* c, const uint32_t* cluster_of_li
 int risltc_proposal_cache(risltc_context_t* c, uint32_t log2_entries, float cell_size, float prior,
                           uint32_t per_light) {
     if (!c) return RISLTC_E_ARG;
+    // graphs of learned batches hold the cache's device pointers
+    ++c->generation;
     if (log2_entries == 0) {
         c->vcache.release();
         c->vcache_vis.release();
@@ -1469,10 +1473,13 @@ struct BatchPlan {
     size_t bvh_capacity = 0;
     size_t lds = 0;
     bool exact = false, shadows = false;
+    bool learned = false, cached = false;  // learned views; the visibility cache is refreshed first
+    rl::VisCacheView cache = {};
     // whether a graph captured with plan `o` can run this plan's descriptors
     bool fits(const BatchPlan& o) const {
         return tile_off == o.tile_off && block_off == o.block_off && samples == o.samples && lds <= o.lds &&
-               exact == o.exact && shadows == o.shadows && bvhs.size() <= o.bvh_capacity;
+               exact == o.exact && shadows == o.shadows && learned == o.learned && cached == o.cached &&
+               bvhs.size() <= o.bvh_capacity;
     }
 };
 
@@ -1486,10 +1493,24 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
         if (!w.visibility || !w.out) return fail(c, RISLTC_E_ARG, "view %u: null buffer", v);
         int e = check_settings(c, &w.constants, &w.settings);
         if (e) return e;
-        if (w.settings.proposal != RISLTC_PROPOSAL_UNIFORM)
-            return fail(c, RISLTC_E_UNSUPPORTED, "view %u: batches use the reference's target", v);
         if (w.settings.polygon_sampling_technique != RISLTC_SAMPLE_POLYGON_LTC_CP)
             return fail(c, RISLTC_E_UNSUPPORTED, "view %u: batches run sample_polygon_ltc_cp only", v);
+        const bool learned = w.settings.proposal != RISLTC_PROPOSAL_UNIFORM;
+        if (learned != (views[0].settings.proposal != RISLTC_PROPOSAL_UNIFORM))
+            return fail(c, RISLTC_E_UNSUPPORTED, "a batch is either all learned or all plain");
+        if (learned) {
+            if (w.light_set != 0 || w.scene != 0)
+                return fail(c, RISLTC_E_UNSUPPORTED, "view %u: learned views use light set 0 and scene 0", v);
+            if (w.settings.light_sampling != RISLTC_LIGHT_RESERVOIR)
+                return fail(c, RISLTC_E_UNSUPPORTED, "view %u: the learned pass is a RIS pass", v);
+            if (c->cluster_count == 0) return fail(c, RISLTC_E_STATE, "no light clusters");
+            if (w.settings.proposal == RISLTC_PROPOSAL_LEARNED && !w.proposal_table)
+                return fail(c, RISLTC_E_ARG, "view %u: no proposal table", v);
+            if (w.settings.proposal == RISLTC_PROPOSAL_CACHE && !c->vcache.p)
+                return fail(c, RISLTC_E_STATE, "no visibility cache");
+            if (c->vcache.p && c->vcache_per_light && c->vcache_slots != c->light_count)
+                return fail(c, RISLTC_E_STATE, "the visibility cache was made for %u lights", c->vcache_slots);
+        }
         if ((w.settings.exact_math != 0) != (views[0].settings.exact_math != 0))
             return fail(c, RISLTC_E_UNSUPPORTED, "a batch has one exact_math for all views");
         off[v] = p.samples;
@@ -1498,6 +1519,9 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
         p.shadows = p.shadows || w.settings.shadow_rays != 0;
     }
     p.exact = views[0].settings.exact_math != 0;
+    p.learned = views[0].settings.proposal != RISLTC_PROPOSAL_UNIFORM;
+    p.cached = p.learned && c->vcache.p != nullptr;
+    if (p.cached) p.cache = vcache_view(c);
     if (allocate) {
         HIPC(c, B.samples.ensure(p.samples * rl::kSampleWords * sizeof(float)));
         HIPC(c, B.rays.ensure(p.samples * 2 * sizeof(rl::ShadowRay)));
@@ -1528,12 +1552,21 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
         a.ray_slots = B.ray_slots.as<uint32_t>();
         a.ray_count = B.counter.as<uint32_t>();
         a.slot_base = (uint32_t)(off[v] * 2);
+        if (p.learned) {
+            a.learned_pass = 1u;
+            a.proposal = w.settings.proposal == RISLTC_PROPOSAL_LEARNED ? w.proposal_table : nullptr;
+            if (p.cached) a.cache = p.cache;
+        }
         p.lds = std::max(p.lds, rl::shade_lds_bytes(a));
         p.tile_off.push_back(p.tile_off.back() + rl::shade_tiles(a));
         p.shade.push_back(a);
         rl::ResolveArgs r;
         fill_resolve(&w.constants, &w.settings, w.visibility, a.samples, B.hits.as<uint32_t>() + off[v] * 2, w.out,
                      SR.triangle_count, r);
+        if (p.cached) {
+            r.cluster_of_light = c->cluster_of_light.as<uint8_t>();
+            r.cache = p.cache;
+        }
         const uint32_t P = w.constants.viewport_size[0] * w.constants.viewport_size[1];
         p.block_off.push_back(p.block_off.back() + (P + 255u) / 256u);
         p.resolve.push_back(r);
@@ -1566,8 +1599,9 @@ static int launch_plan(risltc_context_t* c, BatchBuffers& B, const BatchPlan& p,
     const uint32_t n = (uint32_t)p.shade.size();
     HIPC(c, rl::launch_zero_u32(B.counter.as<uint32_t>(), kCounterWords, st));
     if (timed) HIPC(c, hipEventRecord(c->ev[0], st));
+    if (p.cached) HIPC(c, rl::launch_refresh_cache(p.cache, st));
     HIPC(c, rl::launch_shade_batch(B.shade_args.as<rl::ShadeArgs>(), B.tile_offsets.as<uint32_t>(), n, p.tiles, p.lds,
-                                   p.exact, st));
+                                   p.exact, p.learned, st));
     if (timed) HIPC(c, hipEventRecord(c->ev[1], st));
     if (p.shadows)
         HIPC(c, rl::launch_trace(trace_args(c, B.bvh_table.as<rl::BvhView>(), B.rays.as<rl::ShadowRay>(),

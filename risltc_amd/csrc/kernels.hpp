@@ -246,7 +246,7 @@ uint32_t shade_tiles(const ShadeArgs& a);
 size_t shade_lds_bytes(const ShadeArgs& a);
 // views and their tile offsets (count + 1 entries) in device memory
 hipError_t launch_shade_batch(const ShadeArgs* d_views, const uint32_t* d_tile_offsets, uint32_t count,
-                              uint32_t total_tiles, size_t lds, bool exact, hipStream_t s);
+                              uint32_t total_tiles, size_t lds, bool exact, bool learned, hipStream_t s);
 hipError_t launch_resolve_batch(const ResolveArgs* d_views, const uint32_t* d_block_offsets, uint32_t count,
                                 uint32_t total_blocks, hipStream_t s);
 hipError_t launch_trace(const TraceArgs& a, hipStream_t s);

@@ -1944,14 +1944,14 @@ __device__ __forceinline__ uint32_t view_of(const uint32_t* offsets, uint32_t n,
 
 // Many views in one launch: work-group b shades tile b - tile_offsets[v] of
 // view v, with v's descriptor read from device memory (so a captured graph
-// can be replayed with new cameras and noise seeds).
-template <bool X>
+// can be replayed with new cameras and noise seeds). LEARNED: every view of
+// the launch runs the learned pass.
+template <bool X, bool LEARNED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-shade_batch_kernel(const ShadeArgs* views, const uint32_t* tile_offsets,
-                                                          uint32_t count) {
+shade_batch_kernel(const ShadeArgs* views, const uint32_t* tile_offsets, uint32_t count) {
     extern __shared__ float4 sh_spheres[];
     const uint32_t v = view_of(tile_offsets, count, blockIdx.x);
-    shade_tile<X, false, false>(views[v], blockIdx.x - tile_offsets[v], sh_spheres);
+    shade_tile<X, false, LEARNED>(views[v], blockIdx.x - tile_offsets[v], sh_spheres);
 }
 
 // ---- resolve: sum the estimates in the reference's order -------------------
@@ -2123,14 +2123,19 @@ hipError_t launch_shade_baseline(const ShadeArgs& a, uint32_t technique, bool ex
 }
 
 hipError_t launch_shade_batch(const ShadeArgs* d_views, const uint32_t* d_tile_offsets, uint32_t count,
-                              uint32_t total_tiles, size_t lds, bool exact, hipStream_t s) {
+                              uint32_t total_tiles, size_t lds, bool exact, bool learned, hipStream_t s) {
     if (count == 0 || total_tiles == 0) return hipSuccess;
-    if (exact)
-        hipLaunchKernelGGL(shade_batch_kernel<true>, dim3(total_tiles), dim3(256), lds, s, d_views, d_tile_offsets,
-                           count);
-    else
-        hipLaunchKernelGGL(shade_batch_kernel<false>, dim3(total_tiles), dim3(256), lds, s, d_views, d_tile_offsets,
-                           count);
+    dim3 grid(total_tiles), block(256);
+    if (learned) {
+        if (exact)
+            hipLaunchKernelGGL((shade_batch_kernel<true, true>), grid, block, lds, s, d_views, d_tile_offsets, count);
+        else
+            hipLaunchKernelGGL((shade_batch_kernel<false, true>), grid, block, lds, s, d_views, d_tile_offsets, count);
+    } else if (exact) {
+        hipLaunchKernelGGL((shade_batch_kernel<true, false>), grid, block, lds, s, d_views, d_tile_offsets, count);
+    } else {
+        hipLaunchKernelGGL((shade_batch_kernel<false, false>), grid, block, lds, s, d_views, d_tile_offsets, count);
+    }
     return hipGetLastError();
 }
 

@@ -103,9 +103,13 @@ def test_batch_rejects_bad_views(batch):
     bad = ctx.view(c, s, vis, out, light_set=7)
     with pytest.raises(risltc_amd.RisltcError, match="E_ARG"):
         ctx.shade_batch([bad])
+    # learned views need clusters (this context has none) and cannot be mixed
+    # with plain views
     learned = ctx.view(c, batch["scene"].settings(proposal=abi.PROPOSAL_LEARNED), vis, out)
-    with pytest.raises(risltc_amd.RisltcError, match="E_UNSUPPORTED"):
+    with pytest.raises(risltc_amd.RisltcError, match="E_STATE"):
         ctx.shade_batch([learned])
+    with pytest.raises(risltc_amd.RisltcError, match="E_UNSUPPORTED"):
+        ctx.shade_batch([batch["views"][0], learned])
     exact = ctx.view(c, batch["scene"].settings(exact_math=1), vis, out)
     with pytest.raises(risltc_amd.RisltcError, match="E_UNSUPPORTED"):
         ctx.shade_batch([batch["views"][1], exact])
@@ -285,3 +289,98 @@ def test_batch_with_several_meshes(small_scene, ltc_table):
     finally:
         for c_ in (ctx, ref_b, ref_ab):
             c_.close()
+
+
+def test_learned_batch_equals_single_views(small_scene, ltc_table):
+    """Learned views (MLP source with per-view tables, and the cache-only
+    mode) in one batch and in a graph: each view's frame equals its
+    risltc_shade call from the same cache state bit for bit, and the batch
+    adds to the shared visibility cache exactly the counts of the single
+    calls together."""
+    import risltc_amd
+    import torch
+    sc = small_scene
+    ctx = risltc_amd.Context(0)
+    try:
+        ctx.load_scene(sc, *ltc_table)
+        ctx.cluster_lights(16, 10)
+        ctx.proposal_init(3)
+        ctx.proposal_cache(12, 1.0, 1.0, per_light=True)
+        specs = view_specs(sc)
+        shapes = [(specs[0][0], 256, 256, 2, 8), (specs[1][0], 200, 120, 1, 16), (specs[2][0], 97, 61, 2, 4)]
+        bufs = []
+        for i, (cam, w, h, spp, M) in enumerate(shapes):
+            c = constants(ctx, sc, cam, w, h, seed=i + 1)
+            vis = torch.empty(w * h, dtype=torch.int32, device="cuda")
+            ctx.render_visibility(c, vis)
+            bufs.append((c, vis, w, h, spp, M))
+        # warm the MLP and the cache on view 0
+        c0, vis0, w0, h0, spp0, M0 = bufs[0]
+        tmp = torch.empty(w0 * h0 * 4, dtype=torch.float32, device="cuda")
+        s_warm = sc.settings(proposal=abi.PROPOSAL_LEARNED, sample_count_light=spp0, ris_candidate_count=M0)
+        for f in range(3):
+            cf = constants(ctx, sc, specs[0][0], w0, h0, seed=50 + f)
+            ctx.proposal_update(cf, vis0)
+            ctx.shade(cf, s_warm, vis0, tmp)
+            ctx.proposal_train(cf, vis0, batch=2048, steps=1, lr=1e-2, seed=f)
+        torch.cuda.synchronize()
+        for mode in (abi.PROPOSAL_LEARNED, abi.PROPOSAL_CACHE):
+            print("mode", mode, flush=True)
+            views, outs, tables, settings = [], [], [], []
+            for c, vis, w, h, spp, M in bufs:
+                st = sc.settings(proposal=mode, sample_count_light=spp, ris_candidate_count=M, learned_floor_log2=3)
+                tab = None
+                if mode == abi.PROPOSAL_LEARNED:
+                    tab = torch.empty(w * h * 32, dtype=torch.uint8, device="cuda")
+                    ctx.proposal_update(c, vis, table=tab)
+                out = torch.full((w * h * 4,), float("nan"), dtype=torch.float32, device="cuda")
+                views.append(ctx.view(c, st, vis, out, proposal_table=tab))
+                outs.append(out)
+                tables.append(tab)
+                settings.append(st)
+            torch.cuda.synchronize()
+            counts0 = ctx.proposal_cache_read()
+            ctx.shade_batch(views)
+            torch.cuda.synchronize()
+            counts_b = ctx.proposal_cache_read()
+            inc = np.zeros(counts0.shape, dtype=np.int64)
+            refs = []
+            for (c, vis, w, h, spp, M), st in zip(bufs, settings):
+                ctx.proposal_cache_write(counts0)
+                if mode == abi.PROPOSAL_LEARNED:
+                    ctx.proposal_update(c, vis)
+                ref = torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
+                ctx.shade(c, st, vis, ref)
+                torch.cuda.synchronize()
+                inc += ctx.proposal_cache_read().astype(np.int64) - counts0
+                refs.append(ref)
+            for i, (out, ref) in enumerate(zip(outs, refs)):
+                assert np.array_equal(bits(out), bits(ref)), (mode, i)
+                o = out.cpu().numpy().reshape(-1, 4)[:, :3]
+                assert np.isfinite(o).all() and (o > 0).mean() > 0.05
+            assert inc.sum() > 0
+            assert np.array_equal(counts_b.astype(np.int64), counts0 + inc)
+            # the same views as a graph: replays from the same cache state
+            g = ctx.graph(views)
+            try:
+                for rep in range(3):
+                    ctx.proposal_cache_write(counts0)
+                    for out in outs:
+                        out.fill_(float("nan"))
+                    g.launch()
+                    torch.cuda.synchronize()
+                    for i, (out, ref) in enumerate(zip(outs, refs)):
+                        assert np.array_equal(bits(out), bits(ref)), (mode, rep, i)
+                    assert np.array_equal(ctx.proposal_cache_read(), counts_b)
+                    # eager work between replays (a plain frame, which also counts into the cache)
+                    ctx.shade(bufs[0][0], sc.settings(sample_count_light=2), bufs[0][1], tmp)
+                    torch.cuda.synchronize()
+                # a new cache invalidates the graph
+                ctx.proposal_cache(12, 1.0, 1.0, per_light=True)
+                with pytest.raises(risltc_amd.RisltcError, match="E_STATE"):
+                    g.launch()
+                ctx.proposal_cache_write(counts_b)
+            finally:
+                g.close()
+    finally:
+        ctx.close()
