@@ -23,9 +23,15 @@ resolutions from 320x240 to 1920x1080, M in {8, 16, 32}, 2 spp; 16 views by
 default, --batch-views 64 for the full configuration), shaded by one call per
 view, by one risltc_shade_batch, and by a hipGraph replay of that batch.
 
+The learned_batch object: the learned pass in one batch at the large end
+(16 views of 1920x1080 around a ~4M-triangle city with 16384 lights, 16 spp,
+per-view proposal tables, the shared visibility cache), against the plain
+batch of the same views, and as a graph replay.
+
 The launch shape is autotuned (risltc_autotune) on the frame before timing.
 
   python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-learned] [--no-batch]
+                  [--no-learned-batch]
 """
 import argparse
 import json
@@ -133,6 +139,114 @@ def bench_batch(ctx, ltc_rgba, ltc_rg, steps, warmup, count):
             "msamples_per_s": {"per_view_calls": rate(ms_views), "batch": rate(ms_batch), "graph": rate(ms_graph)}}
 
 
+def bench_learned_batch(ltc_rgba, ltc_rg, steps, warmup, views_n=16, grid=540, lights=16384, spp=16):
+    """The learned pass in one batch at the large end: views_n views at
+    1920x1080 orbiting a procedural city of ~4M triangles with `lights`
+    3-8-gon lights, M = 32, `spp` samples per pixel, shaded as one plain batch
+    and as one learned batch (MLP source with per-view tables + visibility
+    cache, M = 24) and as a graph replay of the learned batch. The MLP is the
+    96-64-64-64-256 network over 256 k-means clusters (the kernels' limits;
+    the configuration asks for width 128 over 1024 clusters), warmed by a few
+    trained single-view frames. Wall time per step; no convergence run (a
+    65536-spp reference of 16 views is out of this bench's time)."""
+    import math
+    import torch
+    import risltc_amd
+    from risltc_amd import abi, synth, convergence
+    t0 = time.time()
+    sc = synth.city_scene(seed=400, light_count=lights, grid=grid, spp=spp, candidates=32)
+    ctx = risltc_amd.Context(0)
+    ctx.load_scene(sc, ltc_rgba, ltc_rg)
+    lk = convergence.DEFAULT_LEARNED
+    ctx.cluster_lights(lk["clusters"], 20)
+    ctx.proposal_init(lk["seed"])
+    # a per-light cache of 16384 lights x 2^16 cells would take 8 GiB: the
+    # cache keeps one slot per cluster here
+    per_light = lk["cache_per_light"] and lights <= 4096
+    ctx.proposal_cache(lk["cache_log2"], lk["cache_cell"], lk["cache_prior"], per_light)
+    centre = np.array([0.0, 40.0, 10.0])
+    W, H = sc.width, sc.height
+    bufs = []
+    for v in range(views_n):
+        phi = 2.0 * math.pi * v / views_n
+        pos = (centre[0] + 100.0 * math.sin(phi), centre[1] - 100.0 * math.cos(phi), 35.0)
+        best = None
+        for rz in (math.pi + phi, math.pi - phi):
+            cam = synth.make_camera(pos, rz, math.pi / 2 - 0.3)
+            c = risltc_amd.write_frame_constants(cam, W, H, sc.mesh.dequantization_factor,
+                                                 sc.mesh.dequantization_summand, ctx.ltc_constants)
+            P = np.array(c.pixel_to_ray_direction_world_space, dtype=np.float64)
+            d = P[:, 0] * (W / 2) + P[:, 1] * (H / 2) + P[:, 2]
+            score = d @ (centre - np.array(pos)) / np.linalg.norm(d)
+            if best is None or score > best[0]:
+                best = (score, cam)
+        cam = best[1]
+        vis = torch.empty(W * H, dtype=torch.int32, device="cuda")
+        out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
+        tab = torch.empty(W * H * 32, dtype=torch.uint8, device="cuda")
+        bufs.append([cam, vis, out, tab])
+        ctx.render_visibility(risltc_amd.write_frame_constants(cam, W, H, sc.mesh.dequantization_factor,
+                                                               sc.mesh.dequantization_summand, ctx.ltc_constants),
+                              vis)
+    plain_s = sc.settings()
+    learned_s = sc.settings(proposal=abi.PROPOSAL_LEARNED, ris_candidate_count=lk["candidates"],
+                            learned_floor_log2=lk["floor_log2"], skip_training_records=1)
+    # warm the MLP and the cache on single-view frames of view 0
+    cam0, vis0 = bufs[0][0], bufs[0][1]
+    for f in range(4):
+        c = risltc_amd.write_frame_constants(cam0, W, H, sc.mesh.dequantization_factor,
+                                             sc.mesh.dequantization_summand, ctx.ltc_constants, noise_seed=f,
+                                             animate=True)
+        ctx.proposal_update(c, vis0)
+        ctx.shade(c, sc.settings(proposal=abi.PROPOSAL_LEARNED, ris_candidate_count=lk["candidates"],
+                                 learned_floor_log2=lk["floor_log2"]), vis0, bufs[0][2])
+        ctx.proposal_train(c, vis0, batch=lk["batch"], steps=1, lr=lk["lr"], seed=f)
+    plain, learned = [], []
+    for cam, vis, out, tab in bufs:
+        c = risltc_amd.write_frame_constants(cam, W, H, sc.mesh.dequantization_factor,
+                                             sc.mesh.dequantization_summand, ctx.ltc_constants, noise_seed=9,
+                                             animate=True)
+        ctx.proposal_update(c, vis, table=tab)
+        plain.append(ctx.view(c, plain_s, vis, out))
+        learned.append(ctx.view(c, learned_s, vis, out, proposal_table=tab))
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+    samples = views_n * W * H * spp
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) * 1e3 / steps
+
+    ms_plain = timed(lambda: ctx.shade_batch(plain))
+    k_plain = ctx.last_timings()
+    ms_learned = timed(lambda: ctx.shade_batch(learned))
+    k_learned = ctx.last_timings()
+    g = ctx.graph(learned)
+    try:
+        ms_graph = timed(g.launch)
+    finally:
+        g.close()
+    ctx.close()
+    rate = lambda ms: round(samples / (ms * 1e-3) / 1e6, 2)
+    km = lambda k: {"shade": round(k[0], 2), "trace": round(k[1], 2), "resolve": round(k[2], 2)}
+    return {"views": views_n, "resolution": [W, H], "triangles": int(sc.mesh.triangle_count), "lights": lights,
+            "clusters": lk["clusters"], "mlp": "96-64-64-64-256", "spp": spp,
+            "cache_slots": "light" if per_light else "cluster",
+            "candidates": {"plain": 32, "learned": lk["candidates"]}, "samples_per_step": samples,
+            "setup_s": round(setup_s, 1),
+            "ms_per_step": {"plain_batch": round(ms_plain, 2), "learned_batch": round(ms_learned, 2),
+                            "learned_graph": round(ms_graph, 2)},
+            "batch_kernel_ms": {"plain": km(k_plain), "learned": km(k_learned)},
+            "msamples_per_s": {"plain_batch": rate(ms_plain), "learned_batch": rate(ms_learned),
+                               "learned_graph": rate(ms_graph)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +260,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-plain time-to-relMSE experiment")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view measurement")
+    ap.add_argument("--no-learned-batch", action="store_true",
+                    help="skip the learned batch at the large end (16 x 1080p views, ~4M triangles, 16384 lights)")
     ap.add_argument("--batch-views", type=int, default=16,
                     help="views in the batched measurement (the batched configuration has 64)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
@@ -249,6 +365,11 @@ def main():
         batch = bench_batch(ctx, rgba, rg, args.steps, args.warmup, args.batch_views)
         print("batch: %s" % batch, file=sys.stderr, flush=True)
 
+    learned_batch = None
+    if not args.no_learned_batch and args.workload == "city" and not args.exact:
+        learned_batch = bench_learned_batch(rgba, rg, max(1, min(args.steps, 3)), 1)
+        print("learned batch: %s" % learned_batch, file=sys.stderr, flush=True)
+
     learned = None
     if not args.no_learned and args.workload == "city" and not args.exact:
         from risltc_amd import convergence
@@ -306,6 +427,7 @@ def main():
                      "kernel": "shade_kernel", "algorithmic_bytes": alg},
         "valu": valu,
         "batched_views": batch,
+        "learned_batch": learned_batch,
         "learned_vs_plain": learned,
         "cpu_baseline": cpu,
     }
