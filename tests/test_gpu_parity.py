@@ -315,7 +315,7 @@ def test_accumulate_running_mean(small_gpu):
     assert torch.allclose(acc, frames.mean(0), rtol=1e-5, atol=1e-6)
 
 
-BASELINE_CASES = [
+TECHNIQUE_CASES = [
     dict(polygon_sampling_technique=abi.SAMPLE_POLYGON_AREA_TURK),
     dict(polygon_sampling_technique=abi.SAMPLE_POLYGON_AREA_TURK, light_sampling=abi.LIGHT_UNIFORM),
     dict(polygon_sampling_technique=abi.SAMPLE_POLYGON_AREA_TURK, shadow_rays=0, ris_candidate_count=3),
@@ -326,7 +326,7 @@ BASELINE_CASES = [
 ]
 
 
-@pytest.mark.parametrize("case", range(len(BASELINE_CASES)))
+@pytest.mark.parametrize("case", range(len(TECHNIQUE_CASES)))
 def test_baseline_techniques_exact_bitwise(small_gpu, case):
     """The reference's comparison techniques (Turk area sampling and
     projected-solid-angle RIS) in the per-pixel baseline kernel: every
@@ -334,7 +334,7 @@ def test_baseline_techniques_exact_bitwise(small_gpu, case):
     FP32 oracle."""
     g = small_gpu
     kw = dict(exact_math=1, sample_count_light=2, ris_candidate_count=8)
-    kw.update(BASELINE_CASES[case])
+    kw.update(TECHNIQUE_CASES[case])
     print("case", kw, flush=True)
     s = g["scene"].settings(**kw)
     out, rec, cand = _shade(g, s)
