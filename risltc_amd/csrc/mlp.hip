@@ -214,85 +214,106 @@ __global__ void __launch_bounds__(64 * kPropWaves) proposal_kernel(ProposalArgs 
     uint16_t* act = sh.act_in[wave];
     uint16_t* h0 = sh.act_h[wave][0];
     uint16_t* h1 = sh.act_h[wave][1];
-    const uint32_t tiles = (A.pixel_count + 15u) / 16u;
-    for (uint32_t tile = blockIdx.x * kPropWaves + wave; tile < tiles; tile += gridDim.x * kPropWaves) {
-        const uint32_t pixel = tile * 16u + (uint32_t)r;
-        const bool in = pixel < A.pixel_count;
-        bool shaded = false;
-        if (g == 0) {
+    // a wave takes 64 pixels at a time: every lane encodes the features of
+    // one (packed to bf16 pairs in registers), then the MLP runs on the four
+    // 16-pixel sub-tiles in turn, each staged in LDS by its 16 lanes
+    const uint32_t groups = (A.pixel_count + 63u) / 64u;
+    for (uint32_t grp = blockIdx.x * kPropWaves + wave; grp < groups; grp += gridDim.x * kPropWaves) {
+        const uint32_t my_pixel = grp * 64u + (uint32_t)lane;
+        const bool my_in = my_pixel < A.pixel_count;
+        uint32_t packed[kMlpIn / 2];
+        bool my_shaded;
+        {
             float f[kMlpIn];
-            shaded = in && encode_features(A.f, pixel, f);
+            my_shaded = my_in && encode_features(A.f, my_pixel, f);
+            if (!my_in) {
 #pragma unroll
-            for (int k = 0; k < kMlpIn; k += 2)
-                *reinterpret_cast<uint32_t*>(act + r * kLdW1 + k) = bf16_pack2(f[k], f[k + 1]);
-            if (A.features && in)
-                for (int k = 0; k < kMlpIn; ++k) A.features[(size_t)pixel * kMlpIn + k] = act[r * kLdW1 + k];
+                for (int k = 0; k < kMlpIn; ++k) f[k] = 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < kMlpIn; k += 2) packed[k / 2] = bf16_pack2(f[k], f[k + 1]);
         }
-        // every lane of the pixel's column learns whether it is shaded
-        shaded = __shfl(shaded ? 1 : 0, r) != 0;
-        wave_sync();
-        hidden_layer<kMlpIn, kLdW1, kLdW1>(sh.w1, sh.bias, act, h0, lane);
-        wave_sync();
-        hidden_layer<kMlpHid, kLdH, kLdH>(sh.w2, sh.bias + kMlpHid, h0, h1, lane);
-        wave_sync();
-        hidden_layer<kMlpHid, kLdH, kLdH>(sh.w3, sh.bias + 2 * kMlpHid, h1, h0, lane);
-        wave_sync();
-        // output layer: this lane keeps logits of clusters 16 t + 4 g + i of pixel r,
-        // and the 4 largest of them
-        float bv[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-        int bi[4] = {0, 0, 0, 0};
+        if (A.features && my_in) {
+            uint32_t* fo = reinterpret_cast<uint32_t*>(A.features + (size_t)my_pixel * kMlpIn);
+#pragma unroll
+            for (int k = 0; k < kMlpIn / 2; ++k) fo[k] = packed[k];
+        }
+#pragma unroll 1
+        for (int sub = 0; sub < 4; ++sub) {
+            if (g == sub) {
+#pragma unroll
+                for (int k = 0; k < kMlpIn / 2; k += 4)
+                    *reinterpret_cast<uint4*>(act + r * kLdW1 + 2 * k) =
+                        make_uint4(packed[k], packed[k + 1], packed[k + 2], packed[k + 3]);
+            }
+            const uint32_t pixel = grp * 64u + (uint32_t)(16 * sub + r);
+            const bool in = pixel < A.pixel_count;
+            // every lane of the pixel's column learns whether it is shaded
+            const bool shaded = __shfl(my_shaded ? 1 : 0, 16 * sub + r) != 0;
+            wave_sync();
+            hidden_layer<kMlpIn, kLdW1, kLdW1>(sh.w1, sh.bias, act, h0, lane);
+            wave_sync();
+            hidden_layer<kMlpHid, kLdH, kLdH>(sh.w2, sh.bias + kMlpHid, h0, h1, lane);
+            wave_sync();
+            hidden_layer<kMlpHid, kLdH, kLdH>(sh.w3, sh.bias + 2 * kMlpHid, h1, h0, lane);
+            wave_sync();
+            // output layer: this lane keeps logits of clusters 16 t + 4 g + i of pixel r,
+            // and the 4 largest of them
+            float bv[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+            int bi[4] = {0, 0, 0, 0};
 #pragma unroll 2
-        for (int t = 0; t < kMlpOut / 16; ++t) {
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int t = 0; t < kMlpOut / 16; ++t) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int ks = 0; ks < kMlpHid / 32; ++ks)
-                acc = mfma(frag(sh.w4 + (t * 16 + r) * kLdH + ks * 32 + 8 * g), frag(h0 + r * kLdH + ks * 32 + 8 * g),
-                           acc);
+                for (int ks = 0; ks < kMlpHid / 32; ++ks)
+                    acc = mfma(frag(sh.w4 + (t * 16 + r) * kLdH + ks * 32 + 8 * g),
+                               frag(h0 + r * kLdH + ks * 32 + 8 * g), acc);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                int cl = t * 16 + 4 * g + i;
-                float z = acc[i] + sh.bias[3 * kMlpHid + cl];
-                if (A.logits && in) A.logits[(size_t)pixel * kMlpOut + cl] = z;
-                if (sh.csize[cl] == 0u) continue;
-                // insertion into the sorted best-4 (ties keep the smaller id)
-                if (z > bv[3]) {
-                    bv[3] = z;
-                    bi[3] = cl;
+                for (int i = 0; i < 4; ++i) {
+                    int cl = t * 16 + 4 * g + i;
+                    float z = acc[i] + sh.bias[3 * kMlpHid + cl];
+                    if (A.logits && in) A.logits[(size_t)pixel * kMlpOut + cl] = z;
+                    if (sh.csize[cl] == 0u) continue;
+                    // insertion into the sorted best-4 (ties keep the smaller id)
+                    if (z > bv[3]) {
+                        bv[3] = z;
+                        bi[3] = cl;
 #pragma unroll
-                    for (int s = 3; s > 0; --s)
-                        if (bv[s] > bv[s - 1]) {
-                            float tv = bv[s];
-                            bv[s] = bv[s - 1];
-                            bv[s - 1] = tv;
-                            int ti = bi[s];
-                            bi[s] = bi[s - 1];
-                            bi[s - 1] = ti;
-                        }
+                        for (int s = 3; s > 0; --s)
+                            if (bv[s] > bv[s - 1]) {
+                                float tv = bv[s];
+                                bv[s] = bv[s - 1];
+                                bv[s - 1] = tv;
+                                int ti = bi[s];
+                                bi[s] = bi[s - 1];
+                                bi[s - 1] = ti;
+                            }
+                    }
                 }
             }
-        }
-        // the pixel's maximum over its four lanes (lanes r, r+16, r+32, r+48)
-        float zmax = bv[0];
-        zmax = fmax_(zmax, __shfl_xor(zmax, 16));
-        zmax = fmax_(zmax, __shfl_xor(zmax, 32));
-        uint32_t ids = 0, ws = 0;
+            // the pixel's maximum over its four lanes (lanes r, r+16, r+32, r+48)
+            float zmax = bv[0];
+            zmax = fmax_(zmax, __shfl_xor(zmax, 16));
+            zmax = fmax_(zmax, __shfl_xor(zmax, 32));
+            uint32_t ids = 0, ws = 0;
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            uint32_t w = 0;
-            if (shaded && bv[s] > -__builtin_inff()) {
-                float e = __expf(bv[s] - zmax) * 255.0f + 0.5f;
-                w = e >= 255.0f ? 255u : (uint32_t)e;
+            for (int s = 0; s < 4; ++s) {
+                uint32_t w = 0;
+                if (shaded && bv[s] > -__builtin_inff()) {
+                    float e = __expf(bv[s] - zmax) * 255.0f + 0.5f;
+                    w = e >= 255.0f ? 255u : (uint32_t)e;
+                }
+                if (w == 0u) bi[s] = 0;
+                ids |= (uint32_t)bi[s] << (8 * s);
+                ws |= w << (8 * s);
             }
-            if (w == 0u) bi[s] = 0;
-            ids |= (uint32_t)bi[s] << (8 * s);
-            ws |= w << (8 * s);
+            if (in) {
+                uint32_t* o = reinterpret_cast<uint32_t*>(A.out + (size_t)pixel * kPropBytes);
+                o[g] = ids;
+                o[4 + g] = ws;
+            }
+            wave_sync();
         }
-        if (in) {
-            uint32_t* o = reinterpret_cast<uint32_t*>(A.out + (size_t)pixel * kPropBytes);
-            o[g] = ids;
-            o[4 + g] = ws;
-        }
-        wave_sync();
     }
 }
 
@@ -301,8 +322,8 @@ hipError_t launch_proposal(const ProposalArgs& a, hipStream_t s) {
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    uint32_t tiles = (a.pixel_count + 15u) / 16u;
-    uint32_t blocks = (tiles + kPropWaves - 1u) / kPropWaves;
+    uint32_t groups = (a.pixel_count + 63u) / 64u;
+    uint32_t blocks = (groups + kPropWaves - 1u) / kPropWaves;
     uint32_t cap = (uint32_t)(cus > 0 ? cus : 1);
     if (blocks > cap) blocks = cap;
     hipLaunchKernelGGL(proposal_kernel, dim3(blocks), dim3(64 * kPropWaves), 0, s, a);
