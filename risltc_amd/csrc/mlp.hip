@@ -444,27 +444,72 @@ __global__ void batch_sum_kernel(const float* y, uint32_t B, float* out) {
     if (threadIdx.x == 0) *out = part[0];
 }
 
-// dL/dz = (y_b / Y) (softmax(z_b) - onehot(c_b)); per-row loss terms
-__global__ void softmax_grad_kernel(const float* Z, const uint32_t* target, const float* y, const float* ysum,
-                                    uint32_t B, uint16_t* G, uint16_t* Gt, float* row_loss) {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
-    const float* z = Z + (size_t)b * kMlpOut;
-    float zm = -__builtin_inff();
-    for (int n = 0; n < kMlpOut; ++n) zm = fmax_(zm, z[n]);
-    float s = 0.0f;
-    for (int n = 0; n < kMlpOut; ++n) s += __expf(z[n] - zm);
+// dL/dz = (y_b / Y) (softmax(z_b) - onehot(c_b)); per-row loss terms. A
+// work-group takes 64 rows, one wave per row at a time (each lane 4 of the
+// 256 logits, coalesced), and writes the transposed gradient Gt through LDS
+// so that each lane stores one 128-byte line per column.
+constexpr int kSgRows = 64;
+constexpr int kSgLd = kMlpOut + 8;  // u16 row stride of the LDS tile
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax_(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ void __launch_bounds__(256) softmax_grad_kernel(const float* Z, const uint32_t* target, const float* y,
+                                                           const float* ysum, uint32_t B, uint16_t* G, uint16_t* Gt,
+                                                           float* row_loss) {
+    static_assert(kMlpOut == 256, "four logits per lane");
+    __shared__ uint16_t tile[kSgRows * kSgLd];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t b0 = blockIdx.x * kSgRows;  // B is a multiple of kSgRows
     const float Y = *ysum;
-    const float wb = Y > 0.0f ? y[b] / Y : 0.0f;
-    const uint32_t c = target[b];
-    for (int n = 0; n < kMlpOut; ++n) {
-        float p = __expf(z[n] - zm) / s;
-        float d = wb * (p - ((uint32_t)n == c ? 1.0f : 0.0f));
-        uint16_t h = bf16_bits(d);
-        G[(size_t)b * kMlpOut + n] = h;
-        Gt[(size_t)n * B + b] = h;
+    for (uint32_t rr = 0; rr < kSgRows / 4; ++rr) {
+        const uint32_t r = wave * (kSgRows / 4) + rr, b = b0 + r;
+        const float* z = Z + (size_t)b * kMlpOut;
+        float v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = z[lane + 64u * k];
+        const float zm = wave_max(fmax_(fmax_(v[0], v[1]), fmax_(v[2], v[3])));
+        float e[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) e[k] = __expf(v[k] - zm);
+        const float s = wave_sum((e[0] + e[1]) + (e[2] + e[3]));
+        const float wb = Y > 0.0f ? y[b] / Y : 0.0f;
+        const uint32_t c = target[b];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t n = lane + 64u * k;
+            float d = wb * (e[k] / s - (n == c ? 1.0f : 0.0f));
+            uint16_t h = bf16_bits(d);
+            G[(size_t)b * kMlpOut + n] = h;
+            tile[r * kSgLd + n] = h;
+        }
+        // z[c] from the lane that holds it (c is the same for the whole wave)
+        float vc = v[0];
+#pragma unroll
+        for (int k = 1; k < 4; ++k)
+            if ((c >> 6) == (uint32_t)k) vc = v[k];
+        const float zc = __shfl(vc, (int)(c & 63u));
+        if (lane == 0) row_loss[b] = wb > 0.0f ? -wb * (zc - zm - __logf(s)) : 0.0f;
     }
-    row_loss[b] = wb > 0.0f ? -wb * (z[c] - zm - __logf(s)) : 0.0f;
+    __syncthreads();
+    // Gt[n][b0 .. b0 + 63]: thread n writes its column as 8 x 16 bytes
+    const uint32_t n = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < kSgRows; q += 8) {
+        uint32_t w[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            w[j] = (uint32_t)tile[(q + 2 * j) * kSgLd + n] | ((uint32_t)tile[(q + 2 * j + 1) * kSgLd + n] << 16);
+        *reinterpret_cast<uint4*>(Gt + (size_t)n * B + b0 + q) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
 }
 
 // grad[w] = sum over slices (fixed order); bias grads = column sums of G
@@ -625,7 +670,7 @@ hipError_t launch_train_step(const TrainArgs& a, hipStream_t s) {
     gemm<kEpiHidden>(g, 1, s);
     g = GemmArgs{t.H3, a.wf + kPW4, B, kMlpOut, kMlpHid, kMlpHid, kMlpHid, kMlpHid, a.params + kPB4, nullptr, nullptr, t.Z, nullptr};
     gemm<kEpiLogits>(g, 1, s);
-    hipLaunchKernelGGL(softmax_grad_kernel, dim3(B / 256u), dim3(256), 0, s, t.Z, t.target, t.y, t.ysum, B, t.G4,
+    hipLaunchKernelGGL(softmax_grad_kernel, dim3(B / kSgRows), dim3(256), 0, s, t.Z, t.target, t.y, t.ysum, B, t.G4,
                        t.G4t, t.row_loss);
     hipLaunchKernelGGL(loss_sum_kernel, dim3(1), dim3(256), 0, s, t.row_loss, B, a.loss);
     // layer 4: dW4[o][i] = sum_b G4[b][o] H3[b][i]; dH3 = G4 . W4, masked by H3 > 0
