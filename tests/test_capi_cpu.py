@@ -175,3 +175,40 @@ def test_null_context_and_argument_errors(hiplib):
     hiplib.risltc_default_settings(C.byref(s))
     d = abi.default_settings()
     assert bytes(s) == bytes(d)
+
+
+def test_header_layouts_match_python(tmp_path):
+    """The ctypes mirror (risltc_amd/abi.py, risltc_amd/formats.py) has the
+    sizes and field offsets that the C compiler gives the structs of
+    include/risltc/risltc.h."""
+    import shutil
+    from risltc_amd import formats
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    structs = {
+        "risltc_frame_constants_t": abi.FrameConstants, "risltc_settings_t": abi.Settings,
+        "risltc_view_t": abi.View, "risltc_tuning_t": abi.Tuning, "risltc_camera_t": abi.Camera,
+        "risltc_polygonal_light_t": abi.PolygonalLight, "risltc_material_t": abi.Material,
+        "risltc_ltc_constants_t": abi.LtcConstants, "risltc_vks_info_t": formats.VksInfo,
+        "risltc_quicksave_info_t": formats.QuicksaveInfo,
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "risltc/risltc.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f in py._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, f[0], cname, f[0]))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], stdout=subprocess.PIPE, check=True).stdout.decode().split("\n")
+    got = {}
+    for line in out:
+        if line:
+            a, b, v = line.split()
+            got[(a, b)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f in py._fields_:
+            assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])
