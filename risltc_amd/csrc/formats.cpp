@@ -18,6 +18,8 @@ namespace {
 constexpr uint32_t kVksMarker = 0xabcabcu;  // src/scene.c:423
 constexpr uint32_t kVksVersion = 1u;        // src/scene.c:423
 constexpr uint32_t kVksEndMarker = 0xE0FE0Fu;  // src/scene.c:478
+constexpr uint64_t kVksMaxTriangles = 1ull << 36;
+constexpr uint64_t kVksMaxMaterials = 1ull << 24;
 // Bytes of one light record in a quicksave: 20 floats and 2 uint32
 // (POLYGONAL_LIGHT_QUICKSAVE_SIZE, src/polygonal_light.h).
 constexpr size_t kQuicksaveLightBytes = 20 * sizeof(float) + 2 * sizeof(uint32_t);
@@ -80,6 +82,9 @@ extern "C" int risltc_load_vks(const char* path, risltc_vks_info_t* info, uint32
         fprintf(stderr, "The scene file at path %s is completely empty, i.e. it holds 0 triangles.\n", path);
         return RISLTC_E_FORMAT;
     }
+    // counts no file of this format can hold (the byte sizes below would
+    // overflow): a corrupt header
+    if (info->triangle_count > kVksMaxTriangles || info->material_count > kVksMaxMaterials) return RISLTC_E_FORMAT;
     // Material names: uint64 length, then length bytes and a terminating NUL.
     bool want_mesh = positions || normals_and_tex_coords || material_indices;
     uint64_t name_bytes = 0;

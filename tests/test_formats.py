@@ -95,6 +95,14 @@ def test_vks_rejects_malformed(hiplib, tmp_path, kw, code):
     assert _err(formats.load_vks, p) == code
 
 
+def test_vks_rejects_impossible_counts(hiplib, tmp_path):
+    data = bytearray(_hand_vks()[0])
+    data[16:24] = struct.pack("<Q", 1 << 62)  # triangle count
+    p = tmp_path / "huge.vks"
+    p.write_bytes(bytes(data))
+    assert _err(formats.load_vks, p) == 3
+
+
 def test_vks_missing_file(hiplib, tmp_path):
     assert _err(formats.load_vks, tmp_path / "none.vks") == 6
 
