@@ -492,7 +492,8 @@ int risltc_shade_debug(risltc_context_t* context, const risltc_frame_constants_t
  * risltc_shade (learned), then optionally risltc_proposal_train on the
  * frame's own samples. ------------------------------------------------- */
 #define RISLTC_MAX_LIGHT_CLUSTERS 256
-#define RISLTC_PROPOSAL_PARAMS 31168 /* 96-64-64-64-256 MLP, FP32 */
+#define RISLTC_PROPOSAL_PARAMS 31168      /* 96-64-64-64-256 MLP (default), FP32 */
+#define RISLTC_PROPOSAL_PARAMS_WIDE 78464 /* 96-128-128-128-256 MLP, FP32 */
 /* Explicit light clusters: cluster_of_light[i] < cluster_count <= 256. */
 int risltc_set_light_clusters(risltc_context_t* context, const uint32_t* cluster_of_light,
                               uint32_t cluster_count);
@@ -501,9 +502,17 @@ int risltc_set_light_clusters(risltc_context_t* context, const uint32_t* cluster
  * risltc_set_light_clusters. out (host, may be NULL) receives the assignment. */
 int risltc_cluster_lights(risltc_context_t* context, uint32_t cluster_count, uint32_t iterations,
                           uint32_t* out);
+/* Hidden width of the MLP: 64 (default) or 128 (the 96-128-128-128-256
+ * network). Changing it discards the current network; call
+ * risltc_proposal_init or risltc_proposal_set_params next. Parameter arrays
+ * then hold risltc_proposal_param_count floats (RISLTC_PROPOSAL_PARAMS or
+ * RISLTC_PROPOSAL_PARAMS_WIDE), in the same order: every weight matrix
+ * [out][in] followed by its bias. */
+int risltc_proposal_set_width(risltc_context_t* context, uint32_t width);
+uint32_t risltc_proposal_param_count(const risltc_context_t* context);
 /* Deterministic initialisation from a seed; resets the Adam state. */
 int risltc_proposal_init(risltc_context_t* context, uint32_t seed);
-/* Host copies of the FP32 parameters (RISLTC_PROPOSAL_PARAMS floats, every
+/* Host copies of the FP32 parameters (risltc_proposal_param_count floats, every
  * weight matrix stored [out][in] followed by its bias) and of the last
  * training step's gradient; either pointer may be NULL. set resets Adam. */
 int risltc_proposal_set_params(risltc_context_t* context, const float* params);

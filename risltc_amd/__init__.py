@@ -78,6 +78,8 @@ SIGNATURES = {
     "risltc_add_light_set": (C.c_int, [_vp, _vp, _u32, _u32, _u32p]),
     "risltc_set_polygonal_lights": (C.c_int, [_vp, C.POINTER(abi.PolygonalLight), _u32, _u32]),
     "risltc_load_ltc_table": (C.c_int, [_vp, C.c_char_p, _u32]),
+    "risltc_proposal_set_width": (C.c_int, [_vp, _u32]),
+    "risltc_proposal_param_count": (_u32, [_vp]),
     "risltc_add_scene": (C.c_int, [_vp, _u32p, _u16p, _u8p, C.c_uint64, _f32p, _f32p, C.POINTER(abi.Material), _u32,
                                    _u32p]),
     "risltc_render_visibility_scene": (C.c_int, [_vp, _u32, C.POINTER(abi.FrameConstants), _vp, _vp]),
@@ -289,18 +291,28 @@ class Context:
                     "risltc_cluster_lights")
         return out[:n] if n is not None else out
 
-    def proposal_init(self, seed=0):
+    def proposal_init(self, seed=0, width=None):
+        """Initialises the MLP; width 64 (default) or 128 selects the hidden
+        width first (risltc_proposal_set_width)."""
+        if width is not None:
+            self.proposal_set_width(width)
         self._check(self.L.risltc_proposal_init(self.h, seed), "risltc_proposal_init")
+
+    def proposal_set_width(self, width):
+        self._check(self.L.risltc_proposal_set_width(self.h, width), "risltc_proposal_set_width")
+
+    def proposal_param_count(self):
+        return int(self.L.risltc_proposal_param_count(self.h))
 
     def proposal_set_params(self, params):
         p = np.ascontiguousarray(params, dtype=np.float32)
-        assert p.size == abi.PROPOSAL_PARAMS
+        assert p.size == self.proposal_param_count()
         self._check(self.L.risltc_proposal_set_params(self.h, _ptr(p, _f32p)), "risltc_proposal_set_params")
 
     def proposal_get_params(self):
         """(params, gradient of the last training step) as float32 arrays."""
-        p = np.zeros(abi.PROPOSAL_PARAMS, dtype=np.float32)
-        g = np.zeros(abi.PROPOSAL_PARAMS, dtype=np.float32)
+        p = np.zeros(self.proposal_param_count(), dtype=np.float32)
+        g = np.zeros(self.proposal_param_count(), dtype=np.float32)
         self._check(self.L.risltc_proposal_get_params(self.h, _ptr(p, _f32p), _ptr(g, _f32p)),
                     "risltc_proposal_get_params")
         return p, g

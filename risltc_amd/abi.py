@@ -21,6 +21,7 @@ PROPOSAL_UNIFORM, PROPOSAL_LEARNED, PROPOSAL_CACHE = 0, 1, 2
 BVH_SAH, BVH_LBVH = 0, 1
 MAX_LIGHT_CLUSTERS = 256
 PROPOSAL_PARAMS = 31168
+PROPOSAL_PARAMS_WIDE = 78464
 
 # risltc_sample_record_t word offsets
 REC_LIGHT, REC_WSUM, REC_PHAT, REC_W, REC_CLIPD, REC_CLIPS, REC_PSAD, REC_PSAS = range(8)

@@ -24,6 +24,7 @@ static_assert(sizeof(risltc_ltc_constants_t) == 32, "ltc_constants_t is 32 bytes
 static_assert(sizeof(risltc_sample_record_t) == 4 * RISLTC_SAMPLE_RECORD_FLOATS, "sample record size");
 static_assert(sizeof(risltc_candidate_record_t) == 4 * RISLTC_CANDIDATE_RECORD_WORDS, "candidate record size");
 static_assert(rl::kMlpParams == RISLTC_PROPOSAL_PARAMS, "MLP parameter count");
+static_assert(rl::mlp_layout(rl::kMlpHidWide).params == RISLTC_PROPOSAL_PARAMS_WIDE, "wide MLP parameter count");
 static_assert(rl::kMlpOut == RISLTC_MAX_LIGHT_CLUSTERS, "MLP outputs = clusters");
 
 struct DevBuf {
@@ -101,6 +102,7 @@ struct risltc_context_s {
     uint32_t cluster_count = 0;
     DevBuf mlp_params, mlp_m, mlp_v, mlp_grad, mlp_wf, mlp_wb, mlp_loss, train_scratch;
     bool mlp_ready = false;
+    rl::MlpLayout mlp_L = rl::mlp_layout(rl::kMlpHid);  // hidden width and parameter offsets
     uint32_t adam_step = 0;
     DevBuf prop_table, rec_cluster, rec_y;
     uint32_t prop_width = 0, prop_height = 0;
@@ -905,7 +907,8 @@ static uint32_t mix32(uint32_t x) {
 }
 
 static int mlp_alloc(risltc_context_t* c) {
-    const size_t pb = rl::kMlpParams * sizeof(float), hb = rl::kMlpParams * sizeof(uint16_t);
+    const size_t n = (size_t)c->mlp_L.params;
+    const size_t pb = n * sizeof(float), hb = n * sizeof(uint16_t);
     HIPC(c, hipSetDevice(c->device));
     HIPC(c, c->mlp_params.ensure(pb));
     HIPC(c, c->mlp_m.ensure(pb));
@@ -922,13 +925,14 @@ int risltc_proposal_set_params(risltc_context_t* c, const float* params) {
     if (!params) return fail(c, RISLTC_E_ARG, "null params");
     int e = mlp_alloc(c);
     if (e) return e;
-    const size_t pb = rl::kMlpParams * sizeof(float);
+    const size_t pb = (size_t)c->mlp_L.params * sizeof(float);
     HIPC(c, hipMemcpy(c->mlp_params.p, params, pb, hipMemcpyHostToDevice));
-    std::vector<float> zero(rl::kMlpParams, 0.0f);
+    std::vector<float> zero((size_t)c->mlp_L.params, 0.0f);
     HIPC(c, hipMemcpy(c->mlp_m.p, zero.data(), pb, hipMemcpyHostToDevice));
     HIPC(c, hipMemcpy(c->mlp_v.p, zero.data(), pb, hipMemcpyHostToDevice));
     HIPC(c, hipMemcpy(c->mlp_grad.p, zero.data(), pb, hipMemcpyHostToDevice));
-    HIPC(c, rl::launch_mlp_sync(c->mlp_params.as<float>(), c->mlp_wf.as<uint16_t>(), c->mlp_wb.as<uint16_t>(), nullptr));
+    HIPC(c, rl::launch_mlp_sync(c->mlp_L, c->mlp_params.as<float>(), c->mlp_wf.as<uint16_t>(),
+                                c->mlp_wb.as<uint16_t>(), nullptr));
     HIPC(c, hipDeviceSynchronize());
     c->mlp_ready = true;
     c->adam_step = 0;
@@ -939,14 +943,15 @@ int risltc_proposal_init(risltc_context_t* c, uint32_t seed) {
     if (!c) return RISLTC_E_ARG;
     // uniform He initialisation, weights in [-sqrt(6/in), sqrt(6/in)], zero
     // biases; the output layer starts small so the first proposal is flat
-    std::vector<float> p(rl::kMlpParams, 0.0f);
+    const rl::MlpLayout& M = c->mlp_L;
+    std::vector<float> p((size_t)M.params, 0.0f);
     struct L {
         int w, nout, nin;
         float gain;
-    } layers[4] = {{rl::kPW1, rl::kMlpHid, rl::kMlpIn, 1.0f},
-                   {rl::kPW2, rl::kMlpHid, rl::kMlpHid, 1.0f},
-                   {rl::kPW3, rl::kMlpHid, rl::kMlpHid, 1.0f},
-                   {rl::kPW4, rl::kMlpOut, rl::kMlpHid, 0.1f}};
+    } layers[4] = {{M.pw1, M.hid, rl::kMlpIn, 1.0f},
+                   {M.pw2, M.hid, M.hid, 1.0f},
+                   {M.pw3, M.hid, M.hid, 1.0f},
+                   {M.pw4, rl::kMlpOut, M.hid, 0.1f}};
     for (const L& l : layers) {
         const float lim = l.gain * std::sqrt(6.0f / (float)l.nin);
         for (int e = 0; e < l.nout * l.nin; ++e) {
@@ -963,10 +968,24 @@ int risltc_proposal_get_params(risltc_context_t* c, float* out, float* grad_out)
     if (!c->mlp_ready) return fail(c, RISLTC_E_STATE, "the proposal MLP is not initialised");
     HIPC(c, hipSetDevice(c->device));
     HIPC(c, hipDeviceSynchronize());
-    if (out) HIPC(c, hipMemcpy(out, c->mlp_params.p, rl::kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
-    if (grad_out) HIPC(c, hipMemcpy(grad_out, c->mlp_grad.p, rl::kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+    const size_t pb = (size_t)c->mlp_L.params * sizeof(float);
+    if (out) HIPC(c, hipMemcpy(out, c->mlp_params.p, pb, hipMemcpyDeviceToHost));
+    if (grad_out) HIPC(c, hipMemcpy(grad_out, c->mlp_grad.p, pb, hipMemcpyDeviceToHost));
     return RISLTC_OK;
 }
+
+int risltc_proposal_set_width(risltc_context_t* c, uint32_t width) {
+    if (!c) return RISLTC_E_ARG;
+    if (width != (uint32_t)rl::kMlpHid && width != (uint32_t)rl::kMlpHidWide)
+        return fail(c, RISLTC_E_UNSUPPORTED, "hidden width %u: 64 or 128", width);
+    if ((int)width != c->mlp_L.hid) {
+        c->mlp_L = rl::mlp_layout((int)width);
+        c->mlp_ready = false;  // risltc_proposal_init or _set_params next
+    }
+    return RISLTC_OK;
+}
+
+uint32_t risltc_proposal_param_count(const risltc_context_t* c) { return c ? (uint32_t)c->mlp_L.params : 0u; }
 
 static rl::FeatureView feature_view(const risltc_context_t* c, const risltc_frame_constants_t* k, const uint32_t* vis) {
     rl::FeatureView f;
@@ -1003,6 +1022,7 @@ int risltc_proposal_update(risltc_context_t* c, const risltc_frame_constants_t* 
     HIPC(c, c->prop_table.ensure(P * rl::kPropBytes));
     rl::ProposalArgs a;
     memset(&a, 0, sizeof(a));
+    a.L = c->mlp_L;
     a.f = feature_view(c, k, vis);
     a.wf = c->mlp_wf.as<uint16_t>();
     a.params = c->mlp_params.as<float>();
@@ -1036,9 +1056,10 @@ int risltc_proposal_train(risltc_context_t* c, const risltc_frame_constants_t* k
         return fail(c, RISLTC_E_STATE, "the training records are for another viewport");
     hipStream_t st = (hipStream_t)stream;
     HIPC(c, hipSetDevice(c->device));
-    HIPC(c, c->train_scratch.ensure(rl::train_scratch_bytes(batch)));
+    HIPC(c, c->train_scratch.ensure(rl::train_scratch_bytes(batch, c->mlp_L.hid)));
     rl::TrainArgs t;
     memset(&t, 0, sizeof(t));
+    t.L = c->mlp_L;
     t.f = feature_view(c, k, vis);
     t.B = batch;
     t.sample_count = (uint32_t)c->rec_samples;

@@ -63,20 +63,37 @@ constexpr uint64_t kMaxBvhTriangles = 400000000ull;
 constexpr uint32_t kLdsSphereMax = 2048;
 
 // ---- learned RIS proposal ----------------------------------------------------------
-// MLP 96 -> 64 -> 64 -> 64 -> 256 over light clusters. Parameters are one FP32
-// array; every weight matrix is stored [out][in], followed by its bias.
+// MLP 96 -> H -> H -> H -> 256 over light clusters, H = 64 (default) or 128.
+// Parameters are one FP32 array; every weight matrix is stored [out][in],
+// followed by its bias.
 constexpr int kMlpIn = 96, kMlpHid = 64, kMlpOut = 256;
+constexpr int kMlpHidWide = 128;
+struct MlpLayout {
+    int hid;                                   // hidden width
+    int pw1, pb1, pw2, pb2, pw3, pb3, pw4, pb4;  // offsets into the parameters
+    int params;                                // parameter count
+};
+__host__ __device__ constexpr MlpLayout mlp_layout(int h) {
+    return MlpLayout{h,
+                     0,
+                     h * kMlpIn,
+                     h * kMlpIn + h,
+                     h * kMlpIn + h + h * h,
+                     h * kMlpIn + 2 * h + h * h,
+                     h * kMlpIn + 2 * h + 2 * h * h,
+                     h * kMlpIn + 3 * h + 2 * h * h,
+                     h * kMlpIn + 3 * h + 2 * h * h + kMlpOut * h,
+                     h * kMlpIn + 3 * h + 2 * h * h + kMlpOut * h + kMlpOut};
+}
+// the default layout (H = 64)
 constexpr int kPW1 = 0, kPB1 = kPW1 + kMlpHid * kMlpIn;
 constexpr int kPW2 = kPB1 + kMlpHid, kPB2 = kPW2 + kMlpHid * kMlpHid;
 constexpr int kPW3 = kPB2 + kMlpHid, kPB3 = kPW3 + kMlpHid * kMlpHid;
 constexpr int kPW4 = kPB3 + kMlpHid, kPB4 = kPW4 + kMlpOut * kMlpHid;
 constexpr int kMlpParams = kPB4 + kMlpOut;  // 31,168
-// per pixel proposal table: one byte per cluster, v = round(255 sigmoid(z)),
-// the learned probability that a shadow ray toward the cluster is unoccluded.
-// The learned RIS target is p_hat * (floor + v * scale), scale = (1 - floor)
-// / 255: the LTC target mixed with the visibility estimate, never below
-// p_hat * floor.
-// per pixel: the ids of 16 clusters (bytes 0..15) and their weights
+static_assert(mlp_layout(kMlpHid).params == kMlpParams && mlp_layout(kMlpHid).pw4 == kPW4, "default layout");
+static_assert(mlp_layout(kMlpHidWide).params == 78464, "wide layout");
+// per pixel proposal table: the ids of 16 clusters (bytes 0..15) and their weights
 // (bytes 16..31): in each of 4 groups of clusters (c % 16) / 4 = g, the 4
 // with the largest logits, weight round(255 exp(z - z_max))
 constexpr int kPropTop = 16;
@@ -101,6 +118,7 @@ struct FeatureView {
 };
 
 struct ProposalArgs {
+    MlpLayout L;
     FeatureView f;
     const uint16_t* wf;   // bf16 weights, master layout
     const float* params;  // FP32 master (biases)
@@ -138,6 +156,7 @@ hipError_t launch_refresh_cache(const VisCacheView& c, hipStream_t s);
 
 // one optimisation step on a batch of (pixel, cluster, weight) items
 struct TrainArgs {
+    MlpLayout L;
     FeatureView f;
     uint32_t B;                    // batch size, multiple of 256
     uint32_t sample_count;         // pixel-samples in the training records
@@ -156,10 +175,10 @@ struct TrainArgs {
     void* scratch;                 // train_scratch_bytes(B)
     float* loss;                   // out: batch loss (device, one float)
 };
-size_t train_scratch_bytes(uint32_t B);
+size_t train_scratch_bytes(uint32_t B, int hid);
 hipError_t launch_train_step(const TrainArgs& a, hipStream_t s);
 // bf16 copies of the master parameters (after set_params)
-hipError_t launch_mlp_sync(const float* params, uint16_t* wf, uint16_t* wb, hipStream_t s);
+hipError_t launch_mlp_sync(const MlpLayout& L, const float* params, uint16_t* wf, uint16_t* wb, hipStream_t s);
 
 struct ShadeArgs {
     risltc_frame_constants_t c;

@@ -157,27 +157,34 @@ __device__ bool encode_features(const FeatureView& F, uint32_t pixel, float* f) 
 }
 
 // ---- fused inference: features -> MLP -> per-pixel top-16 table -------------------
-constexpr int kLdW1 = kMlpIn + 8, kLdH = kMlpHid + 8;  // bf16 row strides (16-byte multiples)
-constexpr int kPropWaves = 8;
+constexpr int kLdW1 = kMlpIn + 8;  // bf16 row strides (16-byte multiples)
+template <int HID>
+constexpr int ld_h() { return HID + 8; }
 
+// LDS of the inference kernel. Width 64: all weights, 8 waves (135 KB).
+// Width 128: the hidden layers' weights (96 KB) and 4 waves; the output
+// layer's weights (64 KB) are read from global memory, where they stay in L2.
+template <int HID, int WAVES, bool W4LDS>
 struct PropShared {
-    uint16_t w1[kMlpHid * kLdW1];
-    uint16_t w2[kMlpHid * kLdH];
-    uint16_t w3[kMlpHid * kLdH];
-    uint16_t w4[kMlpOut * kLdH];
-    float bias[3 * kMlpHid + kMlpOut];
+    uint16_t w1[HID * kLdW1];
+    uint16_t w2[HID * ld_h<HID>()];
+    uint16_t w3[HID * ld_h<HID>()];
+    uint16_t w4[W4LDS ? kMlpOut * ld_h<HID>() : 8];
+    float bias[3 * HID + kMlpOut];
     uint32_t csize[kMlpOut];
-    uint16_t act_in[kPropWaves][16 * kLdW1];
-    uint16_t act_h[kPropWaves][2][16 * kLdH];
+    uint16_t act_in[WAVES][16 * kLdW1];
+    uint16_t act_h[WAVES][2][16 * ld_h<HID>()];
 };
+static_assert(sizeof(PropShared<64, 8, true>) <= 160 * 1024, "width-64 LDS");
+static_assert(sizeof(PropShared<128, 4, false>) <= 160 * 1024, "width-128 LDS");
 
 // hidden layer on a 16-pixel tile: out[pixel][o] = relu(W[o] . act[pixel] + b[o])
-template <int K, int LDW, int LDA>
+template <int HID, int K, int LDW, int LDA>
 __device__ __forceinline__ void hidden_layer(const uint16_t* W, const float* bias, const uint16_t* act, uint16_t* out,
                                              int lane) {
     const int g = lane >> 4, r = lane & 15;
 #pragma unroll
-    for (int ot = 0; ot < kMlpHid / 16; ++ot) {
+    for (int ot = 0; ot < HID / 16; ++ot) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int ks = 0; ks < K / 32; ++ks)
@@ -185,28 +192,35 @@ __device__ __forceinline__ void hidden_layer(const uint16_t* W, const float* bia
         float v[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) v[i] = fmax_(acc[i] + bias[ot * 16 + 4 * g + i], 0.0f);
-        *reinterpret_cast<uint2*>(out + r * kLdH + ot * 16 + 4 * g) =
+        *reinterpret_cast<uint2*>(out + r * ld_h<HID>() + ot * 16 + 4 * g) =
             make_uint2(bf16_pack2(v[0], v[1]), bf16_pack2(v[2], v[3]));
     }
 }
 
-__global__ void __launch_bounds__(64 * kPropWaves) proposal_kernel(ProposalArgs A) {
-    __shared__ PropShared sh;
+template <int HID, int WAVES, bool W4LDS>
+__global__ void __launch_bounds__(64 * WAVES) proposal_kernel(ProposalArgs A) {
+    constexpr int kLdH = ld_h<HID>();
+    constexpr MlpLayout L = mlp_layout(HID);
+    __shared__ PropShared<HID, WAVES, W4LDS> sh;
     const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // stage the bf16 weights (padded rows) and biases
-    for (int e = tid; e < kMlpHid * kMlpIn; e += blockDim.x) sh.w1[(e / kMlpIn) * kLdW1 + e % kMlpIn] = A.wf[kPW1 + e];
-    for (int e = tid; e < kMlpHid * kMlpHid; e += blockDim.x) {
-        sh.w2[(e / kMlpHid) * kLdH + e % kMlpHid] = A.wf[kPW2 + e];
-        sh.w3[(e / kMlpHid) * kLdH + e % kMlpHid] = A.wf[kPW3 + e];
+    for (int e = tid; e < HID * kMlpIn; e += blockDim.x) sh.w1[(e / kMlpIn) * kLdW1 + e % kMlpIn] = A.wf[L.pw1 + e];
+    for (int e = tid; e < HID * HID; e += blockDim.x) {
+        sh.w2[(e / HID) * kLdH + e % HID] = A.wf[L.pw2 + e];
+        sh.w3[(e / HID) * kLdH + e % HID] = A.wf[L.pw3 + e];
     }
-    for (int e = tid; e < kMlpOut * kMlpHid; e += blockDim.x) sh.w4[(e / kMlpHid) * kLdH + e % kMlpHid] = A.wf[kPW4 + e];
-    for (int e = tid; e < kMlpHid; e += blockDim.x) {
-        sh.bias[e] = A.params[kPB1 + e];
-        sh.bias[kMlpHid + e] = A.params[kPB2 + e];
-        sh.bias[2 * kMlpHid + e] = A.params[kPB3 + e];
+    if constexpr (W4LDS)
+        for (int e = tid; e < kMlpOut * HID; e += blockDim.x) sh.w4[(e / HID) * kLdH + e % HID] = A.wf[L.pw4 + e];
+    // the output layer's weights: [out][in] rows of HID bf16
+    const uint16_t* w4 = W4LDS ? sh.w4 : A.wf + L.pw4;
+    constexpr int kLdW4 = W4LDS ? kLdH : HID;
+    for (int e = tid; e < HID; e += blockDim.x) {
+        sh.bias[e] = A.params[L.pb1 + e];
+        sh.bias[HID + e] = A.params[L.pb2 + e];
+        sh.bias[2 * HID + e] = A.params[L.pb3 + e];
     }
     for (int e = tid; e < kMlpOut; e += blockDim.x) {
-        sh.bias[3 * kMlpHid + e] = A.params[kPB4 + e];
+        sh.bias[3 * HID + e] = A.params[L.pb4 + e];
         sh.csize[e] = (uint32_t)e < A.cluster_count ? A.cluster_offsets[e + 1] - A.cluster_offsets[e] : 0u;
     }
     __syncthreads();
@@ -218,7 +232,7 @@ __global__ void __launch_bounds__(64 * kPropWaves) proposal_kernel(ProposalArgs 
     // one (packed to bf16 pairs in registers), then the MLP runs on the four
     // 16-pixel sub-tiles in turn, each staged in LDS by its 16 lanes
     const uint32_t groups = (A.pixel_count + 63u) / 64u;
-    for (uint32_t grp = blockIdx.x * kPropWaves + wave; grp < groups; grp += gridDim.x * kPropWaves) {
+    for (uint32_t grp = blockIdx.x * WAVES + wave; grp < groups; grp += gridDim.x * WAVES) {
         const uint32_t my_pixel = grp * 64u + (uint32_t)lane;
         const bool my_in = my_pixel < A.pixel_count;
         uint32_t packed[kMlpIn / 2];
@@ -251,11 +265,11 @@ __global__ void __launch_bounds__(64 * kPropWaves) proposal_kernel(ProposalArgs 
             // every lane of the pixel's column learns whether it is shaded
             const bool shaded = __shfl(my_shaded ? 1 : 0, 16 * sub + r) != 0;
             wave_sync();
-            hidden_layer<kMlpIn, kLdW1, kLdW1>(sh.w1, sh.bias, act, h0, lane);
+            hidden_layer<HID, kMlpIn, kLdW1, kLdW1>(sh.w1, sh.bias, act, h0, lane);
             wave_sync();
-            hidden_layer<kMlpHid, kLdH, kLdH>(sh.w2, sh.bias + kMlpHid, h0, h1, lane);
+            hidden_layer<HID, HID, kLdH, kLdH>(sh.w2, sh.bias + HID, h0, h1, lane);
             wave_sync();
-            hidden_layer<kMlpHid, kLdH, kLdH>(sh.w3, sh.bias + 2 * kMlpHid, h1, h0, lane);
+            hidden_layer<HID, HID, kLdH, kLdH>(sh.w3, sh.bias + 2 * HID, h1, h0, lane);
             wave_sync();
             // output layer: this lane keeps logits of clusters 16 t + 4 g + i of pixel r,
             // and the 4 largest of them
@@ -265,13 +279,13 @@ __global__ void __launch_bounds__(64 * kPropWaves) proposal_kernel(ProposalArgs 
             for (int t = 0; t < kMlpOut / 16; ++t) {
                 f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int ks = 0; ks < kMlpHid / 32; ++ks)
-                    acc = mfma(frag(sh.w4 + (t * 16 + r) * kLdH + ks * 32 + 8 * g),
+                for (int ks = 0; ks < HID / 32; ++ks)
+                    acc = mfma(frag(w4 + (t * 16 + r) * kLdW4 + ks * 32 + 8 * g),
                                frag(h0 + r * kLdH + ks * 32 + 8 * g), acc);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     int cl = t * 16 + 4 * g + i;
-                    float z = acc[i] + sh.bias[3 * kMlpHid + cl];
+                    float z = acc[i] + sh.bias[3 * HID + cl];
                     if (A.logits && in) A.logits[(size_t)pixel * kMlpOut + cl] = z;
                     if (sh.csize[cl] == 0u) continue;
                     // insertion into the sorted best-4 (ties keep the smaller id)
@@ -322,11 +336,17 @@ hipError_t launch_proposal(const ProposalArgs& a, hipStream_t s) {
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
+    const bool wide = a.L.hid == kMlpHidWide;
+    if (!wide && a.L.hid != kMlpHid) return hipErrorInvalidValue;
+    const uint32_t waves = wide ? 4u : 8u;
     uint32_t groups = (a.pixel_count + 63u) / 64u;
-    uint32_t blocks = (groups + kPropWaves - 1u) / kPropWaves;
+    uint32_t blocks = (groups + waves - 1u) / waves;
     uint32_t cap = (uint32_t)(cus > 0 ? cus : 1);
     if (blocks > cap) blocks = cap;
-    hipLaunchKernelGGL(proposal_kernel, dim3(blocks), dim3(64 * kPropWaves), 0, s, a);
+    if (wide)
+        hipLaunchKernelGGL((proposal_kernel<kMlpHidWide, 4, false>), dim3(blocks), dim3(64 * 4), 0, s, a);
+    else
+        hipLaunchKernelGGL((proposal_kernel<kMlpHid, 8, true>), dim3(blocks), dim3(64 * 8), 0, s, a);
     return hipGetLastError();
 }
 
@@ -550,25 +570,25 @@ __global__ void loss_sum_kernel(const float* row_loss, uint32_t B, float* out) {
 }
 
 // bf16 copies of layer weights: wf = master layout [out][in], wb = [in][out]
-__device__ __forceinline__ void layer_of(int e, int& off, int& nout, int& nin) {
-    if (e < kPB1) {
-        off = kPW1, nout = kMlpHid, nin = kMlpIn;
-    } else if (e >= kPW2 && e < kPB2) {
-        off = kPW2, nout = kMlpHid, nin = kMlpHid;
-    } else if (e >= kPW3 && e < kPB3) {
-        off = kPW3, nout = kMlpHid, nin = kMlpHid;
-    } else if (e >= kPW4 && e < kPB4) {
-        off = kPW4, nout = kMlpOut, nin = kMlpHid;
+__device__ __forceinline__ void layer_of(const MlpLayout& L, int e, int& off, int& nout, int& nin) {
+    if (e < L.pb1) {
+        off = L.pw1, nout = L.hid, nin = kMlpIn;
+    } else if (e >= L.pw2 && e < L.pb2) {
+        off = L.pw2, nout = L.hid, nin = L.hid;
+    } else if (e >= L.pw3 && e < L.pb3) {
+        off = L.pw3, nout = L.hid, nin = L.hid;
+    } else if (e >= L.pw4 && e < L.pb4) {
+        off = L.pw4, nout = kMlpOut, nin = L.hid;
     } else {
         off = -1, nout = nin = 0;
     }
 }
 
-__device__ __forceinline__ void write_copies(int e, float v, uint16_t* wf, uint16_t* wb) {
+__device__ __forceinline__ void write_copies(const MlpLayout& L, int e, float v, uint16_t* wf, uint16_t* wb) {
     uint16_t h = bf16_bits(v);
     wf[e] = h;
     int off, nout, nin;
-    layer_of(e, off, nout, nin);
+    layer_of(L, e, off, nout, nin);
     if (off >= 0) {
         int o = (e - off) / nin, i = (e - off) % nin;
         wb[off + i * nout + o] = h;
@@ -579,7 +599,7 @@ __device__ __forceinline__ void write_copies(int e, float v, uint16_t* wf, uint1
 
 __global__ void adam_kernel(TrainArgs A) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= kMlpParams) return;
+    if (e >= A.L.params) return;
     const float gr = A.grad[e];
     const float m = A.beta1 * A.m[e] + (1.0f - A.beta1) * gr;
     const float v = A.beta2 * A.v[e] + (1.0f - A.beta2) * gr * gr;
@@ -589,17 +609,17 @@ __global__ void adam_kernel(TrainArgs A) {
     const float bc2 = 1.0f - __powf(A.beta2, (float)A.step);
     const float p = A.params[e] - A.lr * (m / bc1) / (__builtin_sqrtf(v / bc2) + A.eps);
     A.params[e] = p;
-    write_copies(e, p, A.wf, A.wb);
+    write_copies(A.L, e, p, A.wf, A.wb);
 }
 
-__global__ void sync_kernel(const float* params, uint16_t* wf, uint16_t* wb) {
+__global__ void sync_kernel(MlpLayout L, const float* params, uint16_t* wf, uint16_t* wb) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= kMlpParams) return;
-    write_copies(e, params[e], wf, wb);
+    if (e >= L.params) return;
+    write_copies(L, e, params[e], wf, wb);
 }
 
-hipError_t launch_mlp_sync(const float* params, uint16_t* wf, uint16_t* wb, hipStream_t s) {
-    hipLaunchKernelGGL(sync_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, params, wf, wb);
+hipError_t launch_mlp_sync(const MlpLayout& L, const float* params, uint16_t* wf, uint16_t* wb, hipStream_t s) {
+    hipLaunchKernelGGL(sync_kernel, dim3((L.params + 255) / 256), dim3(256), 0, s, L, params, wf, wb);
     return hipGetLastError();
 }
 
@@ -611,7 +631,7 @@ struct TrainScratch {
     uint32_t* target;
 };
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-TrainScratch carve(void* base, uint32_t B, size_t* total) {
+TrainScratch carve(void* base, uint32_t B, int hid, size_t* total) {
     TrainScratch t;
     size_t off = 0;
     char* p = static_cast<char*>(base);
@@ -622,53 +642,55 @@ TrainScratch carve(void* base, uint32_t B, size_t* total) {
     };
     t.X = (uint16_t*)take(2ull * B * kMlpIn);
     t.Xt = (uint16_t*)take(2ull * B * kMlpIn);
-    t.H1 = (uint16_t*)take(2ull * B * kMlpHid);
-    t.H1t = (uint16_t*)take(2ull * B * kMlpHid);
-    t.H2 = (uint16_t*)take(2ull * B * kMlpHid);
-    t.H2t = (uint16_t*)take(2ull * B * kMlpHid);
-    t.H3 = (uint16_t*)take(2ull * B * kMlpHid);
-    t.H3t = (uint16_t*)take(2ull * B * kMlpHid);
+    t.H1 = (uint16_t*)take(2ull * B * hid);
+    t.H1t = (uint16_t*)take(2ull * B * hid);
+    t.H2 = (uint16_t*)take(2ull * B * hid);
+    t.H2t = (uint16_t*)take(2ull * B * hid);
+    t.H3 = (uint16_t*)take(2ull * B * hid);
+    t.H3t = (uint16_t*)take(2ull * B * hid);
     t.G4 = (uint16_t*)take(2ull * B * kMlpOut);
     t.G4t = (uint16_t*)take(2ull * B * kMlpOut);
-    t.G3 = (uint16_t*)take(2ull * B * kMlpHid);
-    t.G3t = (uint16_t*)take(2ull * B * kMlpHid);
-    t.G2 = (uint16_t*)take(2ull * B * kMlpHid);
-    t.G2t = (uint16_t*)take(2ull * B * kMlpHid);
-    t.G1 = (uint16_t*)take(2ull * B * kMlpHid);
-    t.G1t = (uint16_t*)take(2ull * B * kMlpHid);
+    t.G3 = (uint16_t*)take(2ull * B * hid);
+    t.G3t = (uint16_t*)take(2ull * B * hid);
+    t.G2 = (uint16_t*)take(2ull * B * hid);
+    t.G2t = (uint16_t*)take(2ull * B * hid);
+    t.G1 = (uint16_t*)take(2ull * B * hid);
+    t.G1t = (uint16_t*)take(2ull * B * hid);
     t.Z = (float*)take(4ull * B * kMlpOut);
     t.y = (float*)take(4ull * B);
     t.ysum = (float*)take(4);
     t.row_loss = (float*)take(4ull * B);
     const uint32_t slices = (B + kTrainSlice - 1) / kTrainSlice;
-    t.partial = (float*)take(4ull * slices * kMlpOut * kMlpHid);
+    t.partial = (float*)take(4ull * slices * kMlpOut * hid);
     t.target = (uint32_t*)take(4ull * B);
     if (total) *total = off;
     return t;
 }
 }  // namespace
 
-size_t train_scratch_bytes(uint32_t B) {
+size_t train_scratch_bytes(uint32_t B, int hid) {
     size_t total = 0;
-    carve(nullptr, B, &total);
+    carve(nullptr, B, hid, &total);
     return total;
 }
 
 hipError_t launch_train_step(const TrainArgs& a, hipStream_t s) {
     if (a.B == 0 || a.B % 256u != 0u || a.sample_count == 0) return hipErrorInvalidValue;
-    TrainScratch t = carve(a.scratch, a.B, nullptr);
+    const MlpLayout& L4 = a.L;
+    const uint32_t H = (uint32_t)L4.hid;
+    TrainScratch t = carve(a.scratch, a.B, L4.hid, nullptr);
     const uint32_t B = a.B, slices = (B + kTrainSlice - 1) / kTrainSlice;
     hipLaunchKernelGGL(train_gather_kernel, dim3(B / 256u), dim3(256), 0, s, a, t.X, t.Xt, t.target, t.y);
     hipLaunchKernelGGL(batch_sum_kernel, dim3(1), dim3(256), 0, s, t.y, B, t.ysum);
     // forward
     GemmArgs g{};
-    g = GemmArgs{t.X, a.wf + kPW1, B, kMlpHid, kMlpIn, kMlpIn, kMlpIn, kMlpIn, a.params + kPB1, t.H1, t.H1t, nullptr, nullptr};
+    g = GemmArgs{t.X, a.wf + L4.pw1, B, H, kMlpIn, kMlpIn, kMlpIn, kMlpIn, a.params + L4.pb1, t.H1, t.H1t, nullptr, nullptr};
     gemm<kEpiHidden>(g, 1, s);
-    g = GemmArgs{t.H1, a.wf + kPW2, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, a.params + kPB2, t.H2, t.H2t, nullptr, nullptr};
+    g = GemmArgs{t.H1, a.wf + L4.pw2, B, H, H, H, H, H, a.params + L4.pb2, t.H2, t.H2t, nullptr, nullptr};
     gemm<kEpiHidden>(g, 1, s);
-    g = GemmArgs{t.H2, a.wf + kPW3, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, a.params + kPB3, t.H3, t.H3t, nullptr, nullptr};
+    g = GemmArgs{t.H2, a.wf + L4.pw3, B, H, H, H, H, H, a.params + L4.pb3, t.H3, t.H3t, nullptr, nullptr};
     gemm<kEpiHidden>(g, 1, s);
-    g = GemmArgs{t.H3, a.wf + kPW4, B, kMlpOut, kMlpHid, kMlpHid, kMlpHid, kMlpHid, a.params + kPB4, nullptr, nullptr, t.Z, nullptr};
+    g = GemmArgs{t.H3, a.wf + L4.pw4, B, kMlpOut, H, H, H, H, a.params + L4.pb4, nullptr, nullptr, t.Z, nullptr};
     gemm<kEpiLogits>(g, 1, s);
     hipLaunchKernelGGL(softmax_grad_kernel, dim3(B / kSgRows), dim3(256), 0, s, t.Z, t.target, t.y, t.ysum, B, t.G4,
                        t.G4t, t.row_loss);
@@ -679,8 +701,8 @@ hipError_t launch_train_step(const TrainArgs& a, hipStream_t s) {
         uint32_t nout, nin;
         int w, bias;
     };
-    const Layer layers[4] = {{t.G4t, t.H3t, kMlpOut, kMlpHid, kPW4, kPB4}, {t.G3t, t.H2t, kMlpHid, kMlpHid, kPW3, kPB3},
-                             {t.G2t, t.H1t, kMlpHid, kMlpHid, kPW2, kPB2}, {t.G1t, t.Xt, kMlpHid, kMlpIn, kPW1, kPB1}};
+    const Layer layers[4] = {{t.G4t, t.H3t, kMlpOut, H, L4.pw4, L4.pb4}, {t.G3t, t.H2t, H, H, L4.pw3, L4.pb3},
+                             {t.G2t, t.H1t, H, H, L4.pw2, L4.pb2}, {t.G1t, t.Xt, H, kMlpIn, L4.pw1, L4.pb1}};
     const uint16_t* Gs[4] = {t.G4, t.G3, t.G2, t.G1};
     uint16_t* Gn[4] = {t.G3, t.G2, t.G1, nullptr};
     uint16_t* Gnt[4] = {t.G3t, t.G2t, t.G1t, nullptr};
@@ -700,7 +722,7 @@ hipError_t launch_train_step(const TrainArgs& a, hipStream_t s) {
             gemm<kEpiBackward>(g, 1, s);
         }
     }
-    hipLaunchKernelGGL(adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(adam_kernel, dim3((L4.params + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

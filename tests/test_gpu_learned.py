@@ -35,13 +35,24 @@ def bf16(x):
     return (u.astype(np.uint32) << 16).view(np.float32)
 
 
+def layer_bounds(width):
+    """(start, end) of each layer's weights + bias in the parameter array:
+    every matrix [out][in] followed by its bias."""
+    shapes = [(width, 96), (width, width), (width, width), (256, width)]
+    out, off = [], 0
+    for o, i in shapes:
+        out.append((off, off + o * i + o))
+        off += o * i + o
+    return out
+
+
 def layers(params):
     p = params
-    W1 = p[0:6144].reshape(64, 96); b1 = p[6144:6208]
-    W2 = p[6208:10304].reshape(64, 64); b2 = p[10304:10368]
-    W3 = p[10368:14464].reshape(64, 64); b3 = p[14464:14528]
-    W4 = p[14528:30912].reshape(256, 64); b4 = p[30912:31168]
-    return (W1, b1), (W2, b2), (W3, b3), (W4, b4)
+    width = {abi.PROPOSAL_PARAMS: 64, abi.PROPOSAL_PARAMS_WIDE: 128}[p.size]
+    res = []
+    for (a, b), (o, i) in zip(layer_bounds(width), [(width, 96), (width, width), (width, width), (256, width)]):
+        res.append((p[a:a + o * i].reshape(o, i), p[a + o * i:b]))
+    return tuple(res)
 
 
 def forward(params, X):
@@ -71,19 +82,25 @@ def learned(small_scene, ltc_table):
 
 
 @pytest.fixture(scope="module")
+def learned_wide(small_scene, ltc_table):
+    """The 96-128-128-128-256 network."""
+    return _learned_context(small_scene, ltc_table, width=128)
+
+
+@pytest.fixture(scope="module")
 def learned_odd(ltc_table):
     """61 lights: a per-light cache row is not a multiple of 16 bytes."""
     return _learned_context(synth.small_scene(seed=5, width=96, height=80, light_count=61), ltc_table)
 
 
-def _learned_context(small_scene, ltc_table):
+def _learned_context(small_scene, ltc_table, width=64):
     import risltc_amd
     import torch
     ctx = risltc_amd.Context(0)
     ctx.load_scene(small_scene, *ltc_table)
     K = 16
     col = ctx.cluster_lights(K, 20)
-    ctx.proposal_init(7)
+    ctx.proposal_init(7, width=width)
     consts = ctx.frame_constants()
     P = small_scene.width * small_scene.height
     vis = torch.empty(P, dtype=torch.int32, device="cuda")
@@ -108,6 +125,15 @@ def test_mlp_forward_matches_numpy(learned):
     """Logits within 2e-3 of the larger of |z| and 1 on every pixel: the two
     sides differ only in FP32 summation order and, rarely, in the bf16
     rounding of a hidden activation that this order moves across a tie."""
+    _check_forward(learned)
+
+
+def test_wide_mlp_forward_matches_numpy(learned_wide):
+    assert learned_wide["ctx"].proposal_param_count() == abi.PROPOSAL_PARAMS_WIDE
+    _check_forward(learned_wide)
+
+
+def _check_forward(learned):
     params, _ = learned["ctx"].proposal_get_params()
     vis = learned["vis"].cpu().numpy().view(np.uint32)
     shaded = np.flatnonzero((vis >> 31) == 0)
@@ -121,6 +147,14 @@ def test_mlp_forward_matches_numpy(learned):
 
 
 def test_tables_are_top4_per_lane_group(learned):
+    _check_tables(learned)
+
+
+def test_wide_tables_are_top4_per_lane_group(learned_wide):
+    _check_tables(learned_wide)
+
+
+def _check_tables(learned):
     vis = learned["vis"].cpu().numpy().view(np.uint32)
     shaded = np.flatnonzero((vis >> 31) == 0)
     sizes = np.bincount(learned["col"], minlength=256)
@@ -174,6 +208,14 @@ def test_learned_shading_bitwise_vs_oracle(learned):
 
 
 def test_training_step_gradient_and_adam(learned):
+    _check_training_step(learned)
+
+
+def test_wide_training_step_gradient_and_adam(learned_wide):
+    _check_training_step(learned_wide)
+
+
+def _check_training_step(learned):
     import torch
     g = learned
     sc = g["scene"]
@@ -222,9 +264,10 @@ def test_training_step_gradient_and_adam(learned):
     G1 = bf16((G2 @ bf16(W2).astype(np.float64)) * (H1 > 0)).astype(np.float64)
     ref = np.concatenate([(G1.T @ X).ravel(), G1.sum(0), (G2.T @ H1).ravel(), G2.sum(0), (G3.T @ H2).ravel(),
                           G3.sum(0), (G4.T @ H3).ravel(), G4.sum(0)])
-    for name, a, b in (("W1", 0, 6208), ("W2", 6208, 10368), ("W3", 10368, 14528), ("W4", 14528, 31168)):
+    width = W1.shape[0]
+    for name, (a, b) in zip(("W1", "W2", "W3", "W4"), layer_bounds(width)):
         rel = np.linalg.norm(grad[a:b] - ref[a:b]) / np.linalg.norm(ref[a:b])
-        assert rel < 1e-2, (name, rel)
+        assert rel < 1e-2, (name, width, rel)
     # Adam, step 1: m = (1-b1) g, v = (1-b2) g^2, bias-corrected
     gd = grad.astype(np.float64)
     mhat = (0.1 * gd) / (1 - 0.9)
@@ -390,3 +433,30 @@ def test_learned_compacted_equals_debug(learned, exact):
     finally:
         ctx.proposal_cache(0)
     assert np.array_equal(a.cpu().numpy().view(np.uint32), b.cpu().numpy().view(np.uint32))
+
+
+def test_wide_learned_pass_runs_and_trains(learned_wide):
+    """The learned pass with the wide network: its tables drive the shade
+    kernel (bitwise against the oracle is width-independent: the oracle takes
+    the device's tables), and training lowers the loss."""
+    import torch
+    g = learned_wide
+    ctx, sc = g["ctx"], g["scene"]
+    s = sc.settings(proposal=abi.PROPOSAL_LEARNED, sample_count_light=2, ris_candidate_count=8)
+    out = torch.empty(g["P"] * 4, dtype=torch.float32, device="cuda")
+    ctx.shade(g["consts"], s, g["vis"], out)
+    torch.cuda.synchronize()
+    o = out.cpu().numpy().reshape(-1, 4)[:, :3]
+    assert np.isfinite(o).all() and (o > 0).mean() > 0.3
+    first = ctx.proposal_train(g["consts"], g["vis"], batch=8192, steps=1, lr=3e-3, seed=200)
+    for i in range(30):
+        ctx.proposal_train(g["consts"], g["vis"], batch=8192, steps=1, lr=3e-3, seed=201 + i)
+    last = ctx.proposal_train(g["consts"], g["vis"], batch=8192, steps=1, lr=0.0, seed=200)
+    assert last < 0.9 * first, (first, last)
+    # switching the width discards the network until it is initialised again
+    ctx.proposal_set_width(64)
+    import risltc_amd
+    with pytest.raises(risltc_amd.RisltcError, match="E_STATE"):
+        ctx.proposal_update(g["consts"], g["vis"])
+    with pytest.raises(risltc_amd.RisltcError, match="E_UNSUPPORTED"):
+        ctx.proposal_set_width(96)
