@@ -47,7 +47,7 @@ def accumulate(ctx, scene, vis, frames, seed0, learned=None, checkpoints=(), ref
         ctx.set_tuning(**dict(tuning, lds_lights=lk["lds_lights"]))
     if learned:
         ctx.cluster_lights(learned.get("clusters", 256), 20)
-        ctx.proposal_init(learned.get("seed", 1))
+        ctx.proposal_init(learned.get("seed", 1), width=learned.get("width", 64))
         if learned.get("cache_log2"):
             ctx.proposal_cache(learned["cache_log2"], learned.get("cache_cell", 0.5), learned.get("cache_prior", 2.0),
                                learned.get("cache_per_light", False))
@@ -56,6 +56,8 @@ def accumulate(ctx, scene, vis, frames, seed0, learned=None, checkpoints=(), ref
     finally:
         ctx.proposal_cache(0)
         ctx.set_tuning(**tuning)
+        if learned and learned.get("width", 64) != 64:
+            ctx.proposal_set_width(64)
 
 
 def _frames(ctx, scene, vis, frames, seed0, learned, checkpoints, ref, s, out, acc, P):
@@ -176,11 +178,13 @@ def compare(ctx, scene, vis, target=0.01, ref_frames=16384, plain_frames=320, fr
         out["ablations"] = {}
         for name, lk in (("source_only", dict(learned, cache_log2=0)),
                          ("cache_only_uniform_source", dict(learned, proposal=abi.PROPOSAL_CACHE, candidates=48)),
-                         ("untrained_mlp", dict(learned, lr=0.0))):
+                         ("untrained_mlp", dict(learned, lr=0.0)),
+                         ("mlp_width_128", dict(learned, width=128))):
             r = run(lk, frames)
             out["ablations"][name] = {"time_to_target_ms": r["time_to_target_ms"],
                                       "over_plain": round(r["time_to_target_ms"] / plain["time_to_target_ms"], 4),
-                                      "relmse_at_1024spp": r["curve"].get("1024", [None])[0]}
+                                      "relmse_at_1024spp": r["curve"].get("1024", [None])[0],
+                                      "ms_per_frame": r["ms_per_frame"]}
     if baselines:
         out["baselines"] = {}
         for name, tech in (("area_turk", abi.SAMPLE_POLYGON_AREA_TURK),
