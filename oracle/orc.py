@@ -282,7 +282,7 @@ class Oracle:
         return c
 
     def render_visibility(self, constants, threads=8):
-        out = np.empty(self.scene.width * self.scene.height, dtype=np.uint32)
+        out = np.empty(int(constants.viewport_size[0]) * int(constants.viewport_size[1]), dtype=np.uint32)
         self.L.orc_render_visibility(C.byref(self.bvh), ptr(self.positions, u32p), C.byref(constants),
                                      ptr(out, u32p), threads)
         return out
