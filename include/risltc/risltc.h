@@ -491,10 +491,14 @@ int risltc_shade_debug(risltc_context_t* context, const risltc_frame_constants_t
  * Sequence per frame: risltc_proposal_update (inference, any cadence),
  * risltc_shade (learned), then optionally risltc_proposal_train on the
  * frame's own samples. ------------------------------------------------- */
-#define RISLTC_MAX_LIGHT_CLUSTERS 256
+#define RISLTC_MAX_LIGHT_CLUSTERS 1024
+/* Bytes per pixel of a proposal table: 16 cluster ids and 16 weights; ids
+ * are 1 byte for networks of 256 outputs, 2 bytes for 1024. */
+#define RISLTC_PROPOSAL_TABLE_BYTES(outputs) ((outputs) > 256 ? 48u : 32u)
 #define RISLTC_PROPOSAL_PARAMS 31168      /* 96-64-64-64-256 MLP (default), FP32 */
 #define RISLTC_PROPOSAL_PARAMS_WIDE 78464 /* 96-128-128-128-256 MLP, FP32 */
-/* Explicit light clusters: cluster_of_light[i] < cluster_count <= 256. */
+/* Explicit light clusters: cluster_of_light[i] < cluster_count <= 1024
+ * (the learned source needs cluster_count <= the network's outputs). */
 int risltc_set_light_clusters(risltc_context_t* context, const uint32_t* cluster_of_light,
                               uint32_t cluster_count);
 /* k-means of the light centroids (deterministic start: light floor(k N / K)
@@ -509,6 +513,10 @@ int risltc_cluster_lights(risltc_context_t* context, uint32_t cluster_count, uin
  * RISLTC_PROPOSAL_PARAMS_WIDE), in the same order: every weight matrix
  * [out][in] followed by its bias. */
 int risltc_proposal_set_width(risltc_context_t* context, uint32_t width);
+/* Hidden width (64 or 128) and outputs (256 or 1024 clusters) together;
+ * risltc_proposal_set_width keeps the outputs. The 1024-output network
+ * writes tables with 2-byte cluster ids (RISLTC_PROPOSAL_TABLE_BYTES). */
+int risltc_proposal_set_network(risltc_context_t* context, uint32_t width, uint32_t outputs);
 uint32_t risltc_proposal_param_count(const risltc_context_t* context);
 /* Deterministic initialisation from a seed; resets the Adam state. */
 int risltc_proposal_init(risltc_context_t* context, uint32_t seed);
@@ -518,12 +526,14 @@ int risltc_proposal_init(risltc_context_t* context, uint32_t seed);
 int risltc_proposal_set_params(risltc_context_t* context, const float* params);
 int risltc_proposal_get_params(risltc_context_t* context, float* params, float* gradient);
 /* Inference for every pixel of the frame: writes the per-pixel tables the
- * learned shade reads, 32 bytes: cluster ids in bytes 0..15 and weights in
- * 16..31; in each group g = (c % 16) / 4 of non-empty clusters the four with
+ * learned shade reads, RISLTC_PROPOSAL_TABLE_BYTES(outputs) each: 16 cluster
+ * ids (bytes 0..15, or 2-byte ids in bytes 0..31 for 1024 outputs) and 16
+ * weights (bytes 16..31, or 32..47); slot 4 g + k holds, in each group
+ * g = (c % 16) / 4 of non-empty clusters, the k-th of the four with
  * the largest logits z (descending, ties to the smaller id) with weight
  * round(255 exp(z - z_max)), z_max over all non-empty clusters; weights 0
  * for background. Optional device outputs: a copy of the tables, the 256
- * logits and the 96 bf16 features per pixel. */
+ * (or 1024) logits and the 96 bf16 features per pixel. */
 int risltc_proposal_update(risltc_context_t* context, const risltc_frame_constants_t* constants,
                            const uint32_t* visibility, uint8_t* table_out, float* logits_out,
                            uint16_t* features_out, void* stream);
@@ -544,7 +554,7 @@ int risltc_proposal_records(risltc_context_t* context, uint32_t* cluster_out, fl
 /* Online visibility cache (project extension). 2^log2_entries hashed
  * world-space cells (cell = floor(position / cell_size) per axis, split by
  * the dominant axis and sign of the shading normal) x slots, where a slot is
- * a light cluster (256 slots) or, with per_light = 1, a light (light-count
+ * a light cluster (1024 slots) or, with per_light = 1, a light (light-count
  * slots). Each entry holds two uint32 counts (a unoccluded, b occluded) and
  * one byte, 9 * slots * 2^log2_entries bytes of device memory in all
  * (log2_entries <= 20, counts at most 4 GiB). While a cache and light
@@ -613,7 +623,8 @@ typedef struct risltc_view_s {
     uint32_t scene;               /* 0 or an id from risltc_add_scene */
     const uint8_t* proposal_table; /* device, with settings.proposal =
                                       RISLTC_PROPOSAL_LEARNED: this view's
-                                      table (width*height*32 bytes) from
+                                      table (width*height*
+                                      RISLTC_PROPOSAL_TABLE_BYTES bytes) from
                                       risltc_proposal_update's table_out;
                                       NULL otherwise */
 } risltc_view_t;

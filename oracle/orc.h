@@ -143,9 +143,10 @@ typedef struct orc_scene_s {
     uint32_t ltc_res_roughness, ltc_res_inclination, ltc_fresnel_count;
     /* shadow rays */
     const orc_bvh_t* bvh;
-    /* learned RIS target (project extension; read when settings.proposal is
-     * RISLTC_PROPOSAL_LEARNED): 256 bytes per pixel, one per cluster, as
-     * risltc_proposal_update writes them */
+    /* learned RIS source (project extension; read when settings.proposal is
+     * RISLTC_PROPOSAL_LEARNED): per pixel table as risltc_proposal_update
+     * writes it, proposal_table_bytes (32: 1-byte ids, or 48: 2-byte ids)
+     * per pixel */
     const uint8_t* proposal_table;
     const uint32_t* cluster_of_light;
     const uint32_t* cluster_offsets; /* cluster_count + 1 prefix offsets      */
@@ -160,6 +161,7 @@ typedef struct orc_scene_s {
     float cache_inv_cell;
     uint32_t cache_slots;
     uint32_t cache_per_light;
+    uint32_t proposal_table_bytes; /* 32 or 48 */
 } orc_scene_t;
 
 /* Bytes of the visibility cache from its counts (2 words per entry:

@@ -996,6 +996,15 @@ static uint32_t light_index_from(R u, uint32_t n) {
  * parity: bitwise against the kernel's learned shading given the same tables
  * (tests/test_gpu_learned.py); the mixture is an exact pdf of this draw
  * (tests/test_oracle_estimator.py::test_learned_draw_density). */
+/* table slot s: cluster id and weight (1-byte ids with weights at 16, or
+ * 2-byte ids with weights at 32) */
+static uint32_t table_id(const orc_scene_t* sc, const uint8_t* t, int s) {
+    return sc->proposal_table_bytes == 48u ? (uint32_t)t[2 * s] | ((uint32_t)t[2 * s + 1] << 8) : (uint32_t)t[s];
+}
+static uint32_t table_weight(const orc_scene_t* sc, const uint8_t* t, int s) {
+    return t[(sc->proposal_table_bytes == 48u ? 32 : 16) + s];
+}
+
 static uint32_t learned_draw(const orc_scene_t* sc, const uint8_t* t, uint32_t T, uint32_t* seed, uint32_t N) {
     float a = orc_noise_next(seed);
     float b = orc_noise_next(seed);
@@ -1006,11 +1015,11 @@ static uint32_t learned_draw(const orc_scene_t* sc, const uint8_t* t, uint32_t T
     if (k > T - 1u) k = T - 1u;
     uint32_t acc = 0, c = 0;
     for (int s = 0; s != 16; ++s) {
-        if (k < acc + t[16 + s]) {
-            c = t[s];
+        if (k < acc + table_weight(sc, t, s)) {
+            c = table_id(sc, t, s);
             break;
         }
-        acc += t[16 + s];
+        acc += table_weight(sc, t, s);
     }
     if (c >= sc->cluster_count) return light_index_from((R)b, N);
     uint32_t off = sc->cluster_offsets[c], size = sc->cluster_offsets[c + 1] - off;
@@ -1026,8 +1035,8 @@ static R learned_density(const orc_scene_t* sc, const uint8_t* t, uint32_t T, ui
     R q = K(0.5) / (R)N;
     uint32_t c = sc->cluster_of_light[idx], wc = 0;
     for (int s = 0; s != 16; ++s)
-        if (t[16 + s] != 0u && t[s] == c) {
-            wc = t[16 + s];
+        if (table_weight(sc, t, s) != 0u && table_id(sc, t, s) == c) {
+            wc = table_weight(sc, t, s);
             break;
         }
     if (wc != 0u) {
@@ -1169,10 +1178,12 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
                 V3 chosen_point = v3(K(0.0), K(0.0), K(0.0));
                 const int learned = s->proposal != RISLTC_PROPOSAL_UNIFORM;
                 const uint8_t* tab =
-                    s->proposal == RISLTC_PROPOSAL_LEARNED ? sc->proposal_table + (size_t)pixel * 32 : NULL;
+                    s->proposal == RISLTC_PROPOSAL_LEARNED
+                        ? sc->proposal_table + (size_t)pixel * (sc->proposal_table_bytes == 48u ? 48u : 32u)
+                        : NULL;
                 uint32_t T = 0;
                 if (tab)
-                    for (int k2 = 0; k2 != 16; ++k2) T += tab[16 + k2];
+                    for (int k2 = 0; k2 != 16; ++k2) T += table_weight(sc, tab, k2);
                 const uint8_t* crow = learned && sc->cache_vis ? sc->cache_vis + (size_t)ckey * sc->cache_slots : NULL;
                 const float lf = ldexpf(1.0f, -(int)(s->learned_floor_log2 ? s->learned_floor_log2 : 3u));
                 const float ls = (1.0f - lf) / 255.0f;

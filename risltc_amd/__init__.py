@@ -79,6 +79,7 @@ SIGNATURES = {
     "risltc_set_polygonal_lights": (C.c_int, [_vp, C.POINTER(abi.PolygonalLight), _u32, _u32]),
     "risltc_load_ltc_table": (C.c_int, [_vp, C.c_char_p, _u32]),
     "risltc_proposal_set_width": (C.c_int, [_vp, _u32]),
+    "risltc_proposal_set_network": (C.c_int, [_vp, _u32, _u32]),
     "risltc_proposal_param_count": (_u32, [_vp]),
     "risltc_add_scene": (C.c_int, [_vp, _u32p, _u16p, _u8p, C.c_uint64, _f32p, _f32p, C.POINTER(abi.Material), _u32,
                                    _u32p]),
@@ -197,6 +198,7 @@ class Context:
         self.ltc_constants = None
         self._keep = []
         self._graphs = weakref.WeakSet()
+        self._mlp = (64, 256)  # hidden width, outputs
 
     def close(self):
         if getattr(self, "h", None):
@@ -291,15 +293,19 @@ class Context:
                     "risltc_cluster_lights")
         return out[:n] if n is not None else out
 
-    def proposal_init(self, seed=0, width=None):
-        """Initialises the MLP; width 64 (default) or 128 selects the hidden
-        width first (risltc_proposal_set_width)."""
-        if width is not None:
-            self.proposal_set_width(width)
+    def proposal_init(self, seed=0, width=None, outputs=None):
+        """Initialises the MLP; width 64 (default) or 128 and outputs 256
+        (default) or 1024 select the network first."""
+        if width is not None or outputs is not None:
+            self.proposal_set_network(width or self._mlp[0], outputs or self._mlp[1])
         self._check(self.L.risltc_proposal_init(self.h, seed), "risltc_proposal_init")
 
     def proposal_set_width(self, width):
-        self._check(self.L.risltc_proposal_set_width(self.h, width), "risltc_proposal_set_width")
+        self.proposal_set_network(width, self._mlp[1])
+
+    def proposal_set_network(self, width, outputs):
+        self._check(self.L.risltc_proposal_set_network(self.h, width, outputs), "risltc_proposal_set_network")
+        self._mlp = (width, outputs)
 
     def proposal_param_count(self):
         return int(self.L.risltc_proposal_param_count(self.h))
@@ -346,7 +352,7 @@ class Context:
         it); slots are light clusters, or lights with per_light."""
         self._check(self.L.risltc_proposal_cache(self.h, log2_entries, cell_size, prior_strength,
                                                  1 if per_light else 0), "risltc_proposal_cache")
-        self._cache_shape = ((1 << log2_entries), self._light_count if per_light else 256, 2)
+        self._cache_shape = ((1 << log2_entries), self._light_count if per_light else abi.MAX_LIGHT_CLUSTERS, 2)
 
     def proposal_cache_read(self):
         """The cache as a uint32 array [cells, slots, 2] (unoccluded, occluded)."""

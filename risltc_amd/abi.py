@@ -19,7 +19,7 @@ LIGHT_UNIFORM, LIGHT_RESERVOIR = 0, 1
 SAMPLE_POLYGON_AREA_TURK, SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE, SAMPLE_POLYGON_LTC_CP = 1, 2, 4
 PROPOSAL_UNIFORM, PROPOSAL_LEARNED, PROPOSAL_CACHE = 0, 1, 2
 BVH_SAH, BVH_LBVH = 0, 1
-MAX_LIGHT_CLUSTERS = 256
+MAX_LIGHT_CLUSTERS = 1024
 PROPOSAL_PARAMS = 31168
 PROPOSAL_PARAMS_WIDE = 78464
 

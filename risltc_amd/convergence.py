@@ -47,7 +47,8 @@ def accumulate(ctx, scene, vis, frames, seed0, learned=None, checkpoints=(), ref
         ctx.set_tuning(**dict(tuning, lds_lights=lk["lds_lights"]))
     if learned:
         ctx.cluster_lights(learned.get("clusters", 256), 20)
-        ctx.proposal_init(learned.get("seed", 1), width=learned.get("width", 64))
+        ctx.proposal_init(learned.get("seed", 1), width=learned.get("width", 64),
+                          outputs=learned.get("outputs", 256))
         if learned.get("cache_log2"):
             ctx.proposal_cache(learned["cache_log2"], learned.get("cache_cell", 0.5), learned.get("cache_prior", 2.0),
                                learned.get("cache_per_light", False))
@@ -56,8 +57,8 @@ def accumulate(ctx, scene, vis, frames, seed0, learned=None, checkpoints=(), ref
     finally:
         ctx.proposal_cache(0)
         ctx.set_tuning(**tuning)
-        if learned and learned.get("width", 64) != 64:
-            ctx.proposal_set_width(64)
+        if learned and (learned.get("width", 64), learned.get("outputs", 256)) != (64, 256):
+            ctx.proposal_set_network(64, 256)
 
 
 def _frames(ctx, scene, vis, frames, seed0, learned, checkpoints, ref, s, out, acc, P):

@@ -25,7 +25,8 @@ static_assert(sizeof(risltc_sample_record_t) == 4 * RISLTC_SAMPLE_RECORD_FLOATS,
 static_assert(sizeof(risltc_candidate_record_t) == 4 * RISLTC_CANDIDATE_RECORD_WORDS, "candidate record size");
 static_assert(rl::kMlpParams == RISLTC_PROPOSAL_PARAMS, "MLP parameter count");
 static_assert(rl::mlp_layout(rl::kMlpHidWide).params == RISLTC_PROPOSAL_PARAMS_WIDE, "wide MLP parameter count");
-static_assert(rl::kMlpOut == RISLTC_MAX_LIGHT_CLUSTERS, "MLP outputs = clusters");
+static_assert(rl::kMlpOutWide == RISLTC_MAX_LIGHT_CLUSTERS, "the largest network's outputs = clusters");
+static_assert(rl::kCacheClusterSlots == RISLTC_MAX_LIGHT_CLUSTERS, "cache slots in cluster mode");
 
 struct DevBuf {
     void* p = nullptr;
@@ -744,21 +745,21 @@ int risltc_set_light_clusters(risltc_context_t* c, const uint32_t* cluster_of_li
     const uint32_t n = c->light_count;
     if (n == 0) return fail(c, RISLTC_E_STATE, "risltc_set_lights has not been called");
     std::vector<uint32_t> offsets(cluster_count + 1, 0), sorted(n);
-    std::vector<uint8_t> of(n);
+    std::vector<uint16_t> of(n);
     for (uint32_t i = 0; i < n; ++i) {
         if (cluster_of_light[i] >= cluster_count)
             return fail(c, RISLTC_E_ARG, "light %u in cluster %u of %u", i, cluster_of_light[i], cluster_count);
-        of[i] = (uint8_t)cluster_of_light[i];
+        of[i] = (uint16_t)cluster_of_light[i];
         offsets[cluster_of_light[i] + 1]++;
     }
     for (uint32_t k = 0; k < cluster_count; ++k) offsets[k + 1] += offsets[k];
     std::vector<uint32_t> fill(offsets.begin(), offsets.end() - 1);
     for (uint32_t i = 0; i < n; ++i) sorted[fill[cluster_of_light[i]]++] = i;  // stable by light index
     HIPC(c, hipSetDevice(c->device));
-    HIPC(c, c->cluster_of_light.ensure(n));
+    HIPC(c, c->cluster_of_light.ensure(n * sizeof(uint16_t)));
     HIPC(c, c->cluster_offsets.ensure((cluster_count + 1) * sizeof(uint32_t)));
     HIPC(c, c->cluster_lights.ensure(n * sizeof(uint32_t)));
-    HIPC(c, hipMemcpy(c->cluster_of_light.p, of.data(), n, hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(c->cluster_of_light.p, of.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice));
     HIPC(c, hipMemcpy(c->cluster_offsets.p, offsets.data(), offsets.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIPC(c, hipMemcpy(c->cluster_lights.p, sorted.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
     c->cluster_count = cluster_count;
@@ -783,7 +784,7 @@ int risltc_proposal_cache(risltc_context_t* c, uint32_t log2_entries, float cell
     if (!(prior > 0.0f) || !std::isfinite(prior)) return fail(c, RISLTC_E_ARG, "prior strength %g", (double)prior);
     if (per_light > 1) return fail(c, RISLTC_E_ARG, "per_light %u", per_light);
     if (per_light && c->light_count == 0) return fail(c, RISLTC_E_STATE, "risltc_set_lights has not been called");
-    const uint32_t slots = per_light ? c->light_count : (uint32_t)rl::kMlpOut;
+    const uint32_t slots = per_light ? c->light_count : (uint32_t)rl::kCacheClusterSlots;
     const size_t words = ((size_t)2 * slots) << log2_entries;
     if (words > ((size_t)1 << 30)) return fail(c, RISLTC_E_UNSUPPORTED, "cache larger than 4 GiB");
     HIPC(c, hipSetDevice(c->device));
@@ -951,7 +952,7 @@ int risltc_proposal_init(risltc_context_t* c, uint32_t seed) {
     } layers[4] = {{M.pw1, M.hid, rl::kMlpIn, 1.0f},
                    {M.pw2, M.hid, M.hid, 1.0f},
                    {M.pw3, M.hid, M.hid, 1.0f},
-                   {M.pw4, rl::kMlpOut, M.hid, 0.1f}};
+                   {M.pw4, M.out, M.hid, 0.1f}};
     for (const L& l : layers) {
         const float lim = l.gain * std::sqrt(6.0f / (float)l.nin);
         for (int e = 0; e < l.nout * l.nin; ++e) {
@@ -974,15 +975,23 @@ int risltc_proposal_get_params(risltc_context_t* c, float* out, float* grad_out)
     return RISLTC_OK;
 }
 
-int risltc_proposal_set_width(risltc_context_t* c, uint32_t width) {
+int risltc_proposal_set_network(risltc_context_t* c, uint32_t width, uint32_t outputs) {
     if (!c) return RISLTC_E_ARG;
     if (width != (uint32_t)rl::kMlpHid && width != (uint32_t)rl::kMlpHidWide)
         return fail(c, RISLTC_E_UNSUPPORTED, "hidden width %u: 64 or 128", width);
-    if ((int)width != c->mlp_L.hid) {
-        c->mlp_L = rl::mlp_layout((int)width);
+    if (outputs != (uint32_t)rl::kMlpOut && outputs != (uint32_t)rl::kMlpOutWide)
+        return fail(c, RISLTC_E_UNSUPPORTED, "outputs %u: 256 or 1024", outputs);
+    if ((int)width != c->mlp_L.hid || (int)outputs != c->mlp_L.out) {
+        c->mlp_L = rl::mlp_layout((int)width, (int)outputs);
         c->mlp_ready = false;  // risltc_proposal_init or _set_params next
+        c->prop_width = c->prop_height = 0;  // tables of the previous network
     }
     return RISLTC_OK;
+}
+
+int risltc_proposal_set_width(risltc_context_t* c, uint32_t width) {
+    if (!c) return RISLTC_E_ARG;
+    return risltc_proposal_set_network(c, width, (uint32_t)c->mlp_L.out);
 }
 
 uint32_t risltc_proposal_param_count(const risltc_context_t* c) { return c ? (uint32_t)c->mlp_L.params : 0u; }
@@ -1019,7 +1028,10 @@ int risltc_proposal_update(risltc_context_t* c, const risltc_frame_constants_t* 
     if (P > 0xFFFFFFFFull / 2) return fail(c, RISLTC_E_UNSUPPORTED, "viewport too large");
     hipStream_t st = (hipStream_t)stream;
     HIPC(c, hipSetDevice(c->device));
-    HIPC(c, c->prop_table.ensure(P * rl::kPropBytes));
+    if (c->cluster_count > (uint32_t)c->mlp_L.out)
+        return fail(c, RISLTC_E_STATE, "%u clusters for a network of %d outputs", c->cluster_count, c->mlp_L.out);
+    const size_t tb = (size_t)rl::prop_bytes(c->mlp_L.out);
+    HIPC(c, c->prop_table.ensure(P * tb));
     rl::ProposalArgs a;
     memset(&a, 0, sizeof(a));
     a.L = c->mlp_L;
@@ -1035,7 +1047,7 @@ int risltc_proposal_update(risltc_context_t* c, const risltc_frame_constants_t* 
     HIPC(c, hipEventRecord(c->ev[4], st));
     HIPC(c, rl::launch_proposal(a, st));
     HIPC(c, hipEventRecord(c->ev[5], st));
-    if (table_out) HIPC(c, hipMemcpyAsync(table_out, a.out, P * rl::kPropBytes, hipMemcpyDeviceToDevice, st));
+    if (table_out) HIPC(c, hipMemcpyAsync(table_out, a.out, P * tb, hipMemcpyDeviceToDevice, st));
     c->prop_width = W;
     c->prop_height = H;
     HIPC(c, hipEventSynchronize(c->ev[5]));
@@ -1056,7 +1068,7 @@ int risltc_proposal_train(risltc_context_t* c, const risltc_frame_constants_t* k
         return fail(c, RISLTC_E_STATE, "the training records are for another viewport");
     hipStream_t st = (hipStream_t)stream;
     HIPC(c, hipSetDevice(c->device));
-    HIPC(c, c->train_scratch.ensure(rl::train_scratch_bytes(batch, c->mlp_L.hid)));
+    HIPC(c, c->train_scratch.ensure(rl::train_scratch_bytes(batch, c->mlp_L.hid, c->mlp_L.out)));
     rl::TrainArgs t;
     memset(&t, 0, sizeof(t));
     t.L = c->mlp_L;
@@ -1360,7 +1372,7 @@ static void fill_shade(risltc_context_t* c, const risltc_frame_constants_t* k, c
     uint32_t k2 = s->learned_floor_log2 ? s->learned_floor_log2 : 3u;
     a.learned_floor = std::ldexp(1.0f, -(int)k2);
     a.learned_scale = (1.0f - a.learned_floor) / 255.0f;
-    a.clusters.of_light = c->cluster_of_light.as<uint8_t>();
+    a.clusters.of_light = c->cluster_of_light.as<uint16_t>();
     a.clusters.offsets = c->cluster_offsets.as<uint32_t>();
     a.clusters.lights = c->cluster_lights.as<uint32_t>();
     a.clusters.count = c->cluster_count;
@@ -1438,6 +1450,7 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     a.debug_cands = dbg_c;
     a.learned_pass = learned ? 1u : 0u;
     a.proposal = s->proposal == RISLTC_PROPOSAL_LEARNED ? c->prop_table.as<uint8_t>() : nullptr;
+    a.proposal_ids16 = (a.proposal && c->mlp_L.out > 256) ? 1u : 0u;
     const bool cached = c->vcache.p && c->cluster_count > 0;
     if (cached) {
         if (c->vcache_per_light && c->vcache_slots != c->light_count)
@@ -1464,7 +1477,7 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     r.debug_samples = dbg_s;
     c->rec_samples = 0;
     if (c->cluster_count > 0) {
-        r.cluster_of_light = c->cluster_of_light.as<uint8_t>();
+        r.cluster_of_light = c->cluster_of_light.as<uint16_t>();
         if (!s->skip_training_records) {
             // training records for the proposal: one per pixel-sample
             HIPC(c, c->rec_cluster.ensure(S * sizeof(uint32_t)));
@@ -1495,12 +1508,13 @@ struct BatchPlan {
     size_t lds = 0;
     bool exact = false, shadows = false;
     bool learned = false, cached = false;  // learned views; the visibility cache is refreshed first
+    bool ids16 = false;                     // the views' tables have 2-byte cluster ids
     rl::VisCacheView cache = {};
     // whether a graph captured with plan `o` can run this plan's descriptors
     bool fits(const BatchPlan& o) const {
         return tile_off == o.tile_off && block_off == o.block_off && samples == o.samples && lds <= o.lds &&
                exact == o.exact && shadows == o.shadows && learned == o.learned && cached == o.cached &&
-               bvhs.size() <= o.bvh_capacity;
+               ids16 == o.ids16 && bvhs.size() <= o.bvh_capacity;
     }
 };
 
@@ -1541,6 +1555,9 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
     }
     p.exact = views[0].settings.exact_math != 0;
     p.learned = views[0].settings.proposal != RISLTC_PROPOSAL_UNIFORM;
+    p.ids16 = p.learned && c->mlp_L.out > 256;
+    if (p.learned && c->cluster_count > (uint32_t)c->mlp_L.out)
+        return fail(c, RISLTC_E_STATE, "%u clusters for a network of %d outputs", c->cluster_count, c->mlp_L.out);
     p.cached = p.learned && c->vcache.p != nullptr;
     if (p.cached) p.cache = vcache_view(c);
     if (allocate) {
@@ -1576,6 +1593,7 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
         if (p.learned) {
             a.learned_pass = 1u;
             a.proposal = w.settings.proposal == RISLTC_PROPOSAL_LEARNED ? w.proposal_table : nullptr;
+            a.proposal_ids16 = p.ids16 ? 1u : 0u;
             if (p.cached) a.cache = p.cache;
         }
         p.lds = std::max(p.lds, rl::shade_lds_bytes(a));
@@ -1585,7 +1603,7 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
         fill_resolve(&w.constants, &w.settings, w.visibility, a.samples, B.hits.as<uint32_t>() + off[v] * 2, w.out,
                      SR.triangle_count, r);
         if (p.cached) {
-            r.cluster_of_light = c->cluster_of_light.as<uint8_t>();
+            r.cluster_of_light = c->cluster_of_light.as<uint16_t>();
             r.cache = p.cache;
         }
         const uint32_t P = w.constants.viewport_size[0] * w.constants.viewport_size[1];
@@ -1622,7 +1640,7 @@ static int launch_plan(risltc_context_t* c, BatchBuffers& B, const BatchPlan& p,
     if (timed) HIPC(c, hipEventRecord(c->ev[0], st));
     if (p.cached) HIPC(c, rl::launch_refresh_cache(p.cache, st));
     HIPC(c, rl::launch_shade_batch(B.shade_args.as<rl::ShadeArgs>(), B.tile_offsets.as<uint32_t>(), n, p.tiles, p.lds,
-                                   p.exact, p.learned, st));
+                                   p.exact, p.learned ? (p.ids16 ? 2 : 1) : 0, st));
     if (timed) HIPC(c, hipEventRecord(c->ev[1], st));
     if (p.shadows)
         HIPC(c, rl::launch_trace(trace_args(c, B.bvh_table.as<rl::BvhView>(), B.rays.as<rl::ShadowRay>(),

@@ -67,13 +67,14 @@ constexpr uint32_t kLdsSphereMax = 2048;
 // Parameters are one FP32 array; every weight matrix is stored [out][in],
 // followed by its bias.
 constexpr int kMlpIn = 96, kMlpHid = 64, kMlpOut = 256;
-constexpr int kMlpHidWide = 128;
+constexpr int kMlpHidWide = 128, kMlpOutWide = 1024;
 struct MlpLayout {
     int hid;                                   // hidden width
     int pw1, pb1, pw2, pb2, pw3, pb3, pw4, pb4;  // offsets into the parameters
     int params;                                // parameter count
+    int out;                                   // outputs (cluster capacity)
 };
-__host__ __device__ constexpr MlpLayout mlp_layout(int h) {
+__host__ __device__ constexpr MlpLayout mlp_layout(int h, int o = kMlpOut) {
     return MlpLayout{h,
                      0,
                      h * kMlpIn,
@@ -82,8 +83,9 @@ __host__ __device__ constexpr MlpLayout mlp_layout(int h) {
                      h * kMlpIn + 2 * h + h * h,
                      h * kMlpIn + 2 * h + 2 * h * h,
                      h * kMlpIn + 3 * h + 2 * h * h,
-                     h * kMlpIn + 3 * h + 2 * h * h + kMlpOut * h,
-                     h * kMlpIn + 3 * h + 2 * h * h + kMlpOut * h + kMlpOut};
+                     h * kMlpIn + 3 * h + 2 * h * h + o * h,
+                     h * kMlpIn + 3 * h + 2 * h * h + o * h + o,
+                     o};
 }
 // the default layout (H = 64)
 constexpr int kPW1 = 0, kPB1 = kPW1 + kMlpHid * kMlpIn;
@@ -98,12 +100,18 @@ static_assert(mlp_layout(kMlpHidWide).params == 78464, "wide layout");
 // with the largest logits, weight round(255 exp(z - z_max))
 constexpr int kPropTop = 16;
 constexpr int kPropBytes = 2 * kPropTop;
+// with more than 256 outputs the ids are 2 bytes: ids in bytes 0..31
+// (slot s at 2 s), weights in bytes 32..47
+constexpr int kPropBytes16 = 3 * kPropTop;
+__host__ __device__ constexpr int prop_bytes(int outputs) { return outputs > 256 ? kPropBytes16 : kPropBytes; }
+// clusters a visibility cache in cluster mode has slots for
+constexpr int kCacheClusterSlots = kMlpOutWide;
 // default floor 1/8: scale = (1 - 0.125) / 255 in FP32
 constexpr float kLearnedFloor = 0.125f;
 constexpr float kLearnedScale = 0x1.c1c1c2p-9f;
 
 struct ClusterView {
-    const uint8_t* of_light;   // cluster of each light
+    const uint16_t* of_light;  // cluster of each light
     const uint32_t* offsets;   // count + 1 prefix offsets into lights
     const uint32_t* lights;    // light indices sorted by cluster
     uint32_t count;
@@ -175,7 +183,7 @@ struct TrainArgs {
     void* scratch;                 // train_scratch_bytes(B)
     float* loss;                   // out: batch loss (device, one float)
 };
-size_t train_scratch_bytes(uint32_t B, int hid);
+size_t train_scratch_bytes(uint32_t B, int hid, int out);
 hipError_t launch_train_step(const TrainArgs& a, hipStream_t s);
 // bf16 copies of the master parameters (after set_params)
 hipError_t launch_mlp_sync(const MlpLayout& L, const float* params, uint16_t* wf, uint16_t* wb, hipStream_t s);
@@ -202,9 +210,10 @@ struct ShadeArgs {
     // learned pass (1): three draws per candidate, the learned source when
     // proposal is set, the cache factor in the target when cache.vis is set
     uint32_t learned_pass;
-    // learned source (null = uniform): kPropBytes per pixel, mixed half and
-    // half with the uniform source
+    // learned source (null = uniform): kPropBytes (kPropBytes16 when
+    // proposal_ids16) per pixel, mixed half and half with the uniform source
     const uint8_t* proposal;
+    uint32_t proposal_ids16;
     ClusterView clusters;
     // learned target factor f + v (1 - f) / 255 of the visibility cache
     float learned_floor, learned_scale;  // f and (1 - f) / 255 (FP32)
@@ -247,7 +256,7 @@ struct ResolveArgs {
     // training records (optional): per pixel-sample cluster of the selected
     // light (~0 = no traced shadow ray) and the fraction of its traced shadow
     // rays that were unoccluded
-    const uint8_t* cluster_of_light;
+    const uint16_t* cluster_of_light;
     uint32_t* rec_cluster;
     float* rec_y;
     // visibility cache (optional): every traced shadow ray of a sample is
@@ -264,8 +273,9 @@ hipError_t launch_shade_baseline(const ShadeArgs& a, uint32_t technique, bool ex
 uint32_t shade_tiles(const ShadeArgs& a);
 size_t shade_lds_bytes(const ShadeArgs& a);
 // views and their tile offsets (count + 1 entries) in device memory
+// mode: 0 plain, 1 learned, 2 learned with 2-byte table ids (every view)
 hipError_t launch_shade_batch(const ShadeArgs* d_views, const uint32_t* d_tile_offsets, uint32_t count,
-                              uint32_t total_tiles, size_t lds, bool exact, bool learned, hipStream_t s);
+                              uint32_t total_tiles, size_t lds, bool exact, int mode, hipStream_t s);
 hipError_t launch_resolve_batch(const ResolveArgs* d_views, const uint32_t* d_block_offsets, uint32_t count,
                                 uint32_t total_blocks, hipStream_t s);
 hipError_t launch_trace(const TraceArgs& a, hipStream_t s);

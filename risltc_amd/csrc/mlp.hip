@@ -164,19 +164,21 @@ constexpr int ld_h() { return HID + 8; }
 // LDS of the inference kernel. Width 64: all weights, 8 waves (135 KB).
 // Width 128: the hidden layers' weights (96 KB) and 4 waves; the output
 // layer's weights (64 KB) are read from global memory, where they stay in L2.
-template <int HID, int WAVES, bool W4LDS>
+template <int HID, int OUT, int WAVES, bool W4LDS>
 struct PropShared {
     uint16_t w1[HID * kLdW1];
     uint16_t w2[HID * ld_h<HID>()];
     uint16_t w3[HID * ld_h<HID>()];
-    uint16_t w4[W4LDS ? kMlpOut * ld_h<HID>() : 8];
-    float bias[3 * HID + kMlpOut];
-    uint32_t csize[kMlpOut];
+    uint16_t w4[W4LDS ? OUT * ld_h<HID>() : 8];
+    float bias[3 * HID + OUT];
+    uint32_t csize[OUT];
     uint16_t act_in[WAVES][16 * kLdW1];
     uint16_t act_h[WAVES][2][16 * ld_h<HID>()];
 };
-static_assert(sizeof(PropShared<64, 8, true>) <= 160 * 1024, "width-64 LDS");
-static_assert(sizeof(PropShared<128, 4, false>) <= 160 * 1024, "width-128 LDS");
+static_assert(sizeof(PropShared<64, 256, 8, true>) <= 160 * 1024, "width-64 LDS");
+static_assert(sizeof(PropShared<128, 256, 4, false>) <= 160 * 1024, "width-128 LDS");
+static_assert(sizeof(PropShared<64, 1024, 8, false>) <= 160 * 1024, "width-64, 1024 outputs LDS");
+static_assert(sizeof(PropShared<128, 1024, 4, false>) <= 160 * 1024, "width-128, 1024 outputs LDS");
 
 // hidden layer on a 16-pixel tile: out[pixel][o] = relu(W[o] . act[pixel] + b[o])
 template <int HID, int K, int LDW, int LDA>
@@ -197,11 +199,11 @@ __device__ __forceinline__ void hidden_layer(const uint16_t* W, const float* bia
     }
 }
 
-template <int HID, int WAVES, bool W4LDS>
+template <int HID, int OUT, int WAVES, bool W4LDS>
 __global__ void __launch_bounds__(64 * WAVES) proposal_kernel(ProposalArgs A) {
     constexpr int kLdH = ld_h<HID>();
-    constexpr MlpLayout L = mlp_layout(HID);
-    __shared__ PropShared<HID, WAVES, W4LDS> sh;
+    constexpr MlpLayout L = mlp_layout(HID, OUT);
+    __shared__ PropShared<HID, OUT, WAVES, W4LDS> sh;
     const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // stage the bf16 weights (padded rows) and biases
     for (int e = tid; e < HID * kMlpIn; e += blockDim.x) sh.w1[(e / kMlpIn) * kLdW1 + e % kMlpIn] = A.wf[L.pw1 + e];
@@ -210,7 +212,7 @@ __global__ void __launch_bounds__(64 * WAVES) proposal_kernel(ProposalArgs A) {
         sh.w3[(e / HID) * kLdH + e % HID] = A.wf[L.pw3 + e];
     }
     if constexpr (W4LDS)
-        for (int e = tid; e < kMlpOut * HID; e += blockDim.x) sh.w4[(e / HID) * kLdH + e % HID] = A.wf[L.pw4 + e];
+        for (int e = tid; e < OUT * HID; e += blockDim.x) sh.w4[(e / HID) * kLdH + e % HID] = A.wf[L.pw4 + e];
     // the output layer's weights: [out][in] rows of HID bf16
     const uint16_t* w4 = W4LDS ? sh.w4 : A.wf + L.pw4;
     constexpr int kLdW4 = W4LDS ? kLdH : HID;
@@ -219,7 +221,7 @@ __global__ void __launch_bounds__(64 * WAVES) proposal_kernel(ProposalArgs A) {
         sh.bias[HID + e] = A.params[L.pb2 + e];
         sh.bias[2 * HID + e] = A.params[L.pb3 + e];
     }
-    for (int e = tid; e < kMlpOut; e += blockDim.x) {
+    for (int e = tid; e < OUT; e += blockDim.x) {
         sh.bias[3 * HID + e] = A.params[L.pb4 + e];
         sh.csize[e] = (uint32_t)e < A.cluster_count ? A.cluster_offsets[e + 1] - A.cluster_offsets[e] : 0u;
     }
@@ -276,7 +278,7 @@ __global__ void __launch_bounds__(64 * WAVES) proposal_kernel(ProposalArgs A) {
             float bv[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
             int bi[4] = {0, 0, 0, 0};
 #pragma unroll 2
-            for (int t = 0; t < kMlpOut / 16; ++t) {
+            for (int t = 0; t < OUT / 16; ++t) {
                 f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
                 for (int ks = 0; ks < HID / 32; ++ks)
@@ -286,7 +288,7 @@ __global__ void __launch_bounds__(64 * WAVES) proposal_kernel(ProposalArgs A) {
                 for (int i = 0; i < 4; ++i) {
                     int cl = t * 16 + 4 * g + i;
                     float z = acc[i] + sh.bias[3 * HID + cl];
-                    if (A.logits && in) A.logits[(size_t)pixel * kMlpOut + cl] = z;
+                    if (A.logits && in) A.logits[(size_t)pixel * OUT + cl] = z;
                     if (sh.csize[cl] == 0u) continue;
                     // insertion into the sorted best-4 (ties keep the smaller id)
                     if (z > bv[3]) {
@@ -309,7 +311,7 @@ __global__ void __launch_bounds__(64 * WAVES) proposal_kernel(ProposalArgs A) {
             float zmax = bv[0];
             zmax = fmax_(zmax, __shfl_xor(zmax, 16));
             zmax = fmax_(zmax, __shfl_xor(zmax, 32));
-            uint32_t ids = 0, ws = 0;
+            uint32_t ws = 0;
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 uint32_t w = 0;
@@ -318,13 +320,19 @@ __global__ void __launch_bounds__(64 * WAVES) proposal_kernel(ProposalArgs A) {
                     w = e >= 255.0f ? 255u : (uint32_t)e;
                 }
                 if (w == 0u) bi[s] = 0;
-                ids |= (uint32_t)bi[s] << (8 * s);
                 ws |= w << (8 * s);
             }
             if (in) {
-                uint32_t* o = reinterpret_cast<uint32_t*>(A.out + (size_t)pixel * kPropBytes);
-                o[g] = ids;
-                o[4 + g] = ws;
+                uint32_t* o = reinterpret_cast<uint32_t*>(A.out + (size_t)pixel * prop_bytes(OUT));
+                if constexpr (OUT > 256) {
+                    // slot 4 g + s: id (2 bytes) at 2 (4 g + s), weight at 32 + 4 g + s
+                    o[2 * g] = (uint32_t)bi[0] | ((uint32_t)bi[1] << 16);
+                    o[2 * g + 1] = (uint32_t)bi[2] | ((uint32_t)bi[3] << 16);
+                    o[8 + g] = ws;
+                } else {
+                    o[g] = (uint32_t)bi[0] | ((uint32_t)bi[1] << 8) | ((uint32_t)bi[2] << 16) | ((uint32_t)bi[3] << 24);
+                    o[4 + g] = ws;
+                }
             }
             wave_sync();
         }
@@ -336,17 +344,22 @@ hipError_t launch_proposal(const ProposalArgs& a, hipStream_t s) {
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    const bool wide = a.L.hid == kMlpHidWide;
-    if (!wide && a.L.hid != kMlpHid) return hipErrorInvalidValue;
+    const bool wide = a.L.hid == kMlpHidWide, many = a.L.out == kMlpOutWide;
+    if ((!wide && a.L.hid != kMlpHid) || (!many && a.L.out != kMlpOut)) return hipErrorInvalidValue;
     const uint32_t waves = wide ? 4u : 8u;
     uint32_t groups = (a.pixel_count + 63u) / 64u;
     uint32_t blocks = (groups + waves - 1u) / waves;
     uint32_t cap = (uint32_t)(cus > 0 ? cus : 1);
     if (blocks > cap) blocks = cap;
-    if (wide)
-        hipLaunchKernelGGL((proposal_kernel<kMlpHidWide, 4, false>), dim3(blocks), dim3(64 * 4), 0, s, a);
+    dim3 grid(blocks), block(64 * waves);
+    if (wide && many)
+        hipLaunchKernelGGL((proposal_kernel<kMlpHidWide, kMlpOutWide, 4, false>), grid, block, 0, s, a);
+    else if (wide)
+        hipLaunchKernelGGL((proposal_kernel<kMlpHidWide, kMlpOut, 4, false>), grid, block, 0, s, a);
+    else if (many)
+        hipLaunchKernelGGL((proposal_kernel<kMlpHid, kMlpOutWide, 8, false>), grid, block, 0, s, a);
     else
-        hipLaunchKernelGGL((proposal_kernel<kMlpHid, 8, true>), dim3(blocks), dim3(64 * 8), 0, s, a);
+        hipLaunchKernelGGL((proposal_kernel<kMlpHid, kMlpOut, 8, true>), grid, block, 0, s, a);
     return hipGetLastError();
 }
 
@@ -437,7 +450,7 @@ __global__ void train_gather_kernel(TrainArgs A, uint16_t* X, uint16_t* Xt, uint
     bool ok = encode_features(A.f, pixel, f);
     uint32_t c = A.rec_cluster[item];
     float w = 1.0f;
-    if (!ok || c >= (uint32_t)kMlpOut) {
+    if (!ok || c >= (uint32_t)A.L.out) {
         c = 0u;
         w = 0.0f;
     }
@@ -465,11 +478,11 @@ __global__ void batch_sum_kernel(const float* y, uint32_t B, float* out) {
 }
 
 // dL/dz = (y_b / Y) (softmax(z_b) - onehot(c_b)); per-row loss terms. A
-// work-group takes 64 rows, one wave per row at a time (each lane 4 of the
-// 256 logits, coalesced), and writes the transposed gradient Gt through LDS
-// so that each lane stores one 128-byte line per column.
-constexpr int kSgRows = 64;
-constexpr int kSgLd = kMlpOut + 8;  // u16 row stride of the LDS tile
+// work-group takes ROWS rows, one wave per row at a time (each lane OUT / 64
+// of the logits, coalesced), and writes the transposed gradient Gt through
+// LDS so that each thread stores whole 16-byte pieces of its columns.
+template <int OUT>
+constexpr int sg_rows() { return OUT > 256 ? 16 : 64; }
 
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
@@ -482,53 +495,61 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+template <int OUT>
 __global__ void __launch_bounds__(256) softmax_grad_kernel(const float* Z, const uint32_t* target, const float* y,
                                                            const float* ysum, uint32_t B, uint16_t* G, uint16_t* Gt,
                                                            float* row_loss) {
-    static_assert(kMlpOut == 256, "four logits per lane");
-    __shared__ uint16_t tile[kSgRows * kSgLd];
+    constexpr int ROWS = sg_rows<OUT>(), PER = OUT / 64, LD = OUT + 8;
+    __shared__ uint16_t tile[ROWS * LD];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t b0 = blockIdx.x * kSgRows;  // B is a multiple of kSgRows
+    const uint32_t b0 = blockIdx.x * ROWS;  // B is a multiple of ROWS
     const float Y = *ysum;
-    for (uint32_t rr = 0; rr < kSgRows / 4; ++rr) {
-        const uint32_t r = wave * (kSgRows / 4) + rr, b = b0 + r;
-        const float* z = Z + (size_t)b * kMlpOut;
-        float v[4];
+    for (uint32_t rr = 0; rr < ROWS / 4; ++rr) {
+        const uint32_t r = wave * (ROWS / 4) + rr, b = b0 + r;
+        const float* z = Z + (size_t)b * OUT;
+        float v[PER];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = z[lane + 64u * k];
-        const float zm = wave_max(fmax_(fmax_(v[0], v[1]), fmax_(v[2], v[3])));
-        float e[4];
+        for (int k = 0; k < PER; ++k) v[k] = z[lane + 64u * k];
+        float m = v[0];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) e[k] = __expf(v[k] - zm);
-        const float s = wave_sum((e[0] + e[1]) + (e[2] + e[3]));
+        for (int k = 1; k < PER; ++k) m = fmax_(m, v[k]);
+        const float zm = wave_max(m);
+        float e[PER], part = 0.0f;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            e[k] = __expf(v[k] - zm);
+            part += e[k];
+        }
+        const float s = wave_sum(part);
         const float wb = Y > 0.0f ? y[b] / Y : 0.0f;
         const uint32_t c = target[b];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < PER; ++k) {
             const uint32_t n = lane + 64u * k;
             float d = wb * (e[k] / s - (n == c ? 1.0f : 0.0f));
             uint16_t h = bf16_bits(d);
-            G[(size_t)b * kMlpOut + n] = h;
-            tile[r * kSgLd + n] = h;
+            G[(size_t)b * OUT + n] = h;
+            tile[r * LD + n] = h;
         }
         // z[c] from the lane that holds it (c is the same for the whole wave)
         float vc = v[0];
 #pragma unroll
-        for (int k = 1; k < 4; ++k)
+        for (int k = 1; k < PER; ++k)
             if ((c >> 6) == (uint32_t)k) vc = v[k];
         const float zc = __shfl(vc, (int)(c & 63u));
         if (lane == 0) row_loss[b] = wb > 0.0f ? -wb * (zc - zm - __logf(s)) : 0.0f;
     }
     __syncthreads();
-    // Gt[n][b0 .. b0 + 63]: thread n writes its column as 8 x 16 bytes
-    const uint32_t n = threadIdx.x;
+    // Gt[n][b0 .. b0 + ROWS - 1], 8 rows per 16-byte store
+    for (uint32_t n = threadIdx.x; n < (uint32_t)OUT; n += blockDim.x) {
 #pragma unroll
-    for (int q = 0; q < kSgRows; q += 8) {
-        uint32_t w[4];
+        for (int q = 0; q < ROWS; q += 8) {
+            uint32_t w[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            w[j] = (uint32_t)tile[(q + 2 * j) * kSgLd + n] | ((uint32_t)tile[(q + 2 * j + 1) * kSgLd + n] << 16);
-        *reinterpret_cast<uint4*>(Gt + (size_t)n * B + b0 + q) = make_uint4(w[0], w[1], w[2], w[3]);
+            for (int j = 0; j < 4; ++j)
+                w[j] = (uint32_t)tile[(q + 2 * j) * LD + n] | ((uint32_t)tile[(q + 2 * j + 1) * LD + n] << 16);
+            *reinterpret_cast<uint4*>(Gt + (size_t)n * B + b0 + q) = make_uint4(w[0], w[1], w[2], w[3]);
+        }
     }
 }
 
@@ -578,7 +599,7 @@ __device__ __forceinline__ void layer_of(const MlpLayout& L, int e, int& off, in
     } else if (e >= L.pw3 && e < L.pb3) {
         off = L.pw3, nout = L.hid, nin = L.hid;
     } else if (e >= L.pw4 && e < L.pb4) {
-        off = L.pw4, nout = kMlpOut, nin = L.hid;
+        off = L.pw4, nout = L.out, nin = L.hid;
     } else {
         off = -1, nout = nin = 0;
     }
@@ -631,7 +652,7 @@ struct TrainScratch {
     uint32_t* target;
 };
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-TrainScratch carve(void* base, uint32_t B, int hid, size_t* total) {
+TrainScratch carve(void* base, uint32_t B, int hid, int out, size_t* total) {
     TrainScratch t;
     size_t off = 0;
     char* p = static_cast<char*>(base);
@@ -648,29 +669,29 @@ TrainScratch carve(void* base, uint32_t B, int hid, size_t* total) {
     t.H2t = (uint16_t*)take(2ull * B * hid);
     t.H3 = (uint16_t*)take(2ull * B * hid);
     t.H3t = (uint16_t*)take(2ull * B * hid);
-    t.G4 = (uint16_t*)take(2ull * B * kMlpOut);
-    t.G4t = (uint16_t*)take(2ull * B * kMlpOut);
+    t.G4 = (uint16_t*)take(2ull * B * out);
+    t.G4t = (uint16_t*)take(2ull * B * out);
     t.G3 = (uint16_t*)take(2ull * B * hid);
     t.G3t = (uint16_t*)take(2ull * B * hid);
     t.G2 = (uint16_t*)take(2ull * B * hid);
     t.G2t = (uint16_t*)take(2ull * B * hid);
     t.G1 = (uint16_t*)take(2ull * B * hid);
     t.G1t = (uint16_t*)take(2ull * B * hid);
-    t.Z = (float*)take(4ull * B * kMlpOut);
+    t.Z = (float*)take(4ull * B * out);
     t.y = (float*)take(4ull * B);
     t.ysum = (float*)take(4);
     t.row_loss = (float*)take(4ull * B);
     const uint32_t slices = (B + kTrainSlice - 1) / kTrainSlice;
-    t.partial = (float*)take(4ull * slices * kMlpOut * hid);
+    t.partial = (float*)take(4ull * slices * out * hid);
     t.target = (uint32_t*)take(4ull * B);
     if (total) *total = off;
     return t;
 }
 }  // namespace
 
-size_t train_scratch_bytes(uint32_t B, int hid) {
+size_t train_scratch_bytes(uint32_t B, int hid, int out) {
     size_t total = 0;
-    carve(nullptr, B, hid, &total);
+    carve(nullptr, B, hid, out, &total);
     return total;
 }
 
@@ -678,7 +699,8 @@ hipError_t launch_train_step(const TrainArgs& a, hipStream_t s) {
     if (a.B == 0 || a.B % 256u != 0u || a.sample_count == 0) return hipErrorInvalidValue;
     const MlpLayout& L4 = a.L;
     const uint32_t H = (uint32_t)L4.hid;
-    TrainScratch t = carve(a.scratch, a.B, L4.hid, nullptr);
+    TrainScratch t = carve(a.scratch, a.B, L4.hid, L4.out, nullptr);
+    const uint32_t O = (uint32_t)L4.out;
     const uint32_t B = a.B, slices = (B + kTrainSlice - 1) / kTrainSlice;
     hipLaunchKernelGGL(train_gather_kernel, dim3(B / 256u), dim3(256), 0, s, a, t.X, t.Xt, t.target, t.y);
     hipLaunchKernelGGL(batch_sum_kernel, dim3(1), dim3(256), 0, s, t.y, B, t.ysum);
@@ -690,10 +712,14 @@ hipError_t launch_train_step(const TrainArgs& a, hipStream_t s) {
     gemm<kEpiHidden>(g, 1, s);
     g = GemmArgs{t.H2, a.wf + L4.pw3, B, H, H, H, H, H, a.params + L4.pb3, t.H3, t.H3t, nullptr, nullptr};
     gemm<kEpiHidden>(g, 1, s);
-    g = GemmArgs{t.H3, a.wf + L4.pw4, B, kMlpOut, H, H, H, H, a.params + L4.pb4, nullptr, nullptr, t.Z, nullptr};
+    g = GemmArgs{t.H3, a.wf + L4.pw4, B, O, H, H, H, H, a.params + L4.pb4, nullptr, nullptr, t.Z, nullptr};
     gemm<kEpiLogits>(g, 1, s);
-    hipLaunchKernelGGL(softmax_grad_kernel, dim3(B / kSgRows), dim3(256), 0, s, t.Z, t.target, t.y, t.ysum, B, t.G4,
-                       t.G4t, t.row_loss);
+    if (O > 256u)
+        hipLaunchKernelGGL(softmax_grad_kernel<kMlpOutWide>, dim3(B / sg_rows<kMlpOutWide>()), dim3(256), 0, s, t.Z,
+                           t.target, t.y, t.ysum, B, t.G4, t.G4t, t.row_loss);
+    else
+        hipLaunchKernelGGL(softmax_grad_kernel<kMlpOut>, dim3(B / sg_rows<kMlpOut>()), dim3(256), 0, s, t.Z, t.target,
+                           t.y, t.ysum, B, t.G4, t.G4t, t.row_loss);
     hipLaunchKernelGGL(loss_sum_kernel, dim3(1), dim3(256), 0, s, t.row_loss, B, a.loss);
     // layer 4: dW4[o][i] = sum_b G4[b][o] H3[b][i]; dH3 = G4 . W4, masked by H3 > 0
     struct Layer {
@@ -701,7 +727,7 @@ hipError_t launch_train_step(const TrainArgs& a, hipStream_t s) {
         uint32_t nout, nin;
         int w, bias;
     };
-    const Layer layers[4] = {{t.G4t, t.H3t, kMlpOut, H, L4.pw4, L4.pb4}, {t.G3t, t.H2t, H, H, L4.pw3, L4.pb3},
+    const Layer layers[4] = {{t.G4t, t.H3t, O, H, L4.pw4, L4.pb4}, {t.G3t, t.H2t, H, H, L4.pw3, L4.pb3},
                              {t.G2t, t.H1t, H, H, L4.pw2, L4.pb2}, {t.G1t, t.Xt, H, kMlpIn, L4.pw1, L4.pb1}};
     const uint16_t* Gs[4] = {t.G4, t.G3, t.G2, t.G1};
     uint16_t* Gn[4] = {t.G3, t.G2, t.G1, nullptr};

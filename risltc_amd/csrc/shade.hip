@@ -1050,8 +1050,10 @@ __device__ __forceinline__ uint32_t lcg_jump(uint32_t s, uint32_t d) {
 }
 
 // ---- learned proposal: candidate draws ------------------------------------------
-// Per pixel table (kPropBytes): cluster ids in bytes 0..15, weights in 16..31,
-// T = sum of the weights. A candidate takes two draws a, b:
+// Per pixel table: 16 cluster ids and 16 byte weights, T = sum of the
+// weights. kPropBytes: ids in bytes 0..15, weights in 16..31; kPropBytes16
+// (networks with more than 256 outputs, I16): 2-byte ids in bytes 0..31,
+// weights in 32..47. A candidate takes two draws a, b:
 //   T == 0        -> light_index_from(a, N);
 //   a < 1/2       -> light_index_from(2 a, N)          (the uniform half);
 //   otherwise     -> t = min(T - 1, int((2 a - 1) T)), slot s = first with
@@ -1061,21 +1063,54 @@ __device__ __forceinline__ uint32_t lcg_jump(uint32_t s, uint32_t d) {
 //                    light_index_from(b, N)).
 // Its source density is q = 1/(2N) + w_c / (2 T |c|) (w_c = 0 for clusters
 // not in the table), computed in the same operation order by the oracle.
+template <bool I16>
 struct PropTable {
-    uint32_t ids[4], ws[4];
+    static constexpr int kIdWords = I16 ? 8 : 4;
+    uint32_t ids[kIdWords], ws[4];
     uint32_t T;
 };
 
 __device__ __forceinline__ uint32_t byte_of(const uint32_t* v, int s) { return (v[s >> 2] >> (8 * (s & 3))) & 0xFFu; }
 
+template <bool I16>
+__device__ __forceinline__ uint32_t id_of(const PropTable<I16>& P, int s) {
+    if constexpr (I16) return (P.ids[s >> 1] >> (16 * (s & 1))) & 0xFFFFu;
+    else return byte_of(P.ids, s);
+}
+
+// loads the pixel's table (zeros when tab is null)
+template <bool I16>
+__device__ __forceinline__ void load_prop_table(const uint8_t* tab, PropTable<I16>& P) {
+    P.T = 0u;
+#pragma unroll
+    for (int k = 0; k < PropTable<I16>::kIdWords; ++k) P.ids[k] = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) P.ws[k] = 0u;
+    if (!tab) return;
+    const uint4* t = reinterpret_cast<const uint4*>(tab);
+    uint4 t0 = t[0], t1 = t[1];
+    P.ids[0] = t0.x, P.ids[1] = t0.y, P.ids[2] = t0.z, P.ids[3] = t0.w;
+    if constexpr (I16) {
+        uint4 t2 = t[2];
+        P.ids[4] = t1.x, P.ids[5] = t1.y, P.ids[6] = t1.z, P.ids[7] = t1.w;
+        P.ws[0] = t2.x, P.ws[1] = t2.y, P.ws[2] = t2.z, P.ws[3] = t2.w;
+    } else {
+        P.ws[0] = t1.x, P.ws[1] = t1.y, P.ws[2] = t1.z, P.ws[3] = t1.w;
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < kPropTop; ++s2) P.T += byte_of(P.ws, s2);
+}
+
 struct ClusterLds {
-    const uint8_t* of_light;
+    const uint16_t* of_light;
     const uint32_t* offsets;
     const uint32_t* lights;
     uint32_t count;
 };
 
-__device__ __forceinline__ uint32_t draw_learned(uint32_t& seed, const PropTable& P, const ClusterLds& C, uint32_t N) {
+template <bool I16>
+__device__ __forceinline__ uint32_t draw_learned(uint32_t& seed, const PropTable<I16>& P, const ClusterLds& C,
+                                                 uint32_t N) {
     float a = noise_next(seed);
     float b = noise_next(seed);
     if (P.T == 0u) return light_index_from(a, N);
@@ -1089,7 +1124,7 @@ __device__ __forceinline__ uint32_t draw_learned(uint32_t& seed, const PropTable
     for (int s = 0; s < kPropTop; ++s) {
         uint32_t w = byte_of(P.ws, s);
         if (!found && t < acc + w) {
-            c = byte_of(P.ids, s);
+            c = id_of(P, s);
             found = true;
         }
         acc += w;
@@ -1103,8 +1138,9 @@ __device__ __forceinline__ uint32_t draw_learned(uint32_t& seed, const PropTable
     return C.lights[off + j];
 }
 
-template <bool X>
-__device__ __forceinline__ float learned_density(uint32_t idx, const PropTable& P, const ClusterLds& C, uint32_t N) {
+template <bool X, bool I16>
+__device__ __forceinline__ float learned_density(uint32_t idx, const PropTable<I16>& P, const ClusterLds& C,
+                                                 uint32_t N) {
     float q = 0.5f / (float)N;
     if (P.T == 0u) return 1.0f / (float)N;
     uint32_t c = C.of_light[idx];
@@ -1112,7 +1148,7 @@ __device__ __forceinline__ float learned_density(uint32_t idx, const PropTable& 
 #pragma unroll
     for (int s = kPropTop - 1; s >= 0; --s) {
         uint32_t w = byte_of(P.ws, s);
-        if (w != 0u && byte_of(P.ids, s) == c) wc = w;
+        if (w != 0u && id_of(P, s) == c) wc = w;
     }
     if (wc != 0u) {
         uint32_t size = C.offsets[c + 1] - C.offsets[c];
@@ -1126,7 +1162,7 @@ __device__ __forceinline__ float learned_density(uint32_t idx, const PropTable& 
 // p_hat * (floor + v * scale), scale = (1 - floor) / 255, v = the cache's
 // byte for the pixel's cell and l's slot (255, so a factor of about 1, while
 // the entry is empty). The oracle evaluates the same expression.
-__device__ __forceinline__ float cache_factor(const uint8_t* of_light, uint32_t idx, float floor, float scale,
+__device__ __forceinline__ float cache_factor(const uint16_t* of_light, uint32_t idx, float floor, float scale,
                                               const uint8_t* cache_row, const uint8_t* lds, int32_t lds_row,
                                               uint32_t per_light) {
     const uint32_t slot = per_light ? idx : (uint32_t)of_light[idx];
@@ -1156,7 +1192,7 @@ constexpr uint32_t kRowLdsMax = 1024, kRowsPerWave = 4;
 // bytes of the spheres and, for the learned source with every light staged,
 // the cluster tables (lights by cluster, offsets, cluster of each light)
 __device__ __host__ __forceinline__ size_t cluster_lds_end(uint32_t n_lds, bool cluster_tables, uint32_t clusters) {
-    return (size_t)n_lds * 16u + (cluster_tables ? (size_t)n_lds * 5u + ((size_t)clusters + 1u) * 4u : 0u);
+    return (size_t)n_lds * 16u + (cluster_tables ? (size_t)n_lds * 6u + ((size_t)clusters + 1u) * 4u : 0u);
 }
 
 __device__ __host__ __forceinline__ size_t row_lds_offset(uint32_t n_lds, bool cluster_tables, uint32_t clusters) {
@@ -1266,9 +1302,11 @@ __device__ __forceinline__ uint32_t cache_key(const VisCacheView& C, f3 p, f3 n)
 // waves vertically. The first lds_lights light spheres (all of them when
 // they fit) are staged in LDS; the cluster table too for the learned target
 // when every sphere is staged.
-template <bool X, bool DEBUG, bool LEARNED>
+template <bool X, bool DEBUG, int LMODE>
 __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, float4* sh_spheres) {
-    constexpr bool learned = LEARNED;
+    // LMODE: 0 plain, 1 learned (1-byte table ids), 2 learned (2-byte ids)
+    constexpr bool learned = LMODE != 0;
+    constexpr bool I16 = LMODE == 2;
     ClusterLds cl{A.clusters.of_light, A.clusters.offsets, A.clusters.lights, A.clusters.count};
     const uint32_t n_lds = A.scene.light_count < A.lds_lights ? A.scene.light_count : A.lds_lights;
     const bool all_lds = n_lds == A.scene.light_count;
@@ -1277,10 +1315,10 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
         if constexpr (learned) {
             if (all_lds) {
                 // after the spheres: lights by cluster (u32), offsets (u32),
-                // cluster of light (u8)
+                // cluster of light (u16)
                 uint32_t* l = reinterpret_cast<uint32_t*>(sh_spheres + n_lds);
                 uint32_t* o = l + n_lds;
-                uint8_t* c = reinterpret_cast<uint8_t*>(o + A.clusters.count + 1u);
+                uint16_t* c = reinterpret_cast<uint16_t*>(o + A.clusters.count + 1u);
                 for (uint32_t k = threadIdx.x; k < n_lds; k += blockDim.x) {
                     l[k] = A.clusters.lights[k];
                     c[k] = A.clusters.of_light[k];
@@ -1374,18 +1412,11 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
         Light L;
         // the pixel's proposal table, loaded per sample so that it is not
         // live across the final sample
-        PropTable pt;
-        pt.T = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) pt.ids[k] = pt.ws[k] = 0u;
-        if (learned && shade && A.proposal) {
-            const uint4* t = reinterpret_cast<const uint4*>(A.proposal + (size_t)pixel * kPropBytes);
-            uint4 t0 = t[0], t1 = t[1];
-            pt.ids[0] = t0.x, pt.ids[1] = t0.y, pt.ids[2] = t0.z, pt.ids[3] = t0.w;
-            pt.ws[0] = t1.x, pt.ws[1] = t1.y, pt.ws[2] = t1.z, pt.ws[3] = t1.w;
-#pragma unroll
-            for (int s2 = 0; s2 < kPropTop; ++s2) pt.T += byte_of(pt.ws, s2);
-        }
+        PropTable<I16> pt;
+        load_prop_table<I16>(learned && shade && A.proposal
+                                 ? A.proposal + (size_t)pixel * (I16 ? kPropBytes16 : kPropBytes)
+                                 : nullptr,
+                             pt);
         if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
             if (shade) chosen = (int32_t)light_index_from(noise_next(seed), N);
         } else if constexpr (DEBUG) {
@@ -1506,16 +1537,18 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                     const bool have = my_k != 0xFFFFFFFFu;
                     const uint32_t live_o = (uint32_t)__shfl((int)live, (int)my_o);
                     const uint32_t seed_o = (uint32_t)__shfl((int)seed_chunk, (int)my_o);
-                    PropTable pt_o;
+                    PropTable<I16> pt_o;
                     pt_o.T = 0u;
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) pt_o.ids[k] = pt_o.ws[k] = 0u;
+                    for (int k = 0; k < PropTable<I16>::kIdWords; ++k) pt_o.ids[k] = 0u;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) pt_o.ws[k] = 0u;
                     if constexpr (learned) {
 #pragma unroll
-                        for (int k = 0; k < 4; ++k) {
+                        for (int k = 0; k < PropTable<I16>::kIdWords; ++k)
                             pt_o.ids[k] = (uint32_t)__shfl((int)pt.ids[k], (int)my_o);
-                            pt_o.ws[k] = (uint32_t)__shfl((int)pt.ws[k], (int)my_o);
-                        }
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) pt_o.ws[k] = (uint32_t)__shfl((int)pt.ws[k], (int)my_o);
                         pt_o.T = (uint32_t)__shfl((int)pt.T, (int)my_o);
                     }
                     float p_raw = 0.0f;
@@ -1682,10 +1715,10 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
 // Three waves per SIMD: the register budget is 168 VGPRs (the final-sample
 // code spills); the measured gain over two waves comes from hiding the
 // latency of the candidate evaluation.
-template <bool X, bool DEBUG, bool LEARNED>
+template <bool X, bool DEBUG, int LMODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) shade_kernel(ShadeArgs A) {
     extern __shared__ float4 sh_spheres[];
-    shade_tile<X, DEBUG, LEARNED>(A, blockIdx.x, sh_spheres);
+    shade_tile<X, DEBUG, LMODE>(A, blockIdx.x, sh_spheres);
 }
 
 // the last v with offsets[v] <= i (offsets[0] = 0, n + 1 entries)
@@ -1944,14 +1977,14 @@ __device__ __forceinline__ uint32_t view_of(const uint32_t* offsets, uint32_t n,
 
 // Many views in one launch: work-group b shades tile b - tile_offsets[v] of
 // view v, with v's descriptor read from device memory (so a captured graph
-// can be replayed with new cameras and noise seeds). LEARNED: every view of
-// the launch runs the learned pass.
-template <bool X, bool LEARNED>
+// can be replayed with new cameras and noise seeds). LMODE as for
+// shade_tile: every view of the launch runs the same pass.
+template <bool X, int LMODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 shade_batch_kernel(const ShadeArgs* views, const uint32_t* tile_offsets, uint32_t count) {
     extern __shared__ float4 sh_spheres[];
     const uint32_t v = view_of(tile_offsets, count, blockIdx.x);
-    shade_tile<X, false, LEARNED>(views[v], blockIdx.x - tile_offsets[v], sh_spheres);
+    shade_tile<X, false, LMODE>(views[v], blockIdx.x - tile_offsets[v], sh_spheres);
 }
 
 // ---- resolve: sum the estimates in the reference's order -------------------
@@ -2085,21 +2118,30 @@ hipError_t launch_shade(const ShadeArgs& a, bool exact, hipStream_t s) {
     dim3 grid(shade_tiles(a)), block(256);
     bool dbg = a.debug_samples != nullptr || a.debug_cands != nullptr;
     size_t lds = shade_lds_bytes(a);
-    if (a.learned_pass) {
-        if (exact) {
-            if (dbg) hipLaunchKernelGGL((shade_kernel<true, true, true>), grid, block, lds, s, a);
-            else hipLaunchKernelGGL((shade_kernel<true, false, true>), grid, block, lds, s, a);
+    const int mode = a.learned_pass ? (a.proposal_ids16 ? 2 : 1) : 0;
+#define RISLTC_SHADE_LAUNCH(XX, DD, MM) hipLaunchKernelGGL((shade_kernel<XX, DD, MM>), grid, block, lds, s, a)
+    if (exact) {
+        if (dbg) {
+            if (mode == 2) RISLTC_SHADE_LAUNCH(true, true, 2);
+            else if (mode == 1) RISLTC_SHADE_LAUNCH(true, true, 1);
+            else RISLTC_SHADE_LAUNCH(true, true, 0);
         } else {
-            if (dbg) hipLaunchKernelGGL((shade_kernel<false, true, true>), grid, block, lds, s, a);
-            else hipLaunchKernelGGL((shade_kernel<false, false, true>), grid, block, lds, s, a);
+            if (mode == 2) RISLTC_SHADE_LAUNCH(true, false, 2);
+            else if (mode == 1) RISLTC_SHADE_LAUNCH(true, false, 1);
+            else RISLTC_SHADE_LAUNCH(true, false, 0);
         }
-    } else if (exact) {
-        if (dbg) hipLaunchKernelGGL((shade_kernel<true, true, false>), grid, block, lds, s, a);
-        else hipLaunchKernelGGL((shade_kernel<true, false, false>), grid, block, lds, s, a);
     } else {
-        if (dbg) hipLaunchKernelGGL((shade_kernel<false, true, false>), grid, block, lds, s, a);
-        else hipLaunchKernelGGL((shade_kernel<false, false, false>), grid, block, lds, s, a);
+        if (dbg) {
+            if (mode == 2) RISLTC_SHADE_LAUNCH(false, true, 2);
+            else if (mode == 1) RISLTC_SHADE_LAUNCH(false, true, 1);
+            else RISLTC_SHADE_LAUNCH(false, true, 0);
+        } else {
+            if (mode == 2) RISLTC_SHADE_LAUNCH(false, false, 2);
+            else if (mode == 1) RISLTC_SHADE_LAUNCH(false, false, 1);
+            else RISLTC_SHADE_LAUNCH(false, false, 0);
+        }
     }
+#undef RISLTC_SHADE_LAUNCH
     return hipGetLastError();
 }
 
@@ -2123,19 +2165,21 @@ hipError_t launch_shade_baseline(const ShadeArgs& a, uint32_t technique, bool ex
 }
 
 hipError_t launch_shade_batch(const ShadeArgs* d_views, const uint32_t* d_tile_offsets, uint32_t count,
-                              uint32_t total_tiles, size_t lds, bool exact, bool learned, hipStream_t s) {
+                              uint32_t total_tiles, size_t lds, bool exact, int mode, hipStream_t s) {
     if (count == 0 || total_tiles == 0) return hipSuccess;
     dim3 grid(total_tiles), block(256);
-    if (learned) {
-        if (exact)
-            hipLaunchKernelGGL((shade_batch_kernel<true, true>), grid, block, lds, s, d_views, d_tile_offsets, count);
-        else
-            hipLaunchKernelGGL((shade_batch_kernel<false, true>), grid, block, lds, s, d_views, d_tile_offsets, count);
-    } else if (exact) {
-        hipLaunchKernelGGL((shade_batch_kernel<true, false>), grid, block, lds, s, d_views, d_tile_offsets, count);
+#define RISLTC_BATCH_LAUNCH(XX, MM) \
+    hipLaunchKernelGGL((shade_batch_kernel<XX, MM>), grid, block, lds, s, d_views, d_tile_offsets, count)
+    if (exact) {
+        if (mode == 2) RISLTC_BATCH_LAUNCH(true, 2);
+        else if (mode == 1) RISLTC_BATCH_LAUNCH(true, 1);
+        else RISLTC_BATCH_LAUNCH(true, 0);
     } else {
-        hipLaunchKernelGGL((shade_batch_kernel<false, false>), grid, block, lds, s, d_views, d_tile_offsets, count);
+        if (mode == 2) RISLTC_BATCH_LAUNCH(false, 2);
+        else if (mode == 1) RISLTC_BATCH_LAUNCH(false, 1);
+        else RISLTC_BATCH_LAUNCH(false, 0);
     }
+#undef RISLTC_BATCH_LAUNCH
     return hipGetLastError();
 }
 

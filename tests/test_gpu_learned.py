@@ -35,22 +35,34 @@ def bf16(x):
     return (u.astype(np.uint32) << 16).view(np.float32)
 
 
-def layer_bounds(width):
+def layer_shapes(width, outputs):
+    return [(width, 96), (width, width), (width, width), (outputs, width)]
+
+
+def layer_bounds(width, outputs=256):
     """(start, end) of each layer's weights + bias in the parameter array:
     every matrix [out][in] followed by its bias."""
-    shapes = [(width, 96), (width, width), (width, width), (256, width)]
     out, off = [], 0
-    for o, i in shapes:
+    for o, i in layer_shapes(width, outputs):
         out.append((off, off + o * i + o))
         off += o * i + o
     return out
 
 
+def network_of(size):
+    """(width, outputs) of a parameter array's size."""
+    for w in (64, 128):
+        for o in (256, 1024):
+            if layer_bounds(w, o)[-1][1] == size:
+                return w, o
+    raise ValueError(size)
+
+
 def layers(params):
     p = params
-    width = {abi.PROPOSAL_PARAMS: 64, abi.PROPOSAL_PARAMS_WIDE: 128}[p.size]
+    width, outputs = network_of(p.size)
     res = []
-    for (a, b), (o, i) in zip(layer_bounds(width), [(width, 96), (width, width), (width, width), (256, width)]):
+    for (a, b), (o, i) in zip(layer_bounds(width, outputs), layer_shapes(width, outputs)):
         res.append((p[a:a + o * i].reshape(o, i), p[a + o * i:b]))
     return tuple(res)
 
@@ -88,31 +100,47 @@ def learned_wide(small_scene, ltc_table):
 
 
 @pytest.fixture(scope="module")
+def learned_k1024(ltc_table):
+    """1024 clusters of 1500 lights and the 96-128-128-128-1024 network
+    (tables with 2-byte cluster ids)."""
+    sc = synth.small_scene(seed=6, width=96, height=64, light_count=1500)
+    return _learned_context(sc, ltc_table, width=128, outputs=1024, K=1024)
+
+
+@pytest.fixture(scope="module")
 def learned_odd(ltc_table):
     """61 lights: a per-light cache row is not a multiple of 16 bytes."""
     return _learned_context(synth.small_scene(seed=5, width=96, height=80, light_count=61), ltc_table)
 
 
-def _learned_context(small_scene, ltc_table, width=64):
+def _learned_context(small_scene, ltc_table, width=64, outputs=256, K=16):
     import risltc_amd
     import torch
     ctx = risltc_amd.Context(0)
     ctx.load_scene(small_scene, *ltc_table)
-    K = 16
     col = ctx.cluster_lights(K, 20)
-    ctx.proposal_init(7, width=width)
+    ctx.proposal_init(7, width=width, outputs=outputs)
     consts = ctx.frame_constants()
     P = small_scene.width * small_scene.height
     vis = torch.empty(P, dtype=torch.int32, device="cuda")
     ctx.render_visibility(consts, vis)
-    table = torch.zeros(P * 32, dtype=torch.uint8, device="cuda")
-    logits = torch.zeros(P * 256, dtype=torch.float32, device="cuda")
+    tb = 48 if outputs > 256 else 32
+    table = torch.zeros(P * tb, dtype=torch.uint8, device="cuda")
+    logits = torch.zeros(P * outputs, dtype=torch.float32, device="cuda")
     feats = torch.zeros(P * 96, dtype=torch.int16, device="cuda")
     ctx.proposal_update(consts, vis, table, logits, feats)
     torch.cuda.synchronize()
     return dict(ctx=ctx, col=col, K=K, consts=consts, vis=vis, P=P, scene=small_scene, ltc=ltc_table,
-                table=table.cpu().numpy().reshape(P, 32), logits=logits.cpu().numpy().reshape(P, 256),
+                outputs=outputs, table=table.cpu().numpy().reshape(P, tb),
+                logits=logits.cpu().numpy().reshape(P, outputs),
                 feats=(feats.cpu().numpy().view(np.uint16).astype(np.uint32) << 16).view(np.float32).reshape(P, 96))
+
+
+def table_slots(table, outputs):
+    """(ids, weights) of per-pixel tables: pixels x 16 each."""
+    if outputs > 256:
+        return table[:, :32].copy().view(np.uint16).astype(np.int64), table[:, 32:48].astype(np.int64)
+    return table[:, :16].astype(np.int64), table[:, 16:32].astype(np.int64)
 
 
 def test_clusters_cover_every_light(learned):
@@ -157,34 +185,45 @@ def test_wide_tables_are_top4_per_lane_group(learned_wide):
 def _check_tables(learned):
     vis = learned["vis"].cpu().numpy().view(np.uint32)
     shaded = np.flatnonzero((vis >> 31) == 0)
-    sizes = np.bincount(learned["col"], minlength=256)
+    O = learned["outputs"]
+    sizes = np.bincount(learned["col"], minlength=O)
+    ids, ws = table_slots(learned["table"], O)
     rng = np.random.default_rng(0)
     for p in rng.choice(shaded, 300, replace=False):
         z = learned["logits"][p].astype(np.float64)
-        t = learned["table"][p]
-        zmax = max(z[c] for c in range(256) if sizes[c] > 0)
+        zmax = max(z[c] for c in range(O) if sizes[c] > 0)
         for g in range(4):
-            cand = [c for c in range(256) if (c % 16) // 4 == g and sizes[c] > 0]
+            cand = [c for c in range(O) if (c % 16) // 4 == g and sizes[c] > 0]
             # descending value, ties by smaller id
             best = sorted(cand, key=lambda c: (-z[c], c))[:4]
             for s in range(4):
                 c = best[s] if s < len(best) else 0
                 e = math.exp(z[c] - zmax) * 255.0 + 0.5
                 w = min(255, int(e)) if s < len(best) else 0
-                gw = int(t[16 + 4 * g + s])
+                gw = int(ws[p, 4 * g + s])
                 # __expf is within a few ulps: the rounding may differ by one near .5
                 assert abs(gw - w) <= (1 if abs(e - round(e)) < 1e-3 * e + 1e-3 else 0), (p, g, s, gw, w)
                 if gw > 0:
-                    assert t[4 * g + s] == c
-    assert np.all(learned["table"][(vis >> 31) != 0][:, 16:] == 0)
+                    assert ids[p, 4 * g + s] == c
+    assert np.all(ws[(vis >> 31) != 0] == 0)
 
 
 def test_learned_shading_bitwise_vs_oracle(learned):
     """Exact build with the learned source: candidates, sample records and the
     frame equal the oracle's given the device's tables and clusters."""
+    _check_learned_bitwise(learned)
+
+
+def _check_learned_bitwise(learned):
     import torch
     g = learned
     sc = g["scene"]
+    # the tables of the current network (an earlier test may have trained it)
+    tb = g["table"].shape[1]
+    table = torch.zeros(g["P"] * tb, dtype=torch.uint8, device="cuda")
+    g["ctx"].proposal_update(g["consts"], g["vis"], table)
+    torch.cuda.synchronize()
+    g["table"] = table.cpu().numpy().reshape(g["P"], tb)
     s = sc.settings(exact_math=1, sample_count_light=2, ris_candidate_count=8, proposal=abi.PROPOSAL_LEARNED)
     P = g["P"]
     out = torch.empty(P * 4, dtype=torch.float32, device="cuda")
@@ -204,7 +243,7 @@ def test_learned_shading_bitwise_vs_oracle(learned):
     o = out.cpu().numpy().reshape(P, 4)
     assert np.array_equal(o.view(np.uint32), ref.view(np.uint32))
     # the learned source concentrates candidates on clusters: not all uniform
-    assert (g["table"][:, 16:].astype(int).sum(1) > 0).sum() > 0.3 * P
+    assert (table_slots(g["table"], g["outputs"])[1].sum(1) > 0).sum() > 0.3 * P
 
 
 def test_training_step_gradient_and_adam(learned):
@@ -264,8 +303,8 @@ def _check_training_step(learned):
     G1 = bf16((G2 @ bf16(W2).astype(np.float64)) * (H1 > 0)).astype(np.float64)
     ref = np.concatenate([(G1.T @ X).ravel(), G1.sum(0), (G2.T @ H1).ravel(), G2.sum(0), (G3.T @ H2).ravel(),
                           G3.sum(0), (G4.T @ H3).ravel(), G4.sum(0)])
-    width = W1.shape[0]
-    for name, (a, b) in zip(("W1", "W2", "W3", "W4"), layer_bounds(width)):
+    width, outputs = W1.shape[0], W4.shape[0]
+    for name, (a, b) in zip(("W1", "W2", "W3", "W4"), layer_bounds(width, outputs)):
         rel = np.linalg.norm(grad[a:b] - ref[a:b]) / np.linalg.norm(ref[a:b])
         assert rel < 1e-2, (name, width, rel)
     # Adam, step 1: m = (1-b1) g, v = (1-b2) g^2, bias-corrected
@@ -303,7 +342,7 @@ def test_cached_learned_shading_bitwise_vs_oracle(learned, learned_odd, per_ligh
     rng = np.random.default_rng(77)
     N = sc.light_count
     used = N if per_light else g["K"]
-    counts = np.zeros((1 << log2, N if per_light else 256, 2), dtype=np.uint32)
+    counts = np.zeros((1 << log2, N if per_light else abi.MAX_LIGHT_CLUSTERS, 2), dtype=np.uint32)
     counts[:, :used] = rng.integers(0, 60, (1 << log2, used, 2))
     ctx.proposal_cache(log2, cell, prior, per_light)
     try:
@@ -350,7 +389,7 @@ def test_resolve_counts_every_traced_ray(learned, per_light):
     spp = 2
     ctx.proposal_cache(9, 0.75, 2.0, per_light)
     try:
-        want = np.zeros((1 << 9, sc.light_count if per_light else 256, 2), dtype=np.int64)
+        want = np.zeros((1 << 9, sc.light_count if per_light else abi.MAX_LIGHT_CLUSTERS, 2), dtype=np.int64)
         for frame in range(2):
             c = ctx.frame_constants(noise_seed=frame + 5, animate=True)
             s = sc.settings(sample_count_light=spp, ris_candidate_count=8,
@@ -460,3 +499,40 @@ def test_wide_learned_pass_runs_and_trains(learned_wide):
         ctx.proposal_update(g["consts"], g["vis"])
     with pytest.raises(risltc_amd.RisltcError, match="E_UNSUPPORTED"):
         ctx.proposal_set_width(96)
+
+
+def test_k1024_forward_tables_and_training(learned_k1024):
+    """The 1024-output network: logits, 2-byte-id tables and one training
+    step's gradient against numpy."""
+    g = learned_k1024
+    assert g["ctx"].proposal_param_count() == layer_bounds(128, 1024)[-1][1]
+    _check_forward(g)
+    _check_tables(g)
+    ids, _ = table_slots(g["table"], 1024)
+    assert ids.max() >= 256  # ids beyond one byte occur
+    _check_training_step(g)
+
+
+def test_k1024_learned_shading_bitwise_vs_oracle(learned_k1024):
+    """The learned pass reading 2-byte cluster ids: candidates, records and
+    frame bitwise against the oracle."""
+    g = learned_k1024
+    g["ctx"].proposal_update(g["consts"], g["vis"])
+    _check_learned_bitwise(g)
+
+
+@pytest.mark.parametrize("exact", [0, 1])
+def test_k1024_compacted_equals_debug(learned_k1024, exact):
+    import torch
+    g = learned_k1024
+    ctx, sc = g["ctx"], g["scene"]
+    ctx.proposal_update(g["consts"], g["vis"])
+    s = sc.settings(exact_math=exact, sample_count_light=2, ris_candidate_count=16, proposal=abi.PROPOSAL_LEARNED)
+    P = g["P"]
+    a = torch.empty(P * 4, dtype=torch.float32, device="cuda")
+    b = torch.empty(P * 4, dtype=torch.float32, device="cuda")
+    rec = torch.zeros(P * 2 * abi.SAMPLE_RECORD_FLOATS, dtype=torch.float32, device="cuda")
+    ctx.shade(g["consts"], s, g["vis"], a)
+    ctx.shade_debug(g["consts"], s, g["vis"], b, rec)
+    torch.cuda.synchronize()
+    assert np.array_equal(a.cpu().numpy().view(np.uint32), b.cpu().numpy().view(np.uint32))

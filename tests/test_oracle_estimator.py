@@ -253,7 +253,7 @@ def _mix(h, k):
 def _cache_oracle(scene, ltc_table, log2, cell, prior, rng, per_light=False, bvh=True):
     o, tab, col, sizes = _source_oracle(scene, ltc_table, rng, bvh=bvh)
     N = scene.light_count
-    slots = N if per_light else 256
+    slots = N if per_light else orc.CACHE_CLUSTER_SLOTS
     used = N if per_light else 16
     counts = np.zeros((1 << log2, slots, 2), dtype=np.uint32)
     counts[:, :used] = rng.integers(0, 40, (1 << log2, used, 2))
