@@ -145,10 +145,10 @@ def bench_learned_batch(ltc_rgba, ltc_rg, steps, warmup, views_n=16, grid=540, l
     3-8-gon lights, M = 32, `spp` samples per pixel, shaded as one plain batch
     and as one learned batch (MLP source with per-view tables + visibility
     cache, M = 24) and as a graph replay of the learned batch. The MLP is the
-    96-128-128-128-256 network over 256 k-means clusters (the per-pixel
-    tables hold byte cluster ids, so 256 is the limit where the configuration
-    asks for 1024), warmed by a few trained single-view frames. Wall time per step; no convergence run (a
-    65536-spp reference of 16 views is out of this bench's time)."""
+    96-128-128-128-1024 network over 1024 k-means clusters (per-pixel tables
+    of 2-byte cluster ids), warmed by a few trained single-view frames. Wall
+    time per step; no convergence run (a 65536-spp reference of 16 views is
+    out of this bench's time)."""
     import math
     import torch
     import risltc_amd
@@ -158,8 +158,9 @@ def bench_learned_batch(ltc_rgba, ltc_rg, steps, warmup, views_n=16, grid=540, l
     ctx = risltc_amd.Context(0)
     ctx.load_scene(sc, ltc_rgba, ltc_rg)
     lk = convergence.DEFAULT_LEARNED
-    ctx.cluster_lights(lk["clusters"], 20)
-    ctx.proposal_init(lk["seed"], width=128)
+    clusters, outputs = 1024, 1024
+    ctx.cluster_lights(clusters, 20)
+    ctx.proposal_init(lk["seed"], width=128, outputs=outputs)
     # a per-light cache of 16384 lights x 2^16 cells would take 8 GiB: the
     # cache keeps one slot per cluster here
     per_light = lk["cache_per_light"] and lights <= 4096
@@ -183,7 +184,7 @@ def bench_learned_batch(ltc_rgba, ltc_rg, steps, warmup, views_n=16, grid=540, l
         cam = best[1]
         vis = torch.empty(W * H, dtype=torch.int32, device="cuda")
         out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
-        tab = torch.empty(W * H * 32, dtype=torch.uint8, device="cuda")
+        tab = torch.empty(W * H * abi.proposal_table_bytes(outputs), dtype=torch.uint8, device="cuda")
         bufs.append([cam, vis, out, tab])
         ctx.render_visibility(risltc_amd.write_frame_constants(cam, W, H, sc.mesh.dequantization_factor,
                                                                sc.mesh.dequantization_summand, ctx.ltc_constants),
@@ -236,7 +237,7 @@ def bench_learned_batch(ltc_rgba, ltc_rg, steps, warmup, views_n=16, grid=540, l
     rate = lambda ms: round(samples / (ms * 1e-3) / 1e6, 2)
     km = lambda k: {"shade": round(k[0], 2), "trace": round(k[1], 2), "resolve": round(k[2], 2)}
     return {"views": views_n, "resolution": [W, H], "triangles": int(sc.mesh.triangle_count), "lights": lights,
-            "clusters": lk["clusters"], "mlp": "96-128-128-128-256", "spp": spp,
+            "clusters": clusters, "mlp": "96-128-128-128-%d" % outputs, "spp": spp,
             "cache_slots": "light" if per_light else "cluster",
             "candidates": {"plain": 32, "learned": lk["candidates"]}, "samples_per_step": samples,
             "setup_s": round(setup_s, 1),

@@ -477,8 +477,8 @@ int risltc_shade_debug(risltc_context_t* context, const risltc_frame_constants_t
  * estimate as the target, shading_pass.frag.glsl:723-736). Two learned
  * parts, both trained online on the pass's own output:
  *   - source: an MLP maps each shading point to a distribution over light
- *     clusters (softmax of 256 logits; per pixel the 16 kept clusters of
- *     risltc_proposal_update's table). With settings.proposal =
+ *     clusters (softmax of 256 or 1024 logits; per pixel the 16 kept
+ *     clusters of risltc_proposal_update's table). With settings.proposal =
  *     RISLTC_PROPOSAL_LEARNED, risltc_shade draws each candidate from the
  *     mixture q = 1/(2N) + (1/2) pi_c / |c| of the uniform source and the
  *     table, so every light keeps q >= 1/(2N) and the estimator stays

@@ -23,6 +23,12 @@ MAX_LIGHT_CLUSTERS = 1024
 PROPOSAL_PARAMS = 31168
 PROPOSAL_PARAMS_WIDE = 78464
 
+
+def proposal_table_bytes(outputs):
+    """Bytes per pixel of a proposal table (RISLTC_PROPOSAL_TABLE_BYTES):
+    16 cluster ids and 16 weights, ids of 2 bytes for more than 256 outputs."""
+    return 48 if outputs > 256 else 32
+
 # risltc_sample_record_t word offsets
 REC_LIGHT, REC_WSUM, REC_PHAT, REC_W, REC_CLIPD, REC_CLIPS, REC_PSAD, REC_PSAS = range(8)
 REC_DIR, REC_TMAX, REC_STATE, REC_EVIS, REC_EOCC, REC_COLOR, REC_SEED, REC_POS = 8, 14, 16, 18, 24, 30, 33, 34

@@ -291,21 +291,23 @@ def test_batch_with_several_meshes(small_scene, ltc_table):
             c_.close()
 
 
-def test_learned_batch_equals_single_views(small_scene, ltc_table):
+@pytest.mark.parametrize("K,width,outputs,per_light", [(16, 64, 256, True), (48, 128, 1024, False)])
+def test_learned_batch_equals_single_views(small_scene, ltc_table, K, width, outputs, per_light):
     """Learned views (MLP source with per-view tables, and the cache-only
     mode) in one batch and in a graph: each view's frame equals its
     risltc_shade call from the same cache state bit for bit, and the batch
     adds to the shared visibility cache exactly the counts of the single
-    calls together."""
+    calls together. Byte ids with a per-light cache; 2-byte ids (1024
+    outputs) with a per-cluster cache."""
     import risltc_amd
     import torch
     sc = small_scene
     ctx = risltc_amd.Context(0)
     try:
         ctx.load_scene(sc, *ltc_table)
-        ctx.cluster_lights(16, 10)
-        ctx.proposal_init(3)
-        ctx.proposal_cache(12, 1.0, 1.0, per_light=True)
+        ctx.cluster_lights(K, 10)
+        ctx.proposal_init(3, width=width, outputs=outputs)
+        ctx.proposal_cache(12, 1.0, 1.0, per_light=per_light)
         specs = view_specs(sc)
         shapes = [(specs[0][0], 256, 256, 2, 8), (specs[1][0], 200, 120, 1, 16), (specs[2][0], 97, 61, 2, 4)]
         bufs = []
@@ -331,7 +333,7 @@ def test_learned_batch_equals_single_views(small_scene, ltc_table):
                 st = sc.settings(proposal=mode, sample_count_light=spp, ris_candidate_count=M, learned_floor_log2=3)
                 tab = None
                 if mode == abi.PROPOSAL_LEARNED:
-                    tab = torch.empty(w * h * 32, dtype=torch.uint8, device="cuda")
+                    tab = torch.empty(w * h * abi.proposal_table_bytes(outputs), dtype=torch.uint8, device="cuda")
                     ctx.proposal_update(c, vis, table=tab)
                 out = torch.full((w * h * 4,), float("nan"), dtype=torch.float32, device="cuda")
                 views.append(ctx.view(c, st, vis, out, proposal_table=tab))
@@ -376,7 +378,7 @@ def test_learned_batch_equals_single_views(small_scene, ltc_table):
                     ctx.shade(bufs[0][0], sc.settings(sample_count_light=2), bufs[0][1], tmp)
                     torch.cuda.synchronize()
                 # a new cache invalidates the graph
-                ctx.proposal_cache(12, 1.0, 1.0, per_light=True)
+                ctx.proposal_cache(12, 1.0, 1.0, per_light=per_light)
                 with pytest.raises(risltc_amd.RisltcError, match="E_STATE"):
                     g.launch()
                 ctx.proposal_cache_write(counts_b)
