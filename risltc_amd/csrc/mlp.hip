@@ -180,6 +180,40 @@ static_assert(sizeof(PropShared<128, 256, 4, false>) <= 160 * 1024, "width-128 L
 static_assert(sizeof(PropShared<64, 1024, 8, false>) <= 160 * 1024, "width-64, 1024 outputs LDS");
 static_assert(sizeof(PropShared<128, 1024, 4, false>) <= 160 * 1024, "width-128, 1024 outputs LDS");
 
+// one output tile's logits z[i] of clusters c0 + i (i = 0..3) for one pixel:
+// written to `logits` (when not null) and merged into the sorted best-4 bv/bi
+// of the live clusters, in order of i (ties keep the smaller id). Dead
+// clusters read as -inf, and the tile is skipped when its largest logit does
+// not beat the fourth best, which leaves the same best-4 as testing each.
+__device__ __forceinline__ void merge_top4(const float (&z)[4], const bool (&live)[4], int c0, float* logits,
+                                           float (&bv)[4], int (&bi)[4]) {
+    if (logits) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) logits[i] = z[i];
+    }
+    float zc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) zc[i] = live[i] ? z[i] : -__builtin_inff();
+    if (!(fmax_(fmax_(zc[0], zc[1]), fmax_(zc[2], zc[3])) > bv[3])) return;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (zc[i] > bv[3]) {
+            bv[3] = zc[i];
+            bi[3] = c0 + i;
+#pragma unroll
+            for (int q = 3; q > 0; --q)
+                if (bv[q] > bv[q - 1]) {
+                    float tv = bv[q];
+                    bv[q] = bv[q - 1];
+                    bv[q - 1] = tv;
+                    int ti = bi[q];
+                    bi[q] = bi[q - 1];
+                    bi[q - 1] = ti;
+                }
+        }
+    }
+}
+
 // hidden layer on a 16-pixel tile: out[pixel][o] = relu(W[o] . act[pixel] + b[o])
 template <int HID, int K, int LDW, int LDA>
 __device__ __forceinline__ void hidden_layer(const uint16_t* W, const float* bias, const uint16_t* act, uint16_t* out,
@@ -254,87 +288,192 @@ __global__ void __launch_bounds__(64 * WAVES) proposal_kernel(ProposalArgs A) {
 #pragma unroll
             for (int k = 0; k < kMlpIn / 2; ++k) fo[k] = packed[k];
         }
+        if constexpr (W4LDS) {
+            // output weights in LDS: the MLP runs on the four 16-pixel
+            // sub-tiles in turn
 #pragma unroll 1
-        for (int sub = 0; sub < 4; ++sub) {
-            if (g == sub) {
+            for (int sub = 0; sub < 4; ++sub) {
+                if (g == sub) {
 #pragma unroll
-                for (int k = 0; k < kMlpIn / 2; k += 4)
-                    *reinterpret_cast<uint4*>(act + r * kLdW1 + 2 * k) =
-                        make_uint4(packed[k], packed[k + 1], packed[k + 2], packed[k + 3]);
-            }
-            const uint32_t pixel = grp * 64u + (uint32_t)(16 * sub + r);
-            const bool in = pixel < A.pixel_count;
-            // every lane of the pixel's column learns whether it is shaded
-            const bool shaded = __shfl(my_shaded ? 1 : 0, 16 * sub + r) != 0;
-            wave_sync();
-            hidden_layer<HID, kMlpIn, kLdW1, kLdW1>(sh.w1, sh.bias, act, h0, lane);
-            wave_sync();
-            hidden_layer<HID, HID, kLdH, kLdH>(sh.w2, sh.bias + HID, h0, h1, lane);
-            wave_sync();
-            hidden_layer<HID, HID, kLdH, kLdH>(sh.w3, sh.bias + 2 * HID, h1, h0, lane);
-            wave_sync();
-            // output layer: this lane keeps logits of clusters 16 t + 4 g + i of pixel r,
-            // and the 4 largest of them
-            float bv[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-            int bi[4] = {0, 0, 0, 0};
+                    for (int k = 0; k < kMlpIn / 2; k += 4)
+                        *reinterpret_cast<uint4*>(act + r * kLdW1 + 2 * k) =
+                            make_uint4(packed[k], packed[k + 1], packed[k + 2], packed[k + 3]);
+                }
+                const uint32_t pixel = grp * 64u + (uint32_t)(16 * sub + r);
+                const bool in = pixel < A.pixel_count;
+                // every lane of the pixel's column learns whether it is shaded
+                const bool shaded = __shfl(my_shaded ? 1 : 0, 16 * sub + r) != 0;
+                wave_sync();
+                hidden_layer<HID, kMlpIn, kLdW1, kLdW1>(sh.w1, sh.bias, act, h0, lane);
+                wave_sync();
+                hidden_layer<HID, HID, kLdH, kLdH>(sh.w2, sh.bias + HID, h0, h1, lane);
+                wave_sync();
+                hidden_layer<HID, HID, kLdH, kLdH>(sh.w3, sh.bias + 2 * HID, h1, h0, lane);
+                wave_sync();
+                // output layer: this lane keeps logits of clusters 16 t + 4 g + i of pixel r,
+                // and the 4 largest of them
+                float bv[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+                int bi[4] = {0, 0, 0, 0};
 #pragma unroll 2
-            for (int t = 0; t < OUT / 16; ++t) {
-                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                for (int t = 0; t < OUT / 16; ++t) {
+                    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int ks = 0; ks < HID / 32; ++ks)
-                    acc = mfma(frag(w4 + (t * 16 + r) * kLdW4 + ks * 32 + 8 * g),
-                               frag(h0 + r * kLdH + ks * 32 + 8 * g), acc);
+                    for (int ks = 0; ks < HID / 32; ++ks)
+                        acc = mfma(frag(w4 + (t * 16 + r) * kLdW4 + ks * 32 + 8 * g),
+                                   frag(h0 + r * kLdH + ks * 32 + 8 * g), acc);
+                    const int c0 = t * 16 + 4 * g;
+                    float z[4];
+                    bool live[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        z[i] = acc[i] + sh.bias[3 * HID + c0 + i];
+                        live[i] = sh.csize[c0 + i] != 0u;
+                    }
+                    merge_top4(z, live, c0, A.logits && in ? A.logits + (size_t)pixel * OUT + c0 : nullptr, bv, bi);
+                }
+                // the pixel's maximum over its four lanes (lanes r, r+16, r+32, r+48)
+                float zmax = bv[0];
+                zmax = fmax_(zmax, __shfl_xor(zmax, 16));
+                zmax = fmax_(zmax, __shfl_xor(zmax, 32));
+                uint32_t ws = 0;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    uint32_t w = 0;
+                    if (shaded && bv[s] > -__builtin_inff()) {
+                        float e = __expf(bv[s] - zmax) * 255.0f + 0.5f;
+                        w = e >= 255.0f ? 255u : (uint32_t)e;
+                    }
+                    if (w == 0u) bi[s] = 0;
+                    ws |= w << (8 * s);
+                }
+                if (in) {
+                    uint32_t* o = reinterpret_cast<uint32_t*>(A.out + (size_t)pixel * prop_bytes(OUT));
+                    if constexpr (OUT > 256) {
+                        // slot 4 g + s: id (2 bytes) at 2 (4 g + s), weight at 32 + 4 g + s
+                        o[2 * g] = (uint32_t)bi[0] | ((uint32_t)bi[1] << 16);
+                        o[2 * g + 1] = (uint32_t)bi[2] | ((uint32_t)bi[3] << 16);
+                        o[8 + g] = ws;
+                    } else {
+                        o[g] = (uint32_t)bi[0] | ((uint32_t)bi[1] << 8) | ((uint32_t)bi[2] << 16) | ((uint32_t)bi[3] << 24);
+                        o[4 + g] = ws;
+                    }
+                }
+                wave_sync();
+            }
+        } else {
+            // hidden layers per 16-pixel sub-tile; the last layer's activations of
+            // the four sub-tiles stay in registers as MFMA B fragments (sub-tile s
+            // in hb[3 - s] after the loop), so that the output layer loads each of
+            // its weight fragments once for all 64 pixels
+            bf16x8 hb[4][HID / 32];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int ks = 0; ks < HID / 32; ++ks) hb[q][ks] = bf16x8{};
+#pragma unroll 1
+            for (int sub = 0; sub < 4; ++sub) {
+                if (g == sub) {
+#pragma unroll
+                    for (int k = 0; k < kMlpIn / 2; k += 4)
+                        *reinterpret_cast<uint4*>(act + r * kLdW1 + 2 * k) =
+                            make_uint4(packed[k], packed[k + 1], packed[k + 2], packed[k + 3]);
+                }
+                wave_sync();
+                hidden_layer<HID, kMlpIn, kLdW1, kLdW1>(sh.w1, sh.bias, act, h0, lane);
+                wave_sync();
+                hidden_layer<HID, HID, kLdH, kLdH>(sh.w2, sh.bias + HID, h0, h1, lane);
+                wave_sync();
+                hidden_layer<HID, HID, kLdH, kLdH>(sh.w3, sh.bias + 2 * HID, h1, h0, lane);
+                wave_sync();
+#pragma unroll
+                for (int q = 3; q > 0; --q)
+#pragma unroll
+                    for (int ks = 0; ks < HID / 32; ++ks) hb[q][ks] = hb[q - 1][ks];
+#pragma unroll
+                for (int ks = 0; ks < HID / 32; ++ks) hb[0][ks] = frag(h0 + r * kLdH + ks * 32 + 8 * g);
+                // the next sub-tile overwrites act and h0
+                wave_sync();
+            }
+            // output layer: for sub-tile s this lane keeps the logits of clusters
+            // 16 t + 4 g + i of pixel 16 s + r, and the 4 largest of them
+            float bv[4][4];
+            int bi[4][4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    int cl = t * 16 + 4 * g + i;
-                    float z = acc[i] + sh.bias[3 * HID + cl];
-                    if (A.logits && in) A.logits[(size_t)pixel * OUT + cl] = z;
-                    if (sh.csize[cl] == 0u) continue;
-                    // insertion into the sorted best-4 (ties keep the smaller id)
-                    if (z > bv[3]) {
-                        bv[3] = z;
-                        bi[3] = cl;
+                    bv[s][i] = -__builtin_inff();
+                    bi[s][i] = 0;
+                }
+            bf16x8 wa[HID / 32];
 #pragma unroll
-                        for (int s = 3; s > 0; --s)
-                            if (bv[s] > bv[s - 1]) {
-                                float tv = bv[s];
-                                bv[s] = bv[s - 1];
-                                bv[s - 1] = tv;
-                                int ti = bi[s];
-                                bi[s] = bi[s - 1];
-                                bi[s - 1] = ti;
-                            }
+            for (int ks = 0; ks < HID / 32; ++ks) wa[ks] = frag(w4 + r * kLdW4 + ks * 32 + 8 * g);
+#pragma unroll 2
+            for (int t = 0; t < OUT / 16; ++t) {
+                // the next tile's weight fragments load while this tile's MFMAs run
+                const int tn = t + 1 < OUT / 16 ? t + 1 : t;
+                bf16x8 wn[HID / 32];
+#pragma unroll
+                for (int ks = 0; ks < HID / 32; ++ks) wn[ks] = frag(w4 + (tn * 16 + r) * kLdW4 + ks * 32 + 8 * g);
+                float b4[4];
+                bool live[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    b4[i] = sh.bias[3 * HID + t * 16 + 4 * g + i];
+                    live[i] = sh.csize[t * 16 + 4 * g + i] != 0u;
+                }
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                    for (int ks = 0; ks < HID / 32; ++ks) acc = mfma(wa[ks], hb[3 - s][ks], acc);
+                    const uint32_t pixel = grp * 64u + (uint32_t)(16 * s + r);
+                    const int c0 = t * 16 + 4 * g;
+                    float z[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) z[i] = acc[i] + b4[i];
+                    merge_top4(z, live, c0,
+                               A.logits && pixel < A.pixel_count ? A.logits + (size_t)pixel * OUT + c0 : nullptr,
+                               bv[s], bi[s]);
+                }
+#pragma unroll
+                for (int ks = 0; ks < HID / 32; ++ks) wa[ks] = wn[ks];
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const uint32_t pixel = grp * 64u + (uint32_t)(16 * s + r);
+                const bool in = pixel < A.pixel_count;
+                // every lane of the pixel's column learns whether it is shaded
+                const bool shaded = __shfl(my_shaded ? 1 : 0, 16 * s + r) != 0;
+                // the pixel's maximum over its four lanes (lanes r, r+16, r+32, r+48)
+                float zmax = bv[s][0];
+                zmax = fmax_(zmax, __shfl_xor(zmax, 16));
+                zmax = fmax_(zmax, __shfl_xor(zmax, 32));
+                uint32_t ws = 0;
+                int id[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint32_t w = 0;
+                    if (shaded && bv[s][q] > -__builtin_inff()) {
+                        float e = __expf(bv[s][q] - zmax) * 255.0f + 0.5f;
+                        w = e >= 255.0f ? 255u : (uint32_t)e;
+                    }
+                    id[q] = w == 0u ? 0 : bi[s][q];
+                    ws |= w << (8 * q);
+                }
+                if (in) {
+                    uint32_t* o = reinterpret_cast<uint32_t*>(A.out + (size_t)pixel * prop_bytes(OUT));
+                    if constexpr (OUT > 256) {
+                        // slot 4 g + q: id (2 bytes) at 2 (4 g + q), weight at 32 + 4 g + q
+                        o[2 * g] = (uint32_t)id[0] | ((uint32_t)id[1] << 16);
+                        o[2 * g + 1] = (uint32_t)id[2] | ((uint32_t)id[3] << 16);
+                        o[8 + g] = ws;
+                    } else {
+                        o[g] = (uint32_t)id[0] | ((uint32_t)id[1] << 8) | ((uint32_t)id[2] << 16) | ((uint32_t)id[3] << 24);
+                        o[4 + g] = ws;
                     }
                 }
             }
-            // the pixel's maximum over its four lanes (lanes r, r+16, r+32, r+48)
-            float zmax = bv[0];
-            zmax = fmax_(zmax, __shfl_xor(zmax, 16));
-            zmax = fmax_(zmax, __shfl_xor(zmax, 32));
-            uint32_t ws = 0;
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                uint32_t w = 0;
-                if (shaded && bv[s] > -__builtin_inff()) {
-                    float e = __expf(bv[s] - zmax) * 255.0f + 0.5f;
-                    w = e >= 255.0f ? 255u : (uint32_t)e;
-                }
-                if (w == 0u) bi[s] = 0;
-                ws |= w << (8 * s);
-            }
-            if (in) {
-                uint32_t* o = reinterpret_cast<uint32_t*>(A.out + (size_t)pixel * prop_bytes(OUT));
-                if constexpr (OUT > 256) {
-                    // slot 4 g + s: id (2 bytes) at 2 (4 g + s), weight at 32 + 4 g + s
-                    o[2 * g] = (uint32_t)bi[0] | ((uint32_t)bi[1] << 16);
-                    o[2 * g + 1] = (uint32_t)bi[2] | ((uint32_t)bi[3] << 16);
-                    o[8 + g] = ws;
-                } else {
-                    o[g] = (uint32_t)bi[0] | ((uint32_t)bi[1] << 8) | ((uint32_t)bi[2] << 16) | ((uint32_t)bi[3] << 24);
-                    o[4 + g] = ws;
-                }
-            }
-            wave_sync();
         }
     }
 }
