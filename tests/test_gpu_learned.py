@@ -536,3 +536,19 @@ def test_k1024_compacted_equals_debug(learned_k1024, exact):
     ctx.shade_debug(g["consts"], s, g["vis"], b, rec)
     torch.cuda.synchronize()
     assert np.array_equal(a.cpu().numpy().view(np.uint32), b.cpu().numpy().view(np.uint32))
+
+
+def test_k1024_narrow_network_with_dead_outputs(ltc_table):
+    """The 96-64-64-64-1024 network over 600 clusters (outputs past the
+    cluster count are never kept): logits, 2-byte-id tables, and learned
+    shading bitwise against the oracle."""
+    sc = synth.small_scene(seed=8, width=80, height=48, light_count=1200)
+    g = _learned_context(sc, ltc_table, width=64, outputs=1024, K=600)
+    try:
+        _check_forward(g)
+        _check_tables(g)
+        ids, ws = table_slots(g["table"], 1024)
+        assert ids[ws > 0].max() < 600 and ids.max() >= 256
+        _check_learned_bitwise(g)
+    finally:
+        g["ctx"].close()
