@@ -28,10 +28,16 @@ The learned_batch object: the learned pass in one batch at the large end
 per-view proposal tables, the shared visibility cache), against the plain
 batch of the same views, and as a graph replay.
 
+The large_learned_vs_plain object (--large-convergence, off by default: about
+3.5 minutes): learned vs plain time to relMSE 0.01 on one 1920x1080 view at
+the large end (16384 lights, ~4M triangles, 16 spp per frame) against a
+65536-spp plain reference, for the 1024-cluster width-128 network and the
+default 256-cluster network.
+
 The launch shape is autotuned (risltc_autotune) on the frame before timing.
 
   python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-learned] [--no-batch]
-                  [--no-learned-batch]
+                  [--no-learned-batch] [--large-convergence]
 """
 import argparse
 import json
@@ -248,6 +254,56 @@ def bench_learned_batch(ltc_rgba, ltc_rg, steps, warmup, views_n=16, grid=540, l
                                "learned_graph": rate(ms_graph)}}
 
 
+def bench_large_convergence(ltc_rgba, ltc_rg, frames=128, ref_frames=4096):
+    """Learned vs plain time to relMSE 0.01 at the large end, on one 1920x1080
+    view of the ~4M-triangle city with 16384 lights and 16 spp per frame
+    (M = 32 plain, 24 learned), against a 65536-spp plain reference of that
+    view: the 96-128-128-128-1024 network over 1024 clusters (the large-end
+    configuration's network) and the default 96-64-64-64-256 network over 256
+    clusters, each with a visibility cache of one slot per cluster. About 3.5
+    minutes of GPU time, most of it the reference."""
+    import torch
+    import risltc_amd
+    from risltc_amd import synth, convergence as cv
+    t0 = time.time()
+    sc = synth.city_scene(seed=400, light_count=16384, grid=540, spp=16, candidates=32)
+    ctx = risltc_amd.Context(0)
+    try:
+        ctx.load_scene(sc, ltc_rgba, ltc_rg)
+        P = sc.width * sc.height
+        vis = torch.empty(P, dtype=torch.int32, device="cuda")
+        ctx.render_visibility(ctx.frame_constants(), vis)
+        ref = cv.reference(ctx, sc, vis, ref_frames, chunk=512,
+                           log=lambda m: print(m, file=sys.stderr, flush=True))
+        cps = (4, 8, 16, 32, 64, 128)
+
+        def run(lk):
+            _, _, curve = cv.accumulate(ctx, sc, vis, frames, 0, learned=lk, checkpoints=cps, ref=ref)
+            t, ex = cv.time_to(curve, 0.01)
+            km = cv._frames.kernel_ms
+            return {"time_to_target_ms": round(t, 1), "extrapolated": ex,
+                    "curve": {str(k): [round(v[0], 6), round(v[1], 1)] for k, v in sorted(curve.items())},
+                    "ms_per_frame": {"shade": round(km[0], 3), "trace": round(km[1], 3), "resolve": round(km[2], 3),
+                                     "inference": round(km[3], 3), "training": round(km[4], 3)}}
+
+        plain = run(None)
+        base = dict(cv.DEFAULT_LEARNED, cache_per_light=False)
+        out = {"view": [sc.width, sc.height], "triangles": int(sc.mesh.triangle_count), "lights": sc.light_count,
+               "spp_per_frame": sc.spp, "reference_spp": ref_frames * sc.spp, "target_relmse": 0.01,
+               "plain": plain}
+        for name, lk in (("learned_1024_clusters_w128", dict(base, clusters=1024, width=128, outputs=1024)),
+                         ("learned_256_clusters_w64", dict(base, clusters=256, width=64, outputs=256))):
+            r = run(lk)
+            r["over_plain"] = round(r["time_to_target_ms"] / plain["time_to_target_ms"], 4)
+            out[name] = r
+            print("large end %s: %.1f ms vs plain %.1f ms" % (name, r["time_to_target_ms"],
+                                                              plain["time_to_target_ms"]), file=sys.stderr, flush=True)
+        out["wall_s"] = round(time.time() - t0, 1)
+        return out
+    finally:
+        ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -263,6 +319,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view measurement")
     ap.add_argument("--no-learned-batch", action="store_true",
                     help="skip the learned batch at the large end (16 x 1080p views, ~4M triangles, 16384 lights)")
+    ap.add_argument("--large-convergence", action="store_true",
+                    help="also run learned vs plain time to relMSE 0.01 on one view at the large end "
+                         "(16384 lights, ~4M triangles; about 3.5 minutes)")
     ap.add_argument("--batch-views", type=int, default=16,
                     help="views in the batched measurement (the batched configuration has 64)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
@@ -371,6 +430,10 @@ def main():
         learned_batch = bench_learned_batch(rgba, rg, max(1, min(args.steps, 3)), 1)
         print("learned batch: %s" % learned_batch, file=sys.stderr, flush=True)
 
+    large = None
+    if args.large_convergence and args.workload == "city" and not args.exact:
+        large = bench_large_convergence(rgba, rg)
+
     learned = None
     if not args.no_learned and args.workload == "city" and not args.exact:
         from risltc_amd import convergence
@@ -430,6 +493,7 @@ def main():
         "batched_views": batch,
         "learned_batch": learned_batch,
         "learned_vs_plain": learned,
+        "large_learned_vs_plain": large,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
