@@ -624,7 +624,8 @@ typedef struct risltc_view_s {
     const uint8_t* proposal_table; /* device, with settings.proposal =
                                       RISLTC_PROPOSAL_LEARNED: this view's
                                       table (width*height*
-                                      RISLTC_PROPOSAL_TABLE_BYTES bytes) from
+                                      RISLTC_PROPOSAL_TABLE_BYTES(outputs)
+                                      bytes for the network's outputs) from
                                       risltc_proposal_update's table_out;
                                       NULL otherwise */
 } risltc_view_t;
