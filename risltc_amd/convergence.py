@@ -28,8 +28,9 @@ def accumulate(ctx, scene, vis, frames, seed0, learned=None, checkpoints=(), ref
                technique=abi.SAMPLE_POLYGON_LTC_CP):
     """Accumulates `frames` frames. learned = None (plain) or a dict with
     floor_log2, infer_every, batch, lr, clusters, seed, and optionally
-    cache_log2, cache_cell, cache_prior (the visibility cache) and candidates
-    (M). technique selects the polygon sampling technique of a plain pass.
+    cache_log2, cache_cell, cache_prior (the visibility cache), candidates
+    (M), and warmup_frames / warmup_candidates (the first frames use the
+    uniform source with the cache while the MLP trains on them). technique selects the polygon sampling technique of a plain pass.
     Returns (image, gpu_ms, curve) where curve maps spp -> (relmse, gpu_ms)
     at checkpoints."""
     import torch
@@ -65,14 +66,21 @@ def _frames(ctx, scene, vis, frames, seed0, learned, checkpoints, ref, s, out, a
     gpu = 0.0
     curve = {}
     kernel_ms = [0.0, 0.0, 0.0, 0.0, 0.0]  # shade, trace, resolve, inference, training
+    mlp = bool(learned) and learned.get("proposal", abi.PROPOSAL_LEARNED) == abi.PROPOSAL_LEARNED
+    # warm-up: the first frames draw candidates from the uniform source (the
+    # cache, if any, still weights the target) while the MLP trains on them
+    warm = learned.get("warmup_frames", 0) if mlp else 0
+    m_learned = s.ris_candidate_count
     for f in range(frames):
         c = ctx.frame_constants(noise_seed=seed0 + f, animate=True)
-        if learned and learned.get("proposal", abi.PROPOSAL_LEARNED) == abi.PROPOSAL_LEARNED and \
-                f % learned.get("infer_every", 1) == 0:
+        if mlp:
+            cache_mode = abi.PROPOSAL_CACHE if learned.get("cache_log2") else abi.PROPOSAL_UNIFORM
+            s.proposal = cache_mode if f < warm else abi.PROPOSAL_LEARNED
+            s.ris_candidate_count = learned.get("warmup_candidates", m_learned) if f < warm else m_learned
+        if mlp and f >= warm and ((f - warm) % learned.get("infer_every", 1) == 0):
             ctx.proposal_update(c, vis)
             kernel_ms[3] += ctx.proposal_last_timings()[0]
-        train = bool(learned) and learned.get("proposal", abi.PROPOSAL_LEARNED) == abi.PROPOSAL_LEARNED and \
-            f % learned.get("train_every", 1) == 0
+        train = mlp and f % learned.get("train_every", 1) == 0
         s.skip_training_records = 0 if train else 1
         ctx.shade(c, s, vis, out)
         t = ctx.last_timings()
