@@ -1,4 +1,14 @@
 /*
+ * SPDX-License-Identifier: GPL-3.0-or-later
+ *
+ * This file restates, for testing, algorithms of the risltc renderer
+ * (Copyright (C) 2021, Christoph Peters, Karlsruhe Institute of Technology;
+ * Copyright (C) 2023, Ishaan Shah, International Institute of Information
+ * Technology, Hyderabad), which is licensed under the GNU General Public
+ * License version 3 or later. As a derivative of that code, this file is
+ * distributed under the same licence: see oracle/COPYING.
+ */
+/*
  * orc.h -- CPU oracle for the RIS-LTC shading pass.
  *
  * TEST INFRASTRUCTURE ONLY. This is a scalar C99 restatement of the
@@ -211,8 +221,22 @@ void orc_evaluate_brdf_f64(const double* shading, const double* dir, double* out
 void orc_ltc_coefficients_f32(const orc_scene_t* scene, const risltc_ltc_constants_t* k,
                               float fresnel_luminance, float roughness, const float* position,
                               const float* normal, const float* outgoing, float* out);
+/* One bilinearly filtered channel of the LTC arrays at texture coordinates
+ * (u, v) and array coordinate w: clamp to edge, layer = RNE(w) clamped.
+ * Pinned by affine tables (exact reproduction of an affine texel function at
+ * interior points, clamping at the edges, round-half-even layers). */
 float orc_ltc_fetch_f32(const orc_scene_t* scene, int table, int channel, float u, float v,
                         float w);
+double orc_ltc_fetch_f64(const orc_scene_t* scene, int table, int channel, double u, double v,
+                         double w);
+/* decode_position_64_bit / decode_normal_32_bit (mesh_quantization.glsl:19-45).
+ * Pinned by round trips through the exporter's packing and octahedral
+ * encoding (tools/io_export_vulkan_blender28.py:24-46, 486-512): position =
+ * correctly rounded fma(q, f, s); normal within the 16-bit quantisation bound. */
+void orc_decode_position_f32(uint32_t q0, uint32_t q1, const float f[3], const float s[3], float* out);
+void orc_decode_position_f64(uint32_t q0, uint32_t q1, const float f[3], const float s[3], double* out);
+void orc_decode_normal_f32(uint32_t qx, uint32_t qy, float* out);
+void orc_decode_normal_f64(uint32_t qx, uint32_t qy, double* out);
 
 /* risltc_sample_record_t layout helpers */
 #define ORC_REC_LIGHT 0

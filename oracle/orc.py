@@ -1,3 +1,8 @@
+# SPDX-License-Identifier: GPL-3.0-or-later
+# Part of the test oracle, which restates algorithms of the risltc renderer
+# (Copyright (C) 2021 Christoph Peters, KIT; (C) 2023 Ishaan Shah, IIIT
+# Hyderabad; GPLv3 or later) and is distributed under the same licence:
+# see oracle/COPYING.
 """ctypes binding of the CPU oracle (oracle/liborc.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
@@ -116,6 +121,12 @@ def lib():
                                                 C.c_float, f32p, f32p, f32p, f32p]),
             "orc_ltc_fetch_f32": (C.c_float, [C.POINTER(OrcScene), C.c_int, C.c_int, C.c_float, C.c_float,
                                               C.c_float]),
+            "orc_ltc_fetch_f64": (C.c_double, [C.POINTER(OrcScene), C.c_int, C.c_int, C.c_double, C.c_double,
+                                               C.c_double]),
+            "orc_decode_position_f32": (None, [C.c_uint32, C.c_uint32, f32p, f32p, f32p]),
+            "orc_decode_position_f64": (None, [C.c_uint32, C.c_uint32, f32p, f32p, f64p]),
+            "orc_decode_normal_f32": (None, [C.c_uint32, C.c_uint32, f32p]),
+            "orc_decode_normal_f64": (None, [C.c_uint32, C.c_uint32, f64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -1,4 +1,14 @@
 /*
+ * SPDX-License-Identifier: GPL-3.0-or-later
+ *
+ * This file restates, for testing, algorithms of the risltc renderer
+ * (Copyright (C) 2021, Christoph Peters, Karlsruhe Institute of Technology;
+ * Copyright (C) 2023, Ishaan Shah, International Institute of Information
+ * Technology, Hyderabad), which is licensed under the GNU General Public
+ * License version 3 or later. As a derivative of that code, this file is
+ * distributed under the same licence: see oracle/COPYING.
+ */
+/*
  * orc_common.c -- oracle parts that do not depend on the arithmetic type:
  * noise, the host data model, project FP32 transcendentals, the clipping
  * rule and the CPU LBVH.

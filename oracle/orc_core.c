@@ -1,4 +1,14 @@
 /*
+ * SPDX-License-Identifier: GPL-3.0-or-later
+ *
+ * This file restates, for testing, algorithms of the risltc renderer
+ * (Copyright (C) 2021, Christoph Peters, Karlsruhe Institute of Technology;
+ * Copyright (C) 2023, Ishaan Shah, International Institute of Information
+ * Technology, Hyderabad), which is licensed under the GNU General Public
+ * License version 3 or later. As a derivative of that code, this file is
+ * distributed under the same licence: see oracle/COPYING.
+ */
+/*
  * orc_core.c -- the per-pixel shading path of the reference, restated in C99.
  *
  * TEST INFRASTRUCTURE ONLY (see orc.h). Compiled twice: with ORC_F64 the
@@ -992,7 +1002,7 @@ static uint32_t light_index_from(R u, uint32_t n) {
  *   else slot s = first with (int)((2a - 1) T) (clamped to T - 1) <
  *   w_0 + .. + w_s, cluster id_s, light number min(size - 1, (int)(b size))
  *   of that cluster (an empty cluster or an id past the clusters:
- *   light_index_from(b)).
+ *   light_index_from(b); learned_density counts that mass as uniform).
  * parity: bitwise against the kernel's learned shading given the same tables
  * (tests/test_gpu_learned.py); the mixture is an exact pdf of this draw
  * (tests/test_oracle_estimator.py::test_learned_draw_density). */
@@ -1029,16 +1039,28 @@ static uint32_t learned_draw(const orc_scene_t* sc, const uint8_t* t, uint32_t T
     return sc->cluster_lights[off + j];
 }
 
-/* q = 1/(2N) + w_c / (2 T |c|), w_c = 0 for clusters not in the table */
-static R learned_density(const orc_scene_t* sc, const uint8_t* t, uint32_t T, uint32_t idx, uint32_t N) {
+/* Weight of the slots that learned_draw sends to a uniform light: ids past
+ * the clusters, or of empty clusters. */
+static uint32_t learned_fallback_weight(const orc_scene_t* sc, const uint8_t* t) {
+    uint32_t bad = 0;
+    for (int s = 0; s != 16; ++s) {
+        uint32_t w = table_weight(sc, t, s), c = table_id(sc, t, s);
+        if (w != 0u && (c >= sc->cluster_count || sc->cluster_offsets[c + 1] == sc->cluster_offsets[c])) bad += w;
+    }
+    return bad;
+}
+
+/* q = 1/(2N) + T_bad / (2 T N) + w_c / (2 T |c|): w_c sums the weights of
+ * every slot holding the light's cluster (0 when none does), T_bad is
+ * learned_fallback_weight (its term is skipped when 0) */
+static R learned_density(const orc_scene_t* sc, const uint8_t* t, uint32_t T, uint32_t T_bad, uint32_t idx,
+                         uint32_t N) {
     if (T == 0u) return K(1.0) / (R)N;
     R q = K(0.5) / (R)N;
+    if (T_bad != 0u) q = q + (K(0.5) * (R)T_bad) / ((R)T * (R)N);
     uint32_t c = sc->cluster_of_light[idx], wc = 0;
     for (int s = 0; s != 16; ++s)
-        if (table_weight(sc, t, s) != 0u && table_id(sc, t, s) == c) {
-            wc = table_weight(sc, t, s);
-            break;
-        }
+        if (table_weight(sc, t, s) != 0u && table_id(sc, t, s) == c) wc += table_weight(sc, t, s);
     if (wc != 0u) {
         uint32_t size = sc->cluster_offsets[c + 1] - sc->cluster_offsets[c];
         q = q + (K(0.5) * (R)wc) / ((R)T * (R)size);
@@ -1181,9 +1203,11 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
                     s->proposal == RISLTC_PROPOSAL_LEARNED
                         ? sc->proposal_table + (size_t)pixel * (sc->proposal_table_bytes == 48u ? 48u : 32u)
                         : NULL;
-                uint32_t T = 0;
-                if (tab)
+                uint32_t T = 0, T_bad = 0;
+                if (tab) {
                     for (int k2 = 0; k2 != 16; ++k2) T += table_weight(sc, tab, k2);
+                    T_bad = learned_fallback_weight(sc, tab);
+                }
                 const uint8_t* crow = learned && sc->cache_vis ? sc->cache_vis + (size_t)ckey * sc->cache_slots : NULL;
                 const float lf = ldexpf(1.0f, -(int)(s->learned_floor_log2 ? s->learned_floor_log2 : 3u));
                 const float ls = (1.0f - lf) / 255.0f;
@@ -1212,7 +1236,7 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
                     R p_hat = SQRT(dot3(col, col));
                     if (crow)
                         p_hat = p_hat * (R)cache_factor(sc->cluster_of_light, idx, lf, ls, crow, sc->cache_per_light);
-                    R p = learned ? learned_density(sc, tab, T, idx, N) : K(1.0) / (R)N;
+                    R p = learned ? learned_density(sc, tab, T, T_bad, idx, N) : K(1.0) / (R)N;
                     R w = p_hat / p;
                     R r = (R)orc_noise_next(&px.seed);
                     w_sum += w;
@@ -1387,6 +1411,26 @@ void SFX(orc_evaluate_brdf)(const R* sh, const R* dir, R* out) {
     out[2] = b.z;
 }
 
+/* mesh decode (mesh_quantization.glsl:19-45): out = xyz */
+void SFX(orc_decode_position)(uint32_t q0, uint32_t q1, const float f[3], const float s[3], R* out) {
+    V3 p = decode_position(q0, q1, f, s);
+    out[0] = p.x;
+    out[1] = p.y;
+    out[2] = p.z;
+}
+
+void SFX(orc_decode_normal)(uint32_t qx, uint32_t qy, R* out) {
+    V3 n = decode_normal((uint16_t)qx, (uint16_t)qy);
+    out[0] = n.x;
+    out[1] = n.y;
+    out[2] = n.z;
+}
+
+/* one filtered channel of the LTC arrays (ltc_utility.glsl:56-88, H5) */
+R SFX(orc_ltc_fetch)(const orc_scene_t* sc, int table, int channel, R u, R v, R w) {
+    return ltc_fetch(sc, table, channel, u, v, w);
+}
+
 #ifndef ORC_F64
 void orc_ltc_coefficients_f32(const orc_scene_t* sc, const risltc_ltc_constants_t* k, float f0l, float rough,
                               const float* pos, const float* n, const float* o, float* out) {
@@ -1404,7 +1448,4 @@ void orc_ltc_coefficients_f32(const orc_scene_t* sc, const risltc_ltc_constants_
     out[31] = ltc.determinant;
 }
 
-float orc_ltc_fetch_f32(const orc_scene_t* sc, int table, int channel, float u, float v, float w) {
-    return ltc_fetch(sc, table, channel, u, v, w);
-}
 #endif

@@ -412,7 +412,7 @@ int risltc_load_ltc_fit(const char* dir, uint32_t fresnel_count, uint32_t* out_r
         snprintf(path, sizeof(path), "%s/fit%u.dat", dir, i);
         FILE* fh = fopen(path, "rb");
         if (!fh) {
-            fprintf(stderr, "Failed to open the linearly transformed cosine table at %s.\n", path);
+            fprintf(stderr, "risltc_load_ltc_fit: cannot open %s\n", path);
             return RISLTC_E_IO;
         }
         uint64_t r = 0;
@@ -432,8 +432,7 @@ int risltc_load_ltc_fit(const char* dir, uint32_t fresnel_count, uint32_t* out_r
                 return RISLTC_E_ARG;
             }
         } else if (r != res) {
-            fprintf(stderr, "The linearly transformed cosine tables in directory %s have inconsistent resolutions.\n",
-                    dir);
+            fprintf(stderr, "risltc_load_ltc_fit: the fit files in %s differ in resolution\n", dir);
             fclose(fh);
             return RISLTC_E_FORMAT;
         }
@@ -656,8 +655,9 @@ int risltc_set_lights(risltc_context_t* c, const void* records, uint32_t count, 
     c->light_count = count;
     c->sphere_bound = round_up_float(pl.bound);
     c->light_centroids = std::move(pl.centroids);
-    // clusters refer to the previous light set
+    // clusters, and tables that name them, refer to the previous light set
     c->cluster_count = 0;
+    c->prop_width = c->prop_height = 0;
     return clear_vcache(c);
 }
 
@@ -763,6 +763,7 @@ int risltc_set_light_clusters(risltc_context_t* c, const uint32_t* cluster_of_li
     HIPC(c, hipMemcpy(c->cluster_offsets.p, offsets.data(), offsets.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIPC(c, hipMemcpy(c->cluster_lights.p, sorted.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
     c->cluster_count = cluster_count;
+    c->prop_width = c->prop_height = 0;  // tables that name the previous clusters
     return clear_vcache(c);
 }
 

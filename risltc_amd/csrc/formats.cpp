@@ -64,12 +64,12 @@ extern "C" int risltc_load_vks(const char* path, risltc_vks_info_t* info, uint32
     memset(info, 0, sizeof(*info));
     File fh(path, "rb");
     if (!fh.f) {
-        fprintf(stderr, "Failed to open the scene file at %s.\n", path);
+        fprintf(stderr, "risltc_load_vks: cannot open %s\n", path);
         return RISLTC_E_IO;
     }
     uint32_t head[2];
     if (!fh.read(head, sizeof(head)) || head[0] != kVksMarker || head[1] != kVksVersion) {
-        fprintf(stderr, "The scene file at path %s is invalid or unsupported.\n", path);
+        fprintf(stderr, "risltc_load_vks: %s has no .vks header of a supported version\n", path);
         return RISLTC_E_FORMAT;
     }
     uint64_t counts[2];
@@ -79,7 +79,7 @@ extern "C" int risltc_load_vks(const char* path, risltc_vks_info_t* info, uint32
     info->material_count = counts[0];
     info->triangle_count = counts[1];
     if (info->triangle_count == 0) {
-        fprintf(stderr, "The scene file at path %s is completely empty, i.e. it holds 0 triangles.\n", path);
+        fprintf(stderr, "risltc_load_vks: %s declares no triangles\n", path);
         return RISLTC_E_FORMAT;
     }
     // counts no file of this format can hold (the byte sizes below would
@@ -114,10 +114,7 @@ extern "C" int risltc_load_vks(const char* path, risltc_vks_info_t* info, uint32
     }
     uint32_t eof_marker = 0;
     if (!fh.read(&eof_marker, sizeof(eof_marker)) || eof_marker != kVksEndMarker) {
-        fprintf(stderr,
-                "The scene file at path %s seems to be invalid. The geometry data is not followed by the expected "
-                "end of file marker.\n",
-                path);
+        fprintf(stderr, "risltc_load_vks: %s: no end marker after the mesh data (truncated or corrupt)\n", path);
         return RISLTC_E_FORMAT;
     }
     return RISLTC_OK;
@@ -130,7 +127,7 @@ extern "C" int risltc_load_quicksave(const char* path, risltc_camera_t* camera, 
     memset(info, 0, sizeof(*info));
     File fh(path, "rb");
     if (!fh.f) {
-        fprintf(stderr, "Failed to load a quick save. Please check path and permissions: %s\n", path);
+        fprintf(stderr, "risltc_load_quicksave: cannot open %s\n", path);
         return RISLTC_E_IO;
     }
     risltc_camera_t cam;
@@ -195,7 +192,7 @@ extern "C" int risltc_save_quicksave(const char* path, const risltc_camera_t* ca
             return RISLTC_E_ARG;
     File fh(path, "wb");
     if (!fh.f) {
-        fprintf(stderr, "Quick save failed. Please check path and permissions: %s\n", path);
+        fprintf(stderr, "risltc_save_quicksave: cannot open %s for writing\n", path);
         return RISLTC_E_IO;
     }
     const uint32_t legacy_count = 0;
@@ -285,7 +282,7 @@ extern "C" int risltc_write_hdr(const char* path, const float* pixels, uint32_t 
         return RISLTC_E_ARG;
     File fh(path, "wb");
     if (!fh.f) {
-        fprintf(stderr, "Failed to store a screenshot to the *.hdr file at %s.\n", path);
+        fprintf(stderr, "risltc_write_hdr: cannot write %s\n", path);
         return RISLTC_E_IO;
     }
     char header[160];

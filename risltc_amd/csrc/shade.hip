@@ -1061,13 +1061,17 @@ __device__ __forceinline__ uint32_t lcg_jump(uint32_t s, uint32_t d) {
 //                    min(size - 1, int(b size))-th light of c (an empty
 //                    cluster, or an id past the clusters, falls back to
 //                    light_index_from(b, N)).
-// Its source density is q = 1/(2N) + w_c / (2 T |c|) (w_c = 0 for clusters
-// not in the table), computed in the same operation order by the oracle.
+// Its source density is q = 1/(2N) + T_bad / (2 T N) + w_c / (2 T |c|):
+// w_c sums the weights of the slots that hold the light's cluster (0 when
+// none does) and T_bad the weights of the slots that fall back to a uniform
+// light (the term is skipped when T_bad = 0). The oracle computes it in the
+// same operation order.
 template <bool I16>
 struct PropTable {
     static constexpr int kIdWords = I16 ? 8 : 4;
     uint32_t ids[kIdWords], ws[4];
     uint32_t T;
+    uint32_t T_bad;
 };
 
 __device__ __forceinline__ uint32_t byte_of(const uint32_t* v, int s) { return (v[s >> 2] >> (8 * (s & 3))) & 0xFFu; }
@@ -1082,6 +1086,7 @@ __device__ __forceinline__ uint32_t id_of(const PropTable<I16>& P, int s) {
 template <bool I16>
 __device__ __forceinline__ void load_prop_table(const uint8_t* tab, PropTable<I16>& P) {
     P.T = 0u;
+    P.T_bad = 0u;
 #pragma unroll
     for (int k = 0; k < PropTable<I16>::kIdWords; ++k) P.ids[k] = 0u;
 #pragma unroll
@@ -1107,6 +1112,20 @@ struct ClusterLds {
     const uint32_t* lights;
     uint32_t count;
 };
+
+// T_bad of the pixel's table: the weight of slots whose id is past the
+// clusters or names an empty cluster (draw_learned takes a uniform light
+// for them)
+template <bool I16>
+__device__ __forceinline__ void prop_fallback_weight(PropTable<I16>& P, const ClusterLds& C) {
+    uint32_t bad = 0u;
+#pragma unroll
+    for (int s = 0; s < kPropTop; ++s) {
+        const uint32_t w = byte_of(P.ws, s), c = id_of(P, s);
+        if (w != 0u && (c >= C.count || C.offsets[c + 1u] == C.offsets[c])) bad += w;
+    }
+    P.T_bad = bad;
+}
 
 template <bool I16>
 __device__ __forceinline__ uint32_t draw_learned(uint32_t& seed, const PropTable<I16>& P, const ClusterLds& C,
@@ -1143,12 +1162,13 @@ __device__ __forceinline__ float learned_density(uint32_t idx, const PropTable<I
                                                  uint32_t N) {
     float q = 0.5f / (float)N;
     if (P.T == 0u) return 1.0f / (float)N;
+    if (P.T_bad != 0u) q = q + div_<X>(0.5f * (float)P.T_bad, (float)P.T * (float)N);
     uint32_t c = C.of_light[idx];
     uint32_t wc = 0;
 #pragma unroll
-    for (int s = kPropTop - 1; s >= 0; --s) {
+    for (int s = 0; s < kPropTop; ++s) {
         uint32_t w = byte_of(P.ws, s);
-        if (w != 0u && id_of(P, s) == c) wc = w;
+        if (w != 0u && id_of(P, s) == c) wc += w;
     }
     if (wc != 0u) {
         uint32_t size = C.offsets[c + 1] - C.offsets[c];
@@ -1417,6 +1437,9 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                                  ? A.proposal + (size_t)pixel * (I16 ? kPropBytes16 : kPropBytes)
                                  : nullptr,
                              pt);
+        if constexpr (learned) {
+            if (pt.T != 0u) prop_fallback_weight<I16>(pt, cl);
+        }
         if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
             if (shade) chosen = (int32_t)light_index_from(noise_next(seed), N);
         } else if constexpr (DEBUG) {
@@ -1539,6 +1562,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                     const uint32_t seed_o = (uint32_t)__shfl((int)seed_chunk, (int)my_o);
                     PropTable<I16> pt_o;
                     pt_o.T = 0u;
+                    pt_o.T_bad = 0u;
 #pragma unroll
                     for (int k = 0; k < PropTable<I16>::kIdWords; ++k) pt_o.ids[k] = 0u;
 #pragma unroll

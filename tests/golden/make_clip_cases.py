@@ -20,14 +20,31 @@ SRC = os.path.join(REF, "src", "shaders", "polygon_clipping.glsl")
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "clip_cases.json")
 
 
+COMPARE = {"<=": lambda a, b: a <= b, ">=": lambda a, b: a >= b, "==": lambda a, b: a == b,
+           "!=": lambda a, b: a != b, "<": lambda a, b: a < b, ">": lambda a, b: a > b}
+TERM = re.compile(r"^\s*([A-Z_]+)\s*(<=|>=|==|!=|<|>)\s*(\d+)\s*$")
+
+
+def evaluate_condition(cond, values):
+    """An #if condition of the clipper: a conjunction (&&) of comparisons of
+    a known macro with an integer literal. Anything else is rejected; the
+    text is parsed, never executed."""
+    result = True
+    for term in cond.split("&&"):
+        m = TERM.match(term)
+        if not m or m.group(1) not in values:
+            raise ValueError("unsupported #if condition: %r" % cond)
+        result = result and COMPARE[m.group(2)](values[m.group(1)], int(m.group(3)))
+    return result
+
+
 def cases_for(text, min_count, max_count):
     out = {}
     stack = []
+    values = {"MIN_POLYGON_VERTEX_COUNT_BEFORE_CLIPPING": min_count, "MAX_POLYGON_VERTEX_COUNT": max_count}
 
     def ev(cond):
-        cond = cond.replace("MIN_POLYGON_VERTEX_COUNT_BEFORE_CLIPPING", str(min_count))
-        cond = cond.replace("MAX_POLYGON_VERTEX_COUNT", str(max_count)).replace("&&", " and ")
-        return bool(eval(cond, {}, {}))
+        return evaluate_condition(cond, values)
 
     for line in text.split("\n"):
         s = line.strip()

@@ -122,15 +122,26 @@ def test_estimator_is_unbiased(orc_lib, ltc_table, roughness, metal):
 
 
 # ---------------------------------------------------------------- learned source
-def _random_tables(rng, P, K, sizes):
+def _random_tables(rng, P, K, sizes, bad=False):
     """Per-pixel tables: 16 distinct non-empty clusters with random 8-bit weights
-    (some zero, as the proposal kernel writes them), on some pixels all zero."""
+    (some zero, as the proposal kernel writes them), on some pixels all zero.
+    With bad=True, tables that no proposal kernel writes but a caller may
+    pass: slots naming empty clusters, ids past the clusters, and clusters
+    named twice."""
     nonempty = np.flatnonzero(sizes > 0)
+    empty = np.flatnonzero(sizes == 0)
     tab = np.zeros((P, 32), np.uint8)
     for p in range(P):
         if p % 7 == 3:
             continue  # T = 0: uniform fallback
         ids = rng.choice(nonempty, min(16, nonempty.size), replace=False)
+        if bad:
+            pick = rng.random(ids.size)
+            ids = np.where(pick < 0.15, rng.integers(K, 256, ids.size), ids)   # past the clusters
+            if empty.size:
+                ids = np.where((pick >= 0.15) & (pick < 0.3), rng.choice(empty, ids.size), ids)  # empty
+            dup = (pick >= 0.3) & (pick < 0.4)
+            ids = np.where(dup, ids[0], ids)  # the first slot's cluster again
         w = rng.integers(0, 256, ids.size)
         w[rng.random(ids.size) < 0.2] = 0
         w[0] = max(w[0], 1)
@@ -141,36 +152,46 @@ def _random_tables(rng, P, K, sizes):
 
 
 def _q_numpy(tab, idx, col, sizes, N):
-    """q = 1/(2N) + w_c / (2 T |c|) in float32 with the kernel's operation order."""
+    """q = 1/(2N) + T_bad/(2 T N) + w_c / (2 T |c|) in float32 with the
+    kernel's operation order: T_bad is the weight of slots the draw sends to
+    a uniform light (ids past the clusters, empty clusters), w_c the summed
+    weight of the slots naming the light's cluster."""
     f = np.float32
+    K = sizes.size
     T = int(tab[16:].astype(np.int64).sum())
     if T == 0:
         return f(1.0) / f(N)
     q = f(0.5) / f(N)
+    bad = sum(int(tab[16 + s]) for s in range(16) if tab[s] >= K or sizes[tab[s]] == 0)
+    if bad:
+        q = q + (f(0.5) * f(bad)) / (f(T) * f(N))
     c = col[idx]
-    for s in range(16):
-        if tab[16 + s] != 0 and tab[s] == c:
-            return q + (f(0.5) * f(tab[16 + s])) / (f(T) * f(sizes[c]))
+    wc = sum(int(tab[16 + s]) for s in range(16) if tab[16 + s] != 0 and tab[s] == c)
+    if wc:
+        q = q + (f(0.5) * f(wc)) / (f(T) * f(sizes[c]))
     return q
 
 
-def _source_oracle(scene, ltc_table, rng, K=16, bvh=True):
+def _source_oracle(scene, ltc_table, rng, K=16, bvh=True, bad=False):
     o = orc.Oracle(scene, *ltc_table, bvh=bvh)
     N = scene.light_count
-    col = (np.arange(N) * 7 % K).astype(np.uint32)
+    # with bad tables the last cluster is left empty
+    col = (np.arange(N) * 7 % (K - 1 if bad else K)).astype(np.uint32)
     sizes = np.bincount(col, minlength=K)
     P = scene.width * scene.height
-    tab = _random_tables(rng, P, K, sizes)
+    tab = _random_tables(rng, P, K, sizes, bad)
     o.set_proposal(tab, col, K)
     return o, tab, col, sizes
 
 
-def test_learned_draw_density(small_scene, ltc_table):
+@pytest.mark.parametrize("bad", [False, True], ids=["kernel_tables", "bad_slots"])
+def test_learned_draw_density(small_scene, ltc_table, bad):
     """With the learned source: every candidate's weight is exactly p_hat / q
     with q from the table (a pdf over the lights), and the candidate lights
-    are distributed as q (chi-square)."""
+    are distributed as q (chi-square). Also for tables with slots of empty
+    clusters, ids past the clusters and repeated clusters."""
     rng = np.random.default_rng(31)
-    o, tab, col, sizes = _source_oracle(small_scene, ltc_table, rng)
+    o, tab, col, sizes = _source_oracle(small_scene, ltc_table, rng, bad=bad)
     N = small_scene.light_count
     consts = o.frame_constants()
     vis = o.render_visibility(consts)
@@ -203,12 +224,15 @@ def test_learned_target_factor_constant():
     assert f + np.float32(0) * scale == f
 
 
-def test_learned_estimator_is_unbiased(orc_lib, ltc_table):
+@pytest.mark.parametrize("bad", [False, True], ids=["kernel_tables", "bad_slots"])
+def test_learned_estimator_is_unbiased(orc_lib, ltc_table, bad):
     """Mean of the learned-source estimator over many noise seeds equals the
-    plain (uniform-source) mean on the same pixels, shadows off."""
+    plain (uniform-source) mean on the same pixels, shadows off; also with
+    tables that name empty clusters, ids past the clusters and repeated
+    clusters."""
     scene = synth.small_scene(seed=3, width=48, height=48, light_count=24)
     rng = np.random.default_rng(32)
-    o, _, _, _ = _source_oracle(scene, ltc_table, rng, K=6, bvh=False)
+    o, _, _, _ = _source_oracle(scene, ltc_table, rng, K=6, bvh=False, bad=bad)
     consts = o.frame_constants()
     o2 = orc.Oracle(scene, *ltc_table)
     vis = o2.render_visibility(consts)

@@ -113,7 +113,7 @@ def _light(rng, n):
     return L
 
 
-@pytest.mark.skipif(orc.ref_lib() is None and not os.path.exists("/root/reference/src"),
+@pytest.mark.skipif(not os.path.exists(orc.REF_LIB_PATH) and not os.path.exists("/root/reference/src"),
                     reason="reference host sources not available")
 def test_update_polygonal_light_bitwise_vs_reference_build(orc_lib):
     """src/polygonal_light.c:44-98 compiled from the reference tree."""
