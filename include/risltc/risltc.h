@@ -417,6 +417,25 @@ int risltc_shade(risltc_context_t* context, const risltc_frame_constants_t* cons
                  const risltc_settings_t* settings, const uint32_t* visibility, float* out_rgba32f,
                  void* stream);
 
+/* risltc_shade without device allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory during a frame
+ * (code under stream capture, tight serving loops, hosts that forbid
+ * hipMalloc). The sample records, shadow-ray list, hit words and counters
+ * live in one fixed device array that the library defines, sized for at
+ * most RISLTC_NOALLOC_MAX_SAMPLES pixel-samples (width * height * spp;
+ * about 72 MiB, held by every GPU the library runs on until the process
+ * ends). Larger calls fail with RISLTC_E_UNSUPPORTED. The output equals
+ * risltc_shade's for the same inputs; no training records for
+ * risltc_proposal_train are written. Lights, LTC table, scene, clusters,
+ * proposal tables and visibility cache must be set beforehand (those calls
+ * allocate). Every call on one GPU shares that GPU's array, from every
+ * context: calls that overlap on one GPU (other streams, other threads)
+ * race and must be serialised by the caller. */
+#define RISLTC_NOALLOC_MAX_SAMPLES (1u << 19)
+int risltc_shade_noalloc(risltc_context_t* context, const risltc_frame_constants_t* constants,
+                         const risltc_settings_t* settings, const uint32_t* visibility, float* out_rgba32f,
+                         void* stream);
+
 /* Running mean of frames, the accumulation pass of
  * src/shaders/accum_pass.frag.glsl:45-53: accum = (accum * n + cur) / (n + 1).
  * Device pointers, pixel_count * 4 floats. */

@@ -68,6 +68,8 @@ SIGNATURES = {
     "risltc_shade": (C.c_int, [_vp, C.POINTER(abi.FrameConstants), C.POINTER(abi.Settings), _vp, _vp, _vp]),
     "risltc_shade_debug": (C.c_int, [_vp, C.POINTER(abi.FrameConstants), C.POINTER(abi.Settings), _vp, _vp,
                                      _vp, _vp, _vp]),
+    "risltc_shade_noalloc": (C.c_int, [_vp, C.POINTER(abi.FrameConstants), C.POINTER(abi.Settings), _vp, _vp,
+                                       _vp]),
     "risltc_accumulate": (C.c_int, [_vp, _vp, _vp, _u32, _u32, _vp]),
     "risltc_trace_shadow_rays": (C.c_int, [_vp, _vp, _u32, _vp, _vp]),
     "risltc_last_timings": (C.c_int, [_vp, _f32p]),
@@ -399,6 +401,12 @@ class Context:
     def shade(self, constants, settings, visibility, out, stream=None):
         self._check(self.L.risltc_shade(self.h, C.byref(constants), C.byref(settings), _dptr(visibility),
                                         _dptr(out), _stream(stream)), "risltc_shade")
+
+    def shade_noalloc(self, constants, settings, visibility, out, stream=None):
+        """risltc_shade on the library's fixed scratch array (no device
+        allocation; at most NOALLOC_MAX_SAMPLES pixel-samples)."""
+        self._check(self.L.risltc_shade_noalloc(self.h, C.byref(constants), C.byref(settings), _dptr(visibility),
+                                                _dptr(out), _stream(stream)), "risltc_shade_noalloc")
 
     def shade_debug(self, constants, settings, visibility, out, samples, candidates=None, stream=None):
         self._check(self.L.risltc_shade_debug(self.h, C.byref(constants), C.byref(settings), _dptr(visibility),

@@ -9,6 +9,8 @@ MAX_LIGHT_VERTEX_COUNT = 8
 MAX_POLYGON_VERTEX_COUNT = 9
 SAMPLE_RECORD_FLOATS = 40
 CANDIDATE_RECORD_WORDS = 4
+# pixel-samples of risltc_shade_noalloc's fixed scratch (RISLTC_NOALLOC_MAX_SAMPLES)
+NOALLOC_MAX_SAMPLES = 1 << 19
 
 OK, E_ARG, E_HIP, E_FORMAT, E_UNSUPPORTED, E_STATE, E_IO = range(7)
 ERROR_NAMES = {0: "OK", 1: "E_ARG", 2: "E_HIP", 3: "E_FORMAT", 4: "E_UNSUPPORTED",

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/risltc/risltc.h"
@@ -132,6 +133,8 @@ struct risltc_context_s {
     hipEvent_t ev[8] = {};
     bool timed = false;
     bool learned_ran = false;
+    // this device's copy of the fixed scratch array of risltc_shade_noalloc
+    uint8_t* noalloc_scratch = nullptr;
 };
 
 static int fail(risltc_context_t* c, int code, const char* fmt, ...) {
@@ -468,6 +471,44 @@ int risltc_load_ltc_fit(const char* dir, uint32_t fresnel_count, uint32_t* out_r
 }
 
 // ---- context -----------------------------------------------------------------------
+// ---- the fixed scratch of risltc_shade_noalloc --------------------------------------
+// One file-scope device array holds what risltc_shade allocates per call (sample
+// records, the compacted shadow-ray list, ray slots, hit words, the ray and
+// trace-queue counters) for at most RISLTC_NOALLOC_MAX_SAMPLES pixel-samples:
+// about 72 MiB, held by every GPU that loads this library until the process
+// ends. Each region starts on a 256-byte boundary.
+static const uint32_t kCounterWords = rl::kQueueStride * (1u + rl::kTraceQueues);
+namespace noalloc {
+constexpr size_t align256(size_t x) { return (x + 255u) & ~size_t(255u); }
+constexpr size_t kSamples = RISLTC_NOALLOC_MAX_SAMPLES;
+constexpr size_t kRecordsOff = 0;
+constexpr size_t kRaysOff = align256(kRecordsOff + kSamples * rl::kSampleWords * sizeof(float));
+constexpr size_t kSlotsOff = align256(kRaysOff + kSamples * 2 * sizeof(rl::ShadowRay));
+constexpr size_t kHitsOff = align256(kSlotsOff + kSamples * 2 * sizeof(uint32_t));
+constexpr size_t kCounterOff = align256(kHitsOff + kSamples * 2 * sizeof(uint32_t));
+constexpr size_t kBytes = align256(kCounterOff + kCounterWords * sizeof(uint32_t));
+}  // namespace noalloc
+static __device__ __align__(256) uint8_t scratch[noalloc::kBytes];
+
+// The array's address on each device, resolved once per device with that
+// device current (the host-side name `scratch` is only a handle for the
+// runtime and is never passed to a kernel).
+static std::mutex g_scratch_mutex;
+static uint8_t* g_scratch_of_device[256] = {};
+
+static hipError_t resolve_scratch(int device, uint8_t** out) {
+    if (device < 0 || device >= 256) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_of_device[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_of_device[device] = static_cast<uint8_t*>(p);
+    }
+    *out = g_scratch_of_device[device];
+    return hipSuccess;
+}
+
 int risltc_create(risltc_context_t** out, int device) {
     if (!out) return RISLTC_E_ARG;
     *out = nullptr;
@@ -487,6 +528,12 @@ int risltc_create(risltc_context_t** out, int device) {
             delete c;
             return RISLTC_E_HIP;
         }
+    if (resolve_scratch(device, &c->noalloc_scratch) != hipSuccess) {
+        for (auto& e : c->ev)
+            if (e) hipEventDestroy(e);
+        delete c;
+        return RISLTC_E_HIP;
+    }
     if (const char* v = getenv("RISLTC_TRACE_REFILL")) {
         long r = strtol(v, nullptr, 10);
         if (r >= 1 && r <= 64) c->tuning.trace_refill = (uint32_t)r;
@@ -1394,8 +1441,6 @@ static void fill_resolve(const risltc_frame_constants_t* k, const risltc_setting
     r.out = out;
 }
 
-static const uint32_t kCounterWords = rl::kQueueStride * (1u + rl::kTraceQueues);
-
 static rl::TraceArgs trace_args(risltc_context_t* c, const rl::BvhView* bvhs, rl::ShadowRay* rays, uint32_t* slots,
                                 uint32_t* counter, uint32_t* hits, size_t max_rays) {
     rl::TraceArgs t;
@@ -1410,8 +1455,21 @@ static rl::TraceArgs trace_args(risltc_context_t* c, const rl::BvhView* bvhs, rl
     return t;
 }
 
+// Per-call work buffers of one shading pass.
+struct ShadeWork {
+    float* samples;
+    rl::ShadowRay* rays;
+    uint32_t* ray_slots;
+    uint32_t* hits;
+    uint32_t* counter;
+};
+
+// fixed: the work buffers to use (risltc_shade_noalloc: nothing is
+// allocated and no training records are written), or null to size the
+// context's own
 static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, const risltc_settings_t* s,
-                      const uint32_t* vis, float* out, float* dbg_s, uint32_t* dbg_c, void* stream) {
+                      const uint32_t* vis, float* out, float* dbg_s, uint32_t* dbg_c, void* stream,
+                      const ShadeWork* fixed = nullptr) {
     if (!c) return RISLTC_E_ARG;
     if (!k || !s || !vis || !out) return fail(c, RISLTC_E_ARG, "null argument");
     if (c->light_count == 0) return fail(c, RISLTC_E_STATE, "risltc_set_lights has not been called");
@@ -1431,11 +1489,18 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     const size_t S = (size_t)k->viewport_size[0] * k->viewport_size[1] * s->sample_count_light;
     hipStream_t st = (hipStream_t)stream;
     HIPC(c, hipSetDevice(c->device));
-    HIPC(c, c->samples.ensure(S * rl::kSampleWords * sizeof(float)));
-    HIPC(c, c->rays.ensure(S * 2 * sizeof(rl::ShadowRay)));
-    HIPC(c, c->ray_slots.ensure(S * 2 * sizeof(uint32_t)));
-    HIPC(c, c->hits.ensure(S * 2 * sizeof(uint32_t)));
-    HIPC(c, c->counter.ensure(kCounterWords * sizeof(uint32_t)));
+    ShadeWork wk;
+    if (fixed) {
+        wk = *fixed;
+    } else {
+        HIPC(c, c->samples.ensure(S * rl::kSampleWords * sizeof(float)));
+        HIPC(c, c->rays.ensure(S * 2 * sizeof(rl::ShadowRay)));
+        HIPC(c, c->ray_slots.ensure(S * 2 * sizeof(uint32_t)));
+        HIPC(c, c->hits.ensure(S * 2 * sizeof(uint32_t)));
+        HIPC(c, c->counter.ensure(kCounterWords * sizeof(uint32_t)));
+        wk = ShadeWork{c->samples.as<float>(), c->rays.as<rl::ShadowRay>(), c->ray_slots.as<uint32_t>(),
+                       c->hits.as<uint32_t>(), c->counter.as<uint32_t>()};
+    }
     LightRef L;
     light_ref(c, 0, L);
     SceneRef SR;
@@ -1443,10 +1508,10 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     if (e) return e;
     rl::ShadeArgs a;
     fill_shade(c, k, s, vis, L, SR, a);
-    a.samples = c->samples.as<float>();
-    a.rays = c->rays.as<rl::ShadowRay>();
-    a.ray_slots = c->ray_slots.as<uint32_t>();
-    a.ray_count = c->counter.as<uint32_t>();
+    a.samples = wk.samples;
+    a.rays = wk.rays;
+    a.ray_slots = wk.ray_slots;
+    a.ray_count = wk.counter;
     a.debug_samples = dbg_s;
     a.debug_cands = dbg_c;
     a.learned_pass = learned ? 1u : 0u;
@@ -1458,7 +1523,7 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
             return fail(c, RISLTC_E_STATE, "the visibility cache was made for %u lights", c->vcache_slots);
         a.cache = vcache_view(c);
     }
-    HIPC(c, rl::launch_zero_u32(c->counter.as<uint32_t>(), kCounterWords, st));
+    HIPC(c, rl::launch_zero_u32(wk.counter, kCounterWords, st));
     HIPC(c, hipEventRecord(c->ev[0], st));
     // the cells the previous resolve counted in get their new bytes (timed
     // with the shade kernel)
@@ -1470,16 +1535,16 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     HIPC(c, hipEventRecord(c->ev[1], st));
     if (s->shadow_rays)
         HIPC(c, rl::launch_trace(trace_args(c, c->bvh_table.as<rl::BvhView>(), a.rays, a.ray_slots, a.ray_count,
-                                            c->hits.as<uint32_t>(), S * 2),
+                                            wk.hits, S * 2),
                                  st));
     HIPC(c, hipEventRecord(c->ev[2], st));
     rl::ResolveArgs r;
-    fill_resolve(k, s, vis, a.samples, c->hits.as<uint32_t>(), out, SR.triangle_count, r);
+    fill_resolve(k, s, vis, a.samples, wk.hits, out, SR.triangle_count, r);
     r.debug_samples = dbg_s;
     c->rec_samples = 0;
     if (c->cluster_count > 0) {
         r.cluster_of_light = c->cluster_of_light.as<uint16_t>();
-        if (!s->skip_training_records) {
+        if (!s->skip_training_records && !fixed) {
             // training records for the proposal: one per pixel-sample
             HIPC(c, c->rec_cluster.ensure(S * sizeof(uint32_t)));
             HIPC(c, c->rec_y.ensure(S * sizeof(float)));
@@ -1856,6 +1921,26 @@ int risltc_shade_debug(risltc_context_t* c, const risltc_frame_constants_t* k, c
                        const uint32_t* vis, float* out, float* samples, uint32_t* cands, void* stream) {
     if (c && !samples && !cands) return fail(c, RISLTC_E_ARG, "no debug buffer");
     return shade_impl(c, k, s, vis, out, samples, cands, stream);
+}
+
+int risltc_shade_noalloc(risltc_context_t* c, const risltc_frame_constants_t* k, const risltc_settings_t* s,
+                         const uint32_t* vis, float* out, void* stream) {
+    if (!c) return RISLTC_E_ARG;
+    if (!k || !s) return fail(c, RISLTC_E_ARG, "null argument");
+    const uint64_t S = (uint64_t)k->viewport_size[0] * k->viewport_size[1] * s->sample_count_light;
+    if (S > RISLTC_NOALLOC_MAX_SAMPLES)
+        return fail(c, RISLTC_E_UNSUPPORTED,
+                    "risltc_shade_noalloc: %llu pixel-samples (%u x %u x %u spp) exceed its fixed scratch of %u",
+                    (unsigned long long)S, k->viewport_size[0], k->viewport_size[1], s->sample_count_light,
+                    (unsigned)RISLTC_NOALLOC_MAX_SAMPLES);
+    if (!c->noalloc_scratch) return fail(c, RISLTC_E_STATE, "no scratch address for device %d", c->device);
+    uint8_t* base = c->noalloc_scratch;
+    const ShadeWork wk{reinterpret_cast<float*>(base + noalloc::kRecordsOff),
+                       reinterpret_cast<rl::ShadowRay*>(base + noalloc::kRaysOff),
+                       reinterpret_cast<uint32_t*>(base + noalloc::kSlotsOff),
+                       reinterpret_cast<uint32_t*>(base + noalloc::kHitsOff),
+                       reinterpret_cast<uint32_t*>(base + noalloc::kCounterOff)};
+    return shade_impl(c, k, s, vis, out, nullptr, nullptr, stream, &wk);
 }
 
 int risltc_accumulate(risltc_context_t* c, float* accum, const float* frame, uint32_t count, uint32_t n, void* stream) {

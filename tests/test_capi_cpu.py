@@ -47,6 +47,12 @@ def test_library_exports_every_declared_function(hiplib):
         assert hasattr(hiplib, n)
 
 
+def test_noalloc_limit_matches_header():
+    text = open(HEADER).read()
+    m = re.search(r"#define RISLTC_NOALLOC_MAX_SAMPLES \(1u << (\d+)\)", text)
+    assert m and (1 << int(m.group(1))) == abi.NOALLOC_MAX_SAMPLES
+
+
 def test_struct_layouts():
     assert C.sizeof(abi.FrameConstants) == 256
     offs = {f[0]: getattr(abi.FrameConstants, f[0]).offset for f in abi.FrameConstants._fields_}
