@@ -28,8 +28,8 @@ The learned_batch object: the learned pass in one batch at the large end
 per-view proposal tables, the shared visibility cache), against the plain
 batch of the same views, and as a graph replay.
 
-The large_learned_vs_plain object (--large-convergence, off by default: about
-3.5 minutes): learned vs plain time to relMSE 0.01 on one 1920x1080 view at
+The large_learned_vs_plain object (about 3 minutes; --no-large-convergence
+skips it): learned vs plain time to relMSE 0.01 on one 1920x1080 view at
 the large end (16384 lights, ~4M triangles, 16 spp per frame) against a
 65536-spp plain reference, for the 1024-cluster width-128 network and the
 default 256-cluster network.
@@ -37,7 +37,7 @@ default 256-cluster network.
 The launch shape is autotuned (risltc_autotune) on the frame before timing.
 
   python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-learned] [--no-batch]
-                  [--no-learned-batch] [--large-convergence]
+                  [--no-learned-batch] [--no-large-convergence]
 """
 import argparse
 import json
@@ -72,19 +72,13 @@ def batch_views(ctx, ltc_rgba, ltc_rg, count, seed=7):
     polygons (3-8 vertices, count log-uniform), resolution uniform in
     320x240 .. 1920x1080, M from {8, 16, 32}, 2 spp. Returns the views and
     the buffers they point at."""
-    import math
     import torch
     import risltc_amd
     from risltc_amd import synth
-    rng = np.random.default_rng(seed)
     views, keep = [], []
     samples = 0
-    for v in range(count):
-        grid = int(rng.integers(60, 191))
-        lights = int(round(math.exp(rng.uniform(math.log(16), math.log(8192)))))
-        w, h = int(rng.integers(320, 1921)), int(rng.integers(240, 1081))
-        sc = synth.city_scene(seed=100 + v, width=w, height=h, light_count=lights, grid=grid, spp=2,
-                              candidates=int(rng.choice([8, 16, 32])))
+    for sc in synth.batch_view_scenes(count, seed):
+        w, h = sc.width, sc.height
         sid = ctx.add_scene(sc)
         lid = ctx.add_light_set(sc.light_records, sc.light_count, sc.light_vmax)
         c = risltc_amd.write_frame_constants(sc.camera, w, h, sc.mesh.dequantization_factor,
@@ -153,8 +147,9 @@ def bench_learned_batch(ltc_rgba, ltc_rg, steps, warmup, views_n=16, grid=540, l
     cache, M = 24) and as a graph replay of the learned batch. The MLP is the
     96-128-128-128-1024 network over 1024 k-means clusters (per-pixel tables
     of 2-byte cluster ids), warmed by a few trained single-view frames. Wall
-    time per step; no convergence run (a 65536-spp reference of 16 views is
-    out of this bench's time)."""
+    time per step, learned steps including the refresh of every view's table
+    at the learned pass's inference cadence; no convergence run (a
+    65536-spp reference of 16 views is out of this bench's time)."""
     import math
     import torch
     import risltc_amd
@@ -208,7 +203,7 @@ def bench_learned_batch(ltc_rgba, ltc_rg, steps, warmup, views_n=16, grid=540, l
         ctx.shade(c, sc.settings(proposal=abi.PROPOSAL_LEARNED, ris_candidate_count=lk["candidates"],
                                  learned_floor_log2=lk["floor_log2"]), vis0, bufs[0][2])
         ctx.proposal_train(c, vis0, batch=lk["batch"], steps=1, lr=lk["lr"], seed=f)
-    plain, learned = [], []
+    plain, learned, consts = [], [], []
     for cam, vis, out, tab in bufs:
         c = risltc_amd.write_frame_constants(cam, W, H, sc.mesh.dequantization_factor,
                                              sc.mesh.dequantization_summand, ctx.ltc_constants, noise_seed=9,
@@ -216,9 +211,24 @@ def bench_learned_batch(ltc_rgba, ltc_rg, steps, warmup, views_n=16, grid=540, l
         ctx.proposal_update(c, vis, table=tab)
         plain.append(ctx.view(c, plain_s, vis, out))
         learned.append(ctx.view(c, learned_s, vis, out, proposal_table=tab))
+        consts.append(c)
     torch.cuda.synchronize()
     setup_s = time.time() - t0
     samples = views_n * W * H * spp
+    # a learned step refreshes every view's proposal table at the learned
+    # pass's cadence (every infer_every steps), then shades the batch
+    infer = {"steps": 0, "ms": 0.0}
+
+    def refresh_tables():
+        if infer["steps"] % lk["infer_every"] == 0:
+            for (cam, vis, out, tab), c in zip(bufs, consts):
+                ctx.proposal_update(c, vis, table=tab)
+                infer["ms"] += ctx.proposal_last_timings()[0]
+        infer["steps"] += 1
+
+    def learned_step():
+        refresh_tables()
+        ctx.shade_batch(learned)
 
     def timed(fn):
         for _ in range(warmup):
@@ -232,11 +242,16 @@ def bench_learned_batch(ltc_rgba, ltc_rg, steps, warmup, views_n=16, grid=540, l
 
     ms_plain = timed(lambda: ctx.shade_batch(plain))
     k_plain = ctx.last_timings()
-    ms_learned = timed(lambda: ctx.shade_batch(learned))
+    ms_learned = timed(learned_step)
     k_learned = ctx.last_timings()
+    infer_ms_per_step = infer["ms"] / max(infer["steps"], 1)
     g = ctx.graph(learned)
+
+    def graph_step():
+        refresh_tables()
+        g.launch()
     try:
-        ms_graph = timed(g.launch)
+        ms_graph = timed(graph_step)
     finally:
         g.close()
     ctx.close()
@@ -249,6 +264,9 @@ def bench_learned_batch(ltc_rgba, ltc_rg, steps, warmup, views_n=16, grid=540, l
             "setup_s": round(setup_s, 1),
             "ms_per_step": {"plain_batch": round(ms_plain, 2), "learned_batch": round(ms_learned, 2),
                             "learned_graph": round(ms_graph, 2)},
+            "learned_steps_include": "proposal inference of all %d views every %d steps (%.2f ms per step "
+                                     "on average) + the batch" % (views_n, lk["infer_every"], infer_ms_per_step),
+            "inference_ms_per_step": round(infer_ms_per_step, 3),
             "batch_kernel_ms": {"plain": km(k_plain), "learned": km(k_learned)},
             "msamples_per_s": {"plain_batch": rate(ms_plain), "learned_batch": rate(ms_learned),
                                "learned_graph": rate(ms_graph)}}
@@ -319,9 +337,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view measurement")
     ap.add_argument("--no-learned-batch", action="store_true",
                     help="skip the learned batch at the large end (16 x 1080p views, ~4M triangles, 16384 lights)")
-    ap.add_argument("--large-convergence", action="store_true",
-                    help="also run learned vs plain time to relMSE 0.01 on one view at the large end "
-                         "(16384 lights, ~4M triangles; about 3.5 minutes)")
+    ap.add_argument("--no-large-convergence", action="store_true",
+                    help="skip learned vs plain time to relMSE 0.01 on one view at the large end "
+                         "(16384 lights, ~4M triangles; about 3 minutes)")
     ap.add_argument("--batch-views", type=int, default=16,
                     help="views in the batched measurement (the batched configuration has 64)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
@@ -431,7 +449,7 @@ def main():
         print("learned batch: %s" % learned_batch, file=sys.stderr, flush=True)
 
     large = None
-    if args.large_convergence and args.workload == "city" and not args.exact:
+    if not args.no_large_convergence and args.workload == "city" and not args.exact:
         large = bench_large_convergence(rgba, rg)
 
     learned = None
@@ -449,8 +467,10 @@ def main():
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import orc
-        threads = min(16, os.cpu_count() or 1)
-        oracle = orc.Oracle(scene, rgba, rg)
+        # the host's CPU share: OMP_NUM_THREADS where the host sets it (a GPU
+        # box gives one GPU 16 threads), else every logical CPU
+        threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
+        oracle = orc.Oracle(scene, rgba, rg, timing=True)
         if 0 < args.cpu_pixels < P:
             rng = np.random.default_rng(0)
             pix = np.sort(rng.choice(P, size=args.cpu_pixels, replace=False)).astype(np.uint32)
@@ -462,11 +482,36 @@ def main():
         t1 = time.perf_counter()
         oracle.shade_frame(consts, scene.settings(), vis_h, pixels=pix, threads=threads)
         cpu_s = time.perf_counter() - t1
+        model = None
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
         cpu = {"value": round(count * spp / cpu_s / 1e6, 4), "unit": "Msamples/s", "cores": threads,
                "kind": "port",
-               "sample": "%s, %d spp, M=%d, shadow rays on the oracle's SAH tree; FP32 oracle, OpenMP, "
-                         "%.1f s" % (what, spp, scene.candidates, cpu_s)}
+               "sample": "%s, %d spp, M=%d, shadow rays on the oracle's SAH tree; FP32 oracle built -O3 "
+                         "-march=x86-64-v3 -ffp-contract=off, OpenMP over pixels, %.1f s"
+                         % (what, spp, scene.candidates, cpu_s),
+               "host_logical_cpus": os.cpu_count(), "omp_max_threads": int(oracle.L.orc_omp_max_threads()),
+               "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
 
+    # The HBM roofline of the dominant kernel on algorithmic bytes, as the
+    # record asks; beside it what the counters say limits the kernel: its
+    # VALU issue rate against the CU peak (the binding resource) and its
+    # measured HBM bytes per launch over its time.
+    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                "kernel": "shade_kernel", "algorithmic_bytes": alg}
+    if traffic:
+        measured = traffic / (shade_ms * 1e-3) / 1e9
+        roofline["measured_traffic_gbs"] = round(measured, 2)
+        roofline["measured_traffic_frac"] = round(measured / HBM_PEAK_GBS, 4)
+    if valu:
+        roofline["limiter"] = "valu" if valu["frac"] >= roofline.get("measured_traffic_frac", 0.0) else "hbm"
+        roofline["limiter_frac"] = max(valu["frac"], roofline.get("measured_traffic_frac", 0.0))
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -486,9 +531,7 @@ def main():
                    "samples_per_step": samples, "exact_math": bool(args.exact), "tuning": tuning,
                    "kernel_ms": {"shade": round(kernel_ms[0], 4), "trace": round(kernel_ms[1], 4),
                                  "resolve": round(kernel_ms[2], 4)}},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
-                     "kernel": "shade_kernel", "algorithmic_bytes": alg},
+        "roofline": roofline,
         "valu": valu,
         "batched_views": batch,
         "learned_batch": learned_batch,

@@ -46,6 +46,7 @@ uint32_t orc_noise_seed(uint32_t px, uint32_t py, uint32_t width, uint32_t rando
 float orc_noise_next(uint32_t* seed);
 /* wang_random_number (src/math_utilities.h:50-57) */
 uint32_t orc_wang_random_number(uint32_t seed);
+int orc_omp_max_threads(void);
 void orc_set_noise_constants(uint32_t random_numbers[4], uint32_t* random_seed, int animate);
 
 /* ---------------- project FP32 transcendentals ------------------------- */
@@ -229,6 +230,21 @@ float orc_ltc_fetch_f32(const orc_scene_t* scene, int table, int channel, float 
                         float w);
 double orc_ltc_fetch_f64(const orc_scene_t* scene, int table, int channel, double u, double v,
                          double w);
+/* Mixed precision (the integrator apart from the float shading point):
+ * orc_pixel_setup writes the per-pixel setup of get_shading_data and
+ * get_ltc_coefficients for pixel index `pixel` whose visible triangle is
+ * `prim` (ORC_PIXEL_SETUP_FLOATS floats); orc_ltc_cp_target evaluates the
+ * LTC_CP RIS target |eval_ltc_cp| of one light from a setup in the build's
+ * arithmetic, clip_counts (may be NULL) = vcd | vcs << 8. The FP32 pair
+ * reproduces the FP32 path's candidate targets bit for bit. */
+#define ORC_PIXEL_SETUP_FLOATS 55
+int orc_pixel_setup_f32(const orc_scene_t* scene, const risltc_frame_constants_t* c, uint32_t pixel, uint32_t prim,
+                        float* out);
+int orc_pixel_setup_f64(const orc_scene_t* scene, const risltc_frame_constants_t* c, uint32_t pixel, uint32_t prim,
+                        float* out);
+float orc_ltc_cp_target_f32(const orc_scene_t* scene, const float* setup, uint32_t light, uint32_t* clip_counts);
+double orc_ltc_cp_target_f64(const orc_scene_t* scene, const float* setup, uint32_t light, uint32_t* clip_counts);
+
 /* decode_position_64_bit / decode_normal_32_bit (mesh_quantization.glsl:19-45).
  * Pinned by round trips through the exporter's packing and octahedral
  * encoding (tools/io_export_vulkan_blender28.py:24-46, 486-512): position =

@@ -18,12 +18,15 @@ from risltc_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc.so")
+# the same sources built -O3 for the CPU-baseline timing (identical results)
+LIB_TIMING_PATH = os.path.join(HERE, "liborc_timing.so")
 REF_LIB_PATH = os.path.join(HERE, "_ref", "libref_host.so")
 # Visibility-cache slots per cell when the cache keys by light cluster: one
 # per cluster up to the largest cluster count the library accepts.
 CACHE_CLUSTER_SLOTS = 1024
 
 _lib = None
+_lib_timing = None
 _ref = None
 
 f32p = C.POINTER(C.c_float)
@@ -61,18 +64,32 @@ class OrcScene(C.Structure):
                 ("cache_per_light", C.c_uint32), ("proposal_table_bytes", C.c_uint32)]
 
 
-def lib():
-    global _lib
+def lib(timing=False):
+    """The oracle library; timing=True gives its -O3 build (the CPU
+    baseline), which computes the same results."""
+    global _lib, _lib_timing
+    if timing:
+        if _lib_timing is None:
+            if not os.path.exists(LIB_TIMING_PATH):
+                build()
+            _lib_timing = _bind(C.CDLL(LIB_TIMING_PATH))
+        return _lib_timing
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
+        _lib = _bind(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def _bind(L):
+    if True:
         sig = {
             "orc_murmur_mix": (C.c_uint32, [C.c_uint32, C.c_uint32]),
             "orc_murmur_finalize": (C.c_uint32, [C.c_uint32]),
             "orc_noise_seed": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
             "orc_noise_next": (C.c_float, [u32p]),
             "orc_wang_random_number": (C.c_uint32, [C.c_uint32]),
+            "orc_omp_max_threads": (C.c_int, []),
             "prj_atanf": (C.c_float, [C.c_float]),
             "prj_asinf": (C.c_float, [C.c_float]),
             "prj_acosf": (C.c_float, [C.c_float]),
@@ -125,6 +142,10 @@ def lib():
                                                C.c_double]),
             "orc_decode_position_f32": (None, [C.c_uint32, C.c_uint32, f32p, f32p, f32p]),
             "orc_decode_position_f64": (None, [C.c_uint32, C.c_uint32, f32p, f32p, f64p]),
+            "orc_pixel_setup_f32": (C.c_int, [C.POINTER(OrcScene), C.POINTER(abi.FrameConstants), C.c_uint32,
+                                              C.c_uint32, f32p]),
+            "orc_ltc_cp_target_f32": (C.c_float, [C.POINTER(OrcScene), f32p, C.c_uint32, u32p]),
+            "orc_ltc_cp_target_f64": (C.c_double, [C.POINTER(OrcScene), f32p, C.c_uint32, u32p]),
             "orc_decode_normal_f32": (None, [C.c_uint32, C.c_uint32, f32p]),
             "orc_decode_normal_f64": (None, [C.c_uint32, C.c_uint32, f64p]),
         }
@@ -132,8 +153,7 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        _lib = L
-    return _lib
+    return L
 
 
 def ref_lib():
@@ -180,8 +200,8 @@ def cache_bytes(counts, prior_strength):
 class Oracle:
     """The oracle bound to one synthetic scene (risltc_amd.synth.Scene)."""
 
-    def __init__(self, scene, ltc_rgba16, ltc_rg16, bvh=True, builder=abi.BVH_SAH):
-        self.L = lib()
+    def __init__(self, scene, ltc_rgba16, ltc_rg16, bvh=True, builder=abi.BVH_SAH, timing=False):
+        self.L = lib(timing)
         self.scene = scene
         m = scene.mesh
         self.positions = np.ascontiguousarray(m.positions, dtype=np.uint32)
@@ -329,6 +349,32 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle shade failed: %s" % abi.ERROR_NAMES.get(rc, rc))
         return out, rec, cand
+
+    def ltc_cp_targets(self, constants, visibility, pixels, lights):
+        """RIS targets of the listed (pixel, light) pairs evaluated in FP32
+        and in FP64 from the pixels' FP32 setup, lights shaped (pixels, ...):
+        (t32, t64, clip counts FP32, clip counts FP64)."""
+        pix = np.asarray(pixels, np.int64)
+        li = np.asarray(lights, np.int64)
+        vis = np.asarray(visibility, np.uint32)
+        t32 = np.zeros((pix.size, li.size // max(pix.size, 1)), np.float32)
+        t64 = np.zeros(t32.shape, np.float64)
+        cc32 = np.zeros(t32.shape, np.uint32)
+        cc64 = np.zeros(t32.shape, np.uint32)
+        setup = np.zeros(55, np.float32)
+        one = np.zeros(1, np.uint32)
+        flat_l = li.reshape(t32.shape)
+        sp = ptr(setup, f32p)
+        for i, p in enumerate(pix):
+            rc = self.L.orc_pixel_setup_f32(C.byref(self.cscene), C.byref(constants), int(p), int(vis[p]), sp)
+            if rc != 0:
+                raise RuntimeError("pixel %d has no shading point" % p)
+            for k, l in enumerate(flat_l[i]):
+                t32[i, k] = self.L.orc_ltc_cp_target_f32(C.byref(self.cscene), sp, int(l), ptr(one, u32p))
+                cc32[i, k] = one[0]
+                t64[i, k] = self.L.orc_ltc_cp_target_f64(C.byref(self.cscene), sp, int(l), ptr(one, u32p))
+                cc64[i, k] = one[0]
+        return t32.reshape(li.shape), t64.reshape(li.shape), cc32.reshape(li.shape), cc64.reshape(li.shape)
 
     def shade_frame(self, constants, settings, visibility, pixels=None, threads=8):
         vis = np.ascontiguousarray(visibility, dtype=np.uint32)

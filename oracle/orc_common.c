@@ -17,6 +17,9 @@
  * every a*b+c is two roundings unless fmaf is written.
  */
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -57,6 +60,15 @@ uint32_t orc_noise_seed(uint32_t px, uint32_t py, uint32_t width, uint32_t rando
 float orc_noise_next(uint32_t* seed) {
     *seed = 1664525u * *seed + 1013904223u;
     return (float)(*seed) * (1.0f / 4294967296.0f);
+}
+
+/* threads an OpenMP region of the oracle gets (CPU-baseline reporting) */
+int orc_omp_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* wang_random_number (src/math_utilities.h:50-57) */

@@ -1411,6 +1411,72 @@ void SFX(orc_evaluate_brdf)(const R* sh, const R* dir, R* out) {
     out[2] = b.z;
 }
 
+/* ---- mixed precision: an FP64 target from the FP32 pixel setup ---------------
+ * orc_pixel_setup writes the setup that get_shading_data and
+ * get_ltc_coefficients give the pixel (ORC_PIXEL_SETUP_FLOATS floats: position
+ * 3, world_to_shading 12, w2c 12, w2s_flip 12, w2c_flip 12, diffuse albedo 3,
+ * LTC albedo 1); orc_ltc_cp_target evaluates the LTC_CP target
+ * |eval_ltc_cp| of one light from such a setup in the build's arithmetic. The
+ * FP64 build on the FP32 setup measures the integrator (transform, clip, edge
+ * integral) apart from the float rounding of the shading point itself. */
+static void pack34(const A34* a, float* out) {
+    for (int i = 0; i != 3; ++i)
+        for (int k = 0; k != 4; ++k) out[4 * i + k] = (float)a->m[i][k];
+}
+static void unpack34(const float* in, A34* a) {
+    for (int i = 0; i != 3; ++i)
+        for (int k = 0; k != 4; ++k) a->m[i][k] = (R)in[4 * i + k];
+}
+
+int SFX(orc_pixel_setup)(const orc_scene_t* sc, const risltc_frame_constants_t* c, uint32_t pixel, uint32_t prim,
+                         float* out) {
+    if (!sc || !c || !out || prim == 0xFFFFFFFFu || (prim >> 31) != 0u || (uint64_t)prim >= sc->triangle_count)
+        return RISLTC_E_ARG;
+    const uint32_t W = c->viewport_size[0];
+    pix_t px;
+    const float(*P)[4] = c->pixel_to_ray_direction_world_space;
+    R fx = (R)(pixel % W), fy = (R)(pixel / W);
+    V3 ray = v3(FMA((R)P[0][2], K(1.0), FMA((R)P[0][1], fy, (R)P[0][0] * fx)),
+                FMA((R)P[1][2], K(1.0), FMA((R)P[1][1], fy, (R)P[1][0] * fx)),
+                FMA((R)P[2][2], K(1.0), FMA((R)P[2][1], fy, (R)P[2][0] * fx)));
+    get_shading_data(sc, c, prim, ray, &px.sd);
+    R f0l = dot3(px.sd.fresnel_0, v3(K(0.2126), K(0.7152), K(0.0722)));
+    get_ltc_coefficients(sc, &c->ltc_constants, f0l, px.sd.roughness, px.sd.position, px.sd.normal, px.sd.outgoing,
+                         &px.ltc);
+    px.w2c = mul33_34(&px.ltc.shading_to_cosine, &px.ltc.world_to_shading);
+    px.w2s_flip = px.ltc.world_to_shading;
+    for (int k = 0; k != 4; ++k) px.w2s_flip.m[1][k] = -px.w2s_flip.m[1][k];
+    px.w2c_flip = mul33_34(&px.ltc.shading_to_cosine, &px.w2s_flip);
+    out[0] = (float)px.sd.position.x;
+    out[1] = (float)px.sd.position.y;
+    out[2] = (float)px.sd.position.z;
+    pack34(&px.ltc.world_to_shading, out + 3);
+    pack34(&px.w2c, out + 15);
+    pack34(&px.w2s_flip, out + 27);
+    pack34(&px.w2c_flip, out + 39);
+    out[51] = (float)px.sd.diffuse_albedo.x;
+    out[52] = (float)px.sd.diffuse_albedo.y;
+    out[53] = (float)px.sd.diffuse_albedo.z;
+    out[54] = (float)px.ltc.albedo;
+    return RISLTC_OK;
+}
+
+R SFX(orc_ltc_cp_target)(const orc_scene_t* sc, const float* setup, uint32_t light, uint32_t* clip_counts) {
+    pix_t px;
+    memset(&px, 0, sizeof(px));
+    px.sd.position = v3((R)setup[0], (R)setup[1], (R)setup[2]);
+    unpack34(setup + 3, &px.ltc.world_to_shading);
+    unpack34(setup + 15, &px.w2c);
+    unpack34(setup + 27, &px.w2s_flip);
+    unpack34(setup + 39, &px.w2c_flip);
+    px.sd.diffuse_albedo = v3((R)setup[51], (R)setup[52], (R)setup[53]);
+    px.ltc.albedo = (R)setup[54];
+    light_t l;
+    load_light(sc, light, &l);
+    V3 col = eval_ltc_cp(&px, &l, clip_counts);
+    return SQRT(dot3(col, col));
+}
+
 /* mesh decode (mesh_quantization.glsl:19-45): out = xyz */
 void SFX(orc_decode_position)(uint32_t q0, uint32_t q1, const float f[3], const float s[3], R* out) {
     V3 p = decode_position(q0, q1, f, s);

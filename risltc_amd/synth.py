@@ -374,6 +374,21 @@ def _finish(name, tris, nrm, uvs, mats, materials, lights, camera, width, height
     return Scene(name, mesh, materials, lights, camera, width, height, spp, candidates, seed)
 
 
+def batch_view_scenes(count, seed=7):
+    """Scenes of the batched configuration, one per view: each its own
+    procedural city of ~50k-500k triangles (grid 60-190) and its own light
+    set of 16-8192 polygons (3-8 vertices, count log-uniform), resolution
+    uniform in 320x240 .. 1920x1080, M from {8, 16, 32}, 2 spp. Yields one
+    Scene per view (its width, height and settings are the view's)."""
+    rng = np.random.default_rng(seed)
+    for v in range(count):
+        grid = int(rng.integers(60, 191))
+        lights = int(round(math.exp(rng.uniform(math.log(16), math.log(8192)))))
+        w, h = int(rng.integers(320, 1921)), int(rng.integers(240, 1081))
+        m = int(rng.choice([8, 16, 32]))
+        yield city_scene(seed=100 + v, width=w, height=h, light_count=lights, grid=grid, spp=2, candidates=m)
+
+
 WORKLOADS = {
     "small": dict(fn=small_scene, desc="256x256, 16 boxes + ground (~200 tris), 64 quad lights, M=8, 1 spp"),
     "city": dict(fn=city_scene, desc="1920x1080, ~1M-triangle city, 1024 3-8-gon lights, M=32, 4 spp"),

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import orc
+import parity_rules as PR
 from risltc_amd import abi, synth
 
 pytestmark = pytest.mark.gpu
@@ -61,6 +62,25 @@ def test_city_exact_subset_bitwise(city):
     assert bad.size == 0, (bad.size, city["pix"][bad[:5]])
     # most city pixels are shadowed by buildings, a fifth or so are lit
     assert (got[:, :3].sum(1) > 0).mean() > 0.15
+
+
+def test_city_fast_build_rules(city):
+    """The production build on the 4096-pixel subset of the 1080p frame under
+    the rules of test_gpu_parity.fast_build_rules: noise streams split only
+    at specular PSAs within 1e-6 of zero, clipped vertex counts of every
+    candidate bitwise equal to the FP32 oracle's, RIS choices equal to the
+    FP32 oracle's except where a decision flips between the two builds'
+    normalised weights (at most 0.1% of samples) and differing from the FP64
+    oracle's no more often than IEEE FP32's, LTC targets within 1e-5 of the
+    FP64 integrator for lights above the horizon subtending >= 0.03 sr."""
+    from test_gpu_parity import fast_build_rules
+    s = city["scene"].settings(exact_math=0)
+    g = PR.gpu_debug_subset(city["ctx"], city["consts"], s, city["vis"], city["pix"])
+    r = fast_build_rules(city["oracle"], city["consts"], city["scene"], city["vis_h"], city["pix"], g["out"],
+                         g["rec"], g["cand"], s)
+    print({k: v for k, v in r.items() if k in ("synced", "split_causes", "ltc_checked", "ltc_max", "fp64_mismatches",
+                                               "fp32_oracle_fp64_mismatches", "fp32_oracle_mismatches")})
+    assert r["synced"] > 0.99
 
 
 def test_city_fast_vs_fp64(city):

@@ -1,0 +1,165 @@
+"""The batched, learned and large-end configurations on their own workloads,
+against the oracle.
+
+  * batched: the 64 views of the batched configuration (each its own city
+    mesh of ~50k-500k triangles and its own light set of 16-8192 polygons,
+    320x240 .. 1920x1080, M in {8, 16, 32}, 2 spp) in one risltc_shade_batch:
+    every view bitwise equal to its own risltc_shade call (production
+    build), and in the exact build every view bitwise equal to the FP32
+    oracle on 256 random pixels of the view;
+  * learned: the 1920x1080 city with 1024 lights k-means clustered into 256
+    clusters, the 96-64-64-64-256 network trained online for a few frames,
+    a visibility cache: the logits equal a numpy forward pass on the city's
+    own features, and the exact learned frame equals the oracle given the
+    device's tables, clusters and cache on 4096 random pixels;
+  * large end: one 1920x1080 view of the ~4.1M-triangle city with 16384
+    lights in 1024 clusters, the 96-128-128-128-1024 network, 16 spp, with
+    the same two checks.
+"""
+import numpy as np
+import pytest
+
+import orc
+from risltc_amd import abi, ltcfit, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _subset(vis_h, count, seed):
+    rng = np.random.default_rng(seed)
+    shaded = np.flatnonzero((vis_h >> 31) == 0)
+    return np.sort(rng.choice(shaded, min(count, shaded.size), replace=False)).astype(np.uint32)
+
+
+def test_batched_configuration_64_views(ltc_table):
+    import torch
+    import risltc_amd
+    ctx = risltc_amd.Context(0)
+    try:
+        views_fast, views_exact, refs, keep = [], [], [], []
+        scenes = []
+        for sc in synth.batch_view_scenes(64):
+            w, h = sc.width, sc.height
+            if not scenes:  # the context's own scene, lights and LTC table
+                ctx.load_scene(sc, *ltc_table)
+            sid = ctx.add_scene(sc)
+            lid = ctx.add_light_set(sc.light_records, sc.light_count, sc.light_vmax)
+            c = risltc_amd.write_frame_constants(sc.camera, w, h, sc.mesh.dequantization_factor,
+                                                 sc.mesh.dequantization_summand, ctx.ltc_constants)
+            vis = torch.empty(w * h, dtype=torch.int32, device="cuda")
+            ctx.render_visibility_scene(sid, c, vis)
+            fast = torch.full((w * h * 4,), float("nan"), dtype=torch.float32, device="cuda")
+            exact = torch.full((w * h * 4,), float("nan"), dtype=torch.float32, device="cuda")
+            views_fast.append(ctx.view(c, sc.settings(), vis, fast, light_set=lid, scene=sid))
+            views_exact.append(ctx.view(c, sc.settings(exact_math=1), vis, exact, light_set=lid, scene=sid))
+            scenes.append((sc, c, vis, fast, exact))
+        print("64 views set up", flush=True)
+        ctx.shade_batch(views_fast)
+        torch.cuda.synchronize()
+        fast_bits = [f.cpu().numpy().view(np.uint32) for _, _, _, f, _ in scenes]
+        # one risltc_shade_batch call per view = that view shaded alone
+        for i, v in enumerate(views_fast):
+            ctx.shade_batch([v])
+            torch.cuda.synchronize()
+            assert np.array_equal(scenes[i][3].cpu().numpy().view(np.uint32), fast_bits[i]), i
+        ctx.shade_batch(views_exact)
+        torch.cuda.synchronize()
+        print("batches done", flush=True)
+        lit = 0
+        for i, (sc, c, vis, _, exact) in enumerate(scenes):
+            vis_h = vis.cpu().numpy().view(np.uint32)
+            pix = _subset(vis_h, 256, 1000 + i)
+            oracle = orc.Oracle(sc, *ltc_table)
+            ref, _, _ = oracle.shade(c, sc.settings(exact_math=1), vis_h, pixels=pix)
+            got = exact.cpu().numpy().reshape(-1, 4)[pix]
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (i, sc.width, sc.height)
+            lit += int((got[:, :3].sum(1) > 0).sum())
+            del oracle
+        assert lit > 0.1 * 64 * 256
+    finally:
+        ctx.close()
+
+
+def _learned_checks(sc, ltc_table, clusters, width, outputs, per_light, M, train_frames, seed):
+    """Trains the proposal online for a few frames of the scene's view, then
+    checks the logits against numpy on the view's own features and the exact
+    learned frame against the oracle on 4096 random pixels."""
+    import torch
+    import risltc_amd
+    from test_gpu_learned import forward
+    ctx = risltc_amd.Context(0)
+    try:
+        ctx.load_scene(sc, *ltc_table)
+        col = ctx.cluster_lights(clusters, 20)
+        ctx.proposal_init(seed, width=width, outputs=outputs)
+        log2, cell, prior = 12, 1.0, 1.0
+        ctx.proposal_cache(log2, cell, prior, per_light)
+        P = sc.width * sc.height
+        vis = torch.empty(P, dtype=torch.int32, device="cuda")
+        ctx.render_visibility(ctx.frame_constants(), vis)
+        out = torch.empty(P * 4, dtype=torch.float32, device="cuda")
+        learned = dict(proposal=abi.PROPOSAL_LEARNED, ris_candidate_count=M, learned_floor_log2=7)
+        for f in range(train_frames):
+            c = ctx.frame_constants(noise_seed=f, animate=True)
+            ctx.proposal_update(c, vis)
+            ctx.shade(c, sc.settings(**learned), vis, out)
+            ctx.proposal_train(c, vis, batch=8192, steps=1, lr=0.02, seed=f)
+        torch.cuda.synchronize()
+        print("trained %d frames" % train_frames, flush=True)
+        c = ctx.frame_constants(noise_seed=train_frames + 3, animate=True)
+        vis_h = vis.cpu().numpy().view(np.uint32)
+        pix = _subset(vis_h, 4096, 4096)
+        tb = abi.proposal_table_bytes(outputs)
+        table = torch.zeros(P * tb, dtype=torch.uint8, device="cuda")
+        logits = torch.zeros(P * outputs, dtype=torch.float32, device="cuda")
+        feats = torch.zeros(P * 96, dtype=torch.int16, device="cuda")
+        ctx.proposal_update(c, vis, table, logits, feats)
+        torch.cuda.synchronize()
+        idx = torch.from_numpy(pix.astype(np.int64)).cuda()
+        z = logits.view(P, outputs)[idx].cpu().numpy().astype(np.float64)
+        x = (feats.view(P, 96)[idx].cpu().numpy().view(np.uint16).astype(np.uint32) << 16).view(np.float32)
+        table_h = table.cpu().numpy().reshape(P, tb)
+        del logits, feats
+        torch.cuda.empty_cache()
+        params, _ = ctx.proposal_get_params()
+        _, _, _, Z = forward(params, x)
+        # FP32 summation order, and the bf16 rounding of a hidden activation
+        # that this order moves across a tie, which a trained network's larger
+        # output weights carry further than the fresh network's
+        err = np.abs(z - Z) / np.maximum(np.abs(Z), 1.0)
+        assert err.max() < 5e-3 and np.mean(err < 1e-5) > 0.99, (err.max(), np.mean(err < 1e-5))
+        # the exact learned frame; the cache bytes it reads are made from the
+        # counts of the training frames
+        counts = ctx.proposal_cache_read()
+        s = sc.settings(exact_math=1, **learned)
+        ctx.shade(c, s, vis, out)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy().reshape(P, 4)[pix]
+        bytes_dev = ctx.proposal_cache_table()
+        print("device frame done", flush=True)
+    finally:
+        ctx.close()
+    oracle = orc.Oracle(sc, *ltc_table)
+    oracle.set_proposal(table_h, col, clusters)
+    want = oracle.set_cache(counts, cell, prior, per_light)
+    assert np.array_equal(bytes_dev, want)
+    assert (want != 255).mean() > 0.001  # the training frames filled cache entries
+    ref, _, _ = oracle.shade(c, s, vis_h, pixels=pix)
+    bad = np.flatnonzero(np.any(got.view(np.uint32) != ref.view(np.uint32), axis=1))
+    assert bad.size == 0, (bad.size, pix[bad[:5]])
+    assert (got[:, :3].sum(1) > 0).mean() > 0.1
+    ids_w = table_h[pix, 32 if outputs > 256 else 16:]
+    assert (ids_w.sum(1) > 0).mean() > 0.9  # the tables are in use
+
+
+def test_learned_configuration_city_256_clusters(ltc_table):
+    sc = synth.city_scene()
+    _learned_checks(sc, ltc_table, clusters=256, width=64, outputs=256, per_light=True, M=24, train_frames=4,
+                    seed=1)
+
+
+def test_large_end_configuration_1024_clusters(ltc_table):
+    sc = synth.city_scene(seed=400, light_count=16384, grid=540, spp=16, candidates=32)
+    assert sc.mesh.triangle_count > 4_000_000 and sc.spp == 16
+    _learned_checks(sc, ltc_table, clusters=1024, width=128, outputs=1024, per_light=False, M=24,
+                    train_frames=2, seed=1)

@@ -136,6 +136,112 @@ def _learned_context(small_scene, ltc_table, width=64, outputs=256, K=16):
                 feats=(feats.cpu().numpy().view(np.uint16).astype(np.uint32) << 16).view(np.float32).reshape(P, 96))
 
 
+def numpy_features(scene, consts, vis_h, pix):
+    """The 96 inputs of the MLP per pixel, restated in FP64 numpy from the
+    G-buffer inputs (the ray/triangle barycentrics of get_shading_data,
+    shading_pass.frag.glsl:571-672, interpolated position and normal, view
+    and reflected directions) and the encoding of the proposal kernel:
+    position in the scene box q (3), sin/cos(pi 2^j q) for j = 0..6 (42),
+    normal, view and reflected directions (9), GGX alpha, n.o, diffuse and
+    Fresnel luminances (4), sin/cos(pi 2^j x) of normal and reflection for
+    j = 0, 1 (24), sin/cos(pi o) (6), a constant 1 and seven zeros."""
+    L = orc.lib()
+    pix = np.asarray(pix, np.int64)
+    W = int(consts.viewport_size[0])
+    M = np.array(consts.pixel_to_ray_direction_world_space, np.float64)[:, :3]
+    fx, fy = (pix % W).astype(np.float64), (pix // W).astype(np.float64)
+    ray = M[:, 0][None] * fx[:, None] + M[:, 1][None] * fy[:, None] + M[:, 2][None]
+    prim = vis_h[pix].astype(np.int64)
+    tri = scene.mesh.dequantized().reshape(-1, 3, 3).astype(np.float64)[prim]
+    nuv = scene.mesh.normals_uv.reshape(-1, 3, 4)[prim]
+    nrm = np.zeros((pix.size, 3, 3))
+    for i in range(pix.size):
+        for v in range(3):
+            L.orc_decode_normal_f64(int(nuv[i, v, 0]), int(nuv[i, v, 1]),
+                                    nrm[i, v].ctypes.data_as(orc.f64p))
+    org = np.array(consts.camera_position_world_space, np.float64)
+    e0, e1 = tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]
+    rce1 = np.cross(ray, e1)
+    rcp_det = 1.0 / (e0 * rce1).sum(1)
+    r0 = org - tri[:, 0]
+    by = rcp_det * (r0 * rce1).sum(1)
+    bz = -rcp_det * (ray * np.cross(e0, r0)).sum(1)
+    bx = 1.0 - (by + bz)
+    bc = np.stack([bx, by, bz], 1)[..., None]
+    pos = (tri * bc).sum(1)
+    n = (nrm * bc).sum(1)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    o = org - pos
+    o /= np.linalg.norm(o, axis=1, keepdims=True)
+    ndo = (n * o).sum(1)
+    n = np.where(ndo[:, None] < 0, -n, n)
+    ndo = np.abs(ndo)
+    r = 2.0 * ndo[:, None] * n - o
+    mats = np.asarray(scene.materials, np.float64)[scene.mesh.material_index[prim].astype(np.int64)]
+    metal = mats[:, 5]
+    lum = 0.2126 * mats[:, 0] + 0.7152 * mats[:, 1] + 0.0722 * mats[:, 2]
+    alum = lum * (1.0 - metal)
+    f0l = 0.02 * (1.0 - metal) + lum * metal
+    alpha = np.clip(mats[:, 4] ** 2 * float(consts.roughness_factor), 0.0064, 1.0)
+    # the scene box of the quantisation grid, in float as the library forms it
+    f = np.asarray(scene.mesh.dequantization_factor, np.float32)
+    s_ = np.asarray(scene.mesh.dequantization_summand, np.float32)
+    lo, hi = s_, s_ + f * np.float32(2097151.0)
+    center = (np.float32(0.5) * (hi + lo)).astype(np.float64)
+    inv_half = (np.float32(1.0) / (np.float32(0.5) * (hi - lo))).astype(np.float64)
+    q = (pos - center) * inv_half
+    cols, scales = [q[:, a] for a in range(3)], [0.0] * 3
+    for j in range(7):
+        for a in range(3):
+            cols += [np.sin(np.pi * 2 ** j * q[:, a]), np.cos(np.pi * 2 ** j * q[:, a])]
+            scales += [0.5 * 2 ** j] * 2
+    for v in (n, o, r):
+        cols += [v[:, a] for a in range(3)]
+        scales += [0.0] * 3
+    cols += [alpha, ndo, alum, f0l]
+    scales += [0.0] * 4
+    for j in range(2):
+        for a in range(3):
+            cols += [np.sin(np.pi * 2 ** j * n[:, a]), np.cos(np.pi * 2 ** j * n[:, a]),
+                     np.sin(np.pi * 2 ** j * r[:, a]), np.cos(np.pi * 2 ** j * r[:, a])]
+            scales += [0.5 * 2 ** j] * 4
+    for a in range(3):
+        cols += [np.sin(np.pi * o[:, a]), np.cos(np.pi * o[:, a])]
+        scales += [0.5] * 2
+    cols.append(np.ones(pix.size))
+    scales.append(0.0)
+    X = np.stack(cols, 1)
+    X = np.concatenate([X, np.zeros((pix.size, 96 - X.shape[1]))], 1)
+    return X, np.array(scales + [0.0] * (96 - len(scales)))
+
+
+def test_features_match_numpy_restatement(learned):
+    """The device's features (bf16) equal the FP64 numpy restatement rounded
+    to bf16, to one bf16 step (both sides round a value that sits near a
+    rounding boundary) plus the sin/cos amplification of FP32 position
+    rounding; almost all are equal."""
+    g = learned
+    vis = g["vis"].cpu().numpy().view(np.uint32)
+    shaded = np.flatnonzero((vis >> 31) == 0)
+    pix = np.random.default_rng(3).choice(shaded, 2000, replace=False)
+    X, scale = numpy_features(g["scene"], g["consts"], vis, pix)
+    assert X.shape == (pix.size, 96) and np.all(X[:, 89:] == 0)
+    got = g["feats"][pix].astype(np.float64)
+    want = bf16(X.astype(np.float32)).astype(np.float64)
+    mag = np.maximum(np.abs(got), np.abs(want))
+    ulp = np.where(mag > 0, 2.0 ** (np.floor(np.log2(np.maximum(mag, 1e-30))) - 7), 0.0)
+    tol = ulp + 2 * np.pi * scale[None] * 2e-6 + 1e-6
+    d = np.abs(got - want)
+    assert np.all(d <= tol), (np.unravel_index(np.argmax(d - tol), d.shape), (d - tol).max())
+    # columns without sin/cos amplification are nearly all equal where the
+    # value is not a rounding residue near 0 (e.g. the x, y of a flat
+    # ground's normal: ~1e-10 in FP32, ~1e-17 in FP64)
+    big = mag > 1e-6
+    flat = (scale == 0)[None] & big
+    assert (d[flat] == 0).mean() > 0.97 and (d[big] == 0).mean() > 0.85, ((d[flat] == 0).mean(),
+                                                                          (d[big] == 0).mean())
+
+
 def table_slots(table, outputs):
     """(ids, weights) of per-pixel tables: pixels x 16 each."""
     if outputs > 256:

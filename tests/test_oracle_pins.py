@@ -660,3 +660,20 @@ def test_pixel_rays_match_camera_trigonometry(small_oracle):
         # the reference inverts the projection matrix in FP32 (condition ~ far/near
         # = 2e4), so the directions carry ~1e-4 of rounding error
         assert np.allclose(d / np.linalg.norm(d), expect / np.linalg.norm(expect), atol=1e-3)
+
+
+def test_timing_build_matches_checker_build(small_scene, ltc_table):
+    """The -O3 build timed as the CPU baseline computes the same bits as the
+    checker build (both IEEE, no contraction)."""
+    a = orc.Oracle(small_scene, *ltc_table)
+    b = orc.Oracle(small_scene, *ltc_table, timing=True)
+    c = a.frame_constants()
+    vis = a.render_visibility(c)
+    s = small_scene.settings(sample_count_light=2)
+    pix = np.flatnonzero((vis >> 31) == 0)[::7].astype(np.uint32)
+    ra, _, _ = a.shade(c, s, vis, pixels=pix)
+    rb, _, _ = b.shade(c, s, vis, pixels=pix)
+    assert np.array_equal(ra.view(np.uint32), rb.view(np.uint32))
+    # the OpenMP frame entry point of the timing build: the same bits again
+    fb = b.shade_frame(c, s, vis, pixels=pix, threads=4)
+    assert np.array_equal(fb.view(np.uint32), ra.view(np.uint32))
