@@ -12,10 +12,12 @@ host's cores) on a bounded sample of the same frame.
 The learned_vs_plain object is the second half of the metric: GPU time for
 the plain pass and for the learned pass (MLP + online visibility cache in
 the RIS target) to reach relMSE 0.01 against a 65536-spp plain reference,
-each pass's Msamples/s and per-kernel milliseconds per frame, three
-ablations, and the plain pass with the reference's two comparison
-techniques (Turk area sampling, projected-solid-angle RIS;
-risltc_amd/convergence.py).
+each pass's Msamples/s and per-kernel milliseconds per frame, ablations,
+and the plain pass with the reference's two comparison techniques (Turk
+area sampling, projected-solid-angle RIS; risltc_amd/convergence.py). The
+learned passes first train over LEARN_PATH_FRAMES frames of a seeded camera
+path (GPU time reported apart); the view is held out from that path and
+from the runs that chose the learned settings, with held-out noise seeds.
 
 The batched_views object: views of the batched configuration (each its own
 city mesh of ~50k-500k triangles and its own light set of 16-8192 polygons,
@@ -31,8 +33,9 @@ batch of the same views, and as a graph replay.
 The large_learned_vs_plain object (about 3 minutes; --no-large-convergence
 skips it): learned vs plain time to relMSE 0.01 on one 1920x1080 view at
 the large end (16384 lights, ~4M triangles, 16 spp per frame) against a
-65536-spp plain reference, for the 1024-cluster width-128 network and the
-default 256-cluster network.
+65536-spp plain reference, for the 1024-cluster width-128 network (tables
+of quads of clusters) and the default 256-cluster network, each pretrained
+over a camera path, on a held-out view.
 
 The launch shape is autotuned (risltc_autotune) on the frame before timing.
 
@@ -51,6 +54,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Msamples/s @1080p; GPU time to relMSE 0.01, learned vs plain RIS-LTC"
+# frames of the seeded camera path the proposal trains over before the
+# learned-vs-plain runs (1080p city; and at the large end)
+LEARN_PATH_FRAMES = 200
+LARGE_PATH_FRAMES = 100
 HBM_PEAK_GBS = 8000.0
 # wave64 FP32 VALU instructions: 256 CUs x 4 SIMDs, 4 cycles each, 2.4 GHz
 # (non-packed: the build disables packed FP32), in lane-instructions/s
@@ -277,9 +284,12 @@ def bench_large_convergence(ltc_rgba, ltc_rg, frames=128, ref_frames=4096):
     view of the ~4M-triangle city with 16384 lights and 16 spp per frame
     (M = 32 plain, 24 learned), against a 65536-spp plain reference of that
     view: the 96-128-128-128-1024 network over 1024 clusters (the large-end
-    configuration's network) and the default 96-64-64-64-256 network over 256
-    clusters, each with a visibility cache of one slot per cluster. About 3.5
-    minutes of GPU time, most of it the reference."""
+    configuration's network; its tables sample quads of clusters) and the
+    default 96-64-64-64-256 network over 256 clusters, each with a
+    visibility cache of one slot per cluster, each pretrained over
+    LARGE_PATH_FRAMES frames of a seeded camera path. The view is held out
+    from the path. About 3.5 minutes of GPU time, most of it the
+    reference."""
     import torch
     import risltc_amd
     from risltc_amd import synth, convergence as cv
@@ -289,29 +299,39 @@ def bench_large_convergence(ltc_rgba, ltc_rg, frames=128, ref_frames=4096):
     try:
         ctx.load_scene(sc, ltc_rgba, ltc_rg)
         P = sc.width * sc.height
+        cam = cv.held_out_camera(sc)
+        path = cv.camera_path(sc, LARGE_PATH_FRAMES)
         vis = torch.empty(P, dtype=torch.int32, device="cuda")
-        ctx.render_visibility(ctx.frame_constants(), vis)
-        ref = cv.reference(ctx, sc, vis, ref_frames, chunk=512,
+        ctx.render_visibility(ctx.frame_constants(camera=cam), vis)
+        ref = cv.reference(ctx, sc, vis, ref_frames, chunk=512, camera=cam,
                            log=lambda m: print(m, file=sys.stderr, flush=True))
         cps = (4, 8, 16, 32, 64, 128)
 
-        def run(lk):
-            _, _, curve = cv.accumulate(ctx, sc, vis, frames, 0, learned=lk, checkpoints=cps, ref=ref)
+        def run(lk, use_path=True):
+            _, _, curve = cv.accumulate(ctx, sc, vis, frames, 500000, learned=lk, checkpoints=cps, ref=ref,
+                                        camera=cam, path=path if (lk and use_path) else None)
             t, ex = cv.time_to(curve, 0.01)
             km = cv._frames.kernel_ms
-            return {"time_to_target_ms": round(t, 1), "extrapolated": ex,
-                    "curve": {str(k): [round(v[0], 6), round(v[1], 1)] for k, v in sorted(curve.items())},
-                    "ms_per_frame": {"shade": round(km[0], 3), "trace": round(km[1], 3), "resolve": round(km[2], 3),
-                                     "inference": round(km[3], 3), "training": round(km[4], 3)}}
+            r = {"time_to_target_ms": round(t, 1), "extrapolated": ex,
+                 "curve": {str(k): [round(v[0], 6), round(v[1], 1)] for k, v in sorted(curve.items())},
+                 "ms_per_frame": {"shade": round(km[0], 3), "trace": round(km[1], 3), "resolve": round(km[2], 3),
+                                  "inference": round(km[3], 3), "training": round(km[4], 3)}}
+            if lk and use_path:
+                r["pretrain_frames"] = len(path)
+                r["pretrain_ms"] = round(cv.accumulate.pretrain_ms, 1)
+            return r
 
         plain = run(None)
         base = dict(cv.DEFAULT_LEARNED, cache_per_light=False)
-        out = {"view": [sc.width, sc.height], "triangles": int(sc.mesh.triangle_count), "lights": sc.light_count,
+        out = {"view": [sc.width, sc.height], "camera": "held out (convergence.held_out_camera)",
+               "triangles": int(sc.mesh.triangle_count), "lights": sc.light_count,
                "spp_per_frame": sc.spp, "reference_spp": ref_frames * sc.spp, "target_relmse": 0.01,
                "plain": plain}
-        for name, lk in (("learned_1024_clusters_w128", dict(base, clusters=1024, width=128, outputs=1024)),
-                         ("learned_256_clusters_w64", dict(base, clusters=256, width=64, outputs=256))):
-            r = run(lk)
+        k1024 = dict(base, clusters=1024, width=128, outputs=1024)
+        for name, lk, use_path in (("learned_1024_clusters_w128", k1024, True),
+                                   ("learned_256_clusters_w64", dict(base, clusters=256, width=64, outputs=256), True),
+                                   ("learned_1024_clusters_w128_online_only", k1024, False)):
+            r = run(lk, use_path)
             r["over_plain"] = round(r["time_to_target_ms"] / plain["time_to_target_ms"], 4)
             out[name] = r
             print("large end %s: %.1f ms vs plain %.1f ms" % (name, r["time_to_target_ms"],
@@ -456,7 +476,15 @@ def main():
     if not args.no_learned and args.workload == "city" and not args.exact:
         from risltc_amd import convergence
         t1 = time.time()
-        learned = convergence.compare(ctx, scene, vis, log=lambda m: print(m, file=sys.stderr, flush=True))
+        # evaluated on a view held out from the training path and from the
+        # runs that chose the learned settings, with held-out noise seeds
+        cam = convergence.held_out_camera(scene)
+        vis_eval = torch.empty(P, dtype=torch.int32, device="cuda")
+        ctx.render_visibility(ctx.frame_constants(camera=cam), vis_eval)
+        path = convergence.camera_path(scene, LEARN_PATH_FRAMES)
+        learned = convergence.compare(ctx, scene, vis_eval, camera=cam, path=path, seed0=500000,
+                                      log=lambda m: print(m, file=sys.stderr, flush=True))
+        learned["view"] = "held-out camera (convergence.held_out_camera), noise seeds from 500000"
         learned["wall_s"] = round(time.time() - t1, 1)
         # each pass's shade kernel against the same algorithmic bytes
         for k in ("plain", "learned"):

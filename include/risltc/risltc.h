@@ -511,9 +511,10 @@ int risltc_shade_debug(risltc_context_t* context, const risltc_frame_constants_t
  * risltc_shade (learned), then optionally risltc_proposal_train on the
  * frame's own samples. ------------------------------------------------- */
 #define RISLTC_MAX_LIGHT_CLUSTERS 1024
-/* Bytes per pixel of a proposal table: 16 cluster ids and 16 weights; ids
- * are 1 byte for networks of 256 outputs, 2 bytes for 1024. */
-#define RISLTC_PROPOSAL_TABLE_BYTES(outputs) ((outputs) > 256 ? 48u : 32u)
+/* Bytes per pixel of a proposal table: 16 one-byte group ids and 16 byte
+ * weights, for every network. A group is one cluster for networks of 256
+ * outputs, and the quad of clusters 4 q .. 4 q + 3 for 1024 outputs. */
+#define RISLTC_PROPOSAL_TABLE_BYTES(outputs) 32u
 #define RISLTC_PROPOSAL_PARAMS 31168      /* 96-64-64-64-256 MLP (default), FP32 */
 #define RISLTC_PROPOSAL_PARAMS_WIDE 78464 /* 96-128-128-128-256 MLP, FP32 */
 /* Explicit light clusters: cluster_of_light[i] < cluster_count <= 1024
@@ -522,7 +523,11 @@ int risltc_set_light_clusters(risltc_context_t* context, const uint32_t* cluster
                               uint32_t cluster_count);
 /* k-means of the light centroids (deterministic start: light floor(k N / K)
  * for cluster k; empty clusters restart at the farthest light), then
- * risltc_set_light_clusters. out (host, may be NULL) receives the assignment. */
+ * risltc_set_light_clusters. More than 256 clusters are made in two levels:
+ * ceil(K / 4) clusters of all lights, each split again into up to four
+ * numbered 4 q .. 4 q + 3, so that every quad of consecutive ids is one
+ * compact top-level cluster (the 1024-output network's tables sample quads).
+ * out (host, may be NULL) receives the assignment. */
 int risltc_cluster_lights(risltc_context_t* context, uint32_t cluster_count, uint32_t iterations,
                           uint32_t* out);
 /* Hidden width of the MLP: 64 (default) or 128 (the 96-128-128-128-256
@@ -533,8 +538,8 @@ int risltc_cluster_lights(risltc_context_t* context, uint32_t cluster_count, uin
  * [out][in] followed by its bias. */
 int risltc_proposal_set_width(risltc_context_t* context, uint32_t width);
 /* Hidden width (64 or 128) and outputs (256 or 1024 clusters) together;
- * risltc_proposal_set_width keeps the outputs. The 1024-output network
- * writes tables with 2-byte cluster ids (RISLTC_PROPOSAL_TABLE_BYTES). */
+ * risltc_proposal_set_width keeps the outputs. The 1024-output network's
+ * tables name quads of clusters (risltc_proposal_update). */
 int risltc_proposal_set_network(risltc_context_t* context, uint32_t width, uint32_t outputs);
 uint32_t risltc_proposal_param_count(const risltc_context_t* context);
 /* Deterministic initialisation from a seed; resets the Adam state. */
@@ -545,14 +550,17 @@ int risltc_proposal_init(risltc_context_t* context, uint32_t seed);
 int risltc_proposal_set_params(risltc_context_t* context, const float* params);
 int risltc_proposal_get_params(risltc_context_t* context, float* params, float* gradient);
 /* Inference for every pixel of the frame: writes the per-pixel tables the
- * learned shade reads, RISLTC_PROPOSAL_TABLE_BYTES(outputs) each: 16 cluster
- * ids (bytes 0..15, or 2-byte ids in bytes 0..31 for 1024 outputs) and 16
- * weights (bytes 16..31, or 32..47); slot 4 g + k holds, in each group
- * g = (c % 16) / 4 of non-empty clusters, the k-th of the four with
- * the largest logits z (descending, ties to the smaller id) with weight
- * round(255 exp(z - z_max)), z_max over all non-empty clusters; weights 0
- * for background. Optional device outputs: a copy of the tables, the 256
- * (or 1024) logits and the 96 bf16 features per pixel. */
+ * learned shade reads, RISLTC_PROPOSAL_TABLE_BYTES each: 16 group ids
+ * (bytes 0..15) and 16 weights (bytes 16..31). A group's value is the logit
+ * z of a cluster (256 outputs), or log(sum exp z) over the non-empty
+ * clusters of the quad 4 q .. 4 q + 3 (1024 outputs; id q). Slot 4 g + k
+ * holds, among the groups of lane group g = (c % 16) / 4 with a non-empty
+ * cluster, the k-th of the four with the largest values v (descending, ties
+ * to the smaller id) with weight round(255 exp(v - v_max)), v_max over all
+ * such groups; weights 0 for background. A candidate drawn from a slot
+ * takes one light of the group's clusters uniformly. Optional device
+ * outputs: a copy of the tables, the 256 (or 1024) logits and the 96 bf16
+ * features per pixel. */
 int risltc_proposal_update(risltc_context_t* context, const risltc_frame_constants_t* constants,
                            const uint32_t* visibility, uint8_t* table_out, float* logits_out,
                            uint16_t* features_out, void* stream);

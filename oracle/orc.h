@@ -156,8 +156,7 @@ typedef struct orc_scene_s {
     const orc_bvh_t* bvh;
     /* learned RIS source (project extension; read when settings.proposal is
      * RISLTC_PROPOSAL_LEARNED): per pixel table as risltc_proposal_update
-     * writes it, proposal_table_bytes (32: 1-byte ids, or 48: 2-byte ids)
-     * per pixel */
+     * writes it (32 bytes: 16 group ids, 16 weights) */
     const uint8_t* proposal_table;
     const uint32_t* cluster_of_light;
     const uint32_t* cluster_offsets; /* cluster_count + 1 prefix offsets      */
@@ -172,7 +171,8 @@ typedef struct orc_scene_s {
     float cache_inv_cell;
     uint32_t cache_slots;
     uint32_t cache_per_light;
-    uint32_t proposal_table_bytes; /* 32 or 48 */
+    uint32_t proposal_table_bytes; /* 32 */
+    uint32_t proposal_group_log2;  /* a table slot names 1 << this clusters */
 } orc_scene_t;
 
 /* Bytes of the visibility cache from its counts (2 words per entry:

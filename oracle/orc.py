@@ -61,7 +61,8 @@ class OrcScene(C.Structure):
                 ("proposal_table", u8p), ("cluster_of_light", u32p), ("cluster_offsets", u32p),
                 ("cluster_lights", u32p), ("cluster_count", C.c_uint32), ("cache_vis", u8p),
                 ("cache_mask", C.c_uint32), ("cache_inv_cell", C.c_float), ("cache_slots", C.c_uint32),
-                ("cache_per_light", C.c_uint32), ("proposal_table_bytes", C.c_uint32)]
+                ("cache_per_light", C.c_uint32), ("proposal_table_bytes", C.c_uint32),
+                ("proposal_group_log2", C.c_uint32)]
 
 
 def lib(timing=False):
@@ -246,14 +247,17 @@ class Oracle:
         except Exception:
             pass
 
-    def set_proposal(self, table, cluster_of_light, cluster_count):
-        """Learned-source inputs: per pixel tables (pixels x 32 or 48 uint8,
-        as risltc_proposal_update writes them) and the light clusters; lights
-        are ordered by cluster, then by index (the library's rule)."""
+    def set_proposal(self, table, cluster_of_light, cluster_count, group_log2=0):
+        """Learned-source inputs: per pixel tables (pixels x 32 uint8, as
+        risltc_proposal_update writes them), the light clusters (lights are
+        ordered by cluster, then by index: the library's rule) and log2 of
+        the clusters a table slot names (abi.proposal_group_log2 of the
+        network's outputs)."""
         t = np.asarray(table, dtype=np.uint8)
         width = t.shape[-1] if t.ndim == 2 else 32
-        assert width in (32, 48)
+        assert width == 32
         self.cscene.proposal_table_bytes = width
+        self.cscene.proposal_group_log2 = group_log2
         self.prop_table = np.ascontiguousarray(t).reshape(-1)
         col = np.ascontiguousarray(cluster_of_light, dtype=np.uint32)
         self.prop_of = col

@@ -996,23 +996,35 @@ static uint32_t light_index_from(R u, uint32_t n) {
 }
 
 /* Learned RIS source (project extension; no reference code). Table of the
- * pixel: ids t[0..15], weights t[16..31], T = their sum. Two draws a, b per
- * candidate; every decision in float, as light_index_from:
+ * pixel: group ids t[0..15], weights t[16..31], T = their sum. Group g is the
+ * run of clusters g << G .. ((g + 1) << G) - 1, G = proposal_group_log2 (0:
+ * single clusters; 2: quads of the 1024-output network), whose lights are
+ * the consecutive run cluster_lights[offsets[g << G] .. offsets[min((g + 1)
+ * << G, count)]). Two draws a, b per candidate; every decision in float, as
+ * light_index_from:
  *   T == 0 -> light_index_from(a); a < 1/2 -> light_index_from(2a);
  *   else slot s = first with (int)((2a - 1) T) (clamped to T - 1) <
- *   w_0 + .. + w_s, cluster id_s, light number min(size - 1, (int)(b size))
- *   of that cluster (an empty cluster or an id past the clusters:
+ *   w_0 + .. + w_s, group id_s, light number min(size - 1, (int)(b size))
+ *   of that group (an empty group or an id past the groups:
  *   light_index_from(b); learned_density counts that mass as uniform).
  * parity: bitwise against the kernel's learned shading given the same tables
  * (tests/test_gpu_learned.py); the mixture is an exact pdf of this draw
  * (tests/test_oracle_estimator.py::test_learned_draw_density). */
-/* table slot s: cluster id and weight (1-byte ids with weights at 16, or
- * 2-byte ids with weights at 32) */
-static uint32_t table_id(const orc_scene_t* sc, const uint8_t* t, int s) {
-    return sc->proposal_table_bytes == 48u ? (uint32_t)t[2 * s] | ((uint32_t)t[2 * s + 1] << 8) : (uint32_t)t[s];
+static uint32_t table_id(const uint8_t* t, int s) { return t[s]; }
+static uint32_t table_weight(const uint8_t* t, int s) { return t[16 + s]; }
+
+static uint32_t group_count(const orc_scene_t* sc) {
+    uint32_t G = sc->proposal_group_log2;
+    return (sc->cluster_count + (1u << G) - 1u) >> G;
 }
-static uint32_t table_weight(const orc_scene_t* sc, const uint8_t* t, int s) {
-    return t[(sc->proposal_table_bytes == 48u ? 32 : 16) + s];
+
+/* [first, end) of group g's lights in cluster_lights (g < group_count) */
+static void group_range(const orc_scene_t* sc, uint32_t g, uint32_t* first, uint32_t* end) {
+    uint32_t G = sc->proposal_group_log2;
+    uint32_t c1 = (g + 1u) << G;
+    if (c1 > sc->cluster_count) c1 = sc->cluster_count;
+    *first = sc->cluster_offsets[g << G];
+    *end = sc->cluster_offsets[c1];
 }
 
 static uint32_t learned_draw(const orc_scene_t* sc, const uint8_t* t, uint32_t T, uint32_t* seed, uint32_t N) {
@@ -1023,16 +1035,18 @@ static uint32_t learned_draw(const orc_scene_t* sc, const uint8_t* t, uint32_t T
     float u = (a + a) - 1.0f;
     uint32_t k = (uint32_t)(u * (float)T);
     if (k > T - 1u) k = T - 1u;
-    uint32_t acc = 0, c = 0;
+    uint32_t acc = 0, g = 0;
     for (int s = 0; s != 16; ++s) {
-        if (k < acc + table_weight(sc, t, s)) {
-            c = table_id(sc, t, s);
+        if (k < acc + table_weight(t, s)) {
+            g = table_id(t, s);
             break;
         }
-        acc += table_weight(sc, t, s);
+        acc += table_weight(t, s);
     }
-    if (c >= sc->cluster_count) return light_index_from((R)b, N);
-    uint32_t off = sc->cluster_offsets[c], size = sc->cluster_offsets[c + 1] - off;
+    if (g >= group_count(sc)) return light_index_from((R)b, N);
+    uint32_t off, end;
+    group_range(sc, g, &off, &end);
+    uint32_t size = end - off;
     if (size == 0u) return light_index_from((R)b, N);
     uint32_t j = (uint32_t)(b * (float)size);
     if (j > size - 1u) j = size - 1u;
@@ -1040,30 +1054,38 @@ static uint32_t learned_draw(const orc_scene_t* sc, const uint8_t* t, uint32_t T
 }
 
 /* Weight of the slots that learned_draw sends to a uniform light: ids past
- * the clusters, or of empty clusters. */
+ * the groups, or of empty groups. */
 static uint32_t learned_fallback_weight(const orc_scene_t* sc, const uint8_t* t) {
     uint32_t bad = 0;
     for (int s = 0; s != 16; ++s) {
-        uint32_t w = table_weight(sc, t, s), c = table_id(sc, t, s);
-        if (w != 0u && (c >= sc->cluster_count || sc->cluster_offsets[c + 1] == sc->cluster_offsets[c])) bad += w;
+        uint32_t w = table_weight(t, s), g = table_id(t, s);
+        if (w == 0u) continue;
+        int empty = g >= group_count(sc);
+        if (!empty) {
+            uint32_t off, end;
+            group_range(sc, g, &off, &end);
+            empty = off == end;
+        }
+        if (empty) bad += w;
     }
     return bad;
 }
 
-/* q = 1/(2N) + T_bad / (2 T N) + w_c / (2 T |c|): w_c sums the weights of
- * every slot holding the light's cluster (0 when none does), T_bad is
+/* q = 1/(2N) + T_bad / (2 T N) + w_g / (2 T |g|): w_g sums the weights of
+ * every slot holding the light's group (0 when none does), T_bad is
  * learned_fallback_weight (its term is skipped when 0) */
 static R learned_density(const orc_scene_t* sc, const uint8_t* t, uint32_t T, uint32_t T_bad, uint32_t idx,
                          uint32_t N) {
     if (T == 0u) return K(1.0) / (R)N;
     R q = K(0.5) / (R)N;
     if (T_bad != 0u) q = q + (K(0.5) * (R)T_bad) / ((R)T * (R)N);
-    uint32_t c = sc->cluster_of_light[idx], wc = 0;
+    uint32_t g = sc->cluster_of_light[idx] >> sc->proposal_group_log2, wg = 0;
     for (int s = 0; s != 16; ++s)
-        if (table_weight(sc, t, s) != 0u && table_id(sc, t, s) == c) wc += table_weight(sc, t, s);
-    if (wc != 0u) {
-        uint32_t size = sc->cluster_offsets[c + 1] - sc->cluster_offsets[c];
-        q = q + (K(0.5) * (R)wc) / ((R)T * (R)size);
+        if (table_weight(t, s) != 0u && table_id(t, s) == g) wg += table_weight(t, s);
+    if (wg != 0u) {
+        uint32_t off, end;
+        group_range(sc, g, &off, &end);
+        q = q + (K(0.5) * (R)wg) / ((R)T * (R)(end - off));
     }
     return q;
 }
@@ -1201,11 +1223,11 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
                 const int learned = s->proposal != RISLTC_PROPOSAL_UNIFORM;
                 const uint8_t* tab =
                     s->proposal == RISLTC_PROPOSAL_LEARNED
-                        ? sc->proposal_table + (size_t)pixel * (sc->proposal_table_bytes == 48u ? 48u : 32u)
+                        ? sc->proposal_table + (size_t)pixel * 32u
                         : NULL;
                 uint32_t T = 0, T_bad = 0;
                 if (tab) {
-                    for (int k2 = 0; k2 != 16; ++k2) T += table_weight(sc, tab, k2);
+                    for (int k2 = 0; k2 != 16; ++k2) T += table_weight(tab, k2);
                     T_bad = learned_fallback_weight(sc, tab);
                 }
                 const uint8_t* crow = learned && sc->cache_vis ? sc->cache_vis + (size_t)ckey * sc->cache_slots : NULL;

@@ -28,8 +28,14 @@ PROPOSAL_PARAMS_WIDE = 78464
 
 def proposal_table_bytes(outputs):
     """Bytes per pixel of a proposal table (RISLTC_PROPOSAL_TABLE_BYTES):
-    16 cluster ids and 16 weights, ids of 2 bytes for more than 256 outputs."""
-    return 48 if outputs > 256 else 32
+    16 one-byte group ids and 16 weights, for every network."""
+    return 32
+
+
+def proposal_group_log2(outputs):
+    """log2 of the clusters one table slot names: a cluster for networks of
+    at most 256 outputs, a quad of clusters for more."""
+    return 2 if outputs > 256 else 0
 
 # risltc_sample_record_t word offsets
 REC_LIGHT, REC_WSUM, REC_PHAT, REC_W, REC_CLIPD, REC_CLIPS, REC_PSAD, REC_PSAS = range(8)

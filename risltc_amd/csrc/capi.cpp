@@ -880,6 +880,70 @@ int risltc_proposal_cache_table(risltc_context_t* c, uint8_t* out, uint64_t capa
     return RISLTC_OK;
 }
 
+// Deterministic Lloyd k-means of the points p[3 i] of the listed members into
+// K clusters: centres start at the members k * m / K, every iteration
+// assigns each member to its nearest centre (the first on ties) and moves
+// each centre to its members' mean; an empty cluster restarts at the member
+// farthest from its centre. Returns the cluster of each member.
+static std::vector<uint32_t> lloyd_kmeans(const float* p, const std::vector<uint32_t>& members, uint32_t K,
+                                          uint32_t iterations) {
+    const uint32_t n = (uint32_t)members.size();
+    std::vector<double> ctr(3 * K);
+    for (uint32_t k = 0; k < K; ++k) {
+        uint32_t i = members[(uint32_t)(((uint64_t)k * n) / K)];
+        for (int a = 0; a < 3; ++a) ctr[3 * k + a] = p[3 * i + a];
+    }
+    std::vector<uint32_t> asg(n, 0);
+    auto d2 = [&](uint32_t m, uint32_t k) {
+        const uint32_t i = members[m];
+        double s = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            double t = (double)p[3 * i + a] - ctr[3 * k + a];
+            s += t * t;
+        }
+        return s;
+    };
+    for (uint32_t it = 0; it <= iterations; ++it) {
+        for (uint32_t m = 0; m < n; ++m) {
+            uint32_t best = 0;
+            double bd = d2(m, 0);
+            for (uint32_t k = 1; k < K; ++k) {
+                double d = d2(m, k);
+                if (d < bd) {
+                    bd = d;
+                    best = k;
+                }
+            }
+            asg[m] = best;
+        }
+        if (it == iterations) break;
+        std::vector<double> sum(3 * K, 0.0);
+        std::vector<uint32_t> cnt(K, 0);
+        for (uint32_t m = 0; m < n; ++m) {
+            cnt[asg[m]]++;
+            for (int a = 0; a < 3; ++a) sum[3 * asg[m] + a] += p[3 * members[m] + a];
+        }
+        for (uint32_t k = 0; k < K; ++k) {
+            if (cnt[k] > 0) {
+                for (int a = 0; a < 3; ++a) ctr[3 * k + a] = sum[3 * k + a] / cnt[k];
+                continue;
+            }
+            uint32_t far = 0;
+            double fd = -1.0;
+            for (uint32_t m = 0; m < n; ++m) {
+                double d = d2(m, asg[m]);
+                if (d > fd) {
+                    fd = d;
+                    far = m;
+                }
+            }
+            for (int a = 0; a < 3; ++a) ctr[3 * k + a] = p[3 * members[far] + a];
+            asg[far] = k;
+        }
+    }
+    return asg;
+}
+
 int risltc_cluster_lights(risltc_context_t* c, uint32_t cluster_count, uint32_t iterations, uint32_t* out) {
     if (!c) return RISLTC_E_ARG;
     const uint32_t n = c->light_count;
@@ -889,57 +953,28 @@ int risltc_cluster_lights(risltc_context_t* c, uint32_t cluster_count, uint32_t 
                     RISLTC_MAX_LIGHT_CLUSTERS);
     const float* p = c->light_centroids.data();
     const uint32_t K = cluster_count;
-    std::vector<double> ctr(3 * K);
-    for (uint32_t k = 0; k < K; ++k) {
-        uint32_t i = (uint32_t)(((uint64_t)k * n) / K);
-        for (int a = 0; a < 3; ++a) ctr[3 * k + a] = p[3 * i + a];
-    }
-    std::vector<uint32_t> asg(n, 0);
-    auto d2 = [&](uint32_t i, uint32_t k) {
-        double s = 0.0;
-        for (int a = 0; a < 3; ++a) {
-            double t = (double)p[3 * i + a] - ctr[3 * k + a];
-            s += t * t;
-        }
-        return s;
-    };
-    for (uint32_t it = 0; it <= iterations; ++it) {
-        for (uint32_t i = 0; i < n; ++i) {
-            uint32_t best = 0;
-            double bd = d2(i, 0);
-            for (uint32_t k = 1; k < K; ++k) {
-                double d = d2(i, k);
-                if (d < bd) {
-                    bd = d;
-                    best = k;
-                }
-            }
-            asg[i] = best;
-        }
-        if (it == iterations) break;
-        std::vector<double> sum(3 * K, 0.0);
-        std::vector<uint32_t> cnt(K, 0);
-        for (uint32_t i = 0; i < n; ++i) {
-            cnt[asg[i]]++;
-            for (int a = 0; a < 3; ++a) sum[3 * asg[i] + a] += p[3 * i + a];
-        }
-        for (uint32_t k = 0; k < K; ++k) {
-            if (cnt[k] > 0) {
-                for (int a = 0; a < 3; ++a) ctr[3 * k + a] = sum[3 * k + a] / cnt[k];
-                continue;
-            }
-            // an empty cluster restarts at the light farthest from its centre
-            uint32_t far = 0;
-            double fd = -1.0;
-            for (uint32_t i = 0; i < n; ++i) {
-                double d = d2(i, asg[i]);
-                if (d > fd) {
-                    fd = d;
-                    far = i;
-                }
-            }
-            for (int a = 0; a < 3; ++a) ctr[3 * k + a] = p[3 * far + a];
-            asg[far] = k;
+    std::vector<uint32_t> all(n);
+    for (uint32_t i = 0; i < n; ++i) all[i] = i;
+    std::vector<uint32_t> asg;
+    if (K <= 256) {
+        asg = lloyd_kmeans(p, all, K, iterations);
+    } else {
+        // Two levels: ceil(K / 4) clusters of all lights, then each of those
+        // split into (up to) four, numbered 4 q .. 4 q + 3: a quad of
+        // clusters is one top-level cluster, which the proposal tables of
+        // networks with more than 256 outputs sample as a group.
+        const uint32_t Q = (K + 3u) / 4u;
+        std::vector<uint32_t> top = lloyd_kmeans(p, all, Q, iterations);
+        std::vector<std::vector<uint32_t>> members(Q);
+        for (uint32_t i = 0; i < n; ++i) members[top[i]].push_back(i);
+        asg.assign(n, 0);
+        for (uint32_t q = 0; q < Q; ++q) {
+            const uint32_t m = (uint32_t)members[q].size();
+            uint32_t sub = std::min(4u, K - 4u * q);
+            if (sub > m) sub = m;
+            if (m == 0) continue;
+            std::vector<uint32_t> a = lloyd_kmeans(p, members[q], sub, iterations);
+            for (uint32_t j = 0; j < m; ++j) asg[members[q][j]] = 4u * q + a[j];
         }
     }
     if (out) memcpy(out, asg.data(), n * sizeof(uint32_t));
@@ -1516,7 +1551,7 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     a.debug_cands = dbg_c;
     a.learned_pass = learned ? 1u : 0u;
     a.proposal = s->proposal == RISLTC_PROPOSAL_LEARNED ? c->prop_table.as<uint8_t>() : nullptr;
-    a.proposal_ids16 = (a.proposal && c->mlp_L.out > 256) ? 1u : 0u;
+    a.proposal_group_log2 = rl::prop_group_log2(c->mlp_L.out);
     const bool cached = c->vcache.p && c->cluster_count > 0;
     if (cached) {
         if (c->vcache_per_light && c->vcache_slots != c->light_count)
@@ -1574,13 +1609,13 @@ struct BatchPlan {
     size_t lds = 0;
     bool exact = false, shadows = false;
     bool learned = false, cached = false;  // learned views; the visibility cache is refreshed first
-    bool ids16 = false;                     // the views' tables have 2-byte cluster ids
+    uint32_t group_log2 = 0;                // table slots name groups of 1 << group_log2 clusters
     rl::VisCacheView cache = {};
     // whether a graph captured with plan `o` can run this plan's descriptors
     bool fits(const BatchPlan& o) const {
         return tile_off == o.tile_off && block_off == o.block_off && samples == o.samples && lds <= o.lds &&
                exact == o.exact && shadows == o.shadows && learned == o.learned && cached == o.cached &&
-               ids16 == o.ids16 && bvhs.size() <= o.bvh_capacity;
+               group_log2 == o.group_log2 && bvhs.size() <= o.bvh_capacity;
     }
 };
 
@@ -1621,7 +1656,7 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
     }
     p.exact = views[0].settings.exact_math != 0;
     p.learned = views[0].settings.proposal != RISLTC_PROPOSAL_UNIFORM;
-    p.ids16 = p.learned && c->mlp_L.out > 256;
+    p.group_log2 = p.learned ? rl::prop_group_log2(c->mlp_L.out) : 0u;
     if (p.learned && c->cluster_count > (uint32_t)c->mlp_L.out)
         return fail(c, RISLTC_E_STATE, "%u clusters for a network of %d outputs", c->cluster_count, c->mlp_L.out);
     p.cached = p.learned && c->vcache.p != nullptr;
@@ -1659,7 +1694,7 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
         if (p.learned) {
             a.learned_pass = 1u;
             a.proposal = w.settings.proposal == RISLTC_PROPOSAL_LEARNED ? w.proposal_table : nullptr;
-            a.proposal_ids16 = p.ids16 ? 1u : 0u;
+            a.proposal_group_log2 = p.group_log2;
             if (p.cached) a.cache = p.cache;
         }
         p.lds = std::max(p.lds, rl::shade_lds_bytes(a));
@@ -1706,7 +1741,7 @@ static int launch_plan(risltc_context_t* c, BatchBuffers& B, const BatchPlan& p,
     if (timed) HIPC(c, hipEventRecord(c->ev[0], st));
     if (p.cached) HIPC(c, rl::launch_refresh_cache(p.cache, st));
     HIPC(c, rl::launch_shade_batch(B.shade_args.as<rl::ShadeArgs>(), B.tile_offsets.as<uint32_t>(), n, p.tiles, p.lds,
-                                   p.exact, p.learned ? (p.ids16 ? 2 : 1) : 0, st));
+                                   p.exact, p.learned ? 1 : 0, st));
     if (timed) HIPC(c, hipEventRecord(c->ev[1], st));
     if (p.shadows)
         HIPC(c, rl::launch_trace(trace_args(c, B.bvh_table.as<rl::BvhView>(), B.rays.as<rl::ShadowRay>(),

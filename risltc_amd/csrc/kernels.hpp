@@ -95,15 +95,19 @@ constexpr int kPW4 = kPB3 + kMlpHid, kPB4 = kPW4 + kMlpOut * kMlpHid;
 constexpr int kMlpParams = kPB4 + kMlpOut;  // 31,168
 static_assert(mlp_layout(kMlpHid).params == kMlpParams && mlp_layout(kMlpHid).pw4 == kPW4, "default layout");
 static_assert(mlp_layout(kMlpHidWide).params == 78464, "wide layout");
-// per pixel proposal table: the ids of 16 clusters (bytes 0..15) and their weights
-// (bytes 16..31): in each of 4 groups of clusters (c % 16) / 4 = g, the 4
-// with the largest logits, weight round(255 exp(z - z_max))
+// Per pixel proposal table: 16 slots, each the id of a light group (bytes
+// 0..15) and its weight (bytes 16..31). A group is a cluster for networks of
+// at most 256 outputs, and a quad of clusters 4 q .. 4 q + 3 for the
+// 1024-output network (whose clusters are numbered so that a quad's are
+// neighbours). In each of 4 lane groups g = (c % 16) / 4 of the outputs the
+// table keeps the 4 groups with the largest value -- the logit of a cluster,
+// or log sum exp of the four logits of a quad -- at weight
+// round(255 exp(value - max)).
 constexpr int kPropTop = 16;
 constexpr int kPropBytes = 2 * kPropTop;
-// with more than 256 outputs the ids are 2 bytes: ids in bytes 0..31
-// (slot s at 2 s), weights in bytes 32..47
-constexpr int kPropBytes16 = 3 * kPropTop;
-__host__ __device__ constexpr int prop_bytes(int outputs) { return outputs > 256 ? kPropBytes16 : kPropBytes; }
+__host__ __device__ constexpr int prop_bytes(int) { return kPropBytes; }
+// log2 of the clusters a table slot addresses for a network's outputs
+__host__ __device__ constexpr uint32_t prop_group_log2(int outputs) { return outputs > 256 ? 2u : 0u; }
 // clusters a visibility cache in cluster mode has slots for
 constexpr int kCacheClusterSlots = kMlpOutWide;
 // default floor 1/8: scale = (1 - 0.125) / 255 in FP32
@@ -210,10 +214,11 @@ struct ShadeArgs {
     // learned pass (1): three draws per candidate, the learned source when
     // proposal is set, the cache factor in the target when cache.vis is set
     uint32_t learned_pass;
-    // learned source (null = uniform): kPropBytes (kPropBytes16 when
-    // proposal_ids16) per pixel, mixed half and half with the uniform source
+    // learned source (null = uniform): kPropBytes per pixel, mixed half and
+    // half with the uniform source; a slot's id names the clusters
+    // id << proposal_group_log2 .. ((id + 1) << proposal_group_log2) - 1
     const uint8_t* proposal;
-    uint32_t proposal_ids16;
+    uint32_t proposal_group_log2;
     ClusterView clusters;
     // learned target factor f + v (1 - f) / 255 of the visibility cache
     float learned_floor, learned_scale;  // f and (1 - f) / 255 (FP32)

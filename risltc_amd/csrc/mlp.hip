@@ -214,6 +214,41 @@ __device__ __forceinline__ void merge_top4(const float (&z)[4], const bool (&liv
     }
 }
 
+// The same for a network of more than 256 outputs, whose table slots name
+// quads of clusters: the tile's four logits are the quad c0 / 4 of this
+// lane, merged as one value, log sum exp over its live clusters (-inf when
+// none is live). Ties keep the smaller quad.
+__device__ __forceinline__ void merge_quad(const float (&z)[4], const bool (&live)[4], int c0, float* logits,
+                                           float (&bv)[4], int (&bi)[4]) {
+    if (logits) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) logits[i] = z[i];
+    }
+    float m = -__builtin_inff();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (live[i]) m = fmax_(m, z[i]);
+    if (!(m > -__builtin_inff())) return;
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (live[i]) sum += __expf(z[i] - m);
+    const float v = m + __logf(sum);
+    if (!(v > bv[3])) return;
+    bv[3] = v;
+    bi[3] = c0 >> 2;
+#pragma unroll
+    for (int q = 3; q > 0; --q)
+        if (bv[q] > bv[q - 1]) {
+            float tv = bv[q];
+            bv[q] = bv[q - 1];
+            bv[q - 1] = tv;
+            int ti = bi[q];
+            bi[q] = bi[q - 1];
+            bi[q - 1] = ti;
+        }
+}
+
 // hidden layer on a 16-pixel tile: out[pixel][o] = relu(W[o] . act[pixel] + b[o])
 template <int HID, int K, int LDW, int LDA>
 __device__ __forceinline__ void hidden_layer(const uint16_t* W, const float* bias, const uint16_t* act, uint16_t* out,
@@ -329,7 +364,9 @@ __global__ void __launch_bounds__(64 * WAVES) proposal_kernel(ProposalArgs A) {
                         z[i] = acc[i] + sh.bias[3 * HID + c0 + i];
                         live[i] = sh.csize[c0 + i] != 0u;
                     }
-                    merge_top4(z, live, c0, A.logits && in ? A.logits + (size_t)pixel * OUT + c0 : nullptr, bv, bi);
+                    float* lo = A.logits && in ? A.logits + (size_t)pixel * OUT + c0 : nullptr;
+                    if constexpr (OUT > 256) merge_quad(z, live, c0, lo, bv, bi);
+                    else merge_top4(z, live, c0, lo, bv, bi);
                 }
                 // the pixel's maximum over its four lanes (lanes r, r+16, r+32, r+48)
                 float zmax = bv[0];
@@ -347,16 +384,10 @@ __global__ void __launch_bounds__(64 * WAVES) proposal_kernel(ProposalArgs A) {
                     ws |= w << (8 * s);
                 }
                 if (in) {
-                    uint32_t* o = reinterpret_cast<uint32_t*>(A.out + (size_t)pixel * prop_bytes(OUT));
-                    if constexpr (OUT > 256) {
-                        // slot 4 g + s: id (2 bytes) at 2 (4 g + s), weight at 32 + 4 g + s
-                        o[2 * g] = (uint32_t)bi[0] | ((uint32_t)bi[1] << 16);
-                        o[2 * g + 1] = (uint32_t)bi[2] | ((uint32_t)bi[3] << 16);
-                        o[8 + g] = ws;
-                    } else {
-                        o[g] = (uint32_t)bi[0] | ((uint32_t)bi[1] << 8) | ((uint32_t)bi[2] << 16) | ((uint32_t)bi[3] << 24);
-                        o[4 + g] = ws;
-                    }
+                    // slot 4 g + s: id at byte 4 g + s, weight at 16 + 4 g + s
+                    uint32_t* o = reinterpret_cast<uint32_t*>(A.out + (size_t)pixel * kPropBytes);
+                    o[g] = (uint32_t)bi[0] | ((uint32_t)bi[1] << 8) | ((uint32_t)bi[2] << 16) | ((uint32_t)bi[3] << 24);
+                    o[4 + g] = ws;
                 }
                 wave_sync();
             }
@@ -432,9 +463,9 @@ __global__ void __launch_bounds__(64 * WAVES) proposal_kernel(ProposalArgs A) {
                     float z[4];
 #pragma unroll
                     for (int i = 0; i < 4; ++i) z[i] = acc[i] + b4[i];
-                    merge_top4(z, live, c0,
-                               A.logits && pixel < A.pixel_count ? A.logits + (size_t)pixel * OUT + c0 : nullptr,
-                               bv[s], bi[s]);
+                    float* lo = A.logits && pixel < A.pixel_count ? A.logits + (size_t)pixel * OUT + c0 : nullptr;
+                    if constexpr (OUT > 256) merge_quad(z, live, c0, lo, bv[s], bi[s]);
+                    else merge_top4(z, live, c0, lo, bv[s], bi[s]);
                 }
 #pragma unroll
                 for (int ks = 0; ks < HID / 32; ++ks) wa[ks] = wn[ks];
@@ -462,16 +493,10 @@ __global__ void __launch_bounds__(64 * WAVES) proposal_kernel(ProposalArgs A) {
                     ws |= w << (8 * q);
                 }
                 if (in) {
-                    uint32_t* o = reinterpret_cast<uint32_t*>(A.out + (size_t)pixel * prop_bytes(OUT));
-                    if constexpr (OUT > 256) {
-                        // slot 4 g + q: id (2 bytes) at 2 (4 g + q), weight at 32 + 4 g + q
-                        o[2 * g] = (uint32_t)id[0] | ((uint32_t)id[1] << 16);
-                        o[2 * g + 1] = (uint32_t)id[2] | ((uint32_t)id[3] << 16);
-                        o[8 + g] = ws;
-                    } else {
-                        o[g] = (uint32_t)id[0] | ((uint32_t)id[1] << 8) | ((uint32_t)id[2] << 16) | ((uint32_t)id[3] << 24);
-                        o[4 + g] = ws;
-                    }
+                    // slot 4 g + q: id at byte 4 g + q, weight at 16 + 4 g + q
+                    uint32_t* o = reinterpret_cast<uint32_t*>(A.out + (size_t)pixel * kPropBytes);
+                    o[g] = (uint32_t)id[0] | ((uint32_t)id[1] << 8) | ((uint32_t)id[2] << 16) | ((uint32_t)id[3] << 24);
+                    o[4 + g] = ws;
                 }
             }
         }

@@ -1050,58 +1050,47 @@ __device__ __forceinline__ uint32_t lcg_jump(uint32_t s, uint32_t d) {
 }
 
 // ---- learned proposal: candidate draws ------------------------------------------
-// Per pixel table: 16 cluster ids and 16 byte weights, T = sum of the
-// weights. kPropBytes: ids in bytes 0..15, weights in 16..31; kPropBytes16
-// (networks with more than 256 outputs, I16): 2-byte ids in bytes 0..31,
-// weights in 32..47. A candidate takes two draws a, b:
+// Per pixel table: 16 group ids (bytes 0..15) and 16 byte weights (bytes
+// 16..31), T = the sum of the weights. Group g of the table is the run of
+// clusters g << G .. ((g + 1) << G) - 1 (G = 0: single clusters; G = 2: the
+// quads of the 1024-output network); its lights are the consecutive run
+// lights[offsets[g << G] .. offsets[min((g + 1) << G, count)]). A candidate
+// takes two draws a, b:
 //   T == 0        -> light_index_from(a, N);
 //   a < 1/2       -> light_index_from(2 a, N)          (the uniform half);
 //   otherwise     -> t = min(T - 1, int((2 a - 1) T)), slot s = first with
-//                    t < w_0 + .. + w_s, cluster c = id_s, light = the
-//                    min(size - 1, int(b size))-th light of c (an empty
-//                    cluster, or an id past the clusters, falls back to
+//                    t < w_0 + .. + w_s, group g = id_s, light = the
+//                    min(size - 1, int(b size))-th light of g (an empty
+//                    group, or an id past the groups, falls back to
 //                    light_index_from(b, N)).
-// Its source density is q = 1/(2N) + T_bad / (2 T N) + w_c / (2 T |c|):
-// w_c sums the weights of the slots that hold the light's cluster (0 when
+// Its source density is q = 1/(2N) + T_bad / (2 T N) + w_g / (2 T |g|):
+// w_g sums the weights of the slots that hold the light's group (0 when
 // none does) and T_bad the weights of the slots that fall back to a uniform
 // light (the term is skipped when T_bad = 0). The oracle computes it in the
 // same operation order.
-template <bool I16>
 struct PropTable {
-    static constexpr int kIdWords = I16 ? 8 : 4;
-    uint32_t ids[kIdWords], ws[4];
+    uint32_t ids[4], ws[4];
     uint32_t T;
     uint32_t T_bad;
 };
 
 __device__ __forceinline__ uint32_t byte_of(const uint32_t* v, int s) { return (v[s >> 2] >> (8 * (s & 3))) & 0xFFu; }
 
-template <bool I16>
-__device__ __forceinline__ uint32_t id_of(const PropTable<I16>& P, int s) {
-    if constexpr (I16) return (P.ids[s >> 1] >> (16 * (s & 1))) & 0xFFFFu;
-    else return byte_of(P.ids, s);
-}
+__device__ __forceinline__ uint32_t id_of(const PropTable& P, int s) { return byte_of(P.ids, s); }
 
 // loads the pixel's table (zeros when tab is null)
-template <bool I16>
-__device__ __forceinline__ void load_prop_table(const uint8_t* tab, PropTable<I16>& P) {
+__device__ __forceinline__ void load_prop_table(const uint8_t* tab, PropTable& P) {
     P.T = 0u;
     P.T_bad = 0u;
 #pragma unroll
-    for (int k = 0; k < PropTable<I16>::kIdWords; ++k) P.ids[k] = 0u;
+    for (int k = 0; k < 4; ++k) P.ids[k] = 0u;
 #pragma unroll
     for (int k = 0; k < 4; ++k) P.ws[k] = 0u;
     if (!tab) return;
     const uint4* t = reinterpret_cast<const uint4*>(tab);
     uint4 t0 = t[0], t1 = t[1];
     P.ids[0] = t0.x, P.ids[1] = t0.y, P.ids[2] = t0.z, P.ids[3] = t0.w;
-    if constexpr (I16) {
-        uint4 t2 = t[2];
-        P.ids[4] = t1.x, P.ids[5] = t1.y, P.ids[6] = t1.z, P.ids[7] = t1.w;
-        P.ws[0] = t2.x, P.ws[1] = t2.y, P.ws[2] = t2.z, P.ws[3] = t2.w;
-    } else {
-        P.ws[0] = t1.x, P.ws[1] = t1.y, P.ws[2] = t1.z, P.ws[3] = t1.w;
-    }
+    P.ws[0] = t1.x, P.ws[1] = t1.y, P.ws[2] = t1.z, P.ws[3] = t1.w;
 #pragma unroll
     for (int s2 = 0; s2 < kPropTop; ++s2) P.T += byte_of(P.ws, s2);
 }
@@ -1111,25 +1100,43 @@ struct ClusterLds {
     const uint32_t* offsets;
     const uint32_t* lights;
     uint32_t count;
+    uint32_t group_log2;  // clusters per table group: 1 << group_log2
 };
 
+// the first and one-past-last position of group g's lights (g < groups)
+__device__ __forceinline__ uint2 group_range(const ClusterLds& C, uint32_t g) {
+    const uint32_t c0 = g << C.group_log2;
+    uint32_t c1 = (g + 1u) << C.group_log2;
+    if (c1 > C.count) c1 = C.count;
+    return make_uint2(C.offsets[c0], C.offsets[c1]);
+}
+
+__device__ __forceinline__ uint32_t group_count(const ClusterLds& C) {
+    return (C.count + (1u << C.group_log2) - 1u) >> C.group_log2;
+}
+
 // T_bad of the pixel's table: the weight of slots whose id is past the
-// clusters or names an empty cluster (draw_learned takes a uniform light
-// for them)
-template <bool I16>
-__device__ __forceinline__ void prop_fallback_weight(PropTable<I16>& P, const ClusterLds& C) {
+// groups or names an empty group (draw_learned takes a uniform light for
+// them)
+__device__ __forceinline__ void prop_fallback_weight(PropTable& P, const ClusterLds& C) {
     uint32_t bad = 0u;
+    const uint32_t groups = group_count(C);
 #pragma unroll
     for (int s = 0; s < kPropTop; ++s) {
-        const uint32_t w = byte_of(P.ws, s), c = id_of(P, s);
-        if (w != 0u && (c >= C.count || C.offsets[c + 1u] == C.offsets[c])) bad += w;
+        const uint32_t w = byte_of(P.ws, s), g = id_of(P, s);
+        if (w != 0u) {
+            bool empty = g >= groups;
+            if (!empty) {
+                const uint2 r = group_range(C, g);
+                empty = r.x == r.y;
+            }
+            if (empty) bad += w;
+        }
     }
     P.T_bad = bad;
 }
 
-template <bool I16>
-__device__ __forceinline__ uint32_t draw_learned(uint32_t& seed, const PropTable<I16>& P, const ClusterLds& C,
-                                                 uint32_t N) {
+__device__ __forceinline__ uint32_t draw_learned(uint32_t& seed, const PropTable& P, const ClusterLds& C, uint32_t N) {
     float a = noise_next(seed);
     float b = noise_next(seed);
     if (P.T == 0u) return light_index_from(a, N);
@@ -1137,42 +1144,42 @@ __device__ __forceinline__ uint32_t draw_learned(uint32_t& seed, const PropTable
     float u = (a + a) - 1.0f;
     uint32_t t = (uint32_t)(u * (float)P.T);
     if (t > P.T - 1u) t = P.T - 1u;
-    uint32_t acc = 0, c = 0;
+    uint32_t acc = 0, g = 0;
     bool found = false;
 #pragma unroll
     for (int s = 0; s < kPropTop; ++s) {
         uint32_t w = byte_of(P.ws, s);
         if (!found && t < acc + w) {
-            c = id_of(P, s);
+            g = id_of(P, s);
             found = true;
         }
         acc += w;
     }
-    // a cluster id outside the clusters or an empty cluster: uniform
-    if (c >= C.count) return light_index_from(b, N);
-    uint32_t off = C.offsets[c], size = C.offsets[c + 1] - off;
+    // a group id outside the groups or an empty group: uniform
+    if (g >= group_count(C)) return light_index_from(b, N);
+    const uint2 r = group_range(C, g);
+    const uint32_t size = r.y - r.x;
     if (size == 0u) return light_index_from(b, N);
     uint32_t j = (uint32_t)(b * (float)size);
     if (j > size - 1u) j = size - 1u;
-    return C.lights[off + j];
+    return C.lights[r.x + j];
 }
 
-template <bool X, bool I16>
-__device__ __forceinline__ float learned_density(uint32_t idx, const PropTable<I16>& P, const ClusterLds& C,
-                                                 uint32_t N) {
+template <bool X>
+__device__ __forceinline__ float learned_density(uint32_t idx, const PropTable& P, const ClusterLds& C, uint32_t N) {
     float q = 0.5f / (float)N;
     if (P.T == 0u) return 1.0f / (float)N;
     if (P.T_bad != 0u) q = q + div_<X>(0.5f * (float)P.T_bad, (float)P.T * (float)N);
-    uint32_t c = C.of_light[idx];
-    uint32_t wc = 0;
+    const uint32_t g = (uint32_t)C.of_light[idx] >> C.group_log2;
+    uint32_t wg = 0;
 #pragma unroll
     for (int s = 0; s < kPropTop; ++s) {
         uint32_t w = byte_of(P.ws, s);
-        if (w != 0u && id_of(P, s) == c) wc += w;
+        if (w != 0u && id_of(P, s) == g) wg += w;
     }
-    if (wc != 0u) {
-        uint32_t size = C.offsets[c + 1] - C.offsets[c];
-        q = q + div_<X>(0.5f * (float)wc, (float)P.T * (float)size);
+    if (wg != 0u) {
+        const uint2 r = group_range(C, g);
+        q = q + div_<X>(0.5f * (float)wg, (float)P.T * (float)(r.y - r.x));
     }
     return q;
 }
@@ -1324,10 +1331,10 @@ __device__ __forceinline__ uint32_t cache_key(const VisCacheView& C, f3 p, f3 n)
 // when every sphere is staged.
 template <bool X, bool DEBUG, int LMODE>
 __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, float4* sh_spheres) {
-    // LMODE: 0 plain, 1 learned (1-byte table ids), 2 learned (2-byte ids)
+    // LMODE: 0 plain, 1 learned
     constexpr bool learned = LMODE != 0;
-    constexpr bool I16 = LMODE == 2;
-    ClusterLds cl{A.clusters.of_light, A.clusters.offsets, A.clusters.lights, A.clusters.count};
+    ClusterLds cl{A.clusters.of_light, A.clusters.offsets, A.clusters.lights, A.clusters.count,
+                  A.proposal_group_log2};
     const uint32_t n_lds = A.scene.light_count < A.lds_lights ? A.scene.light_count : A.lds_lights;
     const bool all_lds = n_lds == A.scene.light_count;
     if (n_lds > 0u) {
@@ -1344,7 +1351,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                     c[k] = A.clusters.of_light[k];
                 }
                 for (uint32_t k = threadIdx.x; k <= A.clusters.count; k += blockDim.x) o[k] = A.clusters.offsets[k];
-                cl = ClusterLds{c, o, l, A.clusters.count};
+                cl = ClusterLds{c, o, l, A.clusters.count, A.proposal_group_log2};
             }
         }
         __syncthreads();
@@ -1432,13 +1439,10 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
         Light L;
         // the pixel's proposal table, loaded per sample so that it is not
         // live across the final sample
-        PropTable<I16> pt;
-        load_prop_table<I16>(learned && shade && A.proposal
-                                 ? A.proposal + (size_t)pixel * (I16 ? kPropBytes16 : kPropBytes)
-                                 : nullptr,
-                             pt);
+        PropTable pt;
+        load_prop_table(learned && shade && A.proposal ? A.proposal + (size_t)pixel * kPropBytes : nullptr, pt);
         if constexpr (learned) {
-            if (pt.T != 0u) prop_fallback_weight<I16>(pt, cl);
+            if (pt.T != 0u) prop_fallback_weight(pt, cl);
         }
         if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
             if (shade) chosen = (int32_t)light_index_from(noise_next(seed), N);
@@ -1560,17 +1564,16 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                     const bool have = my_k != 0xFFFFFFFFu;
                     const uint32_t live_o = (uint32_t)__shfl((int)live, (int)my_o);
                     const uint32_t seed_o = (uint32_t)__shfl((int)seed_chunk, (int)my_o);
-                    PropTable<I16> pt_o;
+                    PropTable pt_o;
                     pt_o.T = 0u;
                     pt_o.T_bad = 0u;
 #pragma unroll
-                    for (int k = 0; k < PropTable<I16>::kIdWords; ++k) pt_o.ids[k] = 0u;
+                    for (int k = 0; k < 4; ++k) pt_o.ids[k] = 0u;
 #pragma unroll
                     for (int k = 0; k < 4; ++k) pt_o.ws[k] = 0u;
                     if constexpr (learned) {
 #pragma unroll
-                        for (int k = 0; k < PropTable<I16>::kIdWords; ++k)
-                            pt_o.ids[k] = (uint32_t)__shfl((int)pt.ids[k], (int)my_o);
+                        for (int k = 0; k < 4; ++k) pt_o.ids[k] = (uint32_t)__shfl((int)pt.ids[k], (int)my_o);
 #pragma unroll
                         for (int k = 0; k < 4; ++k) pt_o.ws[k] = (uint32_t)__shfl((int)pt.ws[k], (int)my_o);
                         pt_o.T = (uint32_t)__shfl((int)pt.T, (int)my_o);
@@ -2142,26 +2145,22 @@ hipError_t launch_shade(const ShadeArgs& a, bool exact, hipStream_t s) {
     dim3 grid(shade_tiles(a)), block(256);
     bool dbg = a.debug_samples != nullptr || a.debug_cands != nullptr;
     size_t lds = shade_lds_bytes(a);
-    const int mode = a.learned_pass ? (a.proposal_ids16 ? 2 : 1) : 0;
+    const int mode = a.learned_pass ? 1 : 0;
 #define RISLTC_SHADE_LAUNCH(XX, DD, MM) hipLaunchKernelGGL((shade_kernel<XX, DD, MM>), grid, block, lds, s, a)
     if (exact) {
         if (dbg) {
-            if (mode == 2) RISLTC_SHADE_LAUNCH(true, true, 2);
-            else if (mode == 1) RISLTC_SHADE_LAUNCH(true, true, 1);
+            if (mode == 1) RISLTC_SHADE_LAUNCH(true, true, 1);
             else RISLTC_SHADE_LAUNCH(true, true, 0);
         } else {
-            if (mode == 2) RISLTC_SHADE_LAUNCH(true, false, 2);
-            else if (mode == 1) RISLTC_SHADE_LAUNCH(true, false, 1);
+            if (mode == 1) RISLTC_SHADE_LAUNCH(true, false, 1);
             else RISLTC_SHADE_LAUNCH(true, false, 0);
         }
     } else {
         if (dbg) {
-            if (mode == 2) RISLTC_SHADE_LAUNCH(false, true, 2);
-            else if (mode == 1) RISLTC_SHADE_LAUNCH(false, true, 1);
+            if (mode == 1) RISLTC_SHADE_LAUNCH(false, true, 1);
             else RISLTC_SHADE_LAUNCH(false, true, 0);
         } else {
-            if (mode == 2) RISLTC_SHADE_LAUNCH(false, false, 2);
-            else if (mode == 1) RISLTC_SHADE_LAUNCH(false, false, 1);
+            if (mode == 1) RISLTC_SHADE_LAUNCH(false, false, 1);
             else RISLTC_SHADE_LAUNCH(false, false, 0);
         }
     }
@@ -2195,12 +2194,10 @@ hipError_t launch_shade_batch(const ShadeArgs* d_views, const uint32_t* d_tile_o
 #define RISLTC_BATCH_LAUNCH(XX, MM) \
     hipLaunchKernelGGL((shade_batch_kernel<XX, MM>), grid, block, lds, s, d_views, d_tile_offsets, count)
     if (exact) {
-        if (mode == 2) RISLTC_BATCH_LAUNCH(true, 2);
-        else if (mode == 1) RISLTC_BATCH_LAUNCH(true, 1);
+        if (mode == 1) RISLTC_BATCH_LAUNCH(true, 1);
         else RISLTC_BATCH_LAUNCH(true, 0);
     } else {
-        if (mode == 2) RISLTC_BATCH_LAUNCH(false, 2);
-        else if (mode == 1) RISLTC_BATCH_LAUNCH(false, 1);
+        if (mode == 1) RISLTC_BATCH_LAUNCH(false, 1);
         else RISLTC_BATCH_LAUNCH(false, 0);
     }
 #undef RISLTC_BATCH_LAUNCH

@@ -297,8 +297,8 @@ def test_learned_batch_equals_single_views(small_scene, ltc_table, K, width, out
     mode) in one batch and in a graph: each view's frame equals its
     risltc_shade call from the same cache state bit for bit, and the batch
     adds to the shared visibility cache exactly the counts of the single
-    calls together. Byte ids with a per-light cache; 2-byte ids (1024
-    outputs) with a per-cluster cache."""
+    calls together. Tables of clusters with a per-light cache; tables of
+    quads of clusters (1024 outputs) with a per-cluster cache."""
     import risltc_amd
     import torch
     sc = small_scene

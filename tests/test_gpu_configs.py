@@ -140,7 +140,7 @@ def _learned_checks(sc, ltc_table, clusters, width, outputs, per_light, M, train
     finally:
         ctx.close()
     oracle = orc.Oracle(sc, *ltc_table)
-    oracle.set_proposal(table_h, col, clusters)
+    oracle.set_proposal(table_h, col, clusters, abi.proposal_group_log2(outputs))
     want = oracle.set_cache(counts, cell, prior, per_light)
     assert np.array_equal(bytes_dev, want)
     assert (want != 255).mean() > 0.001  # the training frames filled cache entries
@@ -148,8 +148,8 @@ def _learned_checks(sc, ltc_table, clusters, width, outputs, per_light, M, train
     bad = np.flatnonzero(np.any(got.view(np.uint32) != ref.view(np.uint32), axis=1))
     assert bad.size == 0, (bad.size, pix[bad[:5]])
     assert (got[:, :3].sum(1) > 0).mean() > 0.1
-    ids_w = table_h[pix, 32 if outputs > 256 else 16:]
-    assert (ids_w.sum(1) > 0).mean() > 0.9  # the tables are in use
+    weights = table_h[pix, 16:32].astype(np.int64)
+    assert (weights.sum(1) > 0).mean() > 0.9  # the tables are in use
 
 
 def test_learned_configuration_city_256_clusters(ltc_table):

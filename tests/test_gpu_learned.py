@@ -102,7 +102,7 @@ def learned_wide(small_scene, ltc_table):
 @pytest.fixture(scope="module")
 def learned_k1024(ltc_table):
     """1024 clusters of 1500 lights and the 96-128-128-128-1024 network
-    (tables with 2-byte cluster ids)."""
+    (tables of quads of clusters)."""
     sc = synth.small_scene(seed=6, width=96, height=64, light_count=1500)
     return _learned_context(sc, ltc_table, width=128, outputs=1024, K=1024)
 
@@ -124,7 +124,7 @@ def _learned_context(small_scene, ltc_table, width=64, outputs=256, K=16):
     P = small_scene.width * small_scene.height
     vis = torch.empty(P, dtype=torch.int32, device="cuda")
     ctx.render_visibility(consts, vis)
-    tb = 48 if outputs > 256 else 32
+    tb = abi.proposal_table_bytes(outputs)
     table = torch.zeros(P * tb, dtype=torch.uint8, device="cuda")
     logits = torch.zeros(P * outputs, dtype=torch.float32, device="cuda")
     feats = torch.zeros(P * 96, dtype=torch.int16, device="cuda")
@@ -243,9 +243,8 @@ def test_features_match_numpy_restatement(learned):
 
 
 def table_slots(table, outputs):
-    """(ids, weights) of per-pixel tables: pixels x 16 each."""
-    if outputs > 256:
-        return table[:, :32].copy().view(np.uint16).astype(np.int64), table[:, 32:48].astype(np.int64)
+    """(ids, weights) of per-pixel tables: pixels x 16 each (ids of
+    clusters, or of quads of clusters for more than 256 outputs)."""
     return table[:, :16].astype(np.int64), table[:, 16:32].astype(np.int64)
 
 
@@ -288,7 +287,26 @@ def test_wide_tables_are_top4_per_lane_group(learned_wide):
     _check_tables(learned_wide)
 
 
+def group_values(z, sizes, outputs):
+    """Per table group: (value, lane group, live) -- a cluster's logit for
+    256 outputs; for 1024, log(sum exp z) over the non-empty clusters of the
+    quad 4 q .. 4 q + 3 (the value the proposal kernel ranks)."""
+    if outputs <= 256:
+        ids = np.arange(outputs)
+        return z, (ids % 16) // 4, sizes > 0
+    zq = z.reshape(-1, 4)
+    live = sizes.reshape(-1, 4) > 0
+    m = np.where(live, zq, -np.inf).max(1)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        v = m + np.log(np.where(live, np.exp(zq - m[:, None]), 0.0).sum(1))
+    ids = np.arange(outputs // 4)
+    return np.where(live.any(1), v, -np.inf), ids % 4, live.any(1)
+
+
 def _check_tables(learned):
+    """Per lane group, the four largest group values (descending, ties by
+    the smaller id) at weight round(255 exp(v - v_max)); value differences
+    below 1e-5 may order either way (the kernel's __expf/__logf)."""
     vis = learned["vis"].cpu().numpy().view(np.uint32)
     shaded = np.flatnonzero((vis >> 31) == 0)
     O = learned["outputs"]
@@ -296,21 +314,21 @@ def _check_tables(learned):
     ids, ws = table_slots(learned["table"], O)
     rng = np.random.default_rng(0)
     for p in rng.choice(shaded, 300, replace=False):
-        z = learned["logits"][p].astype(np.float64)
-        zmax = max(z[c] for c in range(O) if sizes[c] > 0)
+        v, lane_group, live = group_values(learned["logits"][p].astype(np.float64), sizes, O)
+        vmax = v[live].max()
         for g in range(4):
-            cand = [c for c in range(O) if (c % 16) // 4 == g and sizes[c] > 0]
-            # descending value, ties by smaller id
-            best = sorted(cand, key=lambda c: (-z[c], c))[:4]
+            cand = np.flatnonzero((lane_group == g) & live)
+            best = sorted(cand, key=lambda c: (-v[c], c))[:4]
             for s in range(4):
                 c = best[s] if s < len(best) else 0
-                e = math.exp(z[c] - zmax) * 255.0 + 0.5
+                e = math.exp(v[c] - vmax) * 255.0 + 0.5
                 w = min(255, int(e)) if s < len(best) else 0
                 gw = int(ws[p, 4 * g + s])
                 # __expf is within a few ulps: the rounding may differ by one near .5
                 assert abs(gw - w) <= (1 if abs(e - round(e)) < 1e-3 * e + 1e-3 else 0), (p, g, s, gw, w)
-                if gw > 0:
-                    assert ids[p, 4 * g + s] == c
+                if gw > 0 and ids[p, 4 * g + s] != c:
+                    # a near-tie ordered the other way
+                    assert abs(v[ids[p, 4 * g + s]] - v[c]) < 1e-5, (p, g, s, ids[p, 4 * g + s], c)
     assert np.all(ws[(vis >> 31) != 0] == 0)
 
 
@@ -338,7 +356,7 @@ def _check_learned_bitwise(learned):
     g["ctx"].shade_debug(g["consts"], s, g["vis"], out, rec, cand)
     torch.cuda.synchronize()
     oracle = orc.Oracle(sc, *g["ltc"])
-    oracle.set_proposal(g["table"], g["col"], g["K"])
+    oracle.set_proposal(g["table"], g["col"], g["K"], abi.proposal_group_log2(g["outputs"]))
     vis = g["vis"].cpu().numpy().view(np.uint32)
     ref, rrec, rcand = oracle.shade(g["consts"], s, vis, records=True, candidates=True)
     c = cand.cpu().numpy().view(np.uint32).reshape(rcand.shape)
@@ -467,7 +485,7 @@ def test_cached_learned_shading_bitwise_vs_oracle(learned, learned_odd, per_ligh
     finally:
         ctx.proposal_cache(0)
     oracle = orc.Oracle(sc, *g["ltc"])
-    oracle.set_proposal(g["table"], g["col"], g["K"])
+    oracle.set_proposal(g["table"], g["col"], g["K"], abi.proposal_group_log2(g["outputs"]))
     want = oracle.set_cache(counts, cell, prior, per_light)
     # the refresh before shading made every byte from the written counts
     assert np.array_equal(table, want)
@@ -608,20 +626,36 @@ def test_wide_learned_pass_runs_and_trains(learned_wide):
 
 
 def test_k1024_forward_tables_and_training(learned_k1024):
-    """The 1024-output network: logits, 2-byte-id tables and one training
-    step's gradient against numpy."""
+    """The 1024-output network: logits, tables of quads of clusters (log sum
+    exp of the quad's logits) and one training step's gradient against
+    numpy."""
     g = learned_k1024
     assert g["ctx"].proposal_param_count() == layer_bounds(128, 1024)[-1][1]
     _check_forward(g)
     _check_tables(g)
-    ids, _ = table_slots(g["table"], 1024)
-    assert ids.max() >= 256  # ids beyond one byte occur
+    ids, ws = table_slots(g["table"], 1024)
+    assert ids[ws > 0].max() < 256 and ids[ws > 0].max() >= 64
     _check_training_step(g)
 
 
+def test_k1024_clusters_form_compact_quads(learned_k1024):
+    """More than 256 clusters come in two levels: each quad of consecutive
+    ids is one top-level k-means cluster, so the quads are a Lloyd fixed
+    point of the lights' centroids (every light's nearest quad mean is its
+    own quad) and every quad has lights."""
+    sc, col = learned_k1024["scene"], learned_k1024["col"].astype(np.int64)
+    cen = np.array([np.asarray(L.world, np.float64)[:L.vertex_count].mean(0) for L in sc.lights])
+    quad = col // 4
+    Q = 256
+    assert np.array_equal(np.unique(quad), np.arange(Q))
+    means = np.stack([cen[quad == q].mean(0) for q in range(Q)])
+    d = ((cen[:, None, :] - means[None]) ** 2).sum(2)
+    assert (d.argmin(1) == quad).mean() > 0.95
+
+
 def test_k1024_learned_shading_bitwise_vs_oracle(learned_k1024):
-    """The learned pass reading 2-byte cluster ids: candidates, records and
-    frame bitwise against the oracle."""
+    """The learned pass drawing from quads of clusters: candidates, records
+    and frame bitwise against the oracle."""
     g = learned_k1024
     g["ctx"].proposal_update(g["consts"], g["vis"])
     _check_learned_bitwise(g)
@@ -645,8 +679,8 @@ def test_k1024_compacted_equals_debug(learned_k1024, exact):
 
 
 def test_k1024_narrow_network_with_dead_outputs(ltc_table):
-    """The 96-64-64-64-1024 network over 600 clusters (outputs past the
-    cluster count are never kept): logits, 2-byte-id tables, and learned
+    """The 96-64-64-64-1024 network over 600 clusters (150 quads; outputs
+    past the cluster count are never kept): logits, tables, and learned
     shading bitwise against the oracle."""
     sc = synth.small_scene(seed=8, width=80, height=48, light_count=1200)
     g = _learned_context(sc, ltc_table, width=64, outputs=1024, K=600)
@@ -654,7 +688,7 @@ def test_k1024_narrow_network_with_dead_outputs(ltc_table):
         _check_forward(g)
         _check_tables(g)
         ids, ws = table_slots(g["table"], 1024)
-        assert ids[ws > 0].max() < 600 and ids.max() >= 256
+        assert ids[ws > 0].max() < 150
         _check_learned_bitwise(g)
     finally:
         g["ctx"].close()

@@ -152,17 +152,18 @@ def _random_tables(rng, P, K, sizes, bad=False):
 
 
 def _q_numpy(tab, idx, col, sizes, N):
-    """q = 1/(2N) + T_bad/(2 T N) + w_c / (2 T |c|) in float32 with the
-    kernel's operation order: T_bad is the weight of slots the draw sends to
-    a uniform light (ids past the clusters, empty clusters), w_c the summed
-    weight of the slots naming the light's cluster."""
+    """q = 1/(2N) + T_bad/(2 T N) + w_g / (2 T |g|) in float32 with the
+    kernel's operation order: table ids name groups (col: the group of each
+    light, sizes: lights per group), T_bad is the weight of slots the draw
+    sends to a uniform light (ids past the groups, empty groups), w_g the
+    summed weight of the slots naming the light's group."""
     f = np.float32
     K = sizes.size
     T = int(tab[16:].astype(np.int64).sum())
     if T == 0:
         return f(1.0) / f(N)
     q = f(0.5) / f(N)
-    bad = sum(int(tab[16 + s]) for s in range(16) if tab[s] >= K or sizes[tab[s]] == 0)
+    bad = sum(int(tab[16 + s]) for s in range(16) if tab[16 + s] != 0 and (tab[s] >= K or sizes[tab[s]] == 0))
     if bad:
         q = q + (f(0.5) * f(bad)) / (f(T) * f(N))
     c = col[idx]
@@ -172,26 +173,34 @@ def _q_numpy(tab, idx, col, sizes, N):
     return q
 
 
-def _source_oracle(scene, ltc_table, rng, K=16, bvh=True, bad=False):
+def _source_oracle(scene, ltc_table, rng, K=16, bvh=True, bad=False, group_log2=0):
+    """An oracle with the learned source: K clusters (lights i -> 7 i mod K;
+    with bad tables the last cluster is empty), tables of random weights over
+    the groups of 1 << group_log2 consecutive clusters. Returns the oracle,
+    the tables, the group of each light and the lights per group."""
     o = orc.Oracle(scene, *ltc_table, bvh=bvh)
     N = scene.light_count
-    # with bad tables the last cluster is left empty
     col = (np.arange(N) * 7 % (K - 1 if bad else K)).astype(np.uint32)
-    sizes = np.bincount(col, minlength=K)
+    groups = (K + (1 << group_log2) - 1) >> group_log2
+    gcol = col >> group_log2
+    sizes = np.bincount(gcol, minlength=groups)
     P = scene.width * scene.height
-    tab = _random_tables(rng, P, K, sizes, bad)
-    o.set_proposal(tab, col, K)
-    return o, tab, col, sizes
+    tab = _random_tables(rng, P, groups, sizes, bad)
+    o.set_proposal(tab, col, K, group_log2)
+    return o, tab, gcol, sizes
 
 
-@pytest.mark.parametrize("bad", [False, True], ids=["kernel_tables", "bad_slots"])
-def test_learned_draw_density(small_scene, ltc_table, bad):
+@pytest.mark.parametrize("bad,group_log2,K", [(False, 0, 16), (True, 0, 16), (False, 2, 32), (True, 2, 30)],
+                         ids=["kernel_tables", "bad_slots", "quads", "quads_bad_slots"])
+def test_learned_draw_density(small_scene, ltc_table, bad, group_log2, K):
     """With the learned source: every candidate's weight is exactly p_hat / q
     with q from the table (a pdf over the lights), and the candidate lights
     are distributed as q (chi-square). Also for tables with slots of empty
-    clusters, ids past the clusters and repeated clusters."""
+    clusters, ids past the clusters and repeated clusters, and for tables
+    whose slots name quads of clusters (1024-output networks; 30 clusters:
+    the last quad has two)."""
     rng = np.random.default_rng(31)
-    o, tab, col, sizes = _source_oracle(small_scene, ltc_table, rng, bad=bad)
+    o, tab, col, sizes = _source_oracle(small_scene, ltc_table, rng, K=K, bad=bad, group_log2=group_log2)
     N = small_scene.light_count
     consts = o.frame_constants()
     vis = o.render_visibility(consts)
@@ -224,15 +233,16 @@ def test_learned_target_factor_constant():
     assert f + np.float32(0) * scale == f
 
 
-@pytest.mark.parametrize("bad", [False, True], ids=["kernel_tables", "bad_slots"])
-def test_learned_estimator_is_unbiased(orc_lib, ltc_table, bad):
+@pytest.mark.parametrize("bad,group_log2,K", [(False, 0, 6), (True, 0, 6), (True, 2, 10)],
+                         ids=["kernel_tables", "bad_slots", "quads_bad_slots"])
+def test_learned_estimator_is_unbiased(orc_lib, ltc_table, bad, group_log2, K):
     """Mean of the learned-source estimator over many noise seeds equals the
     plain (uniform-source) mean on the same pixels, shadows off; also with
     tables that name empty clusters, ids past the clusters and repeated
-    clusters."""
+    clusters, and with tables of quads of clusters."""
     scene = synth.small_scene(seed=3, width=48, height=48, light_count=24)
     rng = np.random.default_rng(32)
-    o, _, _, _ = _source_oracle(scene, ltc_table, rng, K=6, bvh=False, bad=bad)
+    o, _, _, _ = _source_oracle(scene, ltc_table, rng, K=K, bvh=False, bad=bad, group_log2=group_log2)
     consts = o.frame_constants()
     o2 = orc.Oracle(scene, *ltc_table)
     vis = o2.render_visibility(consts)
