@@ -108,3 +108,43 @@ def test_more_lights_than_lds_staging(ltc_table):
         assert (got[shaded, :3].sum(1) > 0).mean() > 0.2
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+def test_non_finite_samples_are_pink(ltc_table, exact):
+    """A light of infinite radiance makes the samples whose candidates see it
+    infinite or NaN; such pixels are written as (1, 0, 0.8) / exposure
+    before the exposure (shading_pass.frag.glsl:764-767). Exact build: the
+    frame equals the oracle's bit for bit. Fast build: the pink pixels are
+    those of the oracle except on at most 0.1% of pixels (a noise stream that
+    splits at a specular PSA within 1e-6 of zero draws other candidates, and
+    a candidate whose LTC integral is zero in one build and a denormal in the
+    other turns inf * 0 into inf * x), and every pixel is finite."""
+    import copy
+    sc = copy.copy(synth.small_scene(seed=0, width=96, height=64))
+    recs = bytearray(sc.light_records)
+    stride = 48 + 16 * sc.light_vmax
+    for k in range(0, sc.light_count, 9):  # every 9th light: radiance (inf, inf, inf)
+        recs[k * stride:k * stride + 12] = np.full(3, np.inf, np.float32).tobytes()
+    sc.light_records = bytes(recs)
+    ctx, oracle = _setup(sc, ltc_table)
+    try:
+        consts = _constants(ctx, sc, sc.width, sc.height)
+        s = sc.settings(exact_math=exact, sample_count_light=2, ris_candidate_count=8)
+        vis = oracle.render_visibility(consts)
+        got = _shade(ctx, consts, s, vis)
+        ref, _, _ = oracle.shade(consts, s, vis)
+        e = np.float32(consts.exposure_factor)
+        pink = np.array([np.float32(1.0) / e * e, np.float32(0.0) / e * e, np.float32(0.8) / e * e, 1.0],
+                        np.float32)
+        is_pink = np.all(got.view(np.uint32) == pink.view(np.uint32), axis=1)
+        ref_pink = np.all(ref.view(np.uint32) == pink.view(np.uint32), axis=1)
+        assert ref_pink.sum() > 50 and is_pink.sum() > 50
+        assert np.isfinite(got).all()
+        if exact:
+            assert np.array_equal(is_pink, ref_pink)
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+        else:
+            assert (is_pink != ref_pink).mean() <= 1e-3, (is_pink.sum(), ref_pink.sum())
+    finally:
+        ctx.close()
