@@ -190,8 +190,15 @@ typedef struct risltc_material_s {
 /* Render settings. The fields mirror render_settings_t (src/main.h:123-156)
  * and the shader defines built from it (src/main.c:962-991); what the
  * reference fixes at compile time is a runtime value here. */
+/* SAMPLE_COUNT: samples per technique of every light evaluation that
+ * samples (the final sample of every technique, the candidates of the area
+ * and projected-solid-angle techniques), averaged
+ * (shading_pass.frag.glsl:365-396, 412-417, 518-559). Each pixel-sample
+ * holds SAMPLE_COUNT records and up to 2 * SAMPLE_COUNT shadow rays; a call
+ * holds at most 2^30 records (pixels x spp x SAMPLE_COUNT). */
+#define RISLTC_MAX_SAMPLE_COUNT 1024u
 typedef struct risltc_settings_s {
-    uint32_t sample_count;               /* SAMPLE_COUNT (must be 1 on the GPU)   */
+    uint32_t sample_count;               /* SAMPLE_COUNT, 1..RISLTC_MAX_SAMPLE_COUNT */
     uint32_t sample_count_light;         /* LIGHT_SAMPLES (samples per pixel)     */
     uint32_t mis_heuristic;              /* RISLTC_MIS_*                          */
     uint32_t light_sampling;             /* RISLTC_LIGHT_*                        */
@@ -422,8 +429,8 @@ int risltc_shade(risltc_context_t* context, const risltc_frame_constants_t* cons
  * (code under stream capture, tight serving loops, hosts that forbid
  * hipMalloc). The sample records, shadow-ray list, hit words and counters
  * live in one fixed device array that the library defines, sized for at
- * most RISLTC_NOALLOC_MAX_SAMPLES pixel-samples (width * height * spp;
- * about 72 MiB, held by every GPU the library runs on until the process
+ * most RISLTC_NOALLOC_MAX_SAMPLES sample records (width * height * spp *
+ * sample_count; about 72 MiB, held by every GPU the library runs on until the process
  * ends). Larger calls fail with RISLTC_E_UNSUPPORTED. The output equals
  * risltc_shade's for the same inputs; no training records for
  * risltc_proposal_train are written. Lights, LTC table, scene, clusters,
@@ -455,7 +462,9 @@ int risltc_trace_shadow_rays(risltc_context_t* context, const float* rays, uint3
  * words at sample_records[(p * spp + j) * RISLTC_SAMPLE_RECORD_FLOATS]
  * (layout: risltc_sample_record_t) and, when candidate_records is not NULL,
  * M records of RISLTC_CANDIDATE_RECORD_WORDS words per pixel sample
- * (risltc_candidate_record_t). Device pointers. */
+ * (risltc_candidate_record_t). Device pointers. With sample_count > 1 the
+ * ray, state and estimate fields describe the first iteration of the
+ * SAMPLE_COUNT loop; color and noise_seed_end cover the whole loop. */
 #define RISLTC_SAMPLE_RECORD_FLOATS 40
 #define RISLTC_CANDIDATE_RECORD_WORDS 4
 typedef struct risltc_sample_record_s {
@@ -473,7 +482,8 @@ typedef struct risltc_sample_record_s {
                                 3 traced miss, 4 untraced (shadow rays off)  */
     float estimate_visible[2][3];  /* MIS estimate if visible                */
     float estimate_occluded[2][3]; /* MIS estimate if occluded               */
-    float color[3];          /* evaluate_polygonal_light_shading_peters()    */
+    float color[3];          /* the light's estimate: shading_pass.frag.glsl
+                                evaluate_polygonal_light_shading[_peters]()  */
     uint32_t noise_seed_end; /* LCG state after this sample                  */
     float position[3];       /* shading position (shadow ray origin)         */
     uint32_t cache_key;      /* visibility-cache cell of the pixel, 0 without */

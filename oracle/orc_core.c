@@ -781,8 +781,10 @@ static V2 noise2(uint32_t* seed) {
 }
 
 /* evaluate_polygonal_light_shading_peters (shading_pass.frag.glsl:292-397),
- * SAMPLE_COUNT = 1, with visibility through the CPU LBVH. rec (may be NULL)
- * receives the per-technique intermediate values.
+ * with visibility through the CPU LBVH: the polygons are prepared once and
+ * the SAMPLE_COUNT loop (365-396) draws one direction per technique per
+ * iteration; the sum is scaled by 1 / SAMPLE_COUNT. rec (may be NULL)
+ * receives the per-technique intermediate values of the first iteration.
  *
  * The same code is the SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE branch of
  * evaluate_polygonal_light_shading (shading_pass.frag.glsl:458-551): with
@@ -790,8 +792,7 @@ static V2 noise2(uint32_t* seed) {
  * integrand is 0, exactly as above; with trace = 0 (the RIS candidates,
  * dummy_vis = false) no ray is traced and a sample above the horizon keeps
  * its unshadowed integrand (get_polygon_radiance_brdf_product, 200-208).
- * Its final "result * (1.0f / SAMPLE_COUNT)" is a multiplication by 1.
- * clip_counts (may be NULL) receives vcd | vcs << 8. */
+ * Both end with "result * (1.0f / SAMPLE_COUNT)" (396, 559). clip_counts (may be NULL) receives vcd | vcs << 8. */
 static V3 eval_peters_ex(pix_t* px, const light_t* l, float* rec, int trace, uint32_t* clip_counts) {
     V3 result = v3(K(0.0), K(0.0), K(0.0));
     R side = plane_side(px->sd.position, l->plane);
@@ -839,69 +840,72 @@ static V3 eval_peters_ex(pix_t* px, const light_t* l, float* rec, int trace, uin
         dw = mul3(dw, rop);
         sw = mul3(sw, rop);
     }
-    V3 dirs[2];
-    dirs[0] = sample_psa(&pd, noise2(&px->seed));
-    dirs[1] = v3(K(0.0), K(0.0), K(0.0));
-    if (ps.psa > K(0.0)) {
-        dirs[1] = sample_psa(&ps, noise2(&px->seed));
-        dirs[1] = normalize3(mul33(&px->ltc.cosine_to_shading, dirs[1]));
-    }
     R ve = (R)px->c->mis_visibility_estimate;
-    for (uint32_t j = 0; j != techniques; ++j) {
-        V3 d = dirs[j];
-        if (d.z <= K(0.0)) continue;
-        R dd = d.z * rcp_d;
-        R sdn = ltc_density(&px->ltc, d, rcp_s);
-        V3 wdir = mul_transposed(w2s, d);
-        int lambert_ok = dot3(px->sd.normal, wdir) > K(0.0);
-        int vis = 0;
-        uint32_t state = 1;
-        R tmax = K(0.0);
-        if (lambert_ok) {
-            tmax = -plane_side(px->sd.position, l->plane) / dot3(wdir, v3(l->plane[0], l->plane[1], l->plane[2]));
-            tmax -= K(1e-3);
-            if (trace && px->s->shadow_rays && px->sc->bvh) {
-                float o[3] = {(float)px->sd.position.x, (float)px->sd.position.y, (float)px->sd.position.z};
-                float dir[3] = {(float)wdir.x, (float)wdir.y, (float)wdir.z};
-                int hit = orc_bvh_any_hit(px->sc->bvh, o, dir, 1.0e-3f, (float)tmax, NULL);
-                vis = !hit;
-                state = hit ? 2u : 3u;
+    const uint32_t iters = px->s->sample_count;
+    for (uint32_t it = 0; it != iters; ++it) {
+        V3 dirs[2];
+        dirs[0] = sample_psa(&pd, noise2(&px->seed));
+        dirs[1] = v3(K(0.0), K(0.0), K(0.0));
+        if (ps.psa > K(0.0)) {
+            dirs[1] = sample_psa(&ps, noise2(&px->seed));
+            dirs[1] = normalize3(mul33(&px->ltc.cosine_to_shading, dirs[1]));
+        }
+        for (uint32_t j = 0; j != techniques; ++j) {
+            V3 d = dirs[j];
+            if (d.z <= K(0.0)) continue;
+            R dd = d.z * rcp_d;
+            R sdn = ltc_density(&px->ltc, d, rcp_s);
+            V3 wdir = mul_transposed(w2s, d);
+            int lambert_ok = dot3(px->sd.normal, wdir) > K(0.0);
+            int vis = 0;
+            uint32_t state = 1;
+            R tmax = K(0.0);
+            if (lambert_ok) {
+                tmax = -plane_side(px->sd.position, l->plane) / dot3(wdir, v3(l->plane[0], l->plane[1], l->plane[2]));
+                tmax -= K(1e-3);
+                if (trace && px->s->shadow_rays && px->sc->bvh) {
+                    float o[3] = {(float)px->sd.position.x, (float)px->sd.position.y, (float)px->sd.position.z};
+                    float dir[3] = {(float)wdir.x, (float)wdir.y, (float)wdir.z};
+                    int hit = orc_bvh_any_hit(px->sc->bvh, o, dir, 1.0e-3f, (float)tmax, NULL);
+                    vis = !hit;
+                    state = hit ? 2u : 3u;
+                } else {
+                    vis = 1;
+                    state = 4u;
+                }
+            }
+            V3 rb_vis = mul3(l->radiance, evaluate_brdf(&px->sd, wdir));
+            V3 integ_vis = sclf3(d.z, rb_vis);
+            V3 integ_occ = sclf3(d.z, v3(K(0.0), K(0.0), K(0.0)));
+            V3 e_vis, e_occ;
+            if (j == 0 && ps.psa <= K(0.0)) {
+                R rd = K(1.0) / dd;
+                e_vis = scl3(integ_vis, rd);
+                e_occ = v3(K(0.0), K(0.0), K(0.0));
+            } else if (j == 0) {
+                e_vis = mis_estimate(heuristic, integ_vis, dw, dd, sw, sdn, ve);
+                e_occ = mis_estimate(heuristic, integ_occ, dw, dd, sw, sdn, ve);
             } else {
-                vis = 1;
-                state = 4u;
+                e_vis = mis_estimate(heuristic, integ_vis, sw, sdn, dw, dd, ve);
+                e_occ = mis_estimate(heuristic, integ_occ, sw, sdn, dw, dd, ve);
+            }
+            result = add3(result, vis ? e_vis : e_occ);
+            if (rec && it == 0) {
+                rec[ORC_REC_DIR + 3 * j + 0] = (float)wdir.x;
+                rec[ORC_REC_DIR + 3 * j + 1] = (float)wdir.y;
+                rec[ORC_REC_DIR + 3 * j + 2] = (float)wdir.z;
+                rec[ORC_REC_TMAX + j] = (float)tmax;
+                memcpy(&rec[ORC_REC_STATE + j], &state, 4);
+                rec[ORC_REC_EVIS + 3 * j + 0] = (float)e_vis.x;
+                rec[ORC_REC_EVIS + 3 * j + 1] = (float)e_vis.y;
+                rec[ORC_REC_EVIS + 3 * j + 2] = (float)e_vis.z;
+                rec[ORC_REC_EOCC + 3 * j + 0] = (float)e_occ.x;
+                rec[ORC_REC_EOCC + 3 * j + 1] = (float)e_occ.y;
+                rec[ORC_REC_EOCC + 3 * j + 2] = (float)e_occ.z;
             }
         }
-        V3 rb_vis = mul3(l->radiance, evaluate_brdf(&px->sd, wdir));
-        V3 integ_vis = sclf3(d.z, rb_vis);
-        V3 integ_occ = sclf3(d.z, v3(K(0.0), K(0.0), K(0.0)));
-        V3 e_vis, e_occ;
-        if (j == 0 && ps.psa <= K(0.0)) {
-            R rd = K(1.0) / dd;
-            e_vis = scl3(integ_vis, rd);
-            e_occ = v3(K(0.0), K(0.0), K(0.0));
-        } else if (j == 0) {
-            e_vis = mis_estimate(heuristic, integ_vis, dw, dd, sw, sdn, ve);
-            e_occ = mis_estimate(heuristic, integ_occ, dw, dd, sw, sdn, ve);
-        } else {
-            e_vis = mis_estimate(heuristic, integ_vis, sw, sdn, dw, dd, ve);
-            e_occ = mis_estimate(heuristic, integ_occ, sw, sdn, dw, dd, ve);
-        }
-        result = add3(result, vis ? e_vis : e_occ);
-        if (rec) {
-            rec[ORC_REC_DIR + 3 * j + 0] = (float)wdir.x;
-            rec[ORC_REC_DIR + 3 * j + 1] = (float)wdir.y;
-            rec[ORC_REC_DIR + 3 * j + 2] = (float)wdir.z;
-            rec[ORC_REC_TMAX + j] = (float)tmax;
-            memcpy(&rec[ORC_REC_STATE + j], &state, 4);
-            rec[ORC_REC_EVIS + 3 * j + 0] = (float)e_vis.x;
-            rec[ORC_REC_EVIS + 3 * j + 1] = (float)e_vis.y;
-            rec[ORC_REC_EVIS + 3 * j + 2] = (float)e_vis.z;
-            rec[ORC_REC_EOCC + 3 * j + 0] = (float)e_occ.x;
-            rec[ORC_REC_EOCC + 3 * j + 1] = (float)e_occ.y;
-            rec[ORC_REC_EOCC + 3 * j + 2] = (float)e_occ.z;
-        }
     }
-    return scl3(result, K(1.0) / K(1.0));
+    return scl3(result, K(1.0) / (R)iters);
 }
 
 static V3 eval_peters(pix_t* px, const light_t* l, float* rec) { return eval_peters_ex(px, l, rec, 1, NULL); }
@@ -949,42 +953,62 @@ static V3 eval_turk(const pix_t* px, const light_t* l, V3 sample, V3* dir, R* tm
     return (density > K(0.0)) ? scl3(rb, lambert / density) : v3(K(0.0), K(0.0), K(0.0));
 }
 
-/* Final sample of the Turk technique: the estimate at the stored point and
- * one shadow ray. Returns the estimate times visibility; rec as for
- * eval_peters (technique slot 0). */
-static V3 final_turk(pix_t* px, const light_t* l, V3 sample, float* rec) {
-    V3 dir;
-    R tmax;
-    int above;
-    V3 col = eval_turk(px, l, sample, &dir, &tmax, &above);
-    uint32_t state = 1;
-    int vis = 0;
-    if (above) {
-        if (px->s->shadow_rays && px->sc->bvh) {
-            float o[3] = {(float)px->sd.position.x, (float)px->sd.position.y, (float)px->sd.position.z};
-            float fd[3] = {(float)dir.x, (float)dir.y, (float)dir.z};
-            int hit = orc_bvh_any_hit(px->sc->bvh, o, fd, 1.0e-3f, (float)tmax, NULL);
-            vis = !hit;
-            state = hit ? 2u : 3u;
-        } else {
-            vis = 1;
-            state = 4u;
+/* Final sample of the Turk technique: the SAMPLE_COUNT loop of
+ * evaluate_polygonal_light_shading (shading_pass.frag.glsl:412-417) with
+ * visibility. draw = 1 (uniform light sampling, 714-720): every iteration
+ * draws its own point; draw = 0 (RIS, 743-756): every iteration evaluates
+ * the reservoir's point `sample`. The loop shares one visibility flag: the
+ * estimate counts only when no shadow ray is occluded (visibility, once
+ * false, stays false and the caller multiplies by it). SAMPLE_COUNT = 1:
+ * the estimate times the visibility of its ray, zero below the horizon.
+ * Returns that estimate; rec as for eval_peters (technique slot 0, the
+ * first iteration). */
+static V3 final_turk(pix_t* px, const light_t* l, V3 sample, float* rec, int draw) {
+    const uint32_t iters = px->s->sample_count;
+    V3 sum = v3(K(0.0), K(0.0), K(0.0)), first = sum;
+    int all_vis = 1, first_vis = 0;
+    for (uint32_t it = 0; it != iters; ++it) {
+        if (draw) sample = turk_point(l, noise2(&px->seed));
+        V3 dir;
+        R tmax;
+        int above;
+        V3 col = eval_turk(px, l, sample, &dir, &tmax, &above);
+        uint32_t state = 1;
+        int vis = 0;
+        if (above) {
+            if (px->s->shadow_rays && px->sc->bvh) {
+                float o[3] = {(float)px->sd.position.x, (float)px->sd.position.y, (float)px->sd.position.z};
+                float fd[3] = {(float)dir.x, (float)dir.y, (float)dir.z};
+                int hit = orc_bvh_any_hit(px->sc->bvh, o, fd, 1.0e-3f, (float)tmax, NULL);
+                vis = !hit;
+                state = hit ? 2u : 3u;
+            } else {
+                vis = 1;
+                state = 4u;
+            }
+            if (!vis) all_vis = 0;
         }
+        if (it == 0) {
+            first = col;
+            first_vis = vis;
+            if (rec) {
+                rec[ORC_REC_POS + 0] = (float)px->sd.position.x;
+                rec[ORC_REC_POS + 1] = (float)px->sd.position.y;
+                rec[ORC_REC_POS + 2] = (float)px->sd.position.z;
+                rec[ORC_REC_DIR + 0] = (float)dir.x;
+                rec[ORC_REC_DIR + 1] = (float)dir.y;
+                rec[ORC_REC_DIR + 2] = (float)dir.z;
+                rec[ORC_REC_TMAX] = (float)tmax;
+                memcpy(&rec[ORC_REC_STATE], &state, 4);
+                rec[ORC_REC_EVIS + 0] = (float)col.x;
+                rec[ORC_REC_EVIS + 1] = (float)col.y;
+                rec[ORC_REC_EVIS + 2] = (float)col.z;
+            }
+        }
+        sum = add3(sum, col);
     }
-    if (rec) {
-        rec[ORC_REC_POS + 0] = (float)px->sd.position.x;
-        rec[ORC_REC_POS + 1] = (float)px->sd.position.y;
-        rec[ORC_REC_POS + 2] = (float)px->sd.position.z;
-        rec[ORC_REC_DIR + 0] = (float)dir.x;
-        rec[ORC_REC_DIR + 1] = (float)dir.y;
-        rec[ORC_REC_DIR + 2] = (float)dir.z;
-        rec[ORC_REC_TMAX] = (float)tmax;
-        memcpy(&rec[ORC_REC_STATE], &state, 4);
-        rec[ORC_REC_EVIS + 0] = (float)col.x;
-        rec[ORC_REC_EVIS + 1] = (float)col.y;
-        rec[ORC_REC_EVIS + 2] = (float)col.z;
-    }
-    return vis ? col : v3(K(0.0), K(0.0), K(0.0));
+    if (iters == 1) return first_vis ? first : v3(K(0.0), K(0.0), K(0.0));
+    return all_vis ? scl3(sum, K(1.0) / (R)iters) : v3(K(0.0), K(0.0), K(0.0));
 }
 
 /* int(u * N) in float arithmetic in both builds (u is a float draw), so the
@@ -1196,8 +1220,7 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
                 load_light(sc, idx, &light);
                 V3 col;
                 if (tech == RISLTC_SAMPLE_POLYGON_AREA_TURK) {
-                    V3 point = turk_point(&light, noise2(&px.seed));
-                    col = final_turk(&px, &light, point, rec);
+                    col = final_turk(&px, &light, v3(K(0.0), K(0.0), K(0.0)), rec, 1);
                 } else {
                     col = eval_peters(&px, &light, rec);
                 }
@@ -1245,11 +1268,19 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
                     uint32_t cc = 0;
                     V3 col, point = v3(K(0.0), K(0.0), K(0.0));
                     if (tech == RISLTC_SAMPLE_POLYGON_AREA_TURK) {
-                        V3 dir;
-                        R tmax;
-                        int above;
-                        point = turk_point(&light, noise2(&px.seed));
-                        col = eval_turk(&px, &light, point, &dir, &tmax, &above);
+                        /* SAMPLE_COUNT points (shading_pass.frag.glsl:412-417);
+                         * the last one is the reservoir's sample */
+                        const uint32_t iters = s->sample_count;
+                        col = v3(K(0.0), K(0.0), K(0.0));
+                        for (uint32_t it = 0; it != iters; ++it) {
+                            V3 dir;
+                            R tmax;
+                            int above;
+                            point = turk_point(&light, noise2(&px.seed));
+                            V3 e = eval_turk(&px, &light, point, &dir, &tmax, &above);
+                            col = it == 0 ? e : add3(col, e);
+                        }
+                        if (iters > 1) col = scl3(col, K(1.0) / (R)iters);
                     } else if (tech == RISLTC_SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE) {
                         col = eval_peters_ex(&px, &light, NULL, 0, &cc);
                     } else {
@@ -1283,7 +1314,7 @@ static void shade_one(const orc_scene_t* sc, const risltc_frame_constants_t* c, 
                 }
                 if (chosen >= 0) {
                     load_light(sc, (uint32_t)chosen, &light);
-                    V3 col = (tech == RISLTC_SAMPLE_POLYGON_AREA_TURK) ? final_turk(&px, &light, chosen_point, rec)
+                    V3 col = (tech == RISLTC_SAMPLE_POLYGON_AREA_TURK) ? final_turk(&px, &light, chosen_point, rec, 0)
                                                                       : eval_peters(&px, &light, rec);
                     R p_hat = sample_value;
                     R Wt = w_sum / ((R)M * p_hat);
@@ -1313,8 +1344,8 @@ int SFX(orc_shade_pixels)(const orc_scene_t* sc, const risltc_frame_constants_t*
                           const uint32_t* visibility, const uint32_t* pixels, uint32_t count, R* out,
                           float* samples, uint32_t* cands) {
     if (!sc || !c || !s || !visibility || !out) return RISLTC_E_ARG;
-    if (s->sample_count != 1 || !orc_technique_supported(s))
-        return RISLTC_E_UNSUPPORTED;
+    if (s->sample_count == 0 || s->sample_count > RISLTC_MAX_SAMPLE_COUNT) return RISLTC_E_ARG;
+    if (!orc_technique_supported(s)) return RISLTC_E_UNSUPPORTED;
     if (s->proposal == RISLTC_PROPOSAL_LEARNED) {
         if (s->light_sampling != RISLTC_LIGHT_RESERVOIR) return RISLTC_E_UNSUPPORTED;
         if (!sc->proposal_table || !sc->cluster_of_light || !sc->cluster_offsets || !sc->cluster_lights ||
@@ -1343,8 +1374,8 @@ int orc_shade_frame_f32(const orc_scene_t* sc, const risltc_frame_constants_t* c
                         const uint32_t* visibility, const uint32_t* pixels, uint32_t count, float* out,
                         int threads) {
     if (!sc || !c || !s || !visibility || !out) return RISLTC_E_ARG;
-    if (s->sample_count != 1 || !orc_technique_supported(s))
-        return RISLTC_E_UNSUPPORTED;
+    if (s->sample_count == 0 || s->sample_count > RISLTC_MAX_SAMPLE_COUNT) return RISLTC_E_ARG;
+    if (!orc_technique_supported(s)) return RISLTC_E_UNSUPPORTED;
     if (s->proposal == RISLTC_PROPOSAL_LEARNED) {
         if (s->light_sampling != RISLTC_LIGHT_RESERVOIR) return RISLTC_E_UNSUPPORTED;
         if (!sc->proposal_table || !sc->cluster_of_light || !sc->cluster_offsets || !sc->cluster_lights ||

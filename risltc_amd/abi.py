@@ -9,8 +9,11 @@ MAX_LIGHT_VERTEX_COUNT = 8
 MAX_POLYGON_VERTEX_COUNT = 9
 SAMPLE_RECORD_FLOATS = 40
 CANDIDATE_RECORD_WORDS = 4
-# pixel-samples of risltc_shade_noalloc's fixed scratch (RISLTC_NOALLOC_MAX_SAMPLES)
+# sample records (pixels x spp x sample_count) of risltc_shade_noalloc's fixed
+# scratch (RISLTC_NOALLOC_MAX_SAMPLES)
 NOALLOC_MAX_SAMPLES = 1 << 19
+# SAMPLE_COUNT range (RISLTC_MAX_SAMPLE_COUNT)
+MAX_SAMPLE_COUNT = 1024
 
 OK, E_ARG, E_HIP, E_FORMAT, E_UNSUPPORTED, E_STATE, E_IO = range(7)
 ERROR_NAMES = {0: "OK", 1: "E_ARG", 2: "E_HIP", 3: "E_FORMAT", 4: "E_UNSUPPORTED",

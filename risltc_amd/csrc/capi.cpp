@@ -474,7 +474,8 @@ int risltc_load_ltc_fit(const char* dir, uint32_t fresnel_count, uint32_t* out_r
 // ---- the fixed scratch of risltc_shade_noalloc --------------------------------------
 // One file-scope device array holds what risltc_shade allocates per call (sample
 // records, the compacted shadow-ray list, ray slots, hit words, the ray and
-// trace-queue counters) for at most RISLTC_NOALLOC_MAX_SAMPLES pixel-samples:
+// trace-queue counters) for at most RISLTC_NOALLOC_MAX_SAMPLES sample records
+// (pixels x spp x SAMPLE_COUNT):
 // about 72 MiB, held by every GPU that loads this library until the process
 // ends. Each region starts on a 256-byte boundary.
 static const uint32_t kCounterWords = rl::kQueueStride * (1u + rl::kTraceQueues);
@@ -1359,12 +1360,17 @@ int risltc_trace_shadow_rays(risltc_context_t* c, const float* rays, uint32_t n,
     return RISLTC_OK;
 }
 
+// records of the final samples: one per pixel, light sample and iteration of
+// the SAMPLE_COUNT loop, each answered by up to two shadow rays
+static size_t sample_records(const risltc_frame_constants_t* k, const risltc_settings_t* s) {
+    return (size_t)k->viewport_size[0] * k->viewport_size[1] * s->sample_count_light * s->sample_count;
+}
+
 static int check_settings(risltc_context_t* c, const risltc_frame_constants_t* k, const risltc_settings_t* s) {
     if (!c->ltc0.p) return fail(c, RISLTC_E_STATE, "risltc_set_ltc_table has not been called");
     if (!c->positions.p) return fail(c, RISLTC_E_STATE, "risltc_set_scene has not been called");
-    if (s->sample_count != 1)
-        return fail(c, RISLTC_E_UNSUPPORTED, "SAMPLE_COUNT %u: the kernels take one sample per technique",
-                    s->sample_count);
+    if (s->sample_count == 0 || s->sample_count > RISLTC_MAX_SAMPLE_COUNT)
+        return fail(c, RISLTC_E_ARG, "sample_count %u outside [1, %u]", s->sample_count, RISLTC_MAX_SAMPLE_COUNT);
     const uint32_t tech = s->polygon_sampling_technique;
     if (tech != RISLTC_SAMPLE_POLYGON_LTC_CP && tech != RISLTC_SAMPLE_POLYGON_AREA_TURK &&
         tech != RISLTC_SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE)
@@ -1385,8 +1391,8 @@ static int check_settings(risltc_context_t* c, const risltc_frame_constants_t* k
         return fail(c, RISLTC_E_ARG, "skip_training_records %u", s->skip_training_records);
     if (s->shadow_rays && !c->bvh.rec) return fail(c, RISLTC_E_STATE, "no hierarchy for shadow rays");
     if (k->viewport_size[0] == 0 || k->viewport_size[1] == 0) return fail(c, RISLTC_E_ARG, "empty viewport");
-    const size_t S = (size_t)k->viewport_size[0] * k->viewport_size[1] * s->sample_count_light;
-    if (S * 2 >= 0x80000000ull) return fail(c, RISLTC_E_UNSUPPORTED, "more than 2^30 samples per call");
+    if (sample_records(k, s) * 2 >= 0x80000000ull)
+        return fail(c, RISLTC_E_UNSUPPORTED, "more than 2^30 sample records (pixels x spp x sample_count) per call");
     return RISLTC_OK;
 }
 
@@ -1428,6 +1434,7 @@ static void fill_shade(risltc_context_t* c, const risltc_frame_constants_t* k, c
     memset(&a, 0, sizeof(a));
     a.c = *k;
     a.spp = s->sample_count_light;
+    a.iters = s->sample_count;
     a.M = s->ris_candidate_count;
     a.mis = s->mis_heuristic;
     a.light_sampling = s->light_sampling;
@@ -1468,6 +1475,7 @@ static void fill_resolve(const risltc_frame_constants_t* k, const risltc_setting
     r.width = k->viewport_size[0];
     r.height = k->viewport_size[1];
     r.spp = s->sample_count_light;
+    r.iters = s->sample_count;
     r.triangle_count = triangle_count;
     r.exposure = k->exposure_factor;
     r.visibility = vis;
@@ -1522,16 +1530,17 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     if (s->proposal == RISLTC_PROPOSAL_CACHE && !c->vcache.p)
         return fail(c, RISLTC_E_STATE, "no visibility cache");
     const size_t S = (size_t)k->viewport_size[0] * k->viewport_size[1] * s->sample_count_light;
+    const size_t R = sample_records(k, s);
     hipStream_t st = (hipStream_t)stream;
     HIPC(c, hipSetDevice(c->device));
     ShadeWork wk;
     if (fixed) {
         wk = *fixed;
     } else {
-        HIPC(c, c->samples.ensure(S * rl::kSampleWords * sizeof(float)));
-        HIPC(c, c->rays.ensure(S * 2 * sizeof(rl::ShadowRay)));
-        HIPC(c, c->ray_slots.ensure(S * 2 * sizeof(uint32_t)));
-        HIPC(c, c->hits.ensure(S * 2 * sizeof(uint32_t)));
+        HIPC(c, c->samples.ensure(R * rl::kSampleWords * sizeof(float)));
+        HIPC(c, c->rays.ensure(R * 2 * sizeof(rl::ShadowRay)));
+        HIPC(c, c->ray_slots.ensure(R * 2 * sizeof(uint32_t)));
+        HIPC(c, c->hits.ensure(R * 2 * sizeof(uint32_t)));
         HIPC(c, c->counter.ensure(kCounterWords * sizeof(uint32_t)));
         wk = ShadeWork{c->samples.as<float>(), c->rays.as<rl::ShadowRay>(), c->ray_slots.as<uint32_t>(),
                        c->hits.as<uint32_t>(), c->counter.as<uint32_t>()};
@@ -1570,7 +1579,7 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     HIPC(c, hipEventRecord(c->ev[1], st));
     if (s->shadow_rays)
         HIPC(c, rl::launch_trace(trace_args(c, c->bvh_table.as<rl::BvhView>(), a.rays, a.ray_slots, a.ray_count,
-                                            wk.hits, S * 2),
+                                            wk.hits, R * 2),
                                  st));
     HIPC(c, hipEventRecord(c->ev[2], st));
     rl::ResolveArgs r;
@@ -1608,13 +1617,15 @@ struct BatchPlan {
     size_t bvh_capacity = 0;
     size_t lds = 0;
     bool exact = false, shadows = false;
+    bool general = false;  // some view has a SAMPLE_COUNT above 1
     bool learned = false, cached = false;  // learned views; the visibility cache is refreshed first
     uint32_t group_log2 = 0;                // table slots name groups of 1 << group_log2 clusters
     rl::VisCacheView cache = {};
     // whether a graph captured with plan `o` can run this plan's descriptors
     bool fits(const BatchPlan& o) const {
         return tile_off == o.tile_off && block_off == o.block_off && samples == o.samples && lds <= o.lds &&
-               exact == o.exact && shadows == o.shadows && learned == o.learned && cached == o.cached &&
+               exact == o.exact && shadows == o.shadows && general == o.general && learned == o.learned &&
+               cached == o.cached &&
                group_log2 == o.group_log2 && bvhs.size() <= o.bvh_capacity;
     }
 };
@@ -1650,9 +1661,11 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
         if ((w.settings.exact_math != 0) != (views[0].settings.exact_math != 0))
             return fail(c, RISLTC_E_UNSUPPORTED, "a batch has one exact_math for all views");
         off[v] = p.samples;
-        p.samples += (size_t)w.constants.viewport_size[0] * w.constants.viewport_size[1] * w.settings.sample_count_light;
-        if (p.samples * 2 >= 0x80000000ull) return fail(c, RISLTC_E_UNSUPPORTED, "more than 2^30 samples per batch");
+        p.samples += sample_records(&w.constants, &w.settings);
+        if (p.samples * 2 >= 0x80000000ull)
+            return fail(c, RISLTC_E_UNSUPPORTED, "more than 2^30 sample records per batch");
         p.shadows = p.shadows || w.settings.shadow_rays != 0;
+        p.general = p.general || w.settings.sample_count > 1u;
     }
     p.exact = views[0].settings.exact_math != 0;
     p.learned = views[0].settings.proposal != RISLTC_PROPOSAL_UNIFORM;
@@ -1741,7 +1754,7 @@ static int launch_plan(risltc_context_t* c, BatchBuffers& B, const BatchPlan& p,
     if (timed) HIPC(c, hipEventRecord(c->ev[0], st));
     if (p.cached) HIPC(c, rl::launch_refresh_cache(p.cache, st));
     HIPC(c, rl::launch_shade_batch(B.shade_args.as<rl::ShadeArgs>(), B.tile_offsets.as<uint32_t>(), n, p.tiles, p.lds,
-                                   p.exact, p.learned ? 1 : 0, st));
+                                   p.exact, p.learned ? 1 : 0, p.general, st));
     if (timed) HIPC(c, hipEventRecord(c->ev[1], st));
     if (p.shadows)
         HIPC(c, rl::launch_trace(trace_args(c, B.bvh_table.as<rl::BvhView>(), B.rays.as<rl::ShadowRay>(),
@@ -1962,12 +1975,12 @@ int risltc_shade_noalloc(risltc_context_t* c, const risltc_frame_constants_t* k,
                          const uint32_t* vis, float* out, void* stream) {
     if (!c) return RISLTC_E_ARG;
     if (!k || !s) return fail(c, RISLTC_E_ARG, "null argument");
-    const uint64_t S = (uint64_t)k->viewport_size[0] * k->viewport_size[1] * s->sample_count_light;
+    const uint64_t S = (uint64_t)k->viewport_size[0] * k->viewport_size[1] * s->sample_count_light * s->sample_count;
     if (S > RISLTC_NOALLOC_MAX_SAMPLES)
         return fail(c, RISLTC_E_UNSUPPORTED,
-                    "risltc_shade_noalloc: %llu pixel-samples (%u x %u x %u spp) exceed its fixed scratch of %u",
+                    "risltc_shade_noalloc: %llu sample records (%u x %u x %u spp x %u) exceed its fixed scratch of %u",
                     (unsigned long long)S, k->viewport_size[0], k->viewport_size[1], s->sample_count_light,
-                    (unsigned)RISLTC_NOALLOC_MAX_SAMPLES);
+                    s->sample_count, (unsigned)RISLTC_NOALLOC_MAX_SAMPLES);
     if (!c->noalloc_scratch) return fail(c, RISLTC_E_STATE, "no scratch address for device %d", c->device);
     uint8_t* base = c->noalloc_scratch;
     const ShadeWork wk{reinterpret_cast<float*>(base + noalloc::kRecordsOff),

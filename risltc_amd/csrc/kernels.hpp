@@ -13,17 +13,21 @@ namespace rl {
 //   [2..7] vertices: 8 vertices x 3 floats, packed (v0.x v0.y v0.z v1.x | ...)
 constexpr int kLightFloat4 = 8;
 
-// Per pixel-sample record written by the shade kernel and read by the
-// resolve kernel: 16 words.
+// Record of one iteration of the SAMPLE_COUNT loop of one pixel-sample
+// (record index = pixel_sample * SAMPLE_COUNT + iteration), written by the
+// shade kernel and read by the resolve kernel: 16 words.
 //   [0] W  (0 when the sample contributes nothing)
-//   [1] flags: bits 0-3 state of technique 0, bits 4-7 technique 1, bit 8 = has contribution
+//   [1] flags: bits 0-3 state of technique 0, bits 4-7 technique 1, bit 8 = has contribution,
+//       bit 9 = joint visibility (area sampling with SAMPLE_COUNT > 1: the sample's
+//       estimates count only when none of its shadow rays is occluded)
 //   [2..7]  estimate if visible, technique 0 then 1 (xyz, xyz)
 //   [8..13] estimate if occluded, technique 0 then 1
-//   [14] spare, [15] spare
+//   [14] selected light (-1 = none), [15] cache key
 constexpr int kSampleWords = 16;
+constexpr uint32_t kFlagContributes = 256u, kFlagJoint = 512u;
 
 // Shadow ray record (compacted list): float4 org.xyz + t_min, float4 dir.xyz + t_max,
-// plus the slot (pixel_sample * 2 + technique) it answers.
+// plus the slot (record index * 2 + technique) it answers.
 struct ShadowRay {
     float4 org;
     float4 dir;
@@ -195,6 +199,7 @@ hipError_t launch_mlp_sync(const MlpLayout& L, const float* params, uint16_t* wf
 struct ShadeArgs {
     risltc_frame_constants_t c;
     uint32_t spp, M, mis, light_sampling, fast_atan, shadow_rays;
+    uint32_t iters;  // SAMPLE_COUNT: final-sample iterations per pixel-sample
     // launch shape (tuned; none changes a result bit): a wave covers
     // 2^tile_w_log2 x (64 >> tile_w_log2) pixels, a work-group 4 waves stacked
     // vertically; candidates are culled in chunks of `chunk` (<= 32); the
@@ -204,7 +209,7 @@ struct ShadeArgs {
     uint32_t bvh_index;        // hierarchy of the view's scene in the trace table
     SceneView scene;
     const uint32_t* visibility;
-    float* samples;            // kSampleWords per pixel-sample
+    float* samples;            // kSampleWords per pixel-sample and iteration
     ShadowRay* rays;           // compacted shadow rays
     uint32_t* ray_slots;       // slot of each compacted ray
     uint32_t* ray_count;       // device counter
@@ -251,6 +256,7 @@ struct TraceArgs {
 
 struct ResolveArgs {
     uint32_t width, height, spp;
+    uint32_t iters;  // SAMPLE_COUNT: records per pixel-sample
     uint64_t triangle_count;
     float exposure;
     const uint32_t* visibility;
@@ -278,9 +284,10 @@ hipError_t launch_shade_baseline(const ShadeArgs& a, uint32_t technique, bool ex
 uint32_t shade_tiles(const ShadeArgs& a);
 size_t shade_lds_bytes(const ShadeArgs& a);
 // views and their tile offsets (count + 1 entries) in device memory
-// mode: 0 plain, 1 learned, 2 learned with 2-byte table ids (every view)
+// mode: 0 plain, 1 learned (every view); general: some view has a
+// SAMPLE_COUNT above 1 (the general instantiation of the tile shader)
 hipError_t launch_shade_batch(const ShadeArgs* d_views, const uint32_t* d_tile_offsets, uint32_t count,
-                              uint32_t total_tiles, size_t lds, bool exact, int mode, hipStream_t s);
+                              uint32_t total_tiles, size_t lds, bool exact, int mode, bool general, hipStream_t s);
 hipError_t launch_resolve_batch(const ResolveArgs* d_views, const uint32_t* d_block_offsets, uint32_t count,
                                 uint32_t total_blocks, hipStream_t s);
 hipError_t launch_trace(const TraceArgs& a, hipStream_t s);

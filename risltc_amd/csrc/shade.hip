@@ -841,10 +841,13 @@ struct FinalOut {
     float psad, psas;
 };
 
-// evaluate_polygonal_light_shading_peters (shading_pass.frag.glsl:292-397)
-// with SAMPLE_COUNT = 1; one projected-solid-angle polygon is live at a time
-// (diffuse prepared and sampled, then specular), which consumes the noise in
-// the reference order.
+// evaluate_polygonal_light_shading_peters (shading_pass.frag.glsl:292-397),
+// one iteration of its SAMPLE_COUNT loop (365-396): the polygons are prepared
+// and one direction per technique is drawn. Calling it k times reproduces the
+// loop, since the preparation draws nothing and gives the same polygons; the
+// caller scales the sum by 1 / k. One projected-solid-angle polygon is live
+// at a time (diffuse prepared and sampled, then specular), which consumes the
+// noise in the reference order.
 template <bool X>
 __device__ __forceinline__ void eval_peters(const ShadeArgs& A, const Pixel& px, const Light& L, uint32_t& seed, FinalOut& F) {
     bool fa = A.fast_atan != 0;
@@ -981,6 +984,91 @@ __device__ __forceinline__ void eval_peters(const ShadeArgs& A, const Pixel& px,
         F.dir[j] = wdir;
         F.tmax[j] = tmax;
         F.state[j] = state;
+    }
+}
+
+__device__ __forceinline__ void clear_final(FinalOut& F) {
+    F.color = mk3(0.0f, 0.0f, 0.0f);
+    F.state[0] = F.state[1] = 0u;
+    F.clipd = F.clips = 0u;
+    F.psad = F.psas = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        F.e_vis[t] = F.e_occ[t] = F.dir[t] = mk3(0.0f, 0.0f, 0.0f);
+        F.tmax[t] = 0.0f;
+    }
+}
+
+// One iteration of a pixel-sample's final sample: its shadow rays are
+// appended to the compacted list (one atomic per wave and technique; every
+// lane of the wave calls this) and its record is written at index `rec_idx`
+// (pixel-sample * SAMPLE_COUNT + iteration).
+__device__ __forceinline__ void put_final(const ShadeArgs& A, uint32_t lane, bool inside, bool contributes,
+                                          const FinalOut& F, f3 pos, size_t rec_idx, float Wt, uint32_t extra,
+                                          int32_t chosen, uint32_t ckey) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        bool emit = contributes && F.state[t] == 5u;
+        uint64_t ballot = __ballot(emit);
+        uint32_t cnt = __popcll(ballot);
+        uint32_t base = 0;
+        if (cnt) {
+            uint32_t first = (uint32_t)__ffsll((unsigned long long)ballot) - 1u;
+            if (lane == first) base = atomicAdd(A.ray_count, cnt);
+            base = __shfl(base, (int)first);
+        }
+        if (emit) {
+            uint32_t off = base + __popcll(ballot & ((1ull << lane) - 1ull));
+            A.rays[off].org = make_float4(pos.x, pos.y, pos.z, __uint_as_float(A.bvh_index));
+            A.rays[off].dir = make_float4(F.dir[t].x, F.dir[t].y, F.dir[t].z, F.tmax[t]);
+            A.ray_slots[off] = A.slot_base + (uint32_t)(rec_idx * 2 + t);
+        }
+    }
+    if (inside) {
+        float4* rec = reinterpret_cast<float4*>(A.samples + rec_idx * kSampleWords);
+        uint32_t flags = (F.state[0] & 15u) | ((F.state[1] & 15u) << 4) | (contributes ? kFlagContributes | extra : 0u);
+        rec[0] = make_float4(Wt, __uint_as_float(flags), F.e_vis[0].x, F.e_vis[0].y);
+        rec[1] = make_float4(F.e_vis[0].z, F.e_vis[1].x, F.e_vis[1].y, F.e_vis[1].z);
+        rec[2] = make_float4(F.e_occ[0].x, F.e_occ[0].y, F.e_occ[0].z, F.e_occ[1].x);
+        rec[3] = make_float4(F.e_occ[1].y, F.e_occ[1].z, __int_as_float(contributes ? chosen : -1),
+                             __uint_as_float(ckey));
+    }
+}
+
+// The debug record of a pixel-sample (RISLTC_SAMPLE_RECORD_FLOATS words):
+// the reservoir, and the first iteration of the final sample.
+__device__ __forceinline__ void put_debug(float* r, const FinalOut& F, int32_t chosen, float w_sum, float sample_value,
+                                          float Wt, bool contributes, f3 pos, uint32_t ckey) {
+    for (int k = 0; k < RISLTC_SAMPLE_RECORD_FLOATS; ++k) r[k] = 0.0f;
+    r[0] = __int_as_float(chosen);
+    r[1] = w_sum;
+    r[2] = sample_value;
+    r[3] = Wt;
+    r[4] = __uint_as_float(F.clipd);
+    r[5] = __uint_as_float(F.clips);
+    r[6] = F.psad;
+    r[7] = F.psas;
+    for (int t = 0; t < 2; ++t) {
+        r[8 + 3 * t] = F.dir[t].x;
+        r[9 + 3 * t] = F.dir[t].y;
+        r[10 + 3 * t] = F.dir[t].z;
+        r[14 + t] = F.tmax[t];
+        r[16 + t] = __uint_as_float(F.state[t]);
+        r[18 + 3 * t] = F.e_vis[t].x;
+        r[19 + 3 * t] = F.e_vis[t].y;
+        r[20 + 3 * t] = F.e_vis[t].z;
+        r[24 + 3 * t] = F.e_occ[t].x;
+        r[25 + 3 * t] = F.e_occ[t].y;
+        r[26 + 3 * t] = F.e_occ[t].z;
+    }
+    r[30] = F.color.x;
+    r[31] = F.color.y;
+    r[32] = F.color.z;
+    r[37] = __uint_as_float(ckey);
+    if (contributes) {
+        r[34] = pos.x;
+        r[35] = pos.y;
+        r[36] = pos.z;
     }
 }
 
@@ -1328,7 +1416,11 @@ __device__ __forceinline__ uint32_t cache_key(const VisCacheView& C, f3 p, f3 n)
 // 2^tile_w_log2 x (64 >> tile_w_log2) pixels; the work-group stacks its 4
 // waves vertically. The first lds_lights light spheres (all of them when
 // they fit) are staged in LDS; the cluster table too for the learned target
-// when every sphere is staged.
+// when every sphere is staged. DEBUG = true is the general instantiation:
+// candidates evaluated per lane, debug records written when their pointers
+// are set, and the SAMPLE_COUNT loop of the final sample; the production
+// instantiation (DEBUG = false) takes SAMPLE_COUNT = 1, so no state is held
+// across a loop there.
 template <bool X, bool DEBUG, int LMODE>
 __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, float4* sh_spheres) {
     // LMODE: 0 plain, 1 learned
@@ -1647,22 +1739,9 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                 }
             }
         }
-        FinalOut F;
-        F.color = mk3(0.0f, 0.0f, 0.0f);
-        F.state[0] = F.state[1] = 0u;
-        F.clipd = F.clips = 0u;
-        F.psad = F.psas = 0.0f;
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            F.e_vis[t] = F.e_occ[t] = F.dir[t] = mk3(0.0f, 0.0f, 0.0f);
-            F.tmax[t] = 0.0f;
-        }
         float Wt = 0.0f;
-        bool contributes = false;
-        if (shade && chosen >= 0) {
-            load_light(A.scene.lights, (uint32_t)chosen, L);
-            eval_peters<X>(A, px, L, seed, F);
-            contributes = true;
+        const bool contributes = shade && chosen >= 0;
+        if (contributes) {
             if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
                 Wt = (float)N;
             } else {
@@ -1670,71 +1749,29 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                 if (sample_value == 0.0f) Wt = 0.0f;
             }
         }
-        // append shadow rays to the compacted list: one atomic per wave
         const size_t sidx = (size_t)pixel * spp + j;
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            bool emit = contributes && F.state[t] == 5u;
-            uint64_t ballot = __ballot(emit);
-            uint32_t cnt = __popcll(ballot);
-            uint32_t base = 0;
-            if (cnt) {
-                uint32_t first = (uint32_t)__ffsll((unsigned long long)ballot) - 1u;
-                if (lane == first) base = atomicAdd(A.ray_count, cnt);
-                base = __shfl(base, (int)first);
+        // the SAMPLE_COUNT loop: every iteration prepares the two polygons
+        // again (the same arithmetic, no draws) and takes one direction per
+        // technique, so the draws follow the reference's loop
+        const uint32_t iters = DEBUG ? A.iters : 1u;
+        for (uint32_t it = 0; it < iters; ++it) {
+            FinalOut F;
+            clear_final(F);
+            // the light is read per iteration so that it is not held across the loop
+            if (contributes) {
+                load_light(A.scene.lights, (uint32_t)chosen, L);
+                eval_peters<X>(A, px, L, seed, F);
             }
-            if (emit) {
-                uint32_t off = base + __popcll(ballot & ((1ull << lane) - 1ull));
-                A.rays[off].org = make_float4(px.pos.x, px.pos.y, px.pos.z, __uint_as_float(A.bvh_index));
-                A.rays[off].dir = make_float4(F.dir[t].x, F.dir[t].y, F.dir[t].z, F.tmax[t]);
-                A.ray_slots[off] = A.slot_base + (uint32_t)(sidx * 2 + t);
+            put_final(A, lane, inside, contributes, F, px.pos, sidx * iters + it, Wt, 0u, chosen, ckey);
+            if constexpr (DEBUG) {
+                if (it == 0u && inside && A.debug_samples)
+                    put_debug(A.debug_samples + sidx * RISLTC_SAMPLE_RECORD_FLOATS, F, shade ? chosen : -1, w_sum,
+                              sample_value, Wt, contributes, px.pos, ckey);
             }
-        }
-        if (inside) {
-            float4* rec = reinterpret_cast<float4*>(A.samples + sidx * kSampleWords);
-            uint32_t flags = (F.state[0] & 15u) | ((F.state[1] & 15u) << 4) | (contributes ? 256u : 0u);
-            rec[0] = make_float4(Wt, __uint_as_float(flags), F.e_vis[0].x, F.e_vis[0].y);
-            rec[1] = make_float4(F.e_vis[0].z, F.e_vis[1].x, F.e_vis[1].y, F.e_vis[1].z);
-            rec[2] = make_float4(F.e_occ[0].x, F.e_occ[0].y, F.e_occ[0].z, F.e_occ[1].x);
-            rec[3] = make_float4(F.e_occ[1].y, F.e_occ[1].z, __int_as_float(contributes ? chosen : -1),
-                                 __uint_as_float(ckey));
         }
         if constexpr (DEBUG) {
-            if (inside && A.debug_samples) {
-                float* r = A.debug_samples + sidx * RISLTC_SAMPLE_RECORD_FLOATS;
-                for (int k = 0; k < RISLTC_SAMPLE_RECORD_FLOATS; ++k) r[k] = 0.0f;
-                r[0] = __int_as_float(shade ? chosen : -1);
-                r[1] = w_sum;
-                r[2] = sample_value;
-                r[3] = Wt;
-                r[4] = __uint_as_float(F.clipd);
-                r[5] = __uint_as_float(F.clips);
-                r[6] = F.psad;
-                r[7] = F.psas;
-                for (int t = 0; t < 2; ++t) {
-                    r[8 + 3 * t] = F.dir[t].x;
-                    r[9 + 3 * t] = F.dir[t].y;
-                    r[10 + 3 * t] = F.dir[t].z;
-                    r[14 + t] = F.tmax[t];
-                    r[16 + t] = __uint_as_float(F.state[t]);
-                    r[18 + 3 * t] = F.e_vis[t].x;
-                    r[19 + 3 * t] = F.e_vis[t].y;
-                    r[20 + 3 * t] = F.e_vis[t].z;
-                    r[24 + 3 * t] = F.e_occ[t].x;
-                    r[25 + 3 * t] = F.e_occ[t].y;
-                    r[26 + 3 * t] = F.e_occ[t].z;
-                }
-                r[30] = F.color.x;
-                r[31] = F.color.y;
-                r[32] = F.color.z;
-                r[33] = __uint_as_float(seed);
-                r[37] = __uint_as_float(ckey);
-                if (contributes) {
-                    r[34] = px.pos.x;
-                    r[35] = px.pos.y;
-                    r[36] = px.pos.z;
-                }
-            }
+            if (inside && A.debug_samples)
+                A.debug_samples[sidx * RISLTC_SAMPLE_RECORD_FLOATS + 33] = __uint_as_float(seed);
         }
     }
 }
@@ -1861,24 +1898,33 @@ __global__ void __launch_bounds__(256) shade_baseline_kernel(ShadeArgs A) {
                 load_light(A.scene.lights, idx, L);
                 f3 col = mk3(0.0f, 0.0f, 0.0f), point = mk3(0.0f, 0.0f, 0.0f);
                 uint32_t cc = 0;
+                // SAMPLE_COUNT samples (shading_pass.frag.glsl:412-417, 518-559),
+                // their mean is the target; the last area sample is the
+                // reservoir's point
                 if constexpr (TECH == RISLTC_SAMPLE_POLYGON_AREA_TURK) {
-                    float u0 = noise_next(seed);
-                    float u1 = noise_next(seed);
-                    point = turk_point<X>(L, u0, u1);
-                    f3 d;
-                    float tm;
-                    bool ab;
-                    col = eval_turk<X>(px, L, point, d, tm, ab);
+                    for (uint32_t it = 0; it < A.iters; ++it) {
+                        float u0 = noise_next(seed);
+                        float u1 = noise_next(seed);
+                        point = turk_point<X>(L, u0, u1);
+                        f3 d;
+                        float tm;
+                        bool ab;
+                        f3 e = eval_turk<X>(px, L, point, d, tm, ab);
+                        col = it == 0u ? e : add3(col, e);
+                    }
                 } else {
                     // the final-sample code without tracing: a sample above
                     // the horizon keeps its unshadowed estimate
-                    FinalOut F;
-                    eval_peters<X>(A, px, L, seed, F);
+                    for (uint32_t it = 0; it < A.iters; ++it) {
+                        FinalOut F;
+                        eval_peters<X>(A, px, L, seed, F);
 #pragma unroll
-                    for (int t = 0; t < 2; ++t)
-                        if (F.state[t] != 0u) col = add3(col, F.state[t] >= 4u ? F.e_vis[t] : F.e_occ[t]);
-                    cc = F.clipd | (F.clips << 8);
+                        for (int t = 0; t < 2; ++t)
+                            if (F.state[t] != 0u) col = add3(col, F.state[t] >= 4u ? F.e_vis[t] : F.e_occ[t]);
+                        cc = F.clipd | (F.clips << 8);
+                    }
                 }
+                if (A.iters > 1u) col = scl3(col, 1.0f / (float)A.iters);
                 float p_hat = sqrt_<X>(dot3(col, col));
                 float w = div_<X>(p_hat, pN);
                 float r = noise_next(seed);
@@ -1898,30 +1944,9 @@ __global__ void __launch_bounds__(256) shade_baseline_kernel(ShadeArgs A) {
                 }
             }
         }
-        FinalOut F;
-        F.color = mk3(0.0f, 0.0f, 0.0f);
-        F.state[0] = F.state[1] = 0u;
-        F.clipd = F.clips = 0u;
-        F.psad = F.psas = 0.0f;
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            F.e_vis[t] = F.e_occ[t] = F.dir[t] = mk3(0.0f, 0.0f, 0.0f);
-            F.tmax[t] = 0.0f;
-        }
         float Wt = 0.0f;
         const bool contributes = shade && chosen >= 0;
         if (contributes) {
-            load_light(A.scene.lights, (uint32_t)chosen, L);
-            if constexpr (TECH == RISLTC_SAMPLE_POLYGON_AREA_TURK) {
-                if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
-                    float u0 = noise_next(seed);
-                    float u1 = noise_next(seed);
-                    chosen_point = turk_point<X>(L, u0, u1);
-                }
-                final_turk<X>(A, px, L, chosen_point, F);
-            } else {
-                eval_peters<X>(A, px, L, seed, F);
-            }
             if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
                 Wt = (float)N;
             } else {
@@ -1930,65 +1955,34 @@ __global__ void __launch_bounds__(256) shade_baseline_kernel(ShadeArgs A) {
             }
         }
         const size_t sidx = (size_t)pixel * spp + j;
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            bool emit = contributes && F.state[t] == 5u;
-            uint64_t ballot = __ballot(emit);
-            uint32_t cnt = __popcll(ballot);
-            uint32_t base = 0;
-            if (cnt) {
-                uint32_t first = (uint32_t)__ffsll((unsigned long long)ballot) - 1u;
-                if (lane == first) base = atomicAdd(A.ray_count, cnt);
-                base = __shfl(base, (int)first);
-            }
-            if (emit) {
-                uint32_t off = base + __popcll(ballot & ((1ull << lane) - 1ull));
-                A.rays[off].org = make_float4(px.pos.x, px.pos.y, px.pos.z, __uint_as_float(A.bvh_index));
-                A.rays[off].dir = make_float4(F.dir[t].x, F.dir[t].y, F.dir[t].z, F.tmax[t]);
-                A.ray_slots[off] = A.slot_base + (uint32_t)(sidx * 2 + t);
-            }
-        }
-        if (inside) {
-            float4* rec = reinterpret_cast<float4*>(A.samples + sidx * kSampleWords);
-            uint32_t flags = (F.state[0] & 15u) | ((F.state[1] & 15u) << 4) | (contributes ? 256u : 0u);
-            rec[0] = make_float4(Wt, __uint_as_float(flags), F.e_vis[0].x, F.e_vis[0].y);
-            rec[1] = make_float4(F.e_vis[0].z, F.e_vis[1].x, F.e_vis[1].y, F.e_vis[1].z);
-            rec[2] = make_float4(F.e_occ[0].x, F.e_occ[0].y, F.e_occ[0].z, F.e_occ[1].x);
-            rec[3] = make_float4(F.e_occ[1].y, F.e_occ[1].z, __int_as_float(contributes ? chosen : -1),
-                                 __uint_as_float(ckey));
-        }
-        if (inside && A.debug_samples) {
-            float* r = A.debug_samples + sidx * RISLTC_SAMPLE_RECORD_FLOATS;
-            for (int k = 0; k < RISLTC_SAMPLE_RECORD_FLOATS; ++k) r[k] = 0.0f;
-            r[0] = __int_as_float(shade ? chosen : -1);
-            r[1] = w_sum;
-            r[2] = sample_value;
-            r[3] = Wt;
-            r[4] = __uint_as_float(F.clipd);
-            r[5] = __uint_as_float(F.clips);
-            r[6] = F.psad;
-            r[7] = F.psas;
-            for (int t = 0; t < 2; ++t) {
-                r[8 + 3 * t] = F.dir[t].x;
-                r[9 + 3 * t] = F.dir[t].y;
-                r[10 + 3 * t] = F.dir[t].z;
-                r[14 + t] = F.tmax[t];
-                r[16 + t] = __uint_as_float(F.state[t]);
-                r[18 + 3 * t] = F.e_vis[t].x;
-                r[19 + 3 * t] = F.e_vis[t].y;
-                r[20 + 3 * t] = F.e_vis[t].z;
-                r[24 + 3 * t] = F.e_occ[t].x;
-                r[25 + 3 * t] = F.e_occ[t].y;
-                r[26 + 3 * t] = F.e_occ[t].z;
-            }
-            r[33] = __uint_as_float(seed);
-            r[37] = __uint_as_float(ckey);
+        // area sampling with SAMPLE_COUNT > 1: one visibility flag for the
+        // whole loop (shading_pass.frag.glsl:412-417 with 743-756 or 714-720)
+        const uint32_t joint = (TECH == RISLTC_SAMPLE_POLYGON_AREA_TURK && A.iters > 1u) ? kFlagJoint : 0u;
+        for (uint32_t it = 0; it < A.iters; ++it) {
+            FinalOut F;
+            clear_final(F);
             if (contributes) {
-                r[34] = px.pos.x;
-                r[35] = px.pos.y;
-                r[36] = px.pos.z;
+                load_light(A.scene.lights, (uint32_t)chosen, L);
+                if constexpr (TECH == RISLTC_SAMPLE_POLYGON_AREA_TURK) {
+                    if (A.light_sampling == RISLTC_LIGHT_UNIFORM) {
+                        float u0 = noise_next(seed);
+                        float u1 = noise_next(seed);
+                        chosen_point = turk_point<X>(L, u0, u1);
+                    }
+                    final_turk<X>(A, px, L, chosen_point, F);
+                    // every iteration evaluates the reservoir's point: its
+                    // first shadow ray answers for all of them
+                    if (A.light_sampling != RISLTC_LIGHT_UNIFORM && it > 0u && F.state[0] == 5u) F.state[0] = 4u;
+                } else {
+                    eval_peters<X>(A, px, L, seed, F);
+                }
             }
+            put_final(A, lane, inside, contributes, F, px.pos, sidx * A.iters + it, Wt, joint, chosen, ckey);
+            if (it == 0u && inside && A.debug_samples)
+                put_debug(A.debug_samples + sidx * RISLTC_SAMPLE_RECORD_FLOATS, F, shade ? chosen : -1, w_sum,
+                          sample_value, Wt, contributes, px.pos, ckey);
         }
+        if (inside && A.debug_samples) A.debug_samples[sidx * RISLTC_SAMPLE_RECORD_FLOATS + 33] = __uint_as_float(seed);
     }
 }
 
@@ -2005,13 +1999,14 @@ __device__ __forceinline__ uint32_t view_of(const uint32_t* offsets, uint32_t n,
 // Many views in one launch: work-group b shades tile b - tile_offsets[v] of
 // view v, with v's descriptor read from device memory (so a captured graph
 // can be replayed with new cameras and noise seeds). LMODE as for
-// shade_tile: every view of the launch runs the same pass.
-template <bool X, int LMODE>
+// shade_tile: every view of the launch runs the same pass. GEN: the general
+// instantiation of shade_tile (batches with a SAMPLE_COUNT above 1).
+template <bool X, int LMODE, bool GEN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 shade_batch_kernel(const ShadeArgs* views, const uint32_t* tile_offsets, uint32_t count) {
     extern __shared__ float4 sh_spheres[];
     const uint32_t v = view_of(tile_offsets, count, blockIdx.x);
-    shade_tile<X, false, LMODE>(views[v], blockIdx.x - tile_offsets[v], sh_spheres);
+    shade_tile<X, GEN, LMODE>(views[v], blockIdx.x - tile_offsets[v], sh_spheres);
 }
 
 // ---- resolve: sum the estimates in the reference's order -------------------
@@ -2034,35 +2029,54 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
         if ((prim >> 31) > 0u) {
             fin = mk3(1.0f, 1.0f, 1.0f);
         } else {
+            const uint32_t K = A.iters;
             for (uint32_t j = 0; j < A.spp; ++j) {
                 size_t sidx = (size_t)pixel * A.spp + j;
-                const float4* rec = reinterpret_cast<const float4*>(A.samples + sidx * kSampleWords);
-                float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+                const float4* rec = reinterpret_cast<const float4*>(A.samples + sidx * K * kSampleWords);
+                float4 r0 = rec[0], r3 = rec[3];
                 uint32_t flags = __float_as_uint(r0.y);
-                if (!(flags & 256u)) {
+                if (!(flags & kFlagContributes)) {
                     if (A.rec_cluster) {
                         A.rec_cluster[sidx] = 0xFFFFFFFFu;
                         A.rec_y[sidx] = 0.0f;
                     }
                     continue;
                 }
-                f3 ev[2] = {mk3(r0.z, r0.w, r1.x), mk3(r1.y, r1.z, r1.w)};
-                f3 eo[2] = {mk3(r2.x, r2.y, r2.z), mk3(r2.w, r3.x, r3.y)};
+                // joint visibility: the sample counts only if none of its
+                // shadow rays is occluded
+                bool all_vis = true;
+                if (flags & kFlagJoint) {
+                    for (uint32_t it = 0; it < K; ++it) {
+                        const uint32_t fl = __float_as_uint(rec[4 * it].y);
+#pragma unroll
+                        for (int t = 0; t < 2; ++t)
+                            if (((fl >> (4 * t)) & 15u) == 5u && A.hits[(sidx * K + it) * 2 + t] != 0u) all_vis = false;
+                    }
+                }
                 f3 result = mk3(0.0f, 0.0f, 0.0f);
                 float* dbg = A.debug_samples ? A.debug_samples + sidx * RISLTC_SAMPLE_RECORD_FLOATS : nullptr;
                 uint32_t traced = 0, unoccluded = 0;
+                // iterations, then techniques: the reference's summation order
+                for (uint32_t it = 0; it < K; ++it) {
+                    const float4* ri = rec + 4 * it;
+                    float4 q0 = ri[0], q1 = ri[1], q2 = ri[2], q3 = ri[3];
+                    const uint32_t fl = __float_as_uint(q0.y);
+                    f3 ev[2] = {mk3(q0.z, q0.w, q1.x), mk3(q1.y, q1.z, q1.w)};
+                    f3 eo[2] = {mk3(q2.x, q2.y, q2.z), mk3(q2.w, q3.x, q3.y)};
 #pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    uint32_t st = (flags >> (4 * t)) & 15u;
-                    if (st == 0u) continue;
-                    bool hit = st == 5u && A.hits[sidx * 2 + t] != 0u;
-                    traced += st == 5u ? 1u : 0u;
-                    unoccluded += (st == 5u && !hit) ? 1u : 0u;
-                    bool vis = st == 4u || (st == 5u && !hit);
-                    result = add3(result, vis ? ev[t] : eo[t]);
-                    if (dbg && st == 5u) dbg[16 + t] = __uint_as_float(hit ? 2u : 3u);
+                    for (int t = 0; t < 2; ++t) {
+                        uint32_t st = (fl >> (4 * t)) & 15u;
+                        if (st == 0u) continue;
+                        bool hit = st == 5u && A.hits[(sidx * K + it) * 2 + t] != 0u;
+                        traced += st == 5u ? 1u : 0u;
+                        unoccluded += (st == 5u && !hit) ? 1u : 0u;
+                        bool vis = (fl & kFlagJoint) ? all_vis : (st == 4u || (st == 5u && !hit));
+                        result = add3(result, vis ? ev[t] : eo[t]);
+                        if (dbg && it == 0u && st == 5u) dbg[16 + t] = __uint_as_float(hit ? 2u : 3u);
+                    }
                 }
-                f3 col = scl3(result, 1.0f / 1.0f);
+                // result * (1 / SAMPLE_COUNT) (shading_pass.frag.glsl:396, 559)
+                f3 col = scl3(result, 1.0f / (float)K);
                 if (dbg) {
                     dbg[30] = col.x;
                     dbg[31] = col.y;
@@ -2143,7 +2157,7 @@ static bool shade_args_ok(const ShadeArgs& a) {
 hipError_t launch_shade(const ShadeArgs& a, bool exact, hipStream_t s) {
     if (!shade_args_ok(a)) return hipErrorInvalidValue;
     dim3 grid(shade_tiles(a)), block(256);
-    bool dbg = a.debug_samples != nullptr || a.debug_cands != nullptr;
+    bool dbg = a.debug_samples != nullptr || a.debug_cands != nullptr || a.iters > 1u;
     size_t lds = shade_lds_bytes(a);
     const int mode = a.learned_pass ? 1 : 0;
 #define RISLTC_SHADE_LAUNCH(XX, DD, MM) hipLaunchKernelGGL((shade_kernel<XX, DD, MM>), grid, block, lds, s, a)
@@ -2188,11 +2202,16 @@ hipError_t launch_shade_baseline(const ShadeArgs& a, uint32_t technique, bool ex
 }
 
 hipError_t launch_shade_batch(const ShadeArgs* d_views, const uint32_t* d_tile_offsets, uint32_t count,
-                              uint32_t total_tiles, size_t lds, bool exact, int mode, hipStream_t s) {
+                              uint32_t total_tiles, size_t lds, bool exact, int mode, bool general, hipStream_t s) {
     if (count == 0 || total_tiles == 0) return hipSuccess;
     dim3 grid(total_tiles), block(256);
 #define RISLTC_BATCH_LAUNCH(XX, MM) \
-    hipLaunchKernelGGL((shade_batch_kernel<XX, MM>), grid, block, lds, s, d_views, d_tile_offsets, count)
+    do { \
+        if (general) \
+            hipLaunchKernelGGL((shade_batch_kernel<XX, MM, true>), grid, block, lds, s, d_views, d_tile_offsets, count); \
+        else \
+            hipLaunchKernelGGL((shade_batch_kernel<XX, MM, false>), grid, block, lds, s, d_views, d_tile_offsets, count); \
+    } while (0)
     if (exact) {
         if (mode == 1) RISLTC_BATCH_LAUNCH(true, 1);
         else RISLTC_BATCH_LAUNCH(true, 0);

@@ -51,6 +51,8 @@ def test_noalloc_limit_matches_header():
     text = open(HEADER).read()
     m = re.search(r"#define RISLTC_NOALLOC_MAX_SAMPLES \(1u << (\d+)\)", text)
     assert m and (1 << int(m.group(1))) == abi.NOALLOC_MAX_SAMPLES
+    m = re.search(r"#define RISLTC_MAX_SAMPLE_COUNT (\d+)u", text)
+    assert m and int(m.group(1)) == abi.MAX_SAMPLE_COUNT
 
 
 def test_struct_layouts():

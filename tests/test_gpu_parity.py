@@ -346,7 +346,7 @@ def test_error_paths(small_gpu, ltc_table):
         ctx.shade(g["consts"], s, g["vis"], out)
     assert e.value.code == abi.E_STATE
     ctx.load_scene(g["scene"], *ltc_table)
-    for bad, code in ((dict(sample_count=2), abi.E_UNSUPPORTED), (dict(polygon_sampling_technique=3),
+    for bad, code in ((dict(sample_count=0), abi.E_ARG), (dict(polygon_sampling_technique=3),
                                                                    abi.E_UNSUPPORTED),
                       (dict(polygon_sampling_technique=0), abi.E_UNSUPPORTED),
                       (dict(polygon_sampling_technique=abi.SAMPLE_POLYGON_AREA_TURK, proposal=abi.PROPOSAL_CACHE),

@@ -495,3 +495,79 @@ def test_baselines_reject_learned_source(small_oracle, small_scene):
     s = small_scene.settings(polygon_sampling_technique=abi.SAMPLE_POLYGON_AREA_TURK, proposal=abi.PROPOSAL_CACHE)
     with pytest.raises(Exception):
         o.shade(consts, s, vis, pixels=np.arange(4, dtype=np.uint32))
+
+
+# ---------------------------------------------------------------- SAMPLE_COUNT
+@pytest.mark.parametrize("technique,triangle_only", [(abi.SAMPLE_POLYGON_LTC_CP, False),
+                                                     (abi.SAMPLE_POLYGON_AREA_TURK, True),
+                                                     (abi.SAMPLE_POLYGON_PROJECTED_SOLID_ANGLE, False)])
+@pytest.mark.parametrize("light_sampling", [abi.LIGHT_UNIFORM, abi.LIGHT_RESERVOIR])
+def test_sample_count_converges(orc_lib, ltc_table, technique, triangle_only, light_sampling):
+    """SAMPLE_COUNT = 3 (shading_pass.frag.glsl:365-396, 412-417, 518-559):
+    every technique still converges to the quadrature of its light (a sum
+    not divided by SAMPLE_COUNT would be three times too large)."""
+    scene = _single_light_scene(0.3, 0.0)
+    o = orc.Oracle(scene, *ltc_table)
+    vis = o.render_visibility(o.frame_constants(exposure=1.0))
+    ground = np.flatnonzero(vis < 2)
+    pix = ground[np.linspace(0, ground.size - 1, 4).astype(int)].astype(np.uint32)
+    s = scene.settings(sample_count_light=128, ris_candidate_count=2, shadow_rays=0, light_sampling=light_sampling,
+                       polygon_sampling_technique=technique, sample_count=3)
+    means = []
+    for seed in range(8):
+        c = o.frame_constants(exposure=1.0, noise_seed=seed, animate=True)
+        out, rec, _ = o.shade(c, s, vis, pixels=pix, fp64=True, records=(seed == 0))
+        means.append(out[:, :3])
+        if seed == 0:
+            pos = rec[:, 0, abi.REC_POS:abi.REC_POS + 3].astype(np.float64)
+    means = np.array(means)
+    est, se = means.mean(0), means.std(0, ddof=1) / math.sqrt(means.shape[0])
+    for p in range(pix.size):
+        ref = _quadrature(scene, pos[p], 1.0, triangle_only)
+        assert np.all(np.abs(est[p] - ref) < 5 * se[p] + 4e-3 * ref), (p, est[p], ref, se[p])
+
+
+def test_sample_count_averages_independent_draws(orc_lib, ltc_table):
+    """With one light and uniform light selection the final sample is the
+    mean of SAMPLE_COUNT independent MIS samples: the variance of a pixel over
+    noise seeds falls by the factor SAMPLE_COUNT (4 against 1; draws reused
+    across iterations would leave it unchanged)."""
+    scene = _single_light_scene(0.3, 0.0)
+    o = orc.Oracle(scene, *ltc_table)
+    vis = o.render_visibility(o.frame_constants(exposure=1.0))
+    ground = np.flatnonzero(vis < 2)
+    pix = ground[np.linspace(0, ground.size - 1, 4).astype(int)].astype(np.uint32)
+    var = {}
+    for k in (1, 4):
+        s = scene.settings(sample_count_light=4, shadow_rays=0, light_sampling=abi.LIGHT_UNIFORM, sample_count=k)
+        outs = [o.shade(o.frame_constants(exposure=1.0, noise_seed=seed, animate=True), s, vis, pixels=pix)[0][:, :3]
+                for seed in range(300)]
+        var[k] = np.array(outs, np.float64).var(0, ddof=1).sum(1)
+    ratio = var[4] / var[1]
+    # the ratio of two variance estimates of 299 degrees of freedom each
+    assert np.all((ratio > 0.25 / 1.5) & (ratio < 0.25 * 1.5)), ratio
+
+
+@pytest.mark.parametrize("light_sampling", [abi.LIGHT_UNIFORM, abi.LIGHT_RESERVOIR])
+def test_area_sampling_shares_one_visibility(small_oracle, small_scene, light_sampling):
+    """Area sampling with SAMPLE_COUNT = 2 and one sample per pixel: the loop
+    of evaluate_polygonal_light_shading keeps one visibility flag
+    (shading_pass.frag.glsl:412-417, get_polygon_visibility 112-129 traces
+    only while it is true) and the caller multiplies by it (720, 755), so a
+    pixel is either its unshadowed value or zero -- never the mean of one
+    visible and one occluded sample. The draws do not depend on visibility,
+    so shadowed and unshadowed frames use the same points."""
+    o = small_oracle
+    consts = o.frame_constants()
+    vis = o.render_visibility(consts)
+    kw = dict(sample_count_light=1, ris_candidate_count=4, light_sampling=light_sampling, sample_count=2,
+              polygon_sampling_technique=abi.SAMPLE_POLYGON_AREA_TURK)
+    shadowed, _, _ = o.shade(consts, small_scene.settings(shadow_rays=1, **kw), vis)
+    free, _, _ = o.shade(consts, small_scene.settings(shadow_rays=0, **kw), vis)
+    a, b = shadowed[:, :3], free[:, :3]
+    same = np.all(a.view(np.uint32) == b.view(np.uint32), axis=1)
+    zero = np.all(a == 0, axis=1)
+    assert np.all(same | zero)
+    lit = np.any(b > 0, axis=1)
+    # the scene has both outcomes among lit pixels
+    assert (lit & same).sum() > 100 and (lit & zero & ~same).sum() > 100
