@@ -233,7 +233,16 @@ def bench_learned_batch(ltc_rgba, ltc_rg, steps, warmup, views_n=16, grid=540, l
                 infer["ms"] += ctx.proposal_last_timings()[0]
         infer["steps"] += 1
 
+    # LDS staging that keeps three work-groups per CU: 1024 spheres for the
+    # plain batch; none for the learned batch, whose cache rows take LDS
+    tuning = ctx.get_tuning().as_dict()
+
+    def plain_step():
+        ctx.set_tuning(**dict(tuning, lds_lights=1024))
+        ctx.shade_batch(plain)
+
     def learned_step():
+        ctx.set_tuning(**dict(tuning, lds_lights=lk["lds_lights"]))
         refresh_tables()
         ctx.shade_batch(learned)
 
@@ -247,7 +256,7 @@ def bench_learned_batch(ltc_rgba, ltc_rg, steps, warmup, views_n=16, grid=540, l
         torch.cuda.synchronize()
         return (time.perf_counter() - t) * 1e3 / steps
 
-    ms_plain = timed(lambda: ctx.shade_batch(plain))
+    ms_plain = timed(plain_step)
     k_plain = ctx.last_timings()
     ms_learned = timed(learned_step)
     k_learned = ctx.last_timings()

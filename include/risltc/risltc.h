@@ -682,7 +682,11 @@ int risltc_shade_batch(risltc_context_t* context, const risltc_view_t* views, ui
  *   cull_chunk   1..32: candidates tested against the bounding spheres per
  *                step of the cull loop;
  *   lds_lights   0..2048: bounding spheres staged in LDS per work-group
- *                (the rest are read from global memory);
+ *                (the rest are read from global memory; default 1024:
+ *                more cost a work-group per CU). The learned pass also
+ *                stages its cluster tables when they take at most 16 KiB;
+ *                with a visibility cache it keeps three work-groups per CU
+ *                at lds_lights = 0;
  *   trace_refill 1..64: idle lanes at which a wave of the persistent
  *                shadow-ray kernel fetches new rays. */
 typedef struct risltc_tuning_s {

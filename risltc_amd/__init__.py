@@ -483,7 +483,7 @@ class Context:
         self._check(self.L.risltc_get_tuning(self.h, C.byref(t)), "risltc_get_tuning")
         return t
 
-    def set_tuning(self, wave_width=16, cull_chunk=32, lds_lights=2048, trace_refill=32):
+    def set_tuning(self, wave_width=16, cull_chunk=32, lds_lights=1024, trace_refill=32):
         t = abi.Tuning(wave_width, cull_chunk, lds_lights, trace_refill)
         self._check(self.L.risltc_set_tuning(self.h, C.byref(t)), "risltc_set_tuning")
 

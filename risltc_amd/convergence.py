@@ -187,12 +187,13 @@ def time_to(curve, target):
 # The learned configuration the experiment uses: learned source (MLP
 # inference every 2 frames, one Adam step of 8192 samples per frame, learning
 # rate 0.02), target floor 1/128 with a per-light visibility cache of 2^16
-# one-metre cells, M = 24 candidates, 512 light spheres staged in LDS (the
-# learned pass's cluster tables and cache rows need LDS too; with all 1024
-# spheres a CU holds two work-groups instead of three).
+# one-metre cells, M = 24 candidates, no light spheres staged in LDS: the
+# learned pass stages its cluster tables and cache rows there, and with
+# spheres too a CU holds two work-groups instead of three (the spheres are
+# read from L2 at almost no cost).
 DEFAULT_LEARNED = dict(floor_log2=7, infer_every=2, train_every=1, batch=8192, lr=2e-2, clusters=256, seed=1,
                        cache_log2=16, cache_cell=1.0, cache_prior=1.0, cache_per_light=True, candidates=24,
-                       lds_lights=512)
+                       lds_lights=0)
 
 
 def reference(ctx, scene, vis, frames, chunk=2048, seed0=1000000, log=None, camera=None):

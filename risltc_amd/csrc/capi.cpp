@@ -117,7 +117,9 @@ struct risltc_context_s {
     uint32_t vcache_log2 = 0, vcache_slots = 0, vcache_per_light = 0;
     float vcache_cell = 1.0f, vcache_prior = 2.0f;
     // launch shape (autotuned; RISLTC_TRACE_REFILL overrides the refill at create)
-    risltc_tuning_t tuning = {16, 32, rl::kLdsSphereMax, 32};
+    // (1024 spheres: 16 KB of LDS, so three work-groups fit a CU with the
+    // pixel area; more cost a work-group per CU)
+    risltc_tuning_t tuning = {16, 32, 1024, 32};
     // extra light sets and meshes for batched views (id = index + 1)
     std::vector<LightSet*> light_sets;
     std::vector<SceneSet*> scene_sets;

@@ -1304,14 +1304,26 @@ __device__ __forceinline__ uint32_t cache_byte(uint2 c, float prior) {
 // the spheres and the cluster table, 16-byte aligned.
 constexpr uint32_t kRowLdsMax = 1024, kRowsPerWave = 4;
 
-// bytes of the spheres and, for the learned source with every light staged,
-// the cluster tables (lights by cluster, offsets, cluster of each light)
-__device__ __host__ __forceinline__ size_t cluster_lds_end(uint32_t n_lds, bool cluster_tables, uint32_t clusters) {
-    return (size_t)n_lds * 16u + (cluster_tables ? (size_t)n_lds * 6u + ((size_t)clusters + 1u) * 4u : 0u);
+// The learned pass stages the cluster tables (lights by cluster u32,
+// offsets u32, cluster of each light u16) after the spheres whenever they
+// take at most kLdsTableMax bytes, however many spheres are staged: the
+// source's draw and density read them for every candidate.
+constexpr uint32_t kLdsTableMax = 16384;
+
+// lights whose cluster tables are staged (0: none)
+__device__ __host__ __forceinline__ uint32_t lds_table_lights(uint32_t learned_pass, uint32_t lights,
+                                                              uint32_t clusters) {
+    const size_t bytes = (size_t)lights * 6u + ((size_t)clusters + 1u) * 4u;
+    return (learned_pass && clusters > 0u && bytes <= kLdsTableMax) ? lights : 0u;
 }
 
-__device__ __host__ __forceinline__ size_t row_lds_offset(uint32_t n_lds, bool cluster_tables, uint32_t clusters) {
-    return (cluster_lds_end(n_lds, cluster_tables, clusters) + 15u) & ~(size_t)15u;
+// bytes of the spheres and the staged cluster tables
+__device__ __host__ __forceinline__ size_t cluster_lds_end(uint32_t n_lds, uint32_t table_lights, uint32_t clusters) {
+    return (size_t)n_lds * 16u + (table_lights ? (size_t)table_lights * 6u + ((size_t)clusters + 1u) * 4u : 0u);
+}
+
+__device__ __host__ __forceinline__ size_t row_lds_offset(uint32_t n_lds, uint32_t table_lights, uint32_t clusters) {
+    return (cluster_lds_end(n_lds, table_lights, clusters) + 15u) & ~(size_t)15u;
 }
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -1390,9 +1402,10 @@ __device__ __forceinline__ uint32_t nth_set_bit64(uint64_t m, uint32_t n) {
 }
 
 // LDS offset of the pixel area: after the spheres, cluster tables and rows
-__device__ __host__ __forceinline__ size_t px_lds_offset(uint32_t n_lds, bool tables, uint32_t clusters, bool rows) {
-    size_t b = rows ? row_lds_offset(n_lds, tables, clusters) + 4u * kRowsPerWave * kRowLdsMax
-                    : cluster_lds_end(n_lds, tables, clusters);
+__device__ __host__ __forceinline__ size_t px_lds_offset(uint32_t n_lds, uint32_t table_lights, uint32_t clusters,
+                                                         bool rows) {
+    size_t b = rows ? row_lds_offset(n_lds, table_lights, clusters) + 4u * kRowsPerWave * kRowLdsMax
+                    : cluster_lds_end(n_lds, table_lights, clusters);
     return (b + 15u) & ~(size_t)15u;
 }
 
@@ -1429,16 +1442,17 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                   A.proposal_group_log2};
     const uint32_t n_lds = A.scene.light_count < A.lds_lights ? A.scene.light_count : A.lds_lights;
     const bool all_lds = n_lds == A.scene.light_count;
-    if (n_lds > 0u) {
+    const uint32_t tab_n = learned ? lds_table_lights(A.learned_pass, A.scene.light_count, A.clusters.count) : 0u;
+    if (n_lds > 0u || tab_n > 0u) {
         for (uint32_t k = threadIdx.x; k < n_lds; k += blockDim.x) sh_spheres[k] = A.scene.spheres[k];
         if constexpr (learned) {
-            if (all_lds) {
+            if (tab_n > 0u) {
                 // after the spheres: lights by cluster (u32), offsets (u32),
                 // cluster of light (u16)
                 uint32_t* l = reinterpret_cast<uint32_t*>(sh_spheres + n_lds);
-                uint32_t* o = l + n_lds;
+                uint32_t* o = l + tab_n;
                 uint16_t* c = reinterpret_cast<uint16_t*>(o + A.clusters.count + 1u);
-                for (uint32_t k = threadIdx.x; k < n_lds; k += blockDim.x) {
+                for (uint32_t k = threadIdx.x; k < tab_n; k += blockDim.x) {
                     l[k] = A.clusters.lights[k];
                     c[k] = A.clusters.of_light[k];
                 }
@@ -1485,7 +1499,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
         if (A.cache.vis) {
             crow = A.cache.vis + (size_t)ckey * A.cache.slots;
             if (A.cache.slots <= kRowLdsMax) {
-                const uint32_t wave_base = (uint32_t)row_lds_offset(n_lds, learned && all_lds, A.clusters.count) +
+                const uint32_t wave_base = (uint32_t)row_lds_offset(n_lds, tab_n, A.clusters.count) +
                                            wave * kRowsPerWave * kRowLdsMax;
                 uint8_t* area = reinterpret_cast<uint8_t*>(sh_spheres) + wave_base;
                 const uint32_t slots = A.cache.slots;
@@ -1520,7 +1534,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
     if constexpr (!DEBUG) {
         const bool rows = learned && A.cache.vis != nullptr && A.cache.slots <= kRowLdsMax;
         pxa = reinterpret_cast<float*>(reinterpret_cast<uint8_t*>(sh_spheres) +
-                                       px_lds_offset(n_lds, learned && all_lds, A.clusters.count, rows)) +
+                                       px_lds_offset(n_lds, tab_n, A.clusters.count, rows)) +
               wave * kPxFloats * 64u;
         px_lds_store(pxa, lane, px);
         wave_lds_sync();
@@ -2142,7 +2156,7 @@ uint32_t shade_tiles(const ShadeArgs& a) {
 
 size_t shade_lds_bytes(const ShadeArgs& a) {
     const uint32_t n = a.scene.light_count < a.lds_lights ? a.scene.light_count : a.lds_lights;
-    const bool tables = a.learned_pass && n == a.scene.light_count;
+    const uint32_t tables = lds_table_lights(a.learned_pass, a.scene.light_count, a.clusters.count);
     // learned pass with a cache: staged rows, kRowsPerWave per wave
     const bool rows = a.learned_pass && a.cache.vis && a.cache.slots <= kRowLdsMax;
     // then the wave's pixels for the compacted evaluation
