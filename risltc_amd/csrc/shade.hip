@@ -1304,13 +1304,15 @@ __device__ __forceinline__ uint32_t cache_byte(uint2 c, float prior) {
 // the spheres and the cluster table, 16-byte aligned.
 constexpr uint32_t kRowLdsMax = 1024, kRowsPerWave = 4;
 
-// The learned pass stages the cluster tables (lights by cluster u32,
-// offsets u32, cluster of each light u16) after the spheres whenever they
-// take at most kLdsTableMax bytes, however many spheres are staged: the
-// source's draw and density read them for every candidate.
+// The learned pass stages its cluster tables after the spheres, however
+// many spheres are staged: the source's draw and density read them for
+// every candidate. The cluster offsets (u32, clusters + 1) always; the
+// lights by cluster (u32) and the cluster of each light (u16) when the
+// three take at most kLdsTableMax bytes. Layout: lights by cluster, offsets,
+// cluster of each light.
 constexpr uint32_t kLdsTableMax = 16384;
 
-// lights whose cluster tables are staged (0: none)
+// lights whose per-light tables are staged (0: none)
 __device__ __host__ __forceinline__ uint32_t lds_table_lights(uint32_t learned_pass, uint32_t lights,
                                                               uint32_t clusters) {
     const size_t bytes = (size_t)lights * 6u + ((size_t)clusters + 1u) * 4u;
@@ -1318,12 +1320,15 @@ __device__ __host__ __forceinline__ uint32_t lds_table_lights(uint32_t learned_p
 }
 
 // bytes of the spheres and the staged cluster tables
-__device__ __host__ __forceinline__ size_t cluster_lds_end(uint32_t n_lds, uint32_t table_lights, uint32_t clusters) {
-    return (size_t)n_lds * 16u + (table_lights ? (size_t)table_lights * 6u + ((size_t)clusters + 1u) * 4u : 0u);
+__device__ __host__ __forceinline__ size_t cluster_lds_end(uint32_t n_lds, uint32_t learned_pass, uint32_t table_lights,
+                                                           uint32_t clusters) {
+    const bool offsets = learned_pass && clusters > 0u;
+    return (size_t)n_lds * 16u + (size_t)table_lights * 6u + (offsets ? ((size_t)clusters + 1u) * 4u : 0u);
 }
 
-__device__ __host__ __forceinline__ size_t row_lds_offset(uint32_t n_lds, uint32_t table_lights, uint32_t clusters) {
-    return (cluster_lds_end(n_lds, table_lights, clusters) + 15u) & ~(size_t)15u;
+__device__ __host__ __forceinline__ size_t row_lds_offset(uint32_t n_lds, uint32_t learned_pass, uint32_t table_lights,
+                                                          uint32_t clusters) {
+    return (cluster_lds_end(n_lds, learned_pass, table_lights, clusters) + 15u) & ~(size_t)15u;
 }
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -1402,10 +1407,10 @@ __device__ __forceinline__ uint32_t nth_set_bit64(uint64_t m, uint32_t n) {
 }
 
 // LDS offset of the pixel area: after the spheres, cluster tables and rows
-__device__ __host__ __forceinline__ size_t px_lds_offset(uint32_t n_lds, uint32_t table_lights, uint32_t clusters,
-                                                         bool rows) {
-    size_t b = rows ? row_lds_offset(n_lds, table_lights, clusters) + 4u * kRowsPerWave * kRowLdsMax
-                    : cluster_lds_end(n_lds, table_lights, clusters);
+__device__ __host__ __forceinline__ size_t px_lds_offset(uint32_t n_lds, uint32_t learned_pass, uint32_t table_lights,
+                                                         uint32_t clusters, bool rows) {
+    size_t b = rows ? row_lds_offset(n_lds, learned_pass, table_lights, clusters) + 4u * kRowsPerWave * kRowLdsMax
+                    : cluster_lds_end(n_lds, learned_pass, table_lights, clusters);
     return (b + 15u) & ~(size_t)15u;
 }
 
@@ -1443,12 +1448,13 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
     const uint32_t n_lds = A.scene.light_count < A.lds_lights ? A.scene.light_count : A.lds_lights;
     const bool all_lds = n_lds == A.scene.light_count;
     const uint32_t tab_n = learned ? lds_table_lights(A.learned_pass, A.scene.light_count, A.clusters.count) : 0u;
-    if (n_lds > 0u || tab_n > 0u) {
+    const bool tab_o = learned && A.learned_pass && A.clusters.count > 0u;
+    if (n_lds > 0u || tab_o) {
         for (uint32_t k = threadIdx.x; k < n_lds; k += blockDim.x) sh_spheres[k] = A.scene.spheres[k];
         if constexpr (learned) {
-            if (tab_n > 0u) {
-                // after the spheres: lights by cluster (u32), offsets (u32),
-                // cluster of light (u16)
+            if (tab_o) {
+                // after the spheres: lights by cluster (u32, when staged),
+                // offsets (u32), cluster of light (u16, when staged)
                 uint32_t* l = reinterpret_cast<uint32_t*>(sh_spheres + n_lds);
                 uint32_t* o = l + tab_n;
                 uint16_t* c = reinterpret_cast<uint16_t*>(o + A.clusters.count + 1u);
@@ -1457,7 +1463,8 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                     c[k] = A.clusters.of_light[k];
                 }
                 for (uint32_t k = threadIdx.x; k <= A.clusters.count; k += blockDim.x) o[k] = A.clusters.offsets[k];
-                cl = ClusterLds{c, o, l, A.clusters.count, A.proposal_group_log2};
+                cl = ClusterLds{tab_n ? c : A.clusters.of_light, o, tab_n ? l : A.clusters.lights, A.clusters.count,
+                                A.proposal_group_log2};
             }
         }
         __syncthreads();
@@ -1499,7 +1506,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
         if (A.cache.vis) {
             crow = A.cache.vis + (size_t)ckey * A.cache.slots;
             if (A.cache.slots <= kRowLdsMax) {
-                const uint32_t wave_base = (uint32_t)row_lds_offset(n_lds, tab_n, A.clusters.count) +
+                const uint32_t wave_base = (uint32_t)row_lds_offset(n_lds, A.learned_pass, tab_n, A.clusters.count) +
                                            wave * kRowsPerWave * kRowLdsMax;
                 uint8_t* area = reinterpret_cast<uint8_t*>(sh_spheres) + wave_base;
                 const uint32_t slots = A.cache.slots;
@@ -1534,7 +1541,8 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
     if constexpr (!DEBUG) {
         const bool rows = learned && A.cache.vis != nullptr && A.cache.slots <= kRowLdsMax;
         pxa = reinterpret_cast<float*>(reinterpret_cast<uint8_t*>(sh_spheres) +
-                                       px_lds_offset(n_lds, tab_n, A.clusters.count, rows)) +
+                                       px_lds_offset(n_lds, learned ? A.learned_pass : 0u, tab_n, A.clusters.count,
+                                                     rows)) +
               wave * kPxFloats * 64u;
         px_lds_store(pxa, lane, px);
         wave_lds_sync();
@@ -2160,7 +2168,7 @@ size_t shade_lds_bytes(const ShadeArgs& a) {
     // learned pass with a cache: staged rows, kRowsPerWave per wave
     const bool rows = a.learned_pass && a.cache.vis && a.cache.slots <= kRowLdsMax;
     // then the wave's pixels for the compacted evaluation
-    return px_lds_offset(n, tables, a.clusters.count, rows) + kPxLdsBytes;
+    return px_lds_offset(n, a.learned_pass, tables, a.clusters.count, rows) + kPxLdsBytes;
 }
 
 static bool shade_args_ok(const ShadeArgs& a) {
