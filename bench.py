@@ -351,6 +351,58 @@ def bench_large_convergence(ltc_rgba, ltc_rg, frames=128, ref_frames=4096):
         ctx.close()
 
 
+def bench_lds_staging(ltc_rgba, ltc_rg, repeats=8):
+    """Shade-kernel time (median of `repeats` frames, HIP events) of the
+    1080p city at its held-out view for each number of light spheres staged
+    in LDS: the plain pass (M = 32; all 1024 spheres take 16 KiB) and the
+    learned pass of the learned configuration (M = 24, tables from a few
+    trained frames, per-light cache rows and the cluster tables in LDS too).
+    With the pixel area (28 KiB) a work-group takes 28 KiB + 16 B per sphere
+    (plain) or that plus 23 KiB (learned); three work-groups fit the CU's
+    160 KiB up to 53 KiB each."""
+    import torch
+    import risltc_amd
+    from risltc_amd import abi, synth, convergence as cv
+    sc = synth.city_scene()
+    ctx = risltc_amd.Context(0)
+    try:
+        ctx.load_scene(sc, ltc_rgba, ltc_rg)
+        cam = cv.held_out_camera(sc)
+        c = ctx.frame_constants(camera=cam)
+        P = sc.width * sc.height
+        vis = torch.empty(P, dtype=torch.int32, device="cuda")
+        ctx.render_visibility(c, vis)
+        out = torch.empty(P * 4, dtype=torch.float32, device="cuda")
+        lk = cv.DEFAULT_LEARNED
+        ctx.cluster_lights(lk["clusters"], 20)
+        ctx.proposal_init(lk["seed"], width=64, outputs=256)
+        ctx.proposal_cache(lk["cache_log2"], lk["cache_cell"], lk["cache_prior"], lk["cache_per_light"])
+        learned = sc.settings(proposal=abi.PROPOSAL_LEARNED, ris_candidate_count=lk["candidates"],
+                              learned_floor_log2=lk["floor_log2"])
+        for f in range(6):
+            cf = ctx.frame_constants(noise_seed=f, animate=True, camera=cam)
+            ctx.proposal_update(cf, vis)
+            ctx.shade(cf, learned, vis, out)
+            ctx.proposal_train(cf, vis, batch=lk["batch"], steps=1, lr=lk["lr"], seed=f)
+        learned.skip_training_records = 1
+        tuning = ctx.get_tuning().as_dict()
+        res = {"view": "held-out camera of the 1920x1080 city (1024 lights)", "plain_shade_ms": {},
+               "learned_shade_ms": {}}
+        for lds in (0, 256, 512, 1024):
+            ctx.set_tuning(**dict(tuning, lds_lights=lds))
+            for key, s in (("plain_shade_ms", sc.settings()), ("learned_shade_ms", learned)):
+                ts = []
+                for _ in range(repeats + 2):
+                    ctx.shade(c, s, vis, out)
+                    torch.cuda.synchronize()
+                    ts.append(ctx.last_timings()[0])
+                res[key][str(lds)] = round(float(np.median(ts[2:])), 4)
+        ctx.set_tuning(**tuning)
+        return res
+    finally:
+        ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -374,6 +426,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--learned-frames", type=int, default=0,
                     help="only run N frames of the learned pass (for profiling its kernels)")
+    ap.add_argument("--lds-ab", action="store_true",
+                    help="only time the shade kernel for each number of light spheres staged in LDS")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (known HBM bytes) for counter scaling")
     args = ap.parse_args()
@@ -383,6 +437,11 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.lds_ab:
+        from risltc_amd import ltcfit
+        rgba, rg, _ = ltcfit.table(51)
+        print(json.dumps({"lds_staging": bench_lds_staging(rgba, rg)}))
+        return
     if args.copy_calibration:
         # an elementwise kernel (not a blit) that reads 1 GiB and writes 1 GiB
         n = 1 << 28
