@@ -2032,6 +2032,15 @@ shade_batch_kernel(const ShadeArgs* views, const uint32_t* tile_offsets, uint32_
 }
 
 // ---- resolve: sum the estimates in the reference's order -------------------
+// adds the n pending (slot, count pair) of one cache cell
+__device__ __forceinline__ void flush_counts(const VisCacheView& C, uint32_t key, const uint32_t (&slot)[4],
+                                             const unsigned long long (&add)[4], uint32_t n) {
+#pragma unroll
+    for (uint32_t q = 0; q < 4u; ++q)
+        if (q < n)
+            atomicAdd(reinterpret_cast<unsigned long long*>(C.counts + (size_t)key * C.slots + slot[q]), add[q]);
+}
+
 __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pixel) {
     if (pixel >= A.width * A.height) return;
     uint32_t prim = A.visibility[pixel];
@@ -2052,6 +2061,10 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
             fin = mk3(1.0f, 1.0f, 1.0f);
         } else {
             const uint32_t K = A.iters;
+            // the cache counts of the pixel's samples (one cell for all of
+            // them), merged per slot before the atomics: up to 4 pending
+            uint32_t pslot[4] = {0u, 0u, 0u, 0u}, np = 0u, pkey = 0u;
+            unsigned long long padd[4] = {0ull, 0ull, 0ull, 0ull};
             for (uint32_t j = 0; j < A.spp; ++j) {
                 size_t sidx = (size_t)pixel * A.spp + j;
                 const float4* rec = reinterpret_cast<const float4*>(A.samples + sidx * K * kSampleWords);
@@ -2117,16 +2130,36 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
                     }
                     if (A.cache.counts && ok) {
                         // (unoccluded, occluded) in one 64-bit add
-                        unsigned long long add =
+                        const unsigned long long add =
                             (unsigned long long)unoccluded | ((unsigned long long)(traced - unoccluded) << 32);
                         const uint32_t slot = A.cache.per_light ? (uint32_t)li : cl;
-                        const uint32_t key = __float_as_uint(r3.w);
-                        atomicAdd(reinterpret_cast<unsigned long long*>(A.cache.counts + (size_t)key * A.cache.slots +
-                                                                        slot),
-                                  add);
-                        if (A.cache.dirty[key] == 0u) A.cache.dirty[key] = 1u;
+                        pkey = __float_as_uint(r3.w);
+                        bool merged = false;
+#pragma unroll
+                        for (uint32_t q = 0; q < 4u; ++q)
+                            if (q < np && pslot[q] == slot) {
+                                padd[q] += add;
+                                merged = true;
+                            }
+                        if (!merged) {
+                            if (np == 4u) {
+                                flush_counts(A.cache, pkey, pslot, padd, np);
+                                np = 0u;
+                            }
+#pragma unroll
+                            for (uint32_t q = 0; q < 4u; ++q)
+                                if (q == np) {
+                                    pslot[q] = slot;
+                                    padd[q] = add;
+                                }
+                            ++np;
+                        }
                     }
                 }
+            }
+            if (np > 0u) {
+                flush_counts(A.cache, pkey, pslot, padd, np);
+                if (A.cache.dirty[pkey] == 0u) A.cache.dirty[pkey] = 1u;
             }
         }
         if (__builtin_isnan(fin.x) || __builtin_isnan(fin.y) || __builtin_isnan(fin.z) || __builtin_isinf(fin.x) ||
