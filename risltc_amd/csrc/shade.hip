@@ -1166,6 +1166,16 @@ __device__ __forceinline__ uint32_t byte_of(const uint32_t* v, int s) { return (
 
 __device__ __forceinline__ uint32_t id_of(const PropTable& P, int s) { return byte_of(P.ids, s); }
 
+// sum of the four bytes of v (v_sad_u8 against zero)
+__device__ __forceinline__ uint32_t byte_sum(uint32_t v) { return __builtin_amdgcn_sad_u8(v, 0u, 0u); }
+
+// 0xFF in every byte of x that is zero, 0 in the others
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t x) {
+    uint32_t y = (x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;  // bit 7 set where the low 7 bits are not all zero
+    y = ~(y | x | 0x7F7F7F7Fu);                     // 0x80 exactly in the zero bytes
+    return (y >> 7) * 0xFFu;
+}
+
 // loads the pixel's table (zeros when tab is null)
 __device__ __forceinline__ void load_prop_table(const uint8_t* tab, PropTable& P) {
     P.T = 0u;
@@ -1180,7 +1190,7 @@ __device__ __forceinline__ void load_prop_table(const uint8_t* tab, PropTable& P
     P.ids[0] = t0.x, P.ids[1] = t0.y, P.ids[2] = t0.z, P.ids[3] = t0.w;
     P.ws[0] = t1.x, P.ws[1] = t1.y, P.ws[2] = t1.z, P.ws[3] = t1.w;
 #pragma unroll
-    for (int s2 = 0; s2 < kPropTop; ++s2) P.T += byte_of(P.ws, s2);
+    for (int k = 0; k < 4; ++k) P.T += byte_sum(P.ws[k]);
 }
 
 struct ClusterLds {
@@ -1232,17 +1242,16 @@ __device__ __forceinline__ uint32_t draw_learned(uint32_t& seed, const PropTable
     float u = (a + a) - 1.0f;
     uint32_t t = (uint32_t)(u * (float)P.T);
     if (t > P.T - 1u) t = P.T - 1u;
-    uint32_t acc = 0, g = 0;
-    bool found = false;
-#pragma unroll
-    for (int s = 0; s < kPropTop; ++s) {
-        uint32_t w = byte_of(P.ws, s);
-        if (!found && t < acc + w) {
-            g = id_of(P, s);
-            found = true;
-        }
-        acc += w;
-    }
+    // the first slot whose running weight sum passes t: first its word of
+    // four slots, then the slot in the word
+    const uint32_t c1 = byte_sum(P.ws[0]), c2 = c1 + byte_sum(P.ws[1]), c3 = c2 + byte_sum(P.ws[2]);
+    const uint32_t wi = (t >= c1 ? 1u : 0u) + (t >= c2 ? 1u : 0u) + (t >= c3 ? 1u : 0u);
+    const uint32_t ww = wi == 0u ? P.ws[0] : wi == 1u ? P.ws[1] : wi == 2u ? P.ws[2] : P.ws[3];
+    const uint32_t iw = wi == 0u ? P.ids[0] : wi == 1u ? P.ids[1] : wi == 2u ? P.ids[2] : P.ids[3];
+    const uint32_t rem = t - (wi == 0u ? 0u : wi == 1u ? c1 : wi == 2u ? c2 : c3);
+    const uint32_t b0 = ww & 0xFFu, b1 = b0 + ((ww >> 8) & 0xFFu), b2 = b1 + ((ww >> 16) & 0xFFu);
+    const uint32_t bi = (rem >= b0 ? 1u : 0u) + (rem >= b1 ? 1u : 0u) + (rem >= b2 ? 1u : 0u);
+    const uint32_t g = (iw >> (8u * bi)) & 0xFFu;
     // a group id outside the groups or an empty group: uniform
     if (g >= group_count(C)) return light_index_from(b, N);
     const uint2 r = group_range(C, g);
@@ -1259,11 +1268,13 @@ __device__ __forceinline__ float learned_density(uint32_t idx, const PropTable& 
     if (P.T == 0u) return 1.0f / (float)N;
     if (P.T_bad != 0u) q = q + div_<X>(0.5f * (float)P.T_bad, (float)P.T * (float)N);
     const uint32_t g = (uint32_t)C.of_light[idx] >> C.group_log2;
+    // the weights of the slots whose id is g, four slots per step (ids are
+    // bytes, so a group past 255 is in no slot)
     uint32_t wg = 0;
+    if (g < 256u) {
+        const uint32_t rep = g * 0x01010101u;
 #pragma unroll
-    for (int s = 0; s < kPropTop; ++s) {
-        uint32_t w = byte_of(P.ws, s);
-        if (w != 0u && id_of(P, s) == g) wg += w;
+        for (int k = 0; k < 4; ++k) wg += byte_sum(P.ws[k] & zero_bytes(P.ids[k] ^ rep));
     }
     if (wg != 0u) {
         const uint2 r = group_range(C, g);
