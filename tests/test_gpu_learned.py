@@ -692,3 +692,39 @@ def test_k1024_narrow_network_with_dead_outputs(ltc_table):
         _check_learned_bitwise(g)
     finally:
         g["ctx"].close()
+
+
+@pytest.mark.parametrize("lds_lights", [0, 13])
+def test_learned_staging_layouts_bitwise_vs_oracle(learned, lds_lights):
+    """The learned pass with its cluster tables and per-light cache rows in
+    LDS and no spheres (0) or a part of them (13 of 64) staged before the
+    tables: the exact frame equals the oracle's given the same tables,
+    clusters and cache."""
+    import torch
+    g = learned
+    sc, ctx, P = g["scene"], g["ctx"], g["P"]
+    rng = np.random.default_rng(29)
+    log2, cell, prior = 9, 0.5, 2.0
+    counts = np.zeros((1 << log2, sc.light_count, 2), dtype=np.uint32)
+    counts[:] = rng.integers(0, 40, counts.shape)
+    before = ctx.get_tuning().as_dict()
+    ctx.proposal_cache(log2, cell, prior, True)
+    try:
+        ctx.set_tuning(**dict(before, lds_lights=lds_lights))
+        ctx.proposal_cache_write(counts)
+        s = sc.settings(exact_math=1, sample_count_light=2, ris_candidate_count=12, proposal=abi.PROPOSAL_LEARNED,
+                        shadow_rays=0)
+        out = torch.empty(P * 4, dtype=torch.float32, device="cuda")
+        ctx.shade(g["consts"], s, g["vis"], out)
+        torch.cuda.synchronize()
+    finally:
+        ctx.set_tuning(**before)
+        ctx.proposal_cache(0)
+    oracle = orc.Oracle(sc, *g["ltc"])
+    oracle.set_proposal(g["table"], g["col"], g["K"], abi.proposal_group_log2(g["outputs"]))
+    oracle.set_cache(counts, cell, prior, True)
+    vis = g["vis"].cpu().numpy().view(np.uint32)
+    ref, _, _ = oracle.shade(g["consts"], s, vis)
+    o = out.cpu().numpy().reshape(P, 4)
+    assert np.array_equal(o.view(np.uint32), ref.view(np.uint32))
+    assert (o[:, :3].sum(1) > 0).mean() > 0.2
