@@ -62,6 +62,8 @@ HBM_PEAK_GBS = 8000.0
 # wave64 FP32 VALU instructions: 256 CUs x 4 SIMDs, 4 cycles each, 2.4 GHz
 # (non-packed: the build disables packed FP32), in lane-instructions/s
 VALU_PEAK_TLANE = 256 * 4 * 16 * 2.4e9 / 1e12
+# dense BF16 MFMA peak of the MI355X
+MFMA_BF16_PEAK_TF = 2500.0
 PMC_FILE = os.path.join(ROOT, "profiles", "shade_city1080_traffic.json")
 
 
@@ -351,6 +353,49 @@ def bench_large_convergence(ltc_rgba, ltc_rg, frames=128, ref_frames=4096):
         ctx.close()
 
 
+def kernel_report(kernel_ms, pixels, samples, rays, shade_alg, learned):
+    """Every kernel of the plain pass against the HBM roofline on its
+    algorithmic bytes (live times), with the measured HBM bytes and VALU
+    issue of the counter profile beside it; and the MLP inference against
+    the BF16 MFMA peak. Algorithmic bytes per launch: shade as the roofline
+    object; trace 40 B per shadow ray (the ray and its slot read, the hit
+    word written); resolve 4 B visibility + 16 B output per pixel, the
+    64-byte record of every pixel-sample and the hit word of every ray."""
+    pmc = json.load(open(PMC_FILE)) if os.path.exists(PMC_FILE) else {}
+    per = dict(pmc.get("kernels", {}))
+    if "bytes_per_launch" in pmc:
+        per["shade_kernel"] = {k: pmc[k] for k in ("bytes_per_launch", "valu_insts_per_launch",
+                                                    "valu_thread_cycles_per_launch")}
+    alg = {"shade_kernel": shade_alg, "trace_kernel": 40 * rays,
+           "resolve_kernel": 20 * pixels + 64 * samples + 4 * rays}
+    out = {"shadow_rays": rays}
+    for i, k in enumerate(("shade_kernel", "trace_kernel", "resolve_kernel")):
+        ms = float(kernel_ms[i])
+        r = {"ms": round(ms, 4), "algorithmic_bytes": alg[k],
+             "hbm_frac": round(alg[k] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 6)}
+        c = per.get(k)
+        if c:
+            r["measured_bytes"] = c["bytes_per_launch"]
+            r["measured_hbm_frac"] = round(c["bytes_per_launch"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+            r["valu_frac"] = round(c["valu_insts_per_launch"] * 64 / (ms * 1e-3) / 1e12 / VALU_PEAK_TLANE, 4)
+            r["active_lanes"] = round(c["valu_thread_cycles_per_launch"] / c["valu_insts_per_launch"] / 64, 3)
+        out[k] = r
+    if learned:
+        lk = learned["learned_config"]
+        per_call = learned["learned"]["ms_per_frame"]["inference"] * lk["infer_every"]
+        h, o = 64, 256
+        flops = 2.0 * pixels * (96 * h + 2 * h * h + h * o)
+        if per_call > 0:
+            tf = flops / (per_call * 1e-3) / 1e12
+            out["proposal_kernel"] = {"ms_per_call": round(per_call, 4), "flop_per_call": flops,
+                                      "tflops": round(tf, 1), "mfma_bf16_frac": round(tf / MFMA_BF16_PEAK_TF, 4),
+                                      "network": "96-64-64-64-256, one 1080p view per call"}
+    out["peaks"] = {"hbm_gbs": HBM_PEAK_GBS, "valu_t_lane_inst_s": round(VALU_PEAK_TLANE, 3),
+                    "mfma_bf16_tflops": MFMA_BF16_PEAK_TF}
+    out["counters_source"] = "profiles/city1080_counters.md"
+    return out
+
+
 def bench_lds_staging(ltc_rgba, ltc_rg, repeats=8):
     """Shade-kernel time (median of `repeats` frames, HIP events) of the
     1080p city at its held-out view for each number of light spheres staged
@@ -504,6 +549,7 @@ def main():
     wall = time.perf_counter() - t_start
     ms_per_step = wall * 1e3 / args.steps
     kernel_ms /= args.steps
+    rays = ctx.last_ray_count()
     samples = P * spp
     value = samples / (ms_per_step * 1e-3) / 1e6
     shade_ms = kernel_ms[0]
@@ -608,6 +654,7 @@ def main():
     if valu:
         roofline["limiter"] = "valu" if valu["frac"] >= roofline.get("measured_traffic_frac", 0.0) else "hbm"
         roofline["limiter_frac"] = max(valu["frac"], roofline.get("measured_traffic_frac", 0.0))
+    kernels = kernel_report(kernel_ms, P, samples, rays, alg, learned)
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -629,6 +676,7 @@ def main():
                                  "resolve": round(kernel_ms[2], 4)}},
         "roofline": roofline,
         "valu": valu,
+        "kernels": kernels,
         "batched_views": batch,
         "learned_batch": learned_batch,
         "learned_vs_plain": learned,

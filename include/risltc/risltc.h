@@ -621,6 +621,11 @@ int risltc_proposal_last_timings(const risltc_context_t* context, float out_ms[2
  * [3] learned-proposal inference. Entries not run are 0. */
 int risltc_last_timings(const risltc_context_t* context, float out_ms[4]);
 
+/* Shadow rays traced by the last risltc_shade or risltc_shade_noalloc call
+ * (waits for it). No counterpart in the reference: the ray count of a frame,
+ * for throughput and roofline figures. RISLTC_E_STATE before any such call. */
+int risltc_last_ray_count(const risltc_context_t* context, uint32_t* out_rays);
+
 /* ------------------------------------------------------------------------ */
 /* Batches of views. No counterpart in the reference, which shades one view
  * per frame (main.c:2050-2055); a batch shades several views (cameras,

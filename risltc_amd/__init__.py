@@ -73,6 +73,7 @@ SIGNATURES = {
     "risltc_accumulate": (C.c_int, [_vp, _vp, _vp, _u32, _u32, _vp]),
     "risltc_trace_shadow_rays": (C.c_int, [_vp, _vp, _u32, _vp, _vp]),
     "risltc_last_timings": (C.c_int, [_vp, _f32p]),
+    "risltc_last_ray_count": (C.c_int, [_vp, C.POINTER(C.c_uint32)]),
     "risltc_proposal_cache": (C.c_int, [_vp, _u32, C.c_float, C.c_float, _u32]),
     "risltc_proposal_cache_read": (C.c_int, [_vp, _u32p, C.c_uint64]),
     "risltc_proposal_cache_write": (C.c_int, [_vp, _u32p, C.c_uint64]),
@@ -421,6 +422,12 @@ class Context:
         n = rays.numel() // 8
         self._check(self.L.risltc_trace_shadow_rays(self.h, _dptr(rays), n, _dptr(hits), _stream(stream)),
                     "risltc_trace_shadow_rays")
+
+    def last_ray_count(self):
+        """Shadow rays traced by the last shade call."""
+        n = C.c_uint32(0)
+        self._check(self.L.risltc_last_ray_count(self.h, C.byref(n)), "risltc_last_ray_count")
+        return n.value
 
     def last_timings(self):
         ms = (C.c_float * 4)()

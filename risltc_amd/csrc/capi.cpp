@@ -134,6 +134,8 @@ struct risltc_context_s {
     // timing
     hipEvent_t ev[8] = {};
     bool timed = false;
+    // the shadow-ray counter of the last risltc_shade or risltc_shade_noalloc call
+    const uint32_t* last_ray_counter = nullptr;
     bool learned_ran = false;
     // this device's copy of the fixed scratch array of risltc_shade_noalloc
     uint8_t* noalloc_scratch = nullptr;
@@ -1605,6 +1607,7 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     HIPC(c, hipEventRecord(c->ev[3], st));
     c->timed = true;
     c->learned_ran = false;
+    c->last_ray_counter = wk.counter;
     return RISLTC_OK;
 }
 
@@ -1786,6 +1789,7 @@ int risltc_shade_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t
     if (e) return e;
     c->timed = true;
     c->learned_ran = false;
+    c->last_ray_counter = nullptr;  // a batch's rays are not counted per call
     return RISLTC_OK;
 }
 
@@ -1999,6 +2003,16 @@ int risltc_accumulate(risltc_context_t* c, float* accum, const float* frame, uin
     if (count == 0) return RISLTC_OK;
     HIPC(c, hipSetDevice(c->device));
     HIPC(c, rl::launch_accumulate(accum, frame, count, n, (hipStream_t)stream));
+    return RISLTC_OK;
+}
+
+int risltc_last_ray_count(const risltc_context_t* c, uint32_t* out) {
+    if (!c || !out) return RISLTC_E_ARG;
+    *out = 0;
+    if (!c->last_ray_counter || !c->timed) return RISLTC_E_STATE;
+    if (hipEventSynchronize(c->ev[3]) != hipSuccess) return RISLTC_E_HIP;
+    if (hipMemcpy(out, c->last_ray_counter, sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return RISLTC_E_HIP;
     return RISLTC_OK;
 }
 
