@@ -1114,27 +1114,25 @@ __device__ __forceinline__ bool culled(const Cull& k, float4 s) {
     return hd < -k.dm && hc < -k.cm;
 }
 
-// The noise LCG advanced by d steps (d < 128) in 7 affine jumps: stepping
-// twice with (a, c) is one step with (a * a, a * c + c).
-struct LcgStep {
-    uint32_t a, c;
+// The noise LCG advanced by d steps (d < 128) in one affine step: d steps
+// take s to A_d * s + C_d, with A_0 = 1, C_0 = 0, A_{d+1} = a * A_d and
+// C_{d+1} = a * C_d + c. The 128 pairs are a constant table (1 KiB).
+struct LcgJumps {
+    uint32_t ac[256];  // A_d at 2d, C_d at 2d + 1
+    constexpr LcgJumps() : ac() {
+        uint32_t a = 1u, c = 0u;
+        for (int d = 0; d < 128; ++d) {
+            ac[2 * d] = a;
+            ac[2 * d + 1] = c;
+            a = 1664525u * a;
+            c = 1664525u * c + 1013904223u;
+        }
+    }
 };
-constexpr LcgStep lcg_pow2(int b) {
-    LcgStep r{1664525u, 1013904223u};
-    for (int i = 0; i < b; ++i) r = LcgStep{r.a * r.a, r.a * r.c + r.c};
-    return r;
-}
+__constant__ __align__(8) LcgJumps c_lcg_jumps = LcgJumps();
 __device__ __forceinline__ uint32_t lcg_jump(uint32_t s, uint32_t d) {
-    constexpr LcgStep j0 = lcg_pow2(0), j1 = lcg_pow2(1), j2 = lcg_pow2(2), j3 = lcg_pow2(3),
-                      j4 = lcg_pow2(4), j5 = lcg_pow2(5), j6 = lcg_pow2(6);
-    if (d & 1u) s = j0.a * s + j0.c;
-    if (d & 2u) s = j1.a * s + j1.c;
-    if (d & 4u) s = j2.a * s + j2.c;
-    if (d & 8u) s = j3.a * s + j3.c;
-    if (d & 16u) s = j4.a * s + j4.c;
-    if (d & 32u) s = j5.a * s + j5.c;
-    if (d & 64u) s = j6.a * s + j6.c;
-    return s;
+    const uint2 j = reinterpret_cast<const uint2*>(c_lcg_jumps.ac)[d];
+    return j.x * s + j.y;
 }
 
 // ---- learned proposal: candidate draws ------------------------------------------
@@ -1645,10 +1643,11 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
         } else {
             // Candidates in chunks: every lane draws its chunk's light
             // indices and cull tests, then the wave evaluates the survivors
-            // compacted (see above) and each owner runs its reservoir over
-            // its survivors in their original order, the noise state jumped
-            // to each. Culled candidates have w = 0 and leave the reservoir
-            // as it was.
+            // compacted (see above): the evaluator jumps the owner's noise
+            // state to the candidate and draws its light and its reservoir
+            // number, and each owner runs its reservoir over its survivors
+            // in their original order. Culled candidates have w = 0 and
+            // leave the reservoir as it was.
             const uint32_t chunk = A.chunk;
             for (uint32_t base = 0; base < M; base += chunk) {
                 const uint32_t cnt = (M - base < chunk) ? M - base : chunk;
@@ -1663,8 +1662,6 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                     }
                 }
                 const uint32_t mycnt = (uint32_t)__popc(live);
-                // owner cursor over its own survivors
-                uint32_t s = seed_chunk, at = 0, live_rem = live;
                 // wave cursor: the next item is rank lvl_off of level lvl
                 uint32_t lvl = 0, lvl_off = 0;
                 uint64_t lvl_mask = __ballot(mycnt > 0u);
@@ -1703,12 +1700,14 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                         for (int k = 0; k < 4; ++k) pt_o.ws[k] = (uint32_t)__shfl((int)pt.ws[k], (int)my_o);
                         pt_o.T = (uint32_t)__shfl((int)pt.T, (int)my_o);
                     }
-                    float p_raw = 0.0f;
+                    float p_raw = 0.0f, r_e = 0.0f;
                     uint32_t idx_e = 0u;
                     if (have) {
                         const uint32_t t = nth_set_bit32(live_o, my_k);
                         uint32_t se = lcg_jump(seed_o, stride * t);
                         idx_e = learned ? draw_learned(se, pt_o, cl, N) : light_index_from(noise_next(se), N);
+                        // the candidate's reservoir draw, taken by its owner
+                        r_e = noise_next(se);
                         Pixel q;
                         px_lds_load_eval<X>(pxa, my_o, q);
                         load_light(A.scene.lights, idx_e, L);
@@ -1728,21 +1727,14 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                             const int src = here ? (int)(ls + rank - oo) : (int)lane;
                             const float ph = __shfl(p_raw, src);
                             const uint32_t idx = (uint32_t)__shfl((int)idx_e, src);
+                            const float r = __shfl(r_e, src);
                             if (here) {
-                                const uint32_t t = (uint32_t)__builtin_ctz(live_rem);
-                                live_rem &= live_rem - 1u;
-                                s = lcg_jump(s, stride * (t - at));
-                                at = t + 1u;
-                                // past the light draws
-                                (void)noise_next(s);
-                                if (learned) (void)noise_next(s);
                                 float p_hat = ph;
                                 if (cached)
                                     p_hat = p_hat * cache_factor(cl.of_light, idx, A.learned_floor, A.learned_scale,
                                                                  crow, reinterpret_cast<const uint8_t*>(sh_spheres),
                                                                  lds_row, A.cache.per_light);
                                 float w = div_<X>(p_hat, learned ? learned_density<X>(idx, pt, cl, N) : pN);
-                                float r = noise_next(s);
                                 w_sum += w;
                                 bool take = X ? (r < w / w_sum) : (r * w_sum < w);
                                 if (w > 0.0f && take) {
