@@ -121,14 +121,30 @@ template <bool X>
 __device__ __forceinline__ float edge_term(f3 a, f3 b) {
     float x = dot3(a, b);
     float y = fabs_(x);
-    float A = 0.8543985f + (0.4965155f + 0.0145206f * y) * y;
-    float B = 3.4175940f + (4.1616724f + y) * y;
-    float v = div_<X>(A, B);
-    float ts = v;
-    // edges longer than 90 degrees are rare: keep the reciprocal square
-    // root off the common path
-    if (!(x > 0.0f)) ts = 0.5f * rsq_<X>(fmax_(1.0f - x * x, 1e-7f)) - v;
-    return (a.x * b.y - a.y * b.x) * ts;
+    if constexpr (X) {
+        float A = 0.8543985f + (0.4965155f + 0.0145206f * y) * y;
+        float B = 3.4175940f + (4.1616724f + y) * y;
+        float v = div_<X>(A, B);
+        float ts = v;
+        if (!(x > 0.0f)) ts = 0.5f * rsq_<X>(fmax_(1.0f - x * x, 1e-7f)) - v;
+        return (a.x * b.y - a.y * b.x) * ts;
+    } else {
+        // the same rational and cross product in fused form; edges longer
+        // than 90 degrees are rare, so the reciprocal square root runs only
+        // when a lane of the wave has one
+        float A = fma_(fma_(0.0145206f, y, 0.4965155f), y, 0.8543985f);
+        float B = fma_(y + 4.1616724f, y, 3.4175940f);
+        float v = div_<X>(A, B);
+        float ts = v;
+        const bool longer = !(x > 0.0f);
+        if (__builtin_amdgcn_ballot_w64(longer) != 0ull) {
+            // (an empty volatile statement keeps the compiler from
+            // speculating this block onto the common path)
+            asm volatile("" ::);
+            if (longer) ts = 0.5f * rsq_<X>(fmax_(fma_(-x, x, 1.0f), 1e-7f)) - v;
+        }
+        return fma_(a.x, b.y, -(a.y * b.x)) * ts;
+    }
 }
 
 // Clip (polygon_clipping.glsl:35-225) + calculate_ltc (polygon_sampling.glsl:
