@@ -155,12 +155,13 @@ __device__ __forceinline__ float edge_term(f3 a, f3 b) {
 template <bool X>
 __device__ __forceinline__ float polygon_ltc(const float* px, const float* py, const float* pz, uint32_t n,
                                              uint32_t& vc) {
+    const uint32_t full = (1u << n) - 1u;
     uint32_t mask = 0;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) mask |= ((uint32_t)i < n && pz[i] > 0.0f) ? (1u << i) : 0u;
+    for (int i = 0; i < 8; ++i) mask |= (pz[i] > 0.0f) ? (1u << i) : 0u;
+    mask &= full;
     vc = 0;
     if (mask == 0) return 0.0f;
-    uint32_t full = (1u << n) - 1u;
     uint32_t k = __builtin_popcount(mask);
     bool clipped = mask != full;
     uint32_t L = clipped ? k + 2u : n;
@@ -196,6 +197,8 @@ __device__ __forceinline__ float polygon_ltc(const float* px, const float* py, c
             f3 P = na;
             if (in_a != in_b) {
                 P = normalize3<X>(iz0<X>(a, b));
+                // on the horizon (0 * finite; the fast build states it)
+                if constexpr (!X) P.z = 0.0f;
                 if (in_b) E = P;
                 else Xp = P;
             }
@@ -249,11 +252,12 @@ __device__ __forceinline__ f3 pick8(const float* vx, const float* vy, const floa
 // with every slot past the count equal to slot 0.
 template <bool X>
 __device__ __forceinline__ uint32_t clip_materialize(float* vx, float* vy, float* vz, uint32_t n) {
+    const uint32_t full = (1u << n) - 1u;
     uint32_t mask = 0;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) mask |= ((uint32_t)i < n && vz[i] > 0.0f) ? (1u << i) : 0u;
+    for (int i = 0; i < 8; ++i) mask |= (vz[i] > 0.0f) ? (1u << i) : 0u;
+    mask &= full;
     if (mask == 0) return 0;
-    uint32_t full = (1u << n) - 1u;
     if (mask == full) {
 #pragma unroll
         for (int i = 1; i < MAXV; ++i)
@@ -795,8 +799,12 @@ __device__ __forceinline__ uint32_t light_index_from(float u, uint32_t n) {
 // evaluate_polygonal_light_shading, LTC_CP branch: the RIS target
 template <bool X>
 __device__ __forceinline__ f3 eval_ltc_cp(const Pixel& px, const Light& L, uint32_t& clip_counts) {
-    float side = plane_side(px.pos, L.plane);
-    bool flip = side < 0.0f;
+    // The mirrored polygon of a shading point behind the light (y negated)
+    // has the same clip masks and negated edge terms, so the same |sum|:
+    // bit for bit, since negation is exact and rounding is symmetric. The
+    // exact build mirrors as the reference does (its cosine-space matrix is
+    // composed with the mirror); the fast build skips it.
+    const bool flip = X && plane_side(px.pos, L.plane) < 0.0f;
     float vx[9], vy[9], vz[9];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
