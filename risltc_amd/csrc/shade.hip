@@ -29,6 +29,14 @@ __constant__ uint8_t c_clip_rot[6 * 256];
 
 constexpr int MAXV = 9;
 
+// four floats in LDS, for reads the compiler must emit as LDS instructions
+typedef float F4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(3))) F4v LdsF4;
+__device__ __forceinline__ float4 lds_f4(const LdsF4* p, uint32_t i) {
+    const F4v v = p[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 struct Pixel {
     f3 pos, n, o;
     float lo;
@@ -252,12 +260,13 @@ __device__ __forceinline__ f3 pick8(const float* vx, const float* vy, const floa
 // with every slot past the count equal to slot 0.
 template <bool X>
 __device__ __forceinline__ uint32_t clip_materialize(float* vx, float* vy, float* vz, uint32_t n) {
-    const uint32_t full = (1u << n) - 1u;
+    // (the count tests per slot keep the register allocation of the final
+    // sample lower than masking with `full` afterwards)
     uint32_t mask = 0;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) mask |= (vz[i] > 0.0f) ? (1u << i) : 0u;
-    mask &= full;
+    for (int i = 0; i < 8; ++i) mask |= ((uint32_t)i < n && vz[i] > 0.0f) ? (1u << i) : 0u;
     if (mask == 0) return 0;
+    const uint32_t full = (1u << n) - 1u;
     if (mask == full) {
 #pragma unroll
         for (int i = 1; i < MAXV; ++i)
@@ -1678,10 +1687,13 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                 const uint32_t seed_chunk = seed;
                 uint32_t live = 0;
                 if (shade) {
+                    const LdsF4* sl = (const LdsF4*)sh_spheres;
                     for (uint32_t t = 0; t < cnt; ++t) {
                         uint32_t idx = learned ? draw_learned(seed, pt, cl, N) : light_index_from(noise_next(seed), N);
                         (void)noise_next(seed);
-                        float4 sp = all_lds ? sh_spheres[idx] : (idx < n_lds ? sh_spheres[idx] : A.scene.spheres[idx]);
+                        // (every sphere in LDS: an LDS read)
+                        float4 sp = all_lds ? lds_f4(sl, idx)
+                                            : (idx < n_lds ? sh_spheres[idx] : A.scene.spheres[idx]);
                         live |= culled(cull, sp) ? 0u : (1u << t);
                     }
                 }
