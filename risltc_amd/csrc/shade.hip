@@ -1390,6 +1390,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 // to shading matrix, the LTC matrix and its albedo.
 constexpr uint32_t kPxFloats = 28;
 constexpr uint32_t kPxLdsBytes = 4u * kPxFloats * 64u * 4u;  // 4 waves
+// after the pixels: each wave's deal table of a round, one word per lane
+// (owner lane | k << 8 of the item the lane evaluates)
+constexpr uint32_t kDealLdsBytes = 4u * 64u * 4u;
 
 __device__ __forceinline__ void px_lds_store(float* b, uint32_t l, const Pixel& p) {
     b[0u * 64u + l] = p.pos.x;
@@ -1440,12 +1443,6 @@ __device__ __forceinline__ uint32_t nth_set_bit32(uint32_t m, uint32_t n) {
         }
     }
     return pos;
-}
-
-__device__ __forceinline__ uint32_t nth_set_bit64(uint64_t m, uint32_t n) {
-    const uint32_t lo = (uint32_t)m, c = (uint32_t)__popc(lo);
-    if (n >= c) return 32u + nth_set_bit32((uint32_t)(m >> 32), n - c);
-    return nth_set_bit32(lo, n);
 }
 
 // LDS offset of the pixel area: after the spheres, cluster tables and rows
@@ -1580,12 +1577,13 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
     const bool cached = learned && A.cache.vis != nullptr;
     // the wave's pixels in LDS for the compacted evaluation (non-debug)
     float* pxa = nullptr;
+    uint32_t* deal = nullptr;
     if constexpr (!DEBUG) {
         const bool rows = learned && A.cache.vis != nullptr && A.cache.slots <= kRowLdsMax;
-        pxa = reinterpret_cast<float*>(reinterpret_cast<uint8_t*>(sh_spheres) +
-                                       px_lds_offset(n_lds, learned ? A.learned_pass : 0u, tab_n, A.clusters.count,
-                                                     rows)) +
-              wave * kPxFloats * 64u;
+        uint8_t* px_base = reinterpret_cast<uint8_t*>(sh_spheres) +
+                           px_lds_offset(n_lds, learned ? A.learned_pass : 0u, tab_n, A.clusters.count, rows);
+        pxa = reinterpret_cast<float*>(px_base) + wave * kPxFloats * 64u;
+        deal = reinterpret_cast<uint32_t*>(px_base + kPxLdsBytes) + wave * 64u;
         px_lds_store(pxa, lane, px);
         wave_lds_sync();
     }
@@ -1702,22 +1700,31 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                 uint32_t lvl = 0, lvl_off = 0;
                 uint64_t lvl_mask = __ballot(mycnt > 0u);
                 while (lvl_mask != 0ull) {
-                    // deal this round's items: lane e takes the e-th remaining item
+                    // deal this round's items: lane e takes the e-th remaining
+                    // item; each owner writes its items' (owner, k) into the
+                    // deal table at the evaluating lane
                     uint32_t my_k = 0xFFFFFFFFu, my_o = lane;
                     {
                         uint32_t kk = lvl, oo = lvl_off, ls = 0;
                         uint64_t m = lvl_mask;
+                        const uint64_t below = (1ull << lane) - 1ull;
                         while (ls < 64u && m != 0ull) {
                             const uint32_t n = (uint32_t)__popcll(m) - oo;
-                            if (lane >= ls && lane - ls < n) {
-                                my_k = kk;
-                                my_o = nth_set_bit64(m, oo + (lane - ls));
-                            }
+                            const uint32_t rank = (uint32_t)__popcll(m & below);
+                            if (((m >> lane) & 1ull) != 0ull && rank >= oo && ls + (rank - oo) < 64u)
+                                deal[ls + (rank - oo)] = lane | (kk << 8);
                             ls += n;
                             kk += 1u;
                             oo = 0u;
                             m = __ballot(mycnt > kk);
                         }
+                        wave_lds_sync();
+                        if (lane < ls) {
+                            const uint32_t v = deal[lane];
+                            my_o = v & 63u;
+                            my_k = v >> 8;
+                        }
+                        wave_lds_sync();
                     }
                     const bool have = my_k != 0xFFFFFFFFu;
                     const uint32_t live_o = (uint32_t)__shfl((int)live, (int)my_o);
@@ -2239,8 +2246,8 @@ size_t shade_lds_bytes(const ShadeArgs& a) {
     const uint32_t tables = lds_table_lights(a.learned_pass, a.scene.light_count, a.clusters.count);
     // learned pass with a cache: staged rows, kRowsPerWave per wave
     const bool rows = a.learned_pass && a.cache.vis && a.cache.slots <= kRowLdsMax;
-    // then the wave's pixels for the compacted evaluation
-    return px_lds_offset(n, a.learned_pass, tables, a.clusters.count, rows) + kPxLdsBytes;
+    // then the wave's pixels and deal tables for the compacted evaluation
+    return px_lds_offset(n, a.learned_pass, tables, a.clusters.count, rows) + kPxLdsBytes + kDealLdsBytes;
 }
 
 static bool shade_args_ok(const ShadeArgs& a) {
