@@ -1427,24 +1427,6 @@ __device__ __forceinline__ void px_lds_load_eval(const float* b, uint32_t l, Pix
     p.ltc_albedo = b[27u * 64u + l];
 }
 
-// position of the n-th (from 0) set bit of m; m has more than n set bits
-__device__ __forceinline__ uint32_t nth_set_bit32(uint32_t m, uint32_t n) {
-    uint32_t pos = 0;
-#pragma unroll
-    for (uint32_t w = 16; w >= 1u; w >>= 1) {
-        const uint32_t lo = m & ((1u << w) - 1u);
-        const uint32_t c = (uint32_t)__popc(lo);
-        if (n >= c) {
-            n -= c;
-            m >>= w;
-            pos += w;
-        } else {
-            m = lo;
-        }
-    }
-    return pos;
-}
-
 // LDS offset of the pixel area: after the spheres, cluster tables and rows
 __device__ __host__ __forceinline__ size_t px_lds_offset(uint32_t n_lds, uint32_t learned_pass, uint32_t table_lights,
                                                          uint32_t clusters, bool rows) {
@@ -1696,14 +1678,17 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                     }
                 }
                 const uint32_t mycnt = (uint32_t)__popc(live);
+                // the owner's survivors not yet dealt (they are dealt in order)
+                uint32_t live_rem = live;
                 // wave cursor: the next item is rank lvl_off of level lvl
                 uint32_t lvl = 0, lvl_off = 0;
                 uint64_t lvl_mask = __ballot(mycnt > 0u);
                 while (lvl_mask != 0ull) {
                     // deal this round's items: lane e takes the e-th remaining
-                    // item; each owner writes its items' (owner, k) into the
-                    // deal table at the evaluating lane
-                    uint32_t my_k = 0xFFFFFFFFu, my_o = lane;
+                    // item; each owner writes its items' (owner, candidate)
+                    // into the deal table at the evaluating lane
+                    uint32_t my_t = 0u, my_o = lane;
+                    bool have = false;
                     {
                         uint32_t kk = lvl, oo = lvl_off, ls = 0;
                         uint64_t m = lvl_mask;
@@ -1711,8 +1696,11 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                         while (ls < 64u && m != 0ull) {
                             const uint32_t n = (uint32_t)__popcll(m) - oo;
                             const uint32_t rank = (uint32_t)__popcll(m & below);
-                            if (((m >> lane) & 1ull) != 0ull && rank >= oo && ls + (rank - oo) < 64u)
-                                deal[ls + (rank - oo)] = lane | (kk << 8);
+                            if (((m >> lane) & 1ull) != 0ull && rank >= oo && ls + (rank - oo) < 64u) {
+                                const uint32_t t = (uint32_t)__builtin_ctz(live_rem);
+                                live_rem &= live_rem - 1u;
+                                deal[ls + (rank - oo)] = lane | (t << 8);
+                            }
                             ls += n;
                             kk += 1u;
                             oo = 0u;
@@ -1722,12 +1710,11 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                         if (lane < ls) {
                             const uint32_t v = deal[lane];
                             my_o = v & 63u;
-                            my_k = v >> 8;
+                            my_t = v >> 8;
+                            have = true;
                         }
                         wave_lds_sync();
                     }
-                    const bool have = my_k != 0xFFFFFFFFu;
-                    const uint32_t live_o = (uint32_t)__shfl((int)live, (int)my_o);
                     const uint32_t seed_o = (uint32_t)__shfl((int)seed_chunk, (int)my_o);
                     PropTable pt_o;
                     pt_o.T = 0u;
@@ -1746,8 +1733,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                     float p_raw = 0.0f, r_e = 0.0f;
                     uint32_t idx_e = 0u;
                     if (have) {
-                        const uint32_t t = nth_set_bit32(live_o, my_k);
-                        uint32_t se = lcg_jump(seed_o, stride * t);
+                        uint32_t se = lcg_jump(seed_o, stride * my_t);
                         idx_e = learned ? draw_learned(se, pt_o, cl, N) : light_index_from(noise_next(se), N);
                         // the candidate's reservoir draw, taken by its owner
                         r_e = noise_next(se);
