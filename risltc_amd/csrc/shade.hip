@@ -1427,6 +1427,19 @@ __device__ __forceinline__ void px_lds_load_eval(const float* b, uint32_t l, Pix
     p.ltc_albedo = b[27u * 64u + l];
 }
 
+// make_cull of the lane's pixel as stored by px_lds_store
+__device__ __forceinline__ Cull cull_from_lds(const float* b, uint32_t l, float bound) {
+    Pixel p = {};
+    p.pos.x = b[0u * 64u + l];
+    p.pos.y = b[1u * 64u + l];
+    p.pos.z = b[2u * 64u + l];
+#pragma unroll
+    for (uint32_t i = 0; i < 12u; ++i) p.w2s[i] = b[(6u + i) * 64u + l];
+    p.s2c[6] = b[24u * 64u + l];
+    p.s2c[8] = b[26u * 64u + l];
+    return make_cull(p, bound);
+}
+
 // LDS offset of the pixel area: after the spheres, cluster tables and rows
 __device__ __host__ __forceinline__ size_t px_lds_offset(uint32_t n_lds, uint32_t learned_pass, uint32_t table_lights,
                                                          uint32_t clusters, bool rows) {
@@ -1505,7 +1518,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
     bool shade = inside && (prim >> 31) == 0u && (uint64_t)prim < A.scene.triangle_count;
 
     Pixel px = {};
-    Cull cull;
+    Cull cull = {};
     uint32_t seed = 0;
     if (shade) {
         const float(*P)[4] = A.c.pixel_to_ray_direction_world_space;
@@ -1514,7 +1527,8 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                      fma_(P[1][2], 1.0f, fma_(P[1][1], fy, P[1][0] * fx)),
                      fma_(P[2][2], 1.0f, fma_(P[2][1], fy, P[2][0] * fx)));
         setup_pixel(A, prim, ray, px);
-        cull = make_cull(px, A.scene.sphere_bound);
+        // (the compacted kernels derive it per sample from the pixel in LDS)
+        if constexpr (DEBUG) cull = make_cull(px, A.scene.sphere_bound);
         seed = murmur_finalize(murmur_mix(murmur_mix(0u, x + y * W), A.c.noise_random_numbers[0]));
     }
     const uint32_t ckey = (shade && A.cache.counts) ? cache_key(A.cache, px.pos, px.n) : 0u;
@@ -1662,6 +1676,9 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
             // in their original order. Culled candidates have w = 0 and
             // leave the reservoir as it was.
             const uint32_t chunk = A.chunk;
+            // the cull planes, from the pixel in LDS: not held in registers
+            // across the final sample
+            const Cull cull_s = cull_from_lds(pxa, lane, A.scene.sphere_bound);
             for (uint32_t base = 0; base < M; base += chunk) {
                 const uint32_t cnt = (M - base < chunk) ? M - base : chunk;
                 const uint32_t seed_chunk = seed;
@@ -1674,7 +1691,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                         // (every sphere in LDS: an LDS read)
                         float4 sp = all_lds ? lds_f4(sl, idx)
                                             : (idx < n_lds ? sh_spheres[idx] : A.scene.spheres[idx]);
-                        live |= culled(cull, sp) ? 0u : (1u << t);
+                        live |= culled(cull_s, sp) ? 0u : (1u << t);
                     }
                 }
                 const uint32_t mycnt = (uint32_t)__popc(live);
