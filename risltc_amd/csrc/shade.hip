@@ -913,6 +913,15 @@ __device__ __forceinline__ void eval_peters(const ShadeArgs& A, const Pixel& px,
     prepare_psa<X>(P, vcd, vx, vy, vz, fa);
     F.psad = P.psa;
     float psad = P.psa;
+    // the diffuse direction first (its two draws precede the specular ones),
+    // so that the specular vertices are not live while it is sampled
+    f3 dirs[2];
+    dirs[0] = mk3(0.0f, 0.0f, 0.0f);
+    if (psad != 0.0f) {
+        float u0 = noise_next(seed);
+        float u1 = noise_next(seed);
+        dirs[0] = sample_psa<X>(P, mk2(u0, u1), fa);
+    }
     // transform for the specular polygon now (the diffuse vertices are dead)
     float m[12];
     world_to_cosine(px, flip, m);
@@ -937,12 +946,6 @@ __device__ __forceinline__ void eval_peters(const ShadeArgs& A, const Pixel& px,
             F.psas = S.psa;
         }
         return;
-    }
-    f3 dirs[2];
-    {
-        float u0 = noise_next(seed);
-        float u1 = noise_next(seed);
-        dirs[0] = sample_psa<X>(P, mk2(u0, u1), fa);
     }
     float psas = 0.0f;
     dirs[1] = mk3(0.0f, 0.0f, 0.0f);
