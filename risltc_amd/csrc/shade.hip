@@ -1042,22 +1042,27 @@ __device__ __forceinline__ void clear_final(FinalOut& F) {
 __device__ __forceinline__ void put_final(const ShadeArgs& A, uint32_t lane, bool inside, bool contributes,
                                           const FinalOut& F, f3 pos, size_t rec_idx, float Wt, uint32_t extra,
                                           int32_t chosen, uint32_t ckey) {
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        bool emit = contributes && F.state[t] == 5u;
-        uint64_t ballot = __ballot(emit);
-        uint32_t cnt = __popcll(ballot);
+    // a lane's two rays (the same light, the same origin) go next to each
+    // other in the list, so they are traced by neighbouring lanes
+    const bool emit0 = contributes && F.state[0] == 5u, emit1 = contributes && F.state[1] == 5u;
+    const uint64_t b0 = __ballot(emit0), b1 = __ballot(emit1);
+    const uint32_t cnt = (uint32_t)(__popcll(b0) + __popcll(b1));
+    if (cnt) {
+        const uint64_t lanes = b0 | b1;
+        const uint32_t first = (uint32_t)__ffsll((unsigned long long)lanes) - 1u;
         uint32_t base = 0;
-        if (cnt) {
-            uint32_t first = (uint32_t)__ffsll((unsigned long long)ballot) - 1u;
-            if (lane == first) base = atomicAdd(A.ray_count, cnt);
-            base = __shfl(base, (int)first);
-        }
-        if (emit) {
-            uint32_t off = base + __popcll(ballot & ((1ull << lane) - 1ull));
-            A.rays[off].org = make_float4(pos.x, pos.y, pos.z, __uint_as_float(A.bvh_index));
-            A.rays[off].dir = make_float4(F.dir[t].x, F.dir[t].y, F.dir[t].z, F.tmax[t]);
-            A.ray_slots[off] = A.slot_base + (uint32_t)(rec_idx * 2 + t);
+        if (lane == first) base = atomicAdd(A.ray_count, cnt);
+        base = (uint32_t)__shfl((int)base, (int)first);
+        const uint64_t below = (1ull << lane) - 1ull;
+        uint32_t off = base + (uint32_t)(__popcll(b0 & below) + __popcll(b1 & below));
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            if (t == 0 ? emit0 : emit1) {
+                A.rays[off].org = make_float4(pos.x, pos.y, pos.z, __uint_as_float(A.bvh_index));
+                A.rays[off].dir = make_float4(F.dir[t].x, F.dir[t].y, F.dir[t].z, F.tmax[t]);
+                A.ray_slots[off] = A.slot_base + (uint32_t)(rec_idx * 2 + t);
+                ++off;
+            }
         }
     }
     if (inside) {
