@@ -258,8 +258,12 @@ __device__ __forceinline__ f3 pick8(const float* vx, const float* vy, const floa
 
 // In place: vx/vy/vz hold n vertices on entry; on return the clipped polygon
 // with every slot past the count equal to slot 0.
-template <bool X>
-__device__ __forceinline__ uint32_t clip_materialize(float* vx, float* vy, float* vz, uint32_t n) {
+// With LDS, the input vertices are indexed through the lane's column of the
+// wave's LDS pixel area (col[f * 64] is field f; 24 fields) instead of
+// selects, which the compiler would otherwise turn into scratch memory.
+template <bool X, bool LDS = false>
+__device__ __forceinline__ uint32_t clip_materialize(float* vx, float* vy, float* vz, uint32_t n,
+                                                     float* col = nullptr) {
     // (the count tests per slot keep the register allocation of the final
     // sample lower than masking with `full` afterwards)
     uint32_t mask = 0;
@@ -286,8 +290,24 @@ __device__ __forceinline__ uint32_t clip_materialize(float* vx, float* vy, float
     uint32_t e = s + k - 1u;
     if (e >= n) e -= n;
     uint32_t e1 = (e + 1u >= n) ? e + 1u - n : e + 1u;
-    f3 E = iz0<X>(pick8(vx, vy, vz, sm1), pick8(vx, vy, vz, s));
-    f3 Xp = iz0<X>(pick8(vx, vy, vz, e), pick8(vx, vy, vz, e1));
+    if constexpr (LDS) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            col[(3 * i + 0) * 64] = vx[i];
+            col[(3 * i + 1) * 64] = vy[i];
+            col[(3 * i + 2) * 64] = vz[i];
+        }
+    }
+    auto pick = [&](uint32_t idx) -> f3 {
+        if constexpr (LDS) {
+            const float* q = col + idx * 192u;
+            return mk3(q[0], q[64], q[128]);
+        } else {
+            return pick8(vx, vy, vz, idx);
+        }
+    };
+    f3 E = iz0<X>(pick(sm1), pick(s));
+    f3 Xp = iz0<X>(pick(e), pick(e1));
     float ox[MAXV], oy[MAXV], oz[MAXV];
 #pragma unroll
     for (int j = 0; j < MAXV; ++j) {
@@ -300,7 +320,7 @@ __device__ __forceinline__ uint32_t clip_materialize(float* vx, float* vy, float
         else {
             uint32_t src = s + c - 1u;
             if (src >= n) src -= n;
-            r = pick8(vx, vy, vz, src);
+            r = pick(src);
         }
         ox[j] = r.x;
         oy[j] = r.y;
@@ -881,8 +901,9 @@ struct FinalOut {
 // caller scales the sum by 1 / k. One projected-solid-angle polygon is live
 // at a time (diffuse prepared and sampled, then specular), which consumes the
 // noise in the reference order.
-template <bool X>
-__device__ __forceinline__ void eval_peters(const ShadeArgs& A, const Pixel& px, const Light& L, uint32_t& seed, FinalOut& F) {
+template <bool X, bool LDS = false>
+__device__ __forceinline__ void eval_peters(const ShadeArgs& A, const Pixel& px, const Light& L, uint32_t& seed, FinalOut& F,
+                                            float* col = nullptr) {
     bool fa = A.fast_atan != 0;
     F.color = mk3(0.0f, 0.0f, 0.0f);
     F.state[0] = F.state[1] = 0u;
@@ -906,7 +927,7 @@ __device__ __forceinline__ void eval_peters(const ShadeArgs& A, const Pixel& px,
     vx[8] = vx[0];
     vy[8] = vy[0];
     vz[8] = vz[0];
-    uint32_t vcd = clip_materialize<X>(vx, vy, vz, L.n);
+    uint32_t vcd = clip_materialize<X, LDS>(vx, vy, vz, L.n, col);
     F.clipd = vcd;
     if (vcd == 0) return;
     Psa P;
@@ -935,7 +956,7 @@ __device__ __forceinline__ void eval_peters(const ShadeArgs& A, const Pixel& px,
     vx[8] = vx[0];
     vy[8] = vy[0];
     vz[8] = vz[0];
-    uint32_t vcs = clip_materialize<X>(vx, vy, vz, L.n);
+    uint32_t vcs = clip_materialize<X, LDS>(vx, vy, vz, L.n, col);
     F.clips = vcs;
     if (psad == 0.0f) {
         // the reference still prepares the specular polygon; only its
@@ -1833,13 +1854,17 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
         // again (the same arithmetic, no draws) and takes one direction per
         // technique, so the draws follow the reference's loop
         const uint32_t iters = DEBUG ? A.iters : 1u;
+        // the compacted kernels lend the lane's column of the LDS pixel area
+        // to the final sample's clipping, and store the pixel again after it
+        if constexpr (!DEBUG) wave_lds_sync();
         for (uint32_t it = 0; it < iters; ++it) {
             FinalOut F;
             clear_final(F);
             // the light is read per iteration so that it is not held across the loop
             if (contributes) {
                 load_light(A.scene.lights, (uint32_t)chosen, L);
-                eval_peters<X>(A, px, L, seed, F);
+                if constexpr (DEBUG) eval_peters<X>(A, px, L, seed, F);
+                else eval_peters<X, true>(A, px, L, seed, F, pxa + lane);
             }
             put_final(A, lane, inside, contributes, F, px.pos, sidx * iters + it, Wt, 0u, chosen, ckey);
             if constexpr (DEBUG) {
@@ -1851,6 +1876,11 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
         if constexpr (DEBUG) {
             if (inside && A.debug_samples)
                 A.debug_samples[sidx * RISLTC_SAMPLE_RECORD_FLOATS + 33] = __uint_as_float(seed);
+        } else {
+            if (j + 1u < spp) {
+                px_lds_store(pxa, lane, px);
+                wave_lds_sync();
+            }
         }
     }
 }
