@@ -360,6 +360,15 @@ void free_bvh(BvhBuild* b) {
 }
 
 // ---- traversal -------------------------------------------------------------------
+// four floats in global memory, for loads the compiler must emit as global
+// (not flat) instructions
+typedef float F4g __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) F4g GlobalF4;
+__device__ __forceinline__ float4 global_f4(const GlobalF4* p, int i) {
+    const F4g v = p[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ bool ray_box(float3 o, float3 inv, float4 bmin, float4 bmax, float tmin, float tmax) {
     float t0 = (bmin.x - o.x) * inv.x, t1 = (bmax.x - o.x) * inv.x;
     tmin = fmax_(tmin, fmin_(t0, t1));
@@ -501,7 +510,9 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs A) {
 #pragma unroll 1
         for (int it = 0; it < 8; ++it) {
             if (active) {
-                float4 a = rec[x], b = rec[x + 1], c = rec[x + 2];
+                // (global-memory loads: the hierarchy is device memory)
+                const GlobalF4* g = (const GlobalF4*)rec;
+                float4 a = global_f4(g, x), b = global_f4(g, x + 1), c = global_f4(g, x + 2);
                 int w = __float_as_int(a.w);
                 bool hit = false;
                 if (w < 0) {
