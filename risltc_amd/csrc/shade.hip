@@ -16,6 +16,7 @@
 // per-candidate and final-sample math. The per-pixel setup (shading frame,
 // LTC lookup) is always exact, so clip decisions agree bit for bit in both.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "devmath.hpp"
 #include "gbuffer.hpp"
@@ -2152,27 +2153,33 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
             // them), merged per slot before the atomics: up to 4 pending
             uint32_t pslot[4] = {0u, 0u, 0u, 0u}, np = 0u, pkey = 0u;
             unsigned long long padd[4] = {0ull, 0ull, 0ull, 0ull};
-            for (uint32_t j = 0; j < A.spp; ++j) {
+            // one sample: its records and hit flags are either loaded here or
+            // (PRE: one iteration, at most 4 samples) were loaded for all of
+            // the pixel's samples together, so that their loads overlap
+            auto sample = [&](uint32_t j, auto pre, const float4* Q, uint2 Hh) {
+                constexpr bool PRE = decltype(pre)::value;
                 size_t sidx = (size_t)pixel * A.spp + j;
                 const float4* rec = reinterpret_cast<const float4*>(A.samples + sidx * K * kSampleWords);
-                float4 r0 = rec[0], r3 = rec[3];
+                float4 r0 = PRE ? Q[0] : rec[0], r3 = PRE ? Q[3] : rec[3];
                 uint32_t flags = __float_as_uint(r0.y);
                 if (!(flags & kFlagContributes)) {
                     if (A.rec_cluster) {
                         A.rec_cluster[sidx] = 0xFFFFFFFFu;
                         A.rec_y[sidx] = 0.0f;
                     }
-                    continue;
+                    return;
                 }
                 // joint visibility: the sample counts only if none of its
                 // shadow rays is occluded
                 bool all_vis = true;
                 if (flags & kFlagJoint) {
                     for (uint32_t it = 0; it < K; ++it) {
-                        const uint32_t fl = __float_as_uint(rec[4 * it].y);
+                        const uint32_t fl = __float_as_uint((PRE && it == 0u) ? Q[0].y : rec[4 * it].y);
 #pragma unroll
-                        for (int t = 0; t < 2; ++t)
-                            if (((fl >> (4 * t)) & 15u) == 5u && A.hits[(sidx * K + it) * 2 + t] != 0u) all_vis = false;
+                        for (int t = 0; t < 2; ++t) {
+                            const uint32_t h = (PRE && it == 0u) ? (t == 0 ? Hh.x : Hh.y) : A.hits[(sidx * K + it) * 2 + t];
+                            if (((fl >> (4 * t)) & 15u) == 5u && h != 0u) all_vis = false;
+                        }
                     }
                 }
                 f3 result = mk3(0.0f, 0.0f, 0.0f);
@@ -2181,7 +2188,8 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
                 // iterations, then techniques: the reference's summation order
                 for (uint32_t it = 0; it < K; ++it) {
                     const float4* ri = rec + 4 * it;
-                    float4 q0 = ri[0], q1 = ri[1], q2 = ri[2], q3 = ri[3];
+                    const bool pq = PRE && it == 0u;
+                    float4 q0 = pq ? Q[0] : ri[0], q1 = pq ? Q[1] : ri[1], q2 = pq ? Q[2] : ri[2], q3 = pq ? Q[3] : ri[3];
                     const uint32_t fl = __float_as_uint(q0.y);
                     f3 ev[2] = {mk3(q0.z, q0.w, q1.x), mk3(q1.y, q1.z, q1.w)};
                     f3 eo[2] = {mk3(q2.x, q2.y, q2.z), mk3(q2.w, q3.x, q3.y)};
@@ -2189,7 +2197,7 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
                     for (int t = 0; t < 2; ++t) {
                         uint32_t st = (fl >> (4 * t)) & 15u;
                         if (st == 0u) continue;
-                        bool hit = st == 5u && A.hits[(sidx * K + it) * 2 + t] != 0u;
+                        bool hit = st == 5u && (pq ? (t == 0 ? Hh.x : Hh.y) : A.hits[(sidx * K + it) * 2 + t]) != 0u;
                         traced += st == 5u ? 1u : 0u;
                         unoccluded += (st == 5u && !hit) ? 1u : 0u;
                         bool vis = (fl & kFlagJoint) ? all_vis : (st == 4u || (st == 5u && !hit));
@@ -2243,6 +2251,25 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
                         }
                     }
                 }
+                        };
+            if (K == 1u && A.spp <= 4u) {
+                float4 Q[4][4];
+                uint2 HH[4];
+#pragma unroll
+                for (uint32_t j = 0; j < 4u; ++j) {
+                    if (j < A.spp) {
+                        const size_t sidx = (size_t)pixel * A.spp + j;
+                        const float4* rec = reinterpret_cast<const float4*>(A.samples + sidx * kSampleWords);
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) Q[j][k] = rec[k];
+                        HH[j] = reinterpret_cast<const uint2*>(A.hits)[sidx];
+                    }
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < 4u; ++j)
+                    if (j < A.spp) sample(j, std::true_type{}, Q[j], HH[j]);
+            } else {
+                for (uint32_t j = 0; j < A.spp; ++j) sample(j, std::false_type{}, nullptr, make_uint2(0u, 0u));
             }
             if (np > 0u) {
                 flush_counts(A.cache, pkey, pslot, padd, np);
