@@ -1886,9 +1886,9 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
     }
 }
 
-// Three waves per SIMD: the register budget is 168 VGPRs (the final-sample
-// code spills); the measured gain over two waves comes from hiding the
-// latency of the candidate evaluation.
+// Three waves per SIMD: the register budget is 168 VGPRs (the compacted
+// kernels spill at most a few of them); the measured gain over two waves
+// comes from hiding the latency of the candidate evaluation.
 template <bool X, bool DEBUG, int LMODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) shade_kernel(ShadeArgs A) {
     extern __shared__ float4 sh_spheres[];
