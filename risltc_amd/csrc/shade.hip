@@ -826,6 +826,16 @@ __device__ __forceinline__ uint32_t light_index_from(float u, uint32_t n) {
     return (uint32_t)idx;
 }
 
+// light_index_from(noise_next(seed), n) with kn = n * 2^-32: the noise value
+// is (float)seed * 2^-32 exactly (a power of two, no underflow), so
+// (float)seed * kn rounds the same real number as u * n; one product fewer.
+__device__ __forceinline__ uint32_t light_index_next(uint32_t& seed, float kn, uint32_t n) {
+    seed = 1664525u * seed + 1013904223u;
+    int idx = (int)((float)seed * kn);
+    if (idx > (int)n - 1) idx = (int)n - 1;
+    return (uint32_t)idx;
+}
+
 // evaluate_polygonal_light_shading, LTC_CP branch: the RIS target
 template <bool X>
 __device__ __forceinline__ f3 eval_ltc_cp(const Pixel& px, const Light& L, uint32_t& clip_counts) {
@@ -1597,6 +1607,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
     }
     const uint32_t N = A.scene.light_count;
     const float pN = 1.0f / (float)N;
+    const float kN = (float)N * (1.0f / 4294967296.0f);
     const uint32_t M = A.M;
     // noise draws per candidate: light (1, or 2 for the learned source) + reservoir
     const uint32_t stride = learned ? 3u : 2u;
@@ -1716,7 +1727,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                 if (shade) {
                     const LdsF4* sl = (const LdsF4*)sh_spheres;
                     for (uint32_t t = 0; t < cnt; ++t) {
-                        uint32_t idx = learned ? draw_learned(seed, pt, cl, N) : light_index_from(noise_next(seed), N);
+                        uint32_t idx = learned ? draw_learned(seed, pt, cl, N) : light_index_next(seed, kN, N);
                         (void)noise_next(seed);
                         // (every sphere in LDS: an LDS read)
                         float4 sp = all_lds ? lds_f4(sl, idx)
@@ -1781,7 +1792,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                     uint32_t idx_e = 0u;
                     if (have) {
                         uint32_t se = lcg_jump(seed_o, stride * my_t);
-                        idx_e = learned ? draw_learned(se, pt_o, cl, N) : light_index_from(noise_next(se), N);
+                        idx_e = learned ? draw_learned(se, pt_o, cl, N) : light_index_next(se, kN, N);
                         // the candidate's reservoir draw, taken by its owner
                         r_e = noise_next(se);
                         Pixel q;
