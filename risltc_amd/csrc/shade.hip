@@ -1433,6 +1433,16 @@ constexpr uint32_t kPxLdsBytes = 4u * kPxFloats * 64u * 4u;  // 4 waves
 // after the pixels: each wave's deal table of a round, one word per lane
 // (owner lane | k << 8 of the item the lane evaluates)
 constexpr uint32_t kDealLdsBytes = 4u * 64u * 4u;
+// the plain kernels' noise jumps (d < 64: two draws for each of at most 32
+// candidates of a chunk) staged in LDS after the deal tables, so that the
+// evaluator's jump waits on LDS rather than global memory
+constexpr uint32_t kJumpLdsEntries = 64, kJumpLdsBytes = kJumpLdsEntries * 8u;
+typedef uint32_t U2v __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(3))) U2v LdsU2;
+__device__ __forceinline__ uint32_t lcg_jump_lds(const uint32_t* tab, uint32_t s, uint32_t d) {
+    const U2v j = ((const LdsU2*)tab)[d];
+    return j.x * s + j.y;
+}
 
 __device__ __forceinline__ void px_lds_store(float* b, uint32_t l, const Pixel& p) {
     b[0u * 64u + l] = p.pos.x;
@@ -1523,8 +1533,17 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
     const bool all_lds = n_lds == A.scene.light_count;
     const uint32_t tab_n = learned ? lds_table_lights(A.learned_pass, A.scene.light_count, A.clusters.count) : 0u;
     const bool tab_o = learned && A.learned_pass && A.clusters.count > 0u;
-    if (n_lds > 0u || tab_o) {
+    // the plain compacted kernels' jump table
+    constexpr bool jt = !learned && !DEBUG;
+    uint32_t* jtab = nullptr;
+    if constexpr (jt)
+        jtab = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(sh_spheres) +
+                                           px_lds_offset(n_lds, 0u, 0u, A.clusters.count, false) + kPxLdsBytes +
+                                           kDealLdsBytes);
+    if (n_lds > 0u || tab_o || jt) {
         for (uint32_t k = threadIdx.x; k < n_lds; k += blockDim.x) sh_spheres[k] = A.scene.spheres[k];
+        if constexpr (jt)
+            for (uint32_t k = threadIdx.x; k < 2u * kJumpLdsEntries; k += blockDim.x) jtab[k] = c_lcg_jumps.ac[k];
         if constexpr (learned) {
             if (tab_o) {
                 // after the spheres: lights by cluster (u32, when staged),
@@ -1791,7 +1810,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                     float p_raw = 0.0f, r_e = 0.0f;
                     uint32_t idx_e = 0u;
                     if (have) {
-                        uint32_t se = lcg_jump(seed_o, stride * my_t);
+                        uint32_t se = jt ? lcg_jump_lds(jtab, seed_o, stride * my_t) : lcg_jump(seed_o, stride * my_t);
                         idx_e = learned ? draw_learned(se, pt_o, cl, N) : light_index_next(se, kN, N);
                         // the candidate's reservoir draw, taken by its owner
                         r_e = noise_next(se);
@@ -2326,7 +2345,8 @@ size_t shade_lds_bytes(const ShadeArgs& a) {
     // learned pass with a cache: staged rows, kRowsPerWave per wave
     const bool rows = a.learned_pass && a.cache.vis && a.cache.slots <= kRowLdsMax;
     // then the wave's pixels and deal tables for the compacted evaluation
-    return px_lds_offset(n, a.learned_pass, tables, a.clusters.count, rows) + kPxLdsBytes + kDealLdsBytes;
+    return px_lds_offset(n, a.learned_pass, tables, a.clusters.count, rows) + kPxLdsBytes + kDealLdsBytes +
+           (a.learned_pass ? 0u : kJumpLdsBytes);
 }
 
 static bool shade_args_ok(const ShadeArgs& a) {
