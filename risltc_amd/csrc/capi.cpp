@@ -1838,7 +1838,7 @@ int risltc_autotune(risltc_context_t* c, const risltc_view_t* views, uint32_t n,
     if (!c) return RISLTC_E_ARG;
     if (repeats == 0 || repeats > 1000) return fail(c, RISLTC_E_ARG, "repeats %u", repeats);
     HIPC(c, hipSetDevice(c->device));
-    const std::vector<uint32_t> space[4] = {{8, 16, 32, 64}, {8, 16, 32}, {0, 512, 1024, 2048}, {16, 24, 32, 48}};
+    const std::vector<uint32_t> space[4] = {{8, 16, 32, 64}, {8, 16, 32}, {0, 512, 1024, 2048}, {16, 24, 32, 40, 48}};
     risltc_tuning_t best = c->tuning;
     auto field = [](risltc_tuning_t& t, int i) -> uint32_t& {
         return i == 0 ? t.wave_width : i == 1 ? t.cull_chunk : i == 2 ? t.lds_lights : t.trace_refill;
