@@ -1439,6 +1439,7 @@ static void fill_shade(risltc_context_t* c, const risltc_frame_constants_t* k, c
     a.c = *k;
     a.spp = s->sample_count_light;
     a.iters = s->sample_count;
+    a.rec_words = rl::kSampleWords;
     a.M = s->ris_candidate_count;
     a.mis = s->mis_heuristic;
     a.light_sampling = s->light_sampling;
@@ -1480,6 +1481,7 @@ static void fill_resolve(const risltc_frame_constants_t* k, const risltc_setting
     r.height = k->viewport_size[1];
     r.spp = s->sample_count_light;
     r.iters = s->sample_count;
+    r.rec_words = rl::kSampleWords;
     r.triangle_count = triangle_count;
     r.exposure = k->exposure_factor;
     r.visibility = vis;
@@ -1500,6 +1502,16 @@ static rl::TraceArgs trace_args(risltc_context_t* c, const rl::BvhView* bvhs, rl
     t.queues = counter + rl::kQueueStride;
     t.refill = c->tuning.trace_refill;
     return t;
+}
+
+// The record size of a call (the buffers are sized for full records): compact
+// when the resolve needs neither the selected light nor the cache key and the
+// occluded estimates are +0 or NaN (every heuristic but the optimal one), for
+// the LTC_CP technique without debug records.
+static uint32_t record_words(const risltc_settings_t* s, bool light_or_key, bool debug) {
+    const bool compact = !light_or_key && !debug && s->mis_heuristic != RISLTC_MIS_OPTIMAL &&
+                         s->polygon_sampling_technique == RISLTC_SAMPLE_POLYGON_LTC_CP;
+    return compact ? rl::kSampleWordsCompact : (uint32_t)rl::kSampleWords;
 }
 
 // Per-call work buffers of one shading pass.
@@ -1571,6 +1583,8 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
             return fail(c, RISLTC_E_STATE, "the visibility cache was made for %u lights", c->vcache_slots);
         a.cache = vcache_view(c);
     }
+    const bool records_out = c->cluster_count > 0 && !s->skip_training_records && !fixed;
+    a.rec_words = record_words(s, records_out || cached, dbg_s != nullptr || dbg_c != nullptr);
     HIPC(c, rl::launch_zero_u32(wk.counter, kCounterWords, st));
     HIPC(c, hipEventRecord(c->ev[0], st));
     // the cells the previous resolve counted in get their new bytes (timed
@@ -1588,6 +1602,7 @@ static int shade_impl(risltc_context_t* c, const risltc_frame_constants_t* k, co
     HIPC(c, hipEventRecord(c->ev[2], st));
     rl::ResolveArgs r;
     fill_resolve(k, s, vis, a.samples, wk.hits, out, SR.triangle_count, r);
+    r.rec_words = a.rec_words;
     r.debug_samples = dbg_s;
     c->rec_samples = 0;
     if (c->cluster_count > 0) {
@@ -1715,12 +1730,14 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
             a.proposal_group_log2 = p.group_log2;
             if (p.cached) a.cache = p.cache;
         }
+        a.rec_words = record_words(&w.settings, p.cached, false);
         p.lds = std::max(p.lds, rl::shade_lds_bytes(a));
         p.tile_off.push_back(p.tile_off.back() + rl::shade_tiles(a));
         p.shade.push_back(a);
         rl::ResolveArgs r;
         fill_resolve(&w.constants, &w.settings, w.visibility, a.samples, B.hits.as<uint32_t>() + off[v] * 2, w.out,
                      SR.triangle_count, r);
+        r.rec_words = a.rec_words;
         if (p.cached) {
             r.cluster_of_light = c->cluster_of_light.as<uint16_t>();
             r.cache = p.cache;

@@ -1098,13 +1098,22 @@ __device__ __forceinline__ void put_final(const ShadeArgs& A, uint32_t lane, boo
         }
     }
     if (inside) {
-        float4* rec = reinterpret_cast<float4*>(A.samples + rec_idx * kSampleWords);
+        float4* rec = reinterpret_cast<float4*>(A.samples + rec_idx * A.rec_words);
         uint32_t flags = (F.state[0] & 15u) | ((F.state[1] & 15u) << 4) | (contributes ? kFlagContributes | extra : 0u);
-        rec[0] = make_float4(Wt, __uint_as_float(flags), F.e_vis[0].x, F.e_vis[0].y);
-        rec[1] = make_float4(F.e_vis[0].z, F.e_vis[1].x, F.e_vis[1].y, F.e_vis[1].z);
-        rec[2] = make_float4(F.e_occ[0].x, F.e_occ[0].y, F.e_occ[0].z, F.e_occ[1].x);
-        rec[3] = make_float4(F.e_occ[1].y, F.e_occ[1].z, __int_as_float(contributes ? chosen : -1),
-                             __uint_as_float(ckey));
+        if (A.rec_words == kSampleWordsCompact) {
+            // the occluded estimates are +0 or NaN: their NaN channels as flags
+            const float eo[6] = {F.e_occ[0].x, F.e_occ[0].y, F.e_occ[0].z, F.e_occ[1].x, F.e_occ[1].y, F.e_occ[1].z};
+#pragma unroll
+            for (int b = 0; b < 6; ++b) flags |= (eo[b] == 0.0f) ? 0u : (1u << (kFlagOccShift + b));
+            rec[0] = make_float4(Wt, __uint_as_float(flags), F.e_vis[0].x, F.e_vis[0].y);
+            rec[1] = make_float4(F.e_vis[0].z, F.e_vis[1].x, F.e_vis[1].y, F.e_vis[1].z);
+        } else {
+            rec[0] = make_float4(Wt, __uint_as_float(flags), F.e_vis[0].x, F.e_vis[0].y);
+            rec[1] = make_float4(F.e_vis[0].z, F.e_vis[1].x, F.e_vis[1].y, F.e_vis[1].z);
+            rec[2] = make_float4(F.e_occ[0].x, F.e_occ[0].y, F.e_occ[0].z, F.e_occ[1].x);
+            rec[3] = make_float4(F.e_occ[1].y, F.e_occ[1].z, __int_as_float(contributes ? chosen : -1),
+                                 __uint_as_float(ckey));
+        }
     }
 }
 
@@ -2179,6 +2188,9 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
             fin = mk3(1.0f, 1.0f, 1.0f);
         } else {
             const uint32_t K = A.iters;
+            // records of A.rec_words words (rw4 float4s)
+            const uint32_t rw4 = A.rec_words / 4u;
+            const bool compact = A.rec_words == kSampleWordsCompact;
             // the cache counts of the pixel's samples (one cell for all of
             // them), merged per slot before the atomics: up to 4 pending
             uint32_t pslot[4] = {0u, 0u, 0u, 0u}, np = 0u, pkey = 0u;
@@ -2189,8 +2201,11 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
             auto sample = [&](uint32_t j, auto pre, const float4* Q, uint2 Hh) {
                 constexpr bool PRE = decltype(pre)::value;
                 size_t sidx = (size_t)pixel * A.spp + j;
-                const float4* rec = reinterpret_cast<const float4*>(A.samples + sidx * K * kSampleWords);
-                float4 r0 = PRE ? Q[0] : rec[0], r3 = PRE ? Q[3] : rec[3];
+                const float4* rec = reinterpret_cast<const float4*>(A.samples + sidx * K * A.rec_words);
+                // (a compact record has no light and key: none is read then)
+                float4 r0 = PRE ? Q[0] : rec[0];
+                float4 r3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (!compact) r3 = PRE ? Q[3] : rec[3];
                 uint32_t flags = __float_as_uint(r0.y);
                 if (!(flags & kFlagContributes)) {
                     if (A.rec_cluster) {
@@ -2204,7 +2219,7 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
                 bool all_vis = true;
                 if (flags & kFlagJoint) {
                     for (uint32_t it = 0; it < K; ++it) {
-                        const uint32_t fl = __float_as_uint((PRE && it == 0u) ? Q[0].y : rec[4 * it].y);
+                        const uint32_t fl = __float_as_uint((PRE && it == 0u) ? Q[0].y : rec[rw4 * it].y);
 #pragma unroll
                         for (int t = 0; t < 2; ++t) {
                             const uint32_t h = (PRE && it == 0u) ? (t == 0 ? Hh.x : Hh.y) : A.hits[(sidx * K + it) * 2 + t];
@@ -2217,10 +2232,22 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
                 uint32_t traced = 0, unoccluded = 0;
                 // iterations, then techniques: the reference's summation order
                 for (uint32_t it = 0; it < K; ++it) {
-                    const float4* ri = rec + 4 * it;
+                    const float4* ri = rec + rw4 * it;
                     const bool pq = PRE && it == 0u;
-                    float4 q0 = pq ? Q[0] : ri[0], q1 = pq ? Q[1] : ri[1], q2 = pq ? Q[2] : ri[2], q3 = pq ? Q[3] : ri[3];
+                    float4 q0 = pq ? Q[0] : ri[0], q1 = pq ? Q[1] : ri[1], q2, q3;
                     const uint32_t fl = __float_as_uint(q0.y);
+                    if (compact) {
+                        // the occluded estimates: +0, NaN where flagged
+                        float e[6];
+#pragma unroll
+                        for (int b = 0; b < 6; ++b)
+                            e[b] = ((fl >> (kFlagOccShift + b)) & 1u) ? __builtin_nanf("") : 0.0f;
+                        q2 = make_float4(e[0], e[1], e[2], e[3]);
+                        q3 = make_float4(e[4], e[5], 0.0f, 0.0f);
+                    } else {
+                        q2 = pq ? Q[2] : ri[2];
+                        q3 = pq ? Q[3] : ri[3];
+                    }
                     f3 ev[2] = {mk3(q0.z, q0.w, q1.x), mk3(q1.y, q1.z, q1.w)};
                     f3 eo[2] = {mk3(q2.x, q2.y, q2.z), mk3(q2.w, q3.x, q3.y)};
 #pragma unroll
@@ -2289,9 +2316,9 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
                 for (uint32_t j = 0; j < 4u; ++j) {
                     if (j < A.spp) {
                         const size_t sidx = (size_t)pixel * A.spp + j;
-                        const float4* rec = reinterpret_cast<const float4*>(A.samples + sidx * kSampleWords);
+                        const float4* rec = reinterpret_cast<const float4*>(A.samples + sidx * A.rec_words);
 #pragma unroll
-                        for (int k = 0; k < 4; ++k) Q[j][k] = rec[k];
+                        for (int k = 0; k < 4; ++k) Q[j][k] = ((uint32_t)k < rw4) ? rec[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                         HH[j] = reinterpret_cast<const uint2*>(A.hits)[sidx];
                     }
                 }
