@@ -360,14 +360,16 @@ def kernel_report(kernel_ms, pixels, samples, rays, shade_alg, learned):
     the BF16 MFMA peak. Algorithmic bytes per launch: shade as the roofline
     object; trace 40 B per shadow ray (the ray and its slot read, the hit
     word written); resolve 4 B visibility + 16 B output per pixel, the
-    64-byte record of every pixel-sample and the hit word of every ray."""
+    record of every pixel-sample (32 B: the plain pass writes the compact
+    record, no light, key or occluded estimates) and the hit word of every
+    ray."""
     pmc = json.load(open(PMC_FILE)) if os.path.exists(PMC_FILE) else {}
     per = dict(pmc.get("kernels", {}))
     if "bytes_per_launch" in pmc:
         per["shade_kernel"] = {k: pmc[k] for k in ("bytes_per_launch", "valu_insts_per_launch",
                                                     "valu_thread_cycles_per_launch")}
     alg = {"shade_kernel": shade_alg, "trace_kernel": 40 * rays,
-           "resolve_kernel": 20 * pixels + 64 * samples + 4 * rays}
+           "resolve_kernel": 20 * pixels + 32 * samples + 4 * rays}
     out = {"shadow_rays": rays}
     for i, k in enumerate(("shade_kernel", "trace_kernel", "resolve_kernel")):
         ms = float(kernel_ms[i])

@@ -1071,6 +1071,8 @@ __device__ __forceinline__ void clear_final(FinalOut& F) {
 // appended to the compacted list (one atomic per wave and technique; every
 // lane of the wave calls this) and its record is written at index `rec_idx`
 // (pixel-sample * SAMPLE_COUNT + iteration).
+// COMPACT: the caller may be asked for compact records (LTC_CP kernels only)
+template <bool COMPACT>
 __device__ __forceinline__ void put_final(const ShadeArgs& A, uint32_t lane, bool inside, bool contributes,
                                           const FinalOut& F, f3 pos, size_t rec_idx, float Wt, uint32_t extra,
                                           int32_t chosen, uint32_t ckey) {
@@ -1100,7 +1102,7 @@ __device__ __forceinline__ void put_final(const ShadeArgs& A, uint32_t lane, boo
     if (inside) {
         float4* rec = reinterpret_cast<float4*>(A.samples + rec_idx * A.rec_words);
         uint32_t flags = (F.state[0] & 15u) | ((F.state[1] & 15u) << 4) | (contributes ? kFlagContributes | extra : 0u);
-        if (A.rec_words == kSampleWordsCompact) {
+        if (COMPACT && A.rec_words == kSampleWordsCompact) {
             // the occluded estimates are +0 or NaN: their NaN channels as flags
             const float eo[6] = {F.e_occ[0].x, F.e_occ[0].y, F.e_occ[0].z, F.e_occ[1].x, F.e_occ[1].y, F.e_occ[1].z};
 #pragma unroll
@@ -1906,7 +1908,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& A, uint32_t tile, fl
                 if constexpr (DEBUG) eval_peters<X>(A, px, L, seed, F);
                 else eval_peters<X, true>(A, px, L, seed, F, pxa + lane);
             }
-            put_final(A, lane, inside, contributes, F, px.pos, sidx * iters + it, Wt, 0u, chosen, ckey);
+            put_final<true>(A, lane, inside, contributes, F, px.pos, sidx * iters + it, Wt, 0u, chosen, ckey);
             if constexpr (DEBUG) {
                 if (it == 0u && inside && A.debug_samples)
                     put_debug(A.debug_samples + sidx * RISLTC_SAMPLE_RECORD_FLOATS, F, shade ? chosen : -1, w_sum,
@@ -2126,7 +2128,7 @@ __global__ void __launch_bounds__(256) shade_baseline_kernel(ShadeArgs A) {
                     eval_peters<X>(A, px, L, seed, F);
                 }
             }
-            put_final(A, lane, inside, contributes, F, px.pos, sidx * A.iters + it, Wt, joint, chosen, ckey);
+            put_final<false>(A, lane, inside, contributes, F, px.pos, sidx * A.iters + it, Wt, joint, chosen, ckey);
             if (it == 0u && inside && A.debug_samples)
                 put_debug(A.debug_samples + sidx * RISLTC_SAMPLE_RECORD_FLOATS, F, shade ? chosen : -1, w_sum,
                           sample_value, Wt, contributes, px.pos, ckey);
@@ -2410,7 +2412,8 @@ hipError_t launch_shade(const ShadeArgs& a, bool exact, hipStream_t s) {
 }
 
 hipError_t launch_shade_baseline(const ShadeArgs& a, uint32_t technique, bool exact, hipStream_t s) {
-    if (!shade_args_ok(a)) return hipErrorInvalidValue;
+    // (the baseline kernels write full records only)
+    if (!shade_args_ok(a) || a.rec_words != (uint32_t)kSampleWords) return hipErrorInvalidValue;
     dim3 grid(shade_tiles(a)), block(256);
     if (technique == RISLTC_SAMPLE_POLYGON_AREA_TURK) {
         if (exact) hipLaunchKernelGGL((shade_baseline_kernel<true, RISLTC_SAMPLE_POLYGON_AREA_TURK>), grid, block, 0, s, a);
