@@ -300,7 +300,10 @@ def test_fast_shading_parity(small_gpu):
 COMPACT_CASES = [dict(exact_math=1, sample_count_light=2, ris_candidate_count=8),
                  dict(exact_math=1, sample_count_light=1, ris_candidate_count=32, mis_heuristic=abi.MIS_BALANCE),
                  dict(exact_math=0, sample_count_light=3, ris_candidate_count=32),
-                 dict(exact_math=0, sample_count_light=1, ris_candidate_count=77, shadow_rays=0)]
+                 dict(exact_math=0, sample_count_light=1, ris_candidate_count=77, shadow_rays=0),
+                 dict(exact_math=1, sample_count_light=2, ris_candidate_count=16, mis_heuristic=abi.MIS_POWER),
+                 dict(exact_math=0, sample_count_light=1, ris_candidate_count=16, mis_heuristic=abi.MIS_WEIGHTED),
+                 dict(exact_math=1, sample_count_light=1, ris_candidate_count=16, mis_heuristic=abi.MIS_OPTIMAL)]
 
 
 @pytest.mark.parametrize("case", range(len(COMPACT_CASES)))
@@ -309,7 +312,11 @@ def test_compacted_evaluation_equals_debug(small_gpu, case, tuning):
     """The non-debug kernels evaluate the surviving candidates compacted over
     the wave; the debug kernels (bitwise equal to the oracle in the exact
     build) evaluate them per lane. The frames are equal bit for bit, in both
-    builds and under a non-default launch shape."""
+    builds and under a non-default launch shape. The non-debug pass also
+    writes the compact 8-word sample record (occluded estimates as NaN flag
+    bits) under every heuristic but the optimal one, the debug pass the full
+    16-word record, so these cases also pin the compact record against the
+    full one."""
     g = small_gpu
     s = g["scene"].settings(**COMPACT_CASES[case])
     ctx = g["ctx"]
