@@ -1504,14 +1504,16 @@ static rl::TraceArgs trace_args(risltc_context_t* c, const rl::BvhView* bvhs, rl
     return t;
 }
 
-// The record size of a call (the buffers are sized for full records): compact
-// when the resolve needs neither the selected light nor the cache key and the
-// occluded estimates are +0 or NaN (every heuristic but the optimal one), for
-// the LTC_CP technique without debug records.
+// The record size of a call (the buffers are sized for full records): for
+// the LTC_CP technique without debug records, under every heuristic but the
+// optimal one (the occluded estimates are then +0 or NaN and travel as flag
+// bits), 8 words, or 12 when the resolve reads the selected light or the
+// cache key; otherwise the full 16.
 static uint32_t record_words(const risltc_settings_t* s, bool light_or_key, bool debug) {
-    const bool compact = !light_or_key && !debug && s->mis_heuristic != RISLTC_MIS_OPTIMAL &&
+    const bool compact = !debug && s->mis_heuristic != RISLTC_MIS_OPTIMAL &&
                          s->polygon_sampling_technique == RISLTC_SAMPLE_POLYGON_LTC_CP;
-    return compact ? rl::kSampleWordsCompact : (uint32_t)rl::kSampleWords;
+    if (!compact) return (uint32_t)rl::kSampleWords;
+    return light_or_key ? rl::kSampleWordsKeyed : rl::kSampleWordsCompact;
 }
 
 // Per-call work buffers of one shading pass.

@@ -23,13 +23,14 @@ constexpr int kLightFloat4 = 8;
 //   [2..7]  estimate if visible, technique 0 then 1 (xyz, xyz)
 //   [8..13] estimate if occluded, technique 0 then 1
 //   [14] selected light (-1 = none), [15] cache key
-// The compact record (8 words: [0..7] above) serves a call that needs neither
-// the light nor the key (no training records, no visibility cache) and whose
-// MIS heuristic is not the optimal one: its occluded estimates are then the
-// heuristic's weight times the zero integrand, +0 or NaN per channel, and
-// flags bits 10 + 3 t + c mark the NaN ones (channel c of technique t).
+// The compact records serve a call whose MIS heuristic is not the optimal
+// one: its occluded estimates are then the heuristic's weight times the zero
+// integrand, +0 or NaN per channel, and flags bits 10 + 3 t + c mark the NaN
+// ones (channel c of technique t). 8 words ([0..7] above) when the call needs
+// neither the light nor the key (no training records, no visibility cache);
+// 12 words ([0..7], then [8] light, [9] key, [10..11] unused) when it does.
 constexpr int kSampleWords = 16;
-constexpr uint32_t kSampleWordsCompact = 8;
+constexpr uint32_t kSampleWordsCompact = 8, kSampleWordsKeyed = 12;
 constexpr uint32_t kFlagContributes = 256u, kFlagJoint = 512u, kFlagOccShift = 10u;
 
 // Shadow ray record (compacted list): float4 org.xyz + t_min, float4 dir.xyz + t_max,
@@ -206,7 +207,7 @@ struct ShadeArgs {
     risltc_frame_constants_t c;
     uint32_t spp, M, mis, light_sampling, fast_atan, shadow_rays;
     uint32_t iters;  // SAMPLE_COUNT: final-sample iterations per pixel-sample
-    uint32_t rec_words;  // record size: kSampleWords, or kSampleWordsCompact
+    uint32_t rec_words;  // record size: kSampleWords, kSampleWordsKeyed or kSampleWordsCompact
     // launch shape (tuned; none changes a result bit): a wave covers
     // 2^tile_w_log2 x (64 >> tile_w_log2) pixels, a work-group 4 waves stacked
     // vertically; candidates are culled in chunks of `chunk` (<= 32); the
@@ -264,7 +265,7 @@ struct TraceArgs {
 struct ResolveArgs {
     uint32_t width, height, spp;
     uint32_t iters;  // SAMPLE_COUNT: records per pixel-sample
-    uint32_t rec_words;  // record size: kSampleWords, or kSampleWordsCompact
+    uint32_t rec_words;  // record size: kSampleWords, kSampleWordsKeyed or kSampleWordsCompact
     uint64_t triangle_count;
     float exposure;
     const uint32_t* visibility;

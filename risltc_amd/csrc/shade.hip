@@ -1102,13 +1102,15 @@ __device__ __forceinline__ void put_final(const ShadeArgs& A, uint32_t lane, boo
     if (inside) {
         float4* rec = reinterpret_cast<float4*>(A.samples + rec_idx * A.rec_words);
         uint32_t flags = (F.state[0] & 15u) | ((F.state[1] & 15u) << 4) | (contributes ? kFlagContributes | extra : 0u);
-        if (COMPACT && A.rec_words == kSampleWordsCompact) {
+        if (COMPACT && A.rec_words != (uint32_t)kSampleWords) {
             // the occluded estimates are +0 or NaN: their NaN channels as flags
             const float eo[6] = {F.e_occ[0].x, F.e_occ[0].y, F.e_occ[0].z, F.e_occ[1].x, F.e_occ[1].y, F.e_occ[1].z};
 #pragma unroll
             for (int b = 0; b < 6; ++b) flags |= (eo[b] == 0.0f) ? 0u : (1u << (kFlagOccShift + b));
             rec[0] = make_float4(Wt, __uint_as_float(flags), F.e_vis[0].x, F.e_vis[0].y);
             rec[1] = make_float4(F.e_vis[0].z, F.e_vis[1].x, F.e_vis[1].y, F.e_vis[1].z);
+            if (A.rec_words == kSampleWordsKeyed)
+                rec[2] = make_float4(__int_as_float(contributes ? chosen : -1), __uint_as_float(ckey), 0.0f, 0.0f);
         } else {
             rec[0] = make_float4(Wt, __uint_as_float(flags), F.e_vis[0].x, F.e_vis[0].y);
             rec[1] = make_float4(F.e_vis[0].z, F.e_vis[1].x, F.e_vis[1].y, F.e_vis[1].z);
@@ -2192,7 +2194,7 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
             const uint32_t K = A.iters;
             // records of A.rec_words words (rw4 float4s)
             const uint32_t rw4 = A.rec_words / 4u;
-            const bool compact = A.rec_words == kSampleWordsCompact;
+            const bool compact = A.rec_words != (uint32_t)kSampleWords;
             // the cache counts of the pixel's samples (one cell for all of
             // them), merged per slot before the atomics: up to 4 pending
             uint32_t pslot[4] = {0u, 0u, 0u, 0u}, np = 0u, pkey = 0u;
@@ -2204,10 +2206,19 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
                 constexpr bool PRE = decltype(pre)::value;
                 size_t sidx = (size_t)pixel * A.spp + j;
                 const float4* rec = reinterpret_cast<const float4*>(A.samples + sidx * K * A.rec_words);
-                // (a compact record has no light and key: none is read then)
+                // the light and key (z, w): in the last float4 of a full
+                // record, the third of a keyed one, none in an 8-word one
+                // (Q[3] holds them in every case)
                 float4 r0 = PRE ? Q[0] : rec[0];
                 float4 r3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (!compact) r3 = PRE ? Q[3] : rec[3];
+                if (PRE) {
+                    r3 = Q[3];
+                } else if (rw4 == 4u) {
+                    r3 = rec[3];
+                } else if (rw4 == 3u) {
+                    const float4 t = rec[2];
+                    r3 = make_float4(0.0f, 0.0f, t.x, t.y);
+                }
                 uint32_t flags = __float_as_uint(r0.y);
                 if (!(flags & kFlagContributes)) {
                     if (A.rec_cluster) {
@@ -2321,6 +2332,7 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
                         const float4* rec = reinterpret_cast<const float4*>(A.samples + sidx * A.rec_words);
 #pragma unroll
                         for (int k = 0; k < 4; ++k) Q[j][k] = ((uint32_t)k < rw4) ? rec[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        if (rw4 == 3u) Q[j][3] = make_float4(0.0f, 0.0f, Q[j][2].x, Q[j][2].y);
                         HH[j] = reinterpret_cast<const uint2*>(A.hits)[sidx];
                     }
                 }
