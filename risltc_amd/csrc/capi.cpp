@@ -1640,6 +1640,7 @@ struct BatchPlan {
     size_t lds = 0;
     bool exact = false, shadows = false;
     bool general = false;  // some view has a SAMPLE_COUNT above 1
+    bool grouped = false;  // the resolve loads samples in groups (rl::resolve_grouped)
     bool learned = false, cached = false;  // learned views; the visibility cache is refreshed first
     uint32_t group_log2 = 0;                // table slots name groups of 1 << group_log2 clusters
     rl::VisCacheView cache = {};
@@ -1746,6 +1747,7 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
         }
         const uint32_t P = w.constants.viewport_size[0] * w.constants.viewport_size[1];
         p.block_off.push_back(p.block_off.back() + (P + 255u) / 256u);
+        p.grouped = p.grouped || rl::resolve_grouped(r);
         p.resolve.push_back(r);
     }
     p.tiles = p.tile_off.back();
@@ -1787,7 +1789,7 @@ static int launch_plan(risltc_context_t* c, BatchBuffers& B, const BatchPlan& p,
                                  st, p.trace_blocks));
     if (timed) HIPC(c, hipEventRecord(c->ev[2], st));
     HIPC(c, rl::launch_resolve_batch(B.resolve_args.as<rl::ResolveArgs>(), B.block_offsets.as<uint32_t>(), n, p.blocks,
-                                     st));
+                                     p.grouped, st));
     if (timed) HIPC(c, hipEventRecord(c->ev[3], st));
     return RISLTC_OK;
 }

@@ -2172,6 +2172,10 @@ __device__ __forceinline__ void flush_counts(const VisCacheView& C, uint32_t key
             atomicAdd(reinterpret_cast<unsigned long long*>(C.counts + (size_t)key * C.slots + slot[q]), add[q]);
 }
 
+// GROUPED: a pixel's samples (one iteration) are loaded in groups of 4
+// whatever their count; otherwise only up to 4 samples are (one group), more
+// are loaded one by one
+template <bool GROUPED>
 __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pixel) {
     if (pixel >= A.width * A.height) return;
     uint32_t prim = A.visibility[pixel];
@@ -2200,8 +2204,8 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
             uint32_t pslot[4] = {0u, 0u, 0u, 0u}, np = 0u, pkey = 0u;
             unsigned long long padd[4] = {0ull, 0ull, 0ull, 0ull};
             // one sample: its records and hit flags are either loaded here or
-            // (PRE: one iteration, at most 4 samples) were loaded for all of
-            // the pixel's samples together, so that their loads overlap
+            // (PRE: one iteration) were loaded with those of up to 3 more of
+            // the pixel's samples, so that their loads overlap
             auto sample = [&](uint32_t j, auto pre, const float4* Q, uint2 Hh) {
                 constexpr bool PRE = decltype(pre)::value;
                 size_t sidx = (size_t)pixel * A.spp + j;
@@ -2322,23 +2326,30 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
                     }
                 }
                         };
-            if (K == 1u && A.spp <= 4u) {
-                float4 Q[4][4];
-                uint2 HH[4];
+            if (K == 1u && (GROUPED || A.spp <= 4u)) {
+                // groups of up to 4 samples, each group's loads issued
+                // together; the samples are summed in order
+                const uint32_t groups = GROUPED ? (A.spp + 3u) / 4u : 1u;
+                for (uint32_t g = 0; g < groups; ++g) {
+                    const uint32_t j0 = 4u * g;
+                    float4 Q[4][4];
+                    uint2 HH[4];
 #pragma unroll
-                for (uint32_t j = 0; j < 4u; ++j) {
-                    if (j < A.spp) {
-                        const size_t sidx = (size_t)pixel * A.spp + j;
-                        const float4* rec = reinterpret_cast<const float4*>(A.samples + sidx * A.rec_words);
+                    for (uint32_t jj = 0; jj < 4u; ++jj) {
+                        if (j0 + jj < A.spp) {
+                            const size_t sidx = (size_t)pixel * A.spp + j0 + jj;
+                            const float4* rec = reinterpret_cast<const float4*>(A.samples + sidx * A.rec_words);
 #pragma unroll
-                        for (int k = 0; k < 4; ++k) Q[j][k] = ((uint32_t)k < rw4) ? rec[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                        if (rw4 == 3u) Q[j][3] = make_float4(0.0f, 0.0f, Q[j][2].x, Q[j][2].y);
-                        HH[j] = reinterpret_cast<const uint2*>(A.hits)[sidx];
+                            for (int k = 0; k < 4; ++k)
+                                Q[jj][k] = ((uint32_t)k < rw4) ? rec[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                            if (rw4 == 3u) Q[jj][3] = make_float4(0.0f, 0.0f, Q[jj][2].x, Q[jj][2].y);
+                            HH[jj] = reinterpret_cast<const uint2*>(A.hits)[sidx];
+                        }
                     }
-                }
 #pragma unroll
-                for (uint32_t j = 0; j < 4u; ++j)
-                    if (j < A.spp) sample(j, std::true_type{}, Q[j], HH[j]);
+                    for (uint32_t jj = 0; jj < 4u; ++jj)
+                        if (j0 + jj < A.spp) sample(j0 + jj, std::true_type{}, Q[jj], HH[jj]);
+                }
             } else {
                 for (uint32_t j = 0; j < A.spp; ++j) sample(j, std::false_type{}, nullptr, make_uint2(0u, 0u));
             }
@@ -2355,14 +2366,16 @@ __device__ __forceinline__ void resolve_pixel(const ResolveArgs& A, uint32_t pix
     reinterpret_cast<float4*>(A.out)[pixel] = out;
 }
 
+template <bool GROUPED>
 __global__ void __launch_bounds__(256) resolve_kernel(ResolveArgs A) {
-    resolve_pixel(A, blockIdx.x * blockDim.x + threadIdx.x);
+    resolve_pixel<GROUPED>(A, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+template <bool GROUPED>
 __global__ void __launch_bounds__(256) resolve_batch_kernel(const ResolveArgs* views, const uint32_t* block_offsets,
                                                             uint32_t count) {
     const uint32_t v = view_of(block_offsets, count, blockIdx.x);
-    resolve_pixel(views[v], (blockIdx.x - block_offsets[v]) * blockDim.x + threadIdx.x);
+    resolve_pixel<GROUPED>(views[v], (blockIdx.x - block_offsets[v]) * blockDim.x + threadIdx.x);
 }
 
 // accum_pass.frag.glsl:45-53: out = fma(prev, n, cur) * (1 / (n + 1))
@@ -2466,15 +2479,23 @@ hipError_t launch_shade_batch(const ShadeArgs* d_views, const uint32_t* d_tile_o
 }
 
 hipError_t launch_resolve_batch(const ResolveArgs* d_views, const uint32_t* d_block_offsets, uint32_t count,
-                                uint32_t total_blocks, hipStream_t s) {
+                                uint32_t total_blocks, bool grouped, hipStream_t s) {
     if (count == 0 || total_blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(resolve_batch_kernel, dim3(total_blocks), dim3(256), 0, s, d_views, d_block_offsets, count);
+    if (grouped)
+        hipLaunchKernelGGL(resolve_batch_kernel<true>, dim3(total_blocks), dim3(256), 0, s, d_views, d_block_offsets,
+                           count);
+    else
+        hipLaunchKernelGGL(resolve_batch_kernel<false>, dim3(total_blocks), dim3(256), 0, s, d_views, d_block_offsets,
+                           count);
     return hipGetLastError();
 }
 
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s) {
     uint32_t n = a.width * a.height;
-    hipLaunchKernelGGL(resolve_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, a);
+    if (resolve_grouped(a))
+        hipLaunchKernelGGL(resolve_kernel<true>, dim3((n + 255u) / 256u), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(resolve_kernel<false>, dim3((n + 255u) / 256u), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -378,6 +378,38 @@ def test_wide_training_step_gradient_and_adam(learned_wide):
     _check_training_step(learned_wide)
 
 
+@pytest.mark.parametrize("spp", [3, 7])
+def test_keyed_records_equal_full_records(learned, spp):
+    """The cached learned pass writes the 12-word keyed sample record (light,
+    key, occluded estimates as flag bits); with debug records it writes the
+    full 16-word one. From the same empty cache both give the same frame and
+    the same cache counts, bit for bit (at 7 samples per pixel through the
+    resolve that loads samples in groups of 4)."""
+    import torch
+    g = learned
+    sc, ctx, P = g["scene"], g["ctx"], g["P"]
+    res = []
+    try:
+        for debug in (False, True):
+            ctx.proposal_cache(9, 0.75, 2.0, True)
+            c = ctx.frame_constants(noise_seed=9, animate=True)
+            s = sc.settings(sample_count_light=spp, ris_candidate_count=8, proposal=abi.PROPOSAL_LEARNED)
+            out = torch.empty(P * 4, dtype=torch.float32, device="cuda")
+            if debug:
+                rec = torch.zeros(P * spp * abi.SAMPLE_RECORD_FLOATS, dtype=torch.float32, device="cuda")
+                ctx.shade_debug(c, s, g["vis"], out, rec)
+            else:
+                ctx.shade(c, s, g["vis"], out)
+            torch.cuda.synchronize()
+            res.append((out.cpu().numpy().copy(), ctx.proposal_cache_read().copy()))
+    finally:
+        ctx.proposal_cache(0)
+    (o0, n0), (o1, n1) = res
+    assert np.array_equal(o0.view(np.uint32), o1.view(np.uint32))
+    assert np.array_equal(n0, n1)
+    assert n0.sum() > 0.2 * P
+
+
 def _check_training_step(learned):
     import torch
     g = learned
@@ -502,15 +534,17 @@ def test_cached_learned_shading_bitwise_vs_oracle(learned, learned_odd, per_ligh
     assert np.array_equal(o.view(np.uint32), ref.view(np.uint32))
 
 
+@pytest.mark.parametrize("spp", [2, 7])
 @pytest.mark.parametrize("per_light", [False, True])
-def test_resolve_counts_every_traced_ray(learned, per_light):
+def test_resolve_counts_every_traced_ray(learned, per_light, spp):
     """Two frames with shadow rays: the cache holds, per (key, slot), the
     unoccluded and occluded shadow rays of the samples the debug records
-    list, summed over both frames."""
+    list, summed over both frames. At 7 samples per pixel the resolve loads
+    them in groups of 4 (the last one ragged) and merges more than 4 pending
+    slots."""
     import torch
     g = learned
     sc, ctx, P = g["scene"], g["ctx"], g["P"]
-    spp = 2
     ctx.proposal_cache(9, 0.75, 2.0, per_light)
     try:
         want = np.zeros((1 << 9, sc.light_count if per_light else abi.MAX_LIGHT_CLUSTERS, 2), dtype=np.int64)
