@@ -301,12 +301,12 @@ hipError_t launch_shade_batch(const ShadeArgs* d_views, const uint32_t* d_tile_o
 // of 4 for any sample count (resolve_grouped of some view)
 hipError_t launch_resolve_batch(const ResolveArgs* d_views, const uint32_t* d_block_offsets, uint32_t count,
                                 uint32_t total_blocks, bool grouped, hipStream_t s);
-// whether a resolve call loads its samples in groups of 4 beyond 4 samples
-// per pixel: with one iteration, when the cache counts or training records
-// make each sample's work long enough for overlapped loads to pay (a plain
-// pass streams faster with the lighter instantiation)
+// whether a resolve call takes the instantiation that loads a pixel's
+// samples in groups of 4 (one iteration): measured faster for up to 4
+// samples per pixel, and kept for the cache counts or training records; a
+// plain pass with more samples streams faster one sample at a time
 inline bool resolve_grouped(const ResolveArgs& a) {
-    return a.iters == 1u && a.spp > 4u && (a.cache.counts != nullptr || a.rec_cluster != nullptr);
+    return a.iters == 1u && (a.spp <= 4u || a.cache.counts != nullptr || a.rec_cluster != nullptr);
 }
 hipError_t launch_trace(const TraceArgs& a, hipStream_t s);
 // the resident grid size of the trace kernel, and a launch with that size
