@@ -1747,7 +1747,9 @@ static int plan_batch(risltc_context_t* c, const risltc_view_t* views, uint32_t 
         }
         const uint32_t P = w.constants.viewport_size[0] * w.constants.viewport_size[1];
         p.block_off.push_back(p.block_off.back() + (P + 255u) / 256u);
-        p.grouped = p.grouped || rl::resolve_grouped(r);
+        // (the batch kernel reads its views' arguments from memory: its
+        // grouped form measured slower at up to 4 samples per pixel)
+        p.grouped = p.grouped || (rl::resolve_grouped(r) && r.spp > 4u);
         p.resolve.push_back(r);
     }
     p.tiles = p.tile_off.back();

@@ -298,7 +298,8 @@ size_t shade_lds_bytes(const ShadeArgs& a);
 hipError_t launch_shade_batch(const ShadeArgs* d_views, const uint32_t* d_tile_offsets, uint32_t count,
                               uint32_t total_tiles, size_t lds, bool exact, int mode, bool general, hipStream_t s);
 // grouped: the resolve instantiation that loads a pixel's samples in groups
-// of 4 for any sample count (resolve_grouped of some view)
+// of 4 for any sample count (some view beyond 4 samples per pixel for which
+// resolve_grouped holds)
 hipError_t launch_resolve_batch(const ResolveArgs* d_views, const uint32_t* d_block_offsets, uint32_t count,
                                 uint32_t total_blocks, bool grouped, hipStream_t s);
 // whether a resolve call takes the instantiation that loads a pixel's
